@@ -1,0 +1,259 @@
+"""Oracle for the FCPW query path -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes front end to ``liboracle.so`` (built from ``fcpw_oracle.c`` by
+``oracle/Makefile``), the plain-C restatement of the reference FCPW CPU path
+(``include/fcpw/aggregates/bvh.inl``, ``geometry/triangles.inl``,
+``geometry/line_segments.inl``, ``fcpw.inl``) and of the reference GPU
+traversal order (``include/fcpw/gpu/bvh.slang``), plus an FP64 brute force.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+package ``fcpw_amd`` never imports it.
+
+Pinning (each function's check lives in tests/test_oracle_*.py):
+  * primitive closest point  -- closed-form Voronoi-region answers and the FP64
+    brute force, which uses a different algorithm (plane projection + edge
+    clamping) from the region test it checks;
+  * BVH build                -- structural invariants (every primitive in one
+    leaf, leaf size, boxes equal the union of children bit for bit, DFS layout)
+    and traversal == brute force within the reference's own 1e-6
+    (tests/aggregate_tests.cpp:562);
+  * CPU and GPU traversal    -- equal to brute force (index except at FP64 ties);
+  * rays                     -- analytic axis-aligned hits and the FP64 brute force;
+  * refit                    -- bit equality with a fresh build's boxes on the
+    same topology.
+Bit equality with the reference binary itself is "parity unpinned": the Eigen
+commit the reference pins is unknown and nothing in its tests fixes FP32 bits.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+HIT_DTYPE = np.dtype([("p", "<f4", (3,)), ("n", "<f4", (3,)), ("uv", "<f4", (2,)),
+                      ("d", "<f4"), ("index", "<i4")])
+assert HIT_DTYPE.itemsize == 40
+
+LONGEST_AXIS_CENTER, SURFACE_AREA, OVERLAP_SURFACE_AREA, VOLUME, OVERLAP_VOLUME = range(5)
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_hitp = np.ctypeslib.ndpointer(dtype=HIT_DTYPE, flags="C_CONTIGUOUS")
+
+
+def _opt(ptr_type):
+    """ndpointer that also accepts None."""
+    class _Opt(ptr_type):
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return ptr_type.from_param(obj)
+    return _Opt
+
+
+def _declare(L):
+    c_int, c_void = ctypes.c_int, ctypes.c_void_p
+    L.oracle_build.restype = c_void
+    L.oracle_build.argtypes = [c_int, _f32p, c_int, _i32p, c_int, c_int, c_int, c_int]
+    L.oracle_free.argtypes = [c_void]
+    for name in ("oracle_num_nodes", "oracle_num_leaves", "oracle_max_depth"):
+        getattr(L, name).restype = c_int
+        getattr(L, name).argtypes = [c_void]
+    L.oracle_export_nodes.argtypes = [c_void, _f32p, _f32p, _i32p, _i32p]
+    L.oracle_export_prim_order.argtypes = [c_void, _i32p]
+    L.oracle_export_index_map.argtypes = [c_void, _i32p]
+    L.oracle_update_vertices.argtypes = [c_void, _f32p, c_int]
+    L.oracle_refit.argtypes = [c_void]
+    oi32 = _opt(_i32p)
+    L.oracle_closest_points_cpu.argtypes = [c_void, _f32p, _f32p, c_int, c_int, c_int, _hitp]
+    L.oracle_closest_points_gpuorder.argtypes = [c_void, _f32p, _f32p, c_int, c_int, c_int, _hitp,
+                                                 oi32, oi32, oi32]
+    L.oracle_intersect_cpu.argtypes = [c_void, _f32p, _f32p, _f32p, c_int, c_int, c_int, _hitp]
+    L.oracle_intersect_gpuorder.argtypes = [c_void, _f32p, _f32p, _f32p, c_int, c_int, c_int, _hitp,
+                                            oi32, oi32]
+    L.oracle_brute_closest_f32.argtypes = [c_void, _f32p, _f32p, c_int, c_int, _hitp]
+    L.oracle_brute_closest_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, c_int, c_int,
+                                           _f64p, _f64p, _i32p]
+    L.oracle_brute_ray_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, _f32p, c_int, c_int,
+                                       _f64p, _i32p]
+    L.oracle_prim_dist_f64.argtypes = [c_int, _f32p, _i32p, _f32p, _i32p, c_int, _f64p]
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if cols is not None:
+        a = a.reshape(-1, cols)
+    return a
+
+
+class Bvh:
+    """Reference Scene<DIM> with one object, built with a non-vectorized BVH."""
+
+    def __init__(self, positions, indices, heuristic=SURFACE_AREA, leaf_size=4, n_buckets=8):
+        self.positions = _f32(positions)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int32)
+        self.dim = self.positions.shape[1]
+        assert self.indices.shape[1] == self.dim, "triangles need dim 3, segments dim 2"
+        self._h = lib().oracle_build(self.dim, self.positions, len(self.positions), self.indices,
+                                     len(self.indices), heuristic, leaf_size, n_buckets)
+        if not self._h:
+            raise ValueError("oracle_build rejected the input")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_free(self._h)
+            self._h = None
+
+    @property
+    def n_nodes(self):
+        return lib().oracle_num_nodes(self._h)
+
+    @property
+    def n_leaves(self):
+        return lib().oracle_num_leaves(self._h)
+
+    @property
+    def max_depth(self):
+        return lib().oracle_max_depth(self._h)
+
+    def nodes(self):
+        n = self.n_nodes
+        pmin = np.zeros((n, 3), np.float32)
+        pmax = np.zeros((n, 3), np.float32)
+        nrefs = np.zeros(n, np.int32)
+        offset = np.zeros(n, np.int32)
+        lib().oracle_export_nodes(self._h, pmin, pmax, nrefs, offset)
+        return pmin, pmax, nrefs, offset
+
+    def prim_order(self):
+        out = np.zeros(len(self.indices), np.int32)
+        lib().oracle_export_prim_order(self._h, out)
+        return out
+
+    def index_map(self):
+        out = np.zeros(len(self.positions), np.int32)
+        lib().oracle_export_index_map(self._h, out)
+        return out
+
+    def update_vertices(self, positions_user_order):
+        p = _f32(positions_user_order, self.dim)
+        lib().oracle_update_vertices(self._h, p, len(p))
+
+    def refit(self):
+        lib().oracle_refit(self._h)
+
+    def _q(self, q):
+        return _f32(q, self.dim)
+
+    def closest_points_cpu(self, q, r2=None, record_normal=False, nthreads=0):
+        q = self._q(q)
+        r2 = np.full(len(q), np.inf, np.float32) if r2 is None else _f32(r2).ravel()
+        out = np.zeros(len(q), HIT_DTYPE)
+        lib().oracle_closest_points_cpu(self._h, q, r2, len(q), int(record_normal), nthreads, out)
+        return out
+
+    def closest_points_gpuorder(self, q, r2=None, record_normal=False, nthreads=0, counters=False):
+        q = self._q(q)
+        r2 = np.full(len(q), np.inf, np.float32) if r2 is None else _f32(r2).ravel()
+        out = np.zeros(len(q), HIT_DTYPE)
+        ni = np.zeros(len(q), np.int32) if counters else None
+        npr = np.zeros(len(q), np.int32) if counters else None
+        npop = np.zeros(len(q), np.int32) if counters else None
+        lib().oracle_closest_points_gpuorder(self._h, q, r2, len(q), int(record_normal), nthreads, out,
+                                             ni, npr, npop)
+        return (out, ni, npr, npop) if counters else out
+
+    def intersect_cpu(self, o, d, tmax=None, occlusion=False, nthreads=0):
+        o, d = self._q(o), self._q(d)
+        tmax = np.full(len(o), np.finfo(np.float32).max, np.float32) if tmax is None else _f32(tmax).ravel()
+        out = np.zeros(len(o), HIT_DTYPE)
+        lib().oracle_intersect_cpu(self._h, o, d, tmax, len(o), int(occlusion), nthreads, out)
+        return out
+
+    def intersect_gpuorder(self, o, d, tmax=None, occlusion=False, nthreads=0, counters=False):
+        o, d = self._q(o), self._q(d)
+        tmax = np.full(len(o), np.finfo(np.float32).max, np.float32) if tmax is None else _f32(tmax).ravel()
+        out = np.zeros(len(o), HIT_DTYPE)
+        ni = np.zeros(len(o), np.int32) if counters else None
+        npr = np.zeros(len(o), np.int32) if counters else None
+        lib().oracle_intersect_gpuorder(self._h, o, d, tmax, len(o), int(occlusion), nthreads, out, ni, npr)
+        return (out, ni, npr) if counters else out
+
+    def brute_closest_f32(self, q, r2=None, nthreads=0):
+        q = self._q(q)
+        r2 = np.full(len(q), np.inf, np.float32) if r2 is None else _f32(r2).ravel()
+        out = np.zeros(len(q), HIT_DTYPE)
+        lib().oracle_brute_closest_f32(self._h, q, r2, len(q), nthreads, out)
+        return out
+
+
+def brute_closest_f64(positions, indices, q, nthreads=0):
+    """FP64 closest point over all primitives: returns (d, p (n,3), index)."""
+    positions = _f32(positions)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    dim = positions.shape[1]
+    q = _f32(q, dim)
+    d = np.zeros(len(q), np.float64)
+    p = np.zeros((len(q), 3), np.float64)
+    idx = np.zeros(len(q), np.int32)
+    lib().oracle_brute_closest_f64(dim, positions, len(positions), indices, len(indices), q, len(q),
+                                   nthreads, d, p, idx)
+    return d, p, idx
+
+
+def brute_ray_f64(positions, indices, o, d, nthreads=0):
+    """FP64 first hit over all primitives (tMax = inf): returns (t, index)."""
+    positions = _f32(positions)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    dim = positions.shape[1]
+    o, d = _f32(o, dim), _f32(d, dim)
+    t = np.zeros(len(o), np.float64)
+    idx = np.zeros(len(o), np.int32)
+    lib().oracle_brute_ray_f64(dim, positions, len(positions), indices, len(indices), o, d, len(o),
+                               nthreads, t, idx)
+    return t, idx
+
+
+def prim_dist_f64(positions, indices, q, prims):
+    """FP64 distance from q[i] to primitive prims[i] (inf where prims[i] < 0)."""
+    positions = _f32(positions)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    dim = positions.shape[1]
+    q = _f32(q, dim)
+    prims = np.ascontiguousarray(prims, dtype=np.int32)
+    out = np.zeros(len(q), np.float64)
+    lib().oracle_prim_dist_f64(dim, positions, indices, q, prims, len(q), out)
+    return out
+
+
+def classify_index_mismatch(positions, indices, q, idx_a, idx_b, rel_tol=1e-6, abs_tol=1e-7):
+    """For queries whose primitive indices differ, decide in FP64 whether the two
+    primitives are equidistant (a tie).  Returns a bool array: True = tie."""
+    da = prim_dist_f64(positions, indices, q, idx_a)
+    db = prim_dist_f64(positions, indices, q, idx_b)
+    scale = np.maximum(np.maximum(da, db), 0.0)
+    return np.abs(da - db) <= rel_tol * scale + abs_tol

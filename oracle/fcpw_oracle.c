@@ -1,0 +1,1383 @@
+/*
+ * fcpw_oracle.c -- TEST INFRASTRUCTURE ONLY (see fcpw_oracle.h).
+ *
+ * Arithmetic conventions, chosen to follow each reference file:
+ *   - CPU-order functions follow the Eigen expressions in the reference C++.
+ *     Eigen reduces a fixed-size 3-vector as c0 + (c1 + c2) (its unroller
+ *     splits the range in halves), so dot, squaredNorm, sum and prod use that
+ *     order here; 2-vectors reduce as c0 + c1.  min/max follow std::min/std::max.
+ *   - GPU-order functions follow the Slang source, whose float3 dot is written
+ *     left to right: (x*x' + y*y') + z*z'.
+ *   - Built with -ffp-contract=off, so no multiply-add is fused.
+ * Which Eigen commit the reference pins is unknown (the submodule is empty), so
+ * bit equality with the reference binary is not claimed; the file:line citations
+ * name the code each function restates.
+ */
+#include "fcpw_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#define MAXF FLT_MAX
+#define LOWF (-FLT_MAX)
+#define EPSF FLT_EPSILON
+#define BVH_MAX_DEPTH 64
+
+typedef struct {
+    float pmin[3], pmax[3];
+} box_t;
+
+typedef struct {
+    box_t box;
+    int32_t offset; /* referenceOffset (leaf) or secondChildOffset (internal) */
+    int32_t nrefs;  /* 0 for internal nodes */
+} node_t;
+
+struct oracle_bvh {
+    int dim, k; /* k = vertices per primitive */
+    int n_vertices, n_prims;
+    float *positions;     /* sorted (leaf-order) vertex positions, dim floats each */
+    int32_t *prim_vidx;   /* per leaf slot: k sorted vertex ids */
+    int32_t *prim_orig;   /* per leaf slot: original primitive index (pIndex) */
+    int32_t *index_map;   /* user vertex -> sorted vertex */
+    node_t *nodes;
+    int n_nodes, n_leaves, max_depth;
+};
+
+/* ----------------------------------------------------------------------------
+ * scalar helpers
+ * ------------------------------------------------------------------------- */
+static inline float smin(float a, float b) { return (b < a) ? b : a; } /* std::min */
+static inline float smax(float a, float b) { return (a < b) ? b : a; } /* std::max */
+
+/* Eigen fixed-size reduction order: 3 -> c0 + (c1 + c2), 2 -> c0 + c1 */
+static inline float esum(const float *c, int dim)
+{
+    return dim == 3 ? c[0] + (c[1] + c[2]) : c[0] + c[1];
+}
+static inline float eprod(const float *c, int dim)
+{
+    return dim == 3 ? c[0] * (c[1] * c[2]) : c[0] * c[1];
+}
+static inline float edot(const float *a, const float *b, int dim)
+{
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < dim; i++) c[i] = a[i] * b[i];
+    return esum(c, dim);
+}
+static inline float enorm(const float *a, int dim) { return sqrtf(edot(a, a, dim)); }
+
+/* Slang float3 dot, left to right; a 2D float3 has z = 0, and adding the zero
+   product leaves the sum unchanged, so the 2-term form is used for dim == 2 */
+static inline float gdot(const float *a, const float *b, int dim)
+{
+    float s = a[0] * b[0] + a[1] * b[1];
+    if (dim == 3) s = s + a[2] * b[2];
+    return s;
+}
+
+/* ----------------------------------------------------------------------------
+ * bounding boxes (bounding_volumes.h:31-168)
+ * ------------------------------------------------------------------------- */
+static inline void box_init(box_t *b)
+{
+    for (int i = 0; i < 3; i++) { b->pmin[i] = MAXF; b->pmax[i] = LOWF; }
+}
+/* expandToInclude(point): inflated by +-FLT_EPSILON (bounding_volumes.h:45-49) */
+static inline void box_expand_point(box_t *b, const float *p, int dim)
+{
+    for (int i = 0; i < dim; i++) {
+        b->pmin[i] = smin(b->pmin[i], p[i] - EPSF);
+        b->pmax[i] = smax(b->pmax[i], p[i] + EPSF);
+    }
+}
+static inline void box_expand_box(box_t *b, const box_t *o, int dim)
+{
+    for (int i = 0; i < dim; i++) {
+        b->pmin[i] = smin(b->pmin[i], o->pmin[i]);
+        b->pmax[i] = smax(b->pmax[i], o->pmax[i]);
+    }
+}
+/* surfaceArea: extent clamped to 1e-5 (bounding_volumes.h:165-168) */
+static float box_surface_area(const box_t *b, int dim)
+{
+    float e[3] = {0.0f, 0.0f, 0.0f}, q[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < dim; i++) e[i] = smax(b->pmax[i] - b->pmin[i], 1e-5f);
+    float P = eprod(e, dim);
+    for (int i = 0; i < dim; i++) q[i] = P / e[i];
+    return 2.0f * esum(q, dim);
+}
+static float box_volume(const box_t *b, int dim)
+{
+    float e[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < dim; i++) e[i] = b->pmax[i] - b->pmin[i];
+    return eprod(e, dim);
+}
+static int box_valid(const box_t *b, int dim)
+{
+    for (int i = 0; i < dim; i++)
+        if (!(b->pmax[i] >= b->pmin[i])) return 0;
+    return 1;
+}
+static box_t box_intersection(const box_t *a, const box_t *b, int dim)
+{
+    box_t r;
+    box_init(&r);
+    for (int i = 0; i < dim; i++) {
+        r.pmin[i] = smax(a->pmin[i], b->pmin[i]);
+        r.pmax[i] = smin(a->pmax[i], b->pmax[i]);
+    }
+    return r;
+}
+/* maxDimension: first index of the largest extent (Eigen maxCoeff(&index)) */
+static int box_max_dimension(const box_t *b, int dim)
+{
+    int idx = 0;
+    float best = b->pmax[0] - b->pmin[0];
+    for (int i = 1; i < dim; i++) {
+        float e = b->pmax[i] - b->pmin[i];
+        if (e > best) { best = e; idx = i; }
+    }
+    return idx;
+}
+/* computeSquaredDistance (bounding_volumes.h:64-69), Eigen order */
+static inline void box_sqdist_cpu(const box_t *b, const float *p, int dim, float *d2min, float *d2max)
+{
+    float a[3], c[3];
+    for (int i = 0; i < dim; i++) {
+        float u = b->pmin[i] - p[i];
+        float v = p[i] - b->pmax[i];
+        a[i] = smax(smax(u, v), 0.0f);
+        c[i] = smin(u, v);
+    }
+    *d2min = edot(a, a, dim);
+    *d2max = edot(c, c, dim);
+}
+/* BoundingBox.overlap (bounding-volumes.slang:43-53), left-to-right dot */
+static inline void box_sqdist_gpu(const box_t *b, const float *p, int dim, float *d2min, float *d2max)
+{
+    float a[3], c[3];
+    for (int i = 0; i < dim; i++) {
+        float u = b->pmin[i] - p[i];
+        float v = p[i] - b->pmax[i];
+        a[i] = fmaxf(fmaxf(u, v), 0.0f);
+        c[i] = fminf(u, v);
+    }
+    *d2min = gdot(a, a, dim);
+    *d2max = gdot(c, c, dim);
+}
+/* CPU slab test without the parallel-slab fix (bounding_volumes.h:99-114) */
+static inline int box_ray_cpu(const box_t *b, const float *o, const float *invd, float tmax, int dim,
+                              float *tmin_out, float *tmax_out)
+{
+    float tn[3], tf[3];
+    for (int i = 0; i < dim; i++) {
+        float t0 = (b->pmin[i] - o[i]) * invd[i];
+        float t1 = (b->pmax[i] - o[i]) * invd[i];
+        tn[i] = smin(t0, t1);
+        tf[i] = smax(t0, t1);
+    }
+    /* Eigen maxCoeff/minCoeff reduce as c0 op (c1 op c2) */
+    float tnm = dim == 3 ? smax(tn[0], smax(tn[1], tn[2])) : smax(tn[0], tn[1]);
+    float tfm = dim == 3 ? smin(tf[0], smin(tf[1], tf[2])) : smin(tf[0], tf[1]);
+    float tNearMax = smax(0.0f, tnm);
+    float tFarMin = smin(tmax, tfm);
+    if (tNearMax > tFarMin) return 0;
+    *tmin_out = tNearMax;
+    *tmax_out = tFarMin;
+    return 1;
+}
+/* GPU slab test with the parallel-slab fix (bounding-volumes.slang:67-106);
+   a 2D float3 box has z = [0,0] and a 2D ray has o.z = d.z = 0, which the fix
+   turns into an unconstrained z slab, so dim == 2 simply skips z */
+static inline int box_ray_gpu(const box_t *b, const float *o, const float *d, const float *dinv, float tmax,
+                              int dim, float *tmin_out, float *tmax_out)
+{
+    float tn[3] = {-MAXF, -MAXF, -MAXF}, tf[3] = {MAXF, MAXF, MAXF};
+    for (int i = 0; i < dim; i++) {
+        float t0 = (b->pmin[i] - o[i]) * dinv[i];
+        float t1 = (b->pmax[i] - o[i]) * dinv[i];
+        int parallel = (d[i] == 0.0f);
+        int within = (o[i] >= b->pmin[i]) && (o[i] <= b->pmax[i]);
+        if (parallel && !within) { *tmin_out = MAXF; *tmax_out = MAXF; return 0; }
+        tn[i] = parallel ? -MAXF : fminf(t0, t1);
+        tf[i] = parallel ? MAXF : fmaxf(t0, t1);
+    }
+    float tNearMax = fmaxf(0.0f, fmaxf(tn[0], fmaxf(tn[1], tn[2])));
+    float tFarMin = fminf(tmax, fminf(tf[0], fminf(tf[1], tf[2])));
+    if (tNearMax > tFarMin) { *tmin_out = MAXF; *tmax_out = MAXF; return 0; }
+    *tmin_out = tNearMax;
+    *tmax_out = tFarMin;
+    return 1;
+}
+
+/* ----------------------------------------------------------------------------
+ * primitives
+ * ------------------------------------------------------------------------- */
+static inline const float *vtx(const oracle_bvh *h, int slot, int c)
+{
+    return h->positions + (size_t)h->prim_vidx[(size_t)slot * h->k + c] * h->dim;
+}
+
+/* Triangle::boundingBox / LineSegment::boundingBox (triangles.inl:12-23) */
+static box_t prim_box(const float *const *v, int k, int dim)
+{
+    box_t b;
+    box_init(&b);
+    for (int c = 0; c < k; c++) box_expand_point(&b, v[c], dim);
+    return b;
+}
+/* Triangle::centroid = (pa + pb + pc)/3 (triangles.inl:25-32); segment (pa + pb)*0.5 */
+static void prim_centroid(const float *const *v, int k, int dim, float *out)
+{
+    for (int i = 0; i < dim; i++)
+        out[i] = (k == 3) ? ((v[0][i] + v[1][i]) + v[2][i]) / 3.0f : (v[0][i] + v[1][i]) * 0.5f;
+}
+
+/* findClosestPointTriangle (triangles.inl:374-458), `dot` selects Eigen or Slang order */
+typedef float (*dot_fn)(const float *, const float *, int);
+static float closest_point_triangle(const float *pa, const float *pb, const float *pc, const float *x,
+                                    float *pt, float *t, dot_fn dot)
+{
+    float ab[3], ac[3], ax[3], bx[3], cx[3], r[3];
+    for (int i = 0; i < 3; i++) { ab[i] = pb[i] - pa[i]; ac[i] = pc[i] - pa[i]; ax[i] = x[i] - pa[i]; }
+    float d1 = dot(ab, ax, 3), d2 = dot(ac, ax, 3);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        t[0] = 1.0f; t[1] = 0.0f;
+        for (int i = 0; i < 3; i++) pt[i] = pa[i];
+        goto done;
+    }
+    for (int i = 0; i < 3; i++) bx[i] = x[i] - pb[i];
+    float d3 = dot(ab, bx, 3), d4 = dot(ac, bx, 3);
+    if (d3 >= 0.0f && d4 <= d3) {
+        t[0] = 0.0f; t[1] = 1.0f;
+        for (int i = 0; i < 3; i++) pt[i] = pb[i];
+        goto done;
+    }
+    for (int i = 0; i < 3; i++) cx[i] = x[i] - pc[i];
+    float d5 = dot(ab, cx, 3), d6 = dot(ac, cx, 3);
+    if (d6 >= 0.0f && d5 <= d6) {
+        t[0] = 0.0f; t[1] = 0.0f;
+        for (int i = 0; i < 3; i++) pt[i] = pc[i];
+        goto done;
+    }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        float v = d1 / (d1 - d3);
+        t[0] = 1.0f - v; t[1] = v;
+        for (int i = 0; i < 3; i++) pt[i] = pa[i] + ab[i] * v;
+        goto done;
+    }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        float w = d2 / (d2 - d6);
+        t[0] = 1.0f - w; t[1] = 0.0f;
+        for (int i = 0; i < 3; i++) pt[i] = pa[i] + ac[i] * w;
+        goto done;
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        t[0] = 0.0f; t[1] = 1.0f - w;
+        for (int i = 0; i < 3; i++) pt[i] = pb[i] + (pc[i] - pb[i]) * w;
+        goto done;
+    }
+    {
+        float denom = 1.0f / (va + vb + vc);
+        float v = vb * denom, w = vc * denom;
+        t[0] = 1.0f - v - w; t[1] = v;
+        for (int i = 0; i < 3; i++) pt[i] = (pa[i] + ab[i] * v) + ac[i] * w;
+    }
+done:
+    for (int i = 0; i < 3; i++) r[i] = x[i] - pt[i];
+    return sqrtf(dot(r, r, 3));
+}
+
+/* findClosestPointLineSegment (line_segments.inl:200-227; geometry.slang:79-103) */
+static float closest_point_segment(const float *pa, const float *pb, const float *x, float *pt, float *t,
+                                   dot_fn dot)
+{
+    float u[2], v[2], r[2];
+    for (int i = 0; i < 2; i++) { u[i] = pb[i] - pa[i]; v[i] = x[i] - pa[i]; }
+    float c1 = dot(u, v, 2);
+    if (c1 <= 0.0f) {
+        pt[0] = pa[0]; pt[1] = pa[1]; *t = 0.0f;
+    } else {
+        float c2 = dot(u, u, 2);
+        if (c2 <= c1) {
+            pt[0] = pb[0]; pt[1] = pb[1]; *t = 1.0f;
+        } else {
+            *t = c1 / c2;
+            for (int i = 0; i < 2; i++) pt[i] = pa[i] + u[i] * (*t);
+        }
+    }
+    for (int i = 0; i < 2; i++) r[i] = x[i] - pt[i];
+    return sqrtf(dot(r, r, 2));
+}
+
+/* closest point to one primitive slot; fills p (3, z = 0 in 2D) and uv (2) */
+static float prim_closest(const oracle_bvh *h, int slot, const float *x, float *p, float *uv, dot_fn dot)
+{
+    if (h->dim == 3)
+        return closest_point_triangle(vtx(h, slot, 0), vtx(h, slot, 1), vtx(h, slot, 2), x, p, uv, dot);
+    p[2] = 0.0f;
+    uv[1] = 0.0f;
+    return closest_point_segment(vtx(h, slot, 0), vtx(h, slot, 1), x, p, &uv[0], dot);
+}
+
+/* normals: CPU Triangle::normal(true) (triangles.inl:48-59) = cross().normalized();
+   LineSegment::normal(true) (line_segments.inl:46-55) = (s.y, -s.x).normalized() */
+static void prim_normal_cpu(const oracle_bvh *h, int slot, float *n)
+{
+    if (h->dim == 3) {
+        const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1), *pc = vtx(h, slot, 2);
+        float a[3], b[3];
+        for (int i = 0; i < 3; i++) { a[i] = pb[i] - pa[i]; b[i] = pc[i] - pa[i]; }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        float z = edot(n, n, 3);
+        if (z > 0.0f) { float s = sqrtf(z); for (int i = 0; i < 3; i++) n[i] = n[i] / s; }
+    } else {
+        const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1);
+        n[0] = pb[1] - pa[1];
+        n[1] = -(pb[0] - pa[0]);
+        n[2] = 0.0f;
+        float z = edot(n, n, 2);
+        if (z > 0.0f) { float s = sqrtf(z); n[0] = n[0] / s; n[1] = n[1] / s; }
+    }
+}
+/* GPU getNormal (geometry.slang:125-131,361-366): normalize(v) written as v / length(v) */
+static void prim_normal_gpu(const oracle_bvh *h, int slot, float *n)
+{
+    if (h->dim == 3) {
+        const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1), *pc = vtx(h, slot, 2);
+        float a[3], b[3];
+        for (int i = 0; i < 3; i++) { a[i] = pb[i] - pa[i]; b[i] = pc[i] - pa[i]; }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        float l = sqrtf(gdot(n, n, 3));
+        for (int i = 0; i < 3; i++) n[i] = n[i] / l;
+    } else {
+        const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1);
+        n[0] = pb[1] - pa[1];
+        n[1] = -(pb[0] - pa[0]);
+        n[2] = 0.0f;
+        float l = sqrtf(gdot(n, n, 2));
+        n[0] = n[0] / l;
+        n[1] = n[1] / l;
+    }
+}
+
+/* Triangle::intersect (triangles.inl:222-260), Eigen order; returns hit and fills out */
+static int ray_triangle_cpu(const float *pa, const float *pb, const float *pc, const float *o, const float *d,
+                            float tmax, int occlusion, oracle_hit *out, int *occluded)
+{
+    float v1[3], v2[3], p[3], s[3], q[3];
+    for (int i = 0; i < 3; i++) { v1[i] = pb[i] - pa[i]; v2[i] = pc[i] - pa[i]; }
+    p[0] = d[1] * v2[2] - d[2] * v2[1];
+    p[1] = d[2] * v2[0] - d[0] * v2[2];
+    p[2] = d[0] * v2[1] - d[1] * v2[0];
+    float det = edot(v1, p, 3);
+    if (fabsf(det) <= EPSF) return 0;
+    float invDet = 1.0f / det;
+    for (int i = 0; i < 3; i++) s[i] = o[i] - pa[i];
+    float v = edot(s, p, 3) * invDet;
+    if (v < 0 || v > 1) return 0;
+    q[0] = s[1] * v1[2] - s[2] * v1[1];
+    q[1] = s[2] * v1[0] - s[0] * v1[2];
+    q[2] = s[0] * v1[1] - s[1] * v1[0];
+    float w = edot(d, q, 3) * invDet;
+    if (w < 0 || v + w > 1) return 0;
+    float t = edot(v2, q, 3) * invDet;
+    if (t >= 0.0f && t <= tmax) {
+        if (occlusion) { *occluded = 1; return 1; }
+        out->d = t;
+        for (int i = 0; i < 3; i++) out->p[i] = (pa[i] + v1[i] * v) + v2[i] * w;
+        float n[3];
+        n[0] = v1[1] * v2[2] - v1[2] * v2[1];
+        n[1] = v1[2] * v2[0] - v1[0] * v2[2];
+        n[2] = v1[0] * v2[1] - v1[1] * v2[0];
+        float z = edot(n, n, 3);
+        if (z > 0.0f) { float sq = sqrtf(z); for (int i = 0; i < 3; i++) n[i] = n[i] / sq; }
+        for (int i = 0; i < 3; i++) out->n[i] = n[i];
+        out->uv[0] = 1.0f - v - w;
+        out->uv[1] = v;
+        return 1;
+    }
+    return 0;
+}
+/* intersectTriangle (geometry.slang:202-249), left-to-right dot */
+static int ray_triangle_gpu(const float *pa, const float *pb, const float *pc, const float *o, const float *d,
+                            float tmax, int occlusion, oracle_hit *out)
+{
+    float v1[3], v2[3], q[3], r[3], s[3];
+    for (int i = 0; i < 3; i++) { v1[i] = pb[i] - pa[i]; v2[i] = pc[i] - pa[i]; }
+    q[0] = d[1] * v2[2] - d[2] * v2[1];
+    q[1] = d[2] * v2[0] - d[0] * v2[2];
+    q[2] = d[0] * v2[1] - d[1] * v2[0];
+    float det = gdot(v1, q, 3);
+    if (fabsf(det) <= EPSF) return 0;
+    float invDet = 1.0f / det;
+    for (int i = 0; i < 3; i++) r[i] = o[i] - pa[i];
+    float v = gdot(r, q, 3) * invDet;
+    if (v < 0.0f || v > 1.0f) return 0;
+    s[0] = r[1] * v1[2] - r[2] * v1[1];
+    s[1] = r[2] * v1[0] - r[0] * v1[2];
+    s[2] = r[0] * v1[1] - r[1] * v1[0];
+    float w = gdot(d, s, 3) * invDet;
+    if (w < 0.0f || v + w > 1.0f) return 0;
+    float t = gdot(v2, s, 3) * invDet;
+    if (t >= 0.0f && t <= tmax) {
+        if (occlusion) return 1;
+        for (int i = 0; i < 3; i++) out->p[i] = (pa[i] + v1[i] * v) + v2[i] * w;
+        float n[3];
+        n[0] = v1[1] * v2[2] - v1[2] * v2[1];
+        n[1] = v1[2] * v2[0] - v1[0] * v2[2];
+        n[2] = v1[0] * v2[1] - v1[1] * v2[0];
+        float l = sqrtf(gdot(n, n, 3));
+        for (int i = 0; i < 3; i++) out->n[i] = n[i] / l;
+        out->uv[0] = 1.0f - v - w;
+        out->uv[1] = v;
+        out->d = t;
+        return 1;
+    }
+    return 0;
+}
+/* LineSegment::intersect (line_segments.inl:140-175) and intersectLineSegment
+   (geometry.slang:37-77) use the same 2D cross products */
+static int ray_segment(const float *pa, const float *pb, const float *o, const float *d, float tmax,
+                       int occlusion, int cpu_normal, oracle_hit *out, int *occluded)
+{
+    float u[2] = {pa[0] - o[0], pa[1] - o[1]};
+    float v[2] = {pb[0] - pa[0], pb[1] - pa[1]};
+    float dv = d[0] * v[1] - d[1] * v[0];
+    if (fabsf(dv) <= EPSF) return 0;
+    float ud = u[0] * d[1] - u[1] * d[0];
+    float s = ud / dv;
+    if (s >= 0.0f && s <= 1.0f) {
+        float uvx = u[0] * v[1] - u[1] * v[0];
+        float t = uvx / dv;
+        if (t >= 0.0f && t <= tmax) {
+            if (occlusion) { if (occluded) *occluded = 1; return 1; }
+            out->d = t;
+            out->p[0] = pa[0] + s * v[0];
+            out->p[1] = pa[1] + s * v[1];
+            out->p[2] = 0.0f;
+            float n[2] = {v[1], -v[0]};
+            float z = cpu_normal ? edot(n, n, 2) : gdot(n, n, 2);
+            float l = sqrtf(z);
+            if (!cpu_normal || z > 0.0f) { n[0] = n[0] / l; n[1] = n[1] / l; }
+            out->n[0] = n[0]; out->n[1] = n[1]; out->n[2] = 0.0f;
+            out->uv[0] = s;
+            out->uv[1] = 0.0f;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------------------------
+ * BVH build (bvh.inl:3-234)
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    oracle_bvh *h;
+    int heuristic, leaf_size, n_buckets;
+    box_t *ref_boxes;
+    float *ref_cent;  /* dim floats per reference */
+    box_t *bkt_box, *rbkt_box;
+    int *bkt_cnt, *rbkt_cnt;
+    int cap;
+} build_ctx;
+
+static void swap_refs(build_ctx *b, int i, int j)
+{
+    int dim = b->h->dim;
+    int32_t t = b->h->prim_orig[i]; b->h->prim_orig[i] = b->h->prim_orig[j]; b->h->prim_orig[j] = t;
+    box_t tb = b->ref_boxes[i]; b->ref_boxes[i] = b->ref_boxes[j]; b->ref_boxes[j] = tb;
+    for (int d = 0; d < dim; d++) {
+        float tc = b->ref_cent[(size_t)i * dim + d];
+        b->ref_cent[(size_t)i * dim + d] = b->ref_cent[(size_t)j * dim + d];
+        b->ref_cent[(size_t)j * dim + d] = tc;
+    }
+}
+
+/* computeSplitCost (bvh.inl:3-37) with packLeaves = false (fcpw.inl:812-819) */
+static float split_cost(build_ctx *b, const box_t *bl, const box_t *br, int nl, int nr)
+{
+    int dim = b->h->dim;
+    float cost = MAXF;
+    if (b->heuristic == ORACLE_SURFACE_AREA) {
+        cost = (float)nl * box_surface_area(bl, dim) + (float)nr * box_surface_area(br, dim);
+    } else if (b->heuristic == ORACLE_OVERLAP_SURFACE_AREA) {
+        box_t bi = box_intersection(bl, br, dim);
+        cost = ((float)nl / box_surface_area(br, dim) + (float)nr / box_surface_area(bl, dim)) *
+               fabsf(box_surface_area(&bi, dim));
+        if (!box_valid(&bi, dim)) cost *= -1;
+    } else if (b->heuristic == ORACLE_VOLUME) {
+        cost = (float)nl * box_volume(bl, dim) + (float)nr * box_volume(br, dim);
+    } else if (b->heuristic == ORACLE_OVERLAP_VOLUME) {
+        box_t bi = box_intersection(bl, br, dim);
+        cost = ((float)nl / box_volume(br, dim) + (float)nr / box_volume(bl, dim)) * fabsf(box_volume(&bi, dim));
+        if (!box_valid(&bi, dim)) cost *= -1;
+    }
+    return cost;
+}
+
+/* computeObjectSplit (bvh.inl:39-111) */
+static void object_split(build_ctx *b, const box_t *bb, const box_t *bc, int start, int end, int *split_dim,
+                         float *split_coord)
+{
+    int dim = b->h->dim, nb = b->n_buckets;
+    float split_cost_best = MAXF;
+    *split_dim = -1;
+    *split_coord = 0.0f;
+    if (b->heuristic != ORACLE_LONGEST_AXIS_CENTER) {
+        float extent[3];
+        for (int d = 0; d < dim; d++) extent[d] = bb->pmax[d] - bb->pmin[d];
+        for (int d = 0; d < dim; d++) {
+            if (extent[d] < 1e-6f) continue;
+            float bucket_width = extent[d] / nb;
+            for (int k = 0; k < nb; k++) { box_init(&b->bkt_box[k]); b->bkt_cnt[k] = 0; }
+            for (int p = start; p < end; p++) {
+                int bi = (int)((b->ref_cent[(size_t)p * dim + d] - bb->pmin[d]) / bucket_width);
+                if (bi < 0) bi = 0;
+                if (bi > nb - 1) bi = nb - 1;
+                box_expand_box(&b->bkt_box[bi], &b->ref_boxes[p], dim);
+                b->bkt_cnt[bi] += 1;
+            }
+            box_t right;
+            box_init(&right);
+            for (int k = nb - 1; k > 0; k--) {
+                box_expand_box(&right, &b->bkt_box[k], dim);
+                b->rbkt_box[k] = right;
+                b->rbkt_cnt[k] = b->bkt_cnt[k];
+                if (k != nb - 1) b->rbkt_cnt[k] += b->rbkt_cnt[k + 1];
+            }
+            box_t left;
+            box_init(&left);
+            int nl = 0;
+            for (int k = 1; k < nb; k++) {
+                box_expand_box(&left, &b->bkt_box[k - 1], dim);
+                nl += b->bkt_cnt[k - 1];
+                if (nl > 0 && b->rbkt_cnt[k] > 0) {
+                    float cost = split_cost(b, &left, &b->rbkt_box[k], nl, b->rbkt_cnt[k]);
+                    if (cost < split_cost_best) {
+                        split_cost_best = cost;
+                        *split_dim = d;
+                        *split_coord = bb->pmin[d] + k * bucket_width;
+                    }
+                }
+            }
+        }
+    }
+    if (*split_dim == -1) {
+        *split_dim = box_max_dimension(bc, dim);
+        *split_coord = (bc->pmin[*split_dim] + bc->pmax[*split_dim]) * 0.5f;
+    }
+}
+
+/* performObjectSplit (bvh.inl:113-140), packLeaves = false */
+static int object_partition(build_ctx *b, int start, int end, int split_dim, float split_coord)
+{
+    int dim = b->h->dim, mid = start;
+    for (int i = start; i < end; i++) {
+        if (b->ref_cent[(size_t)i * dim + split_dim] < split_coord) {
+            swap_refs(b, i, mid);
+            mid++;
+        }
+    }
+    if (mid == start || mid == end) mid = start + (end - start) / 2;
+    return mid;
+}
+
+static int push_node(build_ctx *b, const node_t *n)
+{
+    oracle_bvh *h = b->h;
+    if (h->n_nodes == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 1024;
+        h->nodes = (node_t *)realloc(h->nodes, (size_t)b->cap * sizeof(node_t));
+    }
+    h->nodes[h->n_nodes] = *n;
+    return h->n_nodes++;
+}
+
+/* buildRecursive (bvh.inl:142-209); the parent's secondChildOffset is set when
+   the right child is created, which is what the "touched twice" bookkeeping does */
+static void build_recursive(build_ctx *b, int parent, int is_right, int start, int end, int depth)
+{
+    oracle_bvh *h = b->h;
+    int dim = h->dim;
+    if (depth > h->max_depth) h->max_depth = depth;
+    box_t bb, bc;
+    box_init(&bb);
+    box_init(&bc);
+    for (int p = start; p < end; p++) {
+        box_expand_box(&bb, &b->ref_boxes[p], dim);
+        box_expand_point(&bc, &b->ref_cent[(size_t)p * dim], dim);
+    }
+    node_t node;
+    node.box = bb;
+    int nrefs = end - start;
+    int leaf = (nrefs <= b->leaf_size || depth == BVH_MAX_DEPTH - 2);
+    if (leaf) {
+        node.offset = start;
+        node.nrefs = nrefs;
+        h->n_leaves++;
+    } else {
+        node.offset = 0;
+        node.nrefs = 0;
+    }
+    int cur = push_node(b, &node);
+    if (parent >= 0 && is_right) h->nodes[parent].offset = cur - parent;
+    if (leaf) return;
+    int split_dim;
+    float split_coord;
+    object_split(b, &bb, &bc, start, end, &split_dim, &split_coord);
+    int mid = object_partition(b, start, end, split_dim, split_coord);
+    build_recursive(b, cur, 0, start, mid, depth + 1);
+    build_recursive(b, cur, 1, mid, end, depth + 1);
+}
+
+/* sortTriangleSoupPositions / sortLineSegmentSoupPositions (fcpw.inl:613-689) */
+static void sort_soup_positions(oracle_bvh *h, const float *user_pos, const int32_t *user_idx)
+{
+    int V = h->n_vertices, dim = h->dim, k = h->k;
+    for (int v = 0; v < V; v++) h->index_map[v] = -1;
+    int next = 0;
+    for (int i = 0; i < h->n_nodes; i++) {
+        const node_t *n = &h->nodes[i];
+        for (int j = 0; j < n->nrefs; j++) {
+            int slot = n->offset + j;
+            int orig = h->prim_orig[slot];
+            for (int c = 0; c < k; c++) {
+                int vi = user_idx[(size_t)orig * k + c];
+                if (h->index_map[vi] == -1) {
+                    memcpy(h->positions + (size_t)next * dim, user_pos + (size_t)vi * dim, dim * sizeof(float));
+                    h->index_map[vi] = next++;
+                }
+                h->prim_vidx[(size_t)slot * k + c] = h->index_map[vi];
+            }
+        }
+    }
+}
+
+oracle_bvh *oracle_build(int dim, const float *positions, int n_vertices, const int32_t *indices, int n_prims,
+                         int heuristic, int leaf_size, int n_buckets)
+{
+    if ((dim != 2 && dim != 3) || n_prims <= 0 || n_vertices <= 0) return NULL;
+    oracle_bvh *h = (oracle_bvh *)calloc(1, sizeof(oracle_bvh));
+    h->dim = dim;
+    h->k = dim;
+    h->n_vertices = n_vertices;
+    h->n_prims = n_prims;
+    h->positions = (float *)calloc((size_t)n_vertices * dim, sizeof(float));
+    h->prim_vidx = (int32_t *)malloc((size_t)n_prims * h->k * sizeof(int32_t));
+    h->prim_orig = (int32_t *)malloc((size_t)n_prims * sizeof(int32_t));
+    h->index_map = (int32_t *)malloc((size_t)n_vertices * sizeof(int32_t));
+
+    build_ctx b;
+    memset(&b, 0, sizeof(b));
+    b.h = h;
+    b.heuristic = heuristic;
+    b.leaf_size = leaf_size;
+    b.n_buckets = n_buckets;
+    b.ref_boxes = (box_t *)malloc((size_t)n_prims * sizeof(box_t));
+    b.ref_cent = (float *)malloc((size_t)n_prims * dim * sizeof(float));
+    b.bkt_box = (box_t *)malloc((size_t)n_buckets * sizeof(box_t));
+    b.rbkt_box = (box_t *)malloc((size_t)n_buckets * sizeof(box_t));
+    b.bkt_cnt = (int *)malloc((size_t)n_buckets * sizeof(int));
+    b.rbkt_cnt = (int *)malloc((size_t)n_buckets * sizeof(int));
+    for (int i = 0; i < n_prims; i++) {
+        const float *v[3];
+        for (int c = 0; c < h->k; c++) v[c] = positions + (size_t)indices[(size_t)i * h->k + c] * dim;
+        h->prim_orig[i] = i;
+        b.ref_boxes[i] = prim_box(v, h->k, dim);
+        prim_centroid(v, h->k, dim, b.ref_cent + (size_t)i * dim);
+    }
+    build_recursive(&b, -1, 0, 0, n_prims, 0);
+    sort_soup_positions(h, positions, indices);
+    free(b.ref_boxes); free(b.ref_cent); free(b.bkt_box); free(b.rbkt_box); free(b.bkt_cnt); free(b.rbkt_cnt);
+    return h;
+}
+
+void oracle_free(oracle_bvh *h)
+{
+    if (!h) return;
+    free(h->positions); free(h->prim_vidx); free(h->prim_orig); free(h->index_map); free(h->nodes);
+    free(h);
+}
+
+int oracle_num_nodes(const oracle_bvh *h) { return h->n_nodes; }
+int oracle_num_leaves(const oracle_bvh *h) { return h->n_leaves; }
+int oracle_max_depth(const oracle_bvh *h) { return h->max_depth; }
+
+void oracle_export_nodes(const oracle_bvh *h, float *pmin, float *pmax, int32_t *nrefs, int32_t *offset)
+{
+    for (int i = 0; i < h->n_nodes; i++) {
+        for (int d = 0; d < 3; d++) {
+            pmin[3 * i + d] = d < h->dim ? h->nodes[i].box.pmin[d] : 0.0f;
+            pmax[3 * i + d] = d < h->dim ? h->nodes[i].box.pmax[d] : 0.0f;
+        }
+        nrefs[i] = h->nodes[i].nrefs;
+        offset[i] = h->nodes[i].offset;
+    }
+}
+void oracle_export_prim_order(const oracle_bvh *h, int32_t *prim_order)
+{
+    memcpy(prim_order, h->prim_orig, (size_t)h->n_prims * sizeof(int32_t));
+}
+void oracle_export_index_map(const oracle_bvh *h, int32_t *index_map)
+{
+    memcpy(index_map, h->index_map, (size_t)h->n_vertices * sizeof(int32_t));
+}
+
+/* ----------------------------------------------------------------------------
+ * refit (bvh.inl:548-599) after Scene::updateObjectVertices (fcpw.inl:1128-1143)
+ * ------------------------------------------------------------------------- */
+void oracle_update_vertices(oracle_bvh *h, const float *pos, int n_vertices)
+{
+    for (int v = 0; v < n_vertices && v < h->n_vertices; v++) {
+        int s = h->index_map[v];
+        if (s >= 0) memcpy(h->positions + (size_t)s * h->dim, pos + (size_t)v * h->dim, h->dim * sizeof(float));
+    }
+}
+static void refit_recursive(oracle_bvh *h, int ni)
+{
+    node_t *n = &h->nodes[ni];
+    if (n->nrefs == 0) {
+        refit_recursive(h, ni + 1);
+        refit_recursive(h, ni + n->offset);
+        n->box = h->nodes[ni + 1].box;
+        box_expand_box(&n->box, &h->nodes[ni + n->offset].box, h->dim);
+    } else {
+        box_init(&n->box);
+        for (int p = 0; p < n->nrefs; p++) {
+            const float *v[3];
+            for (int c = 0; c < h->k; c++) v[c] = vtx(h, n->offset + p, c);
+            box_t pb = prim_box(v, h->k, h->dim);
+            box_expand_box(&n->box, &pb, h->dim);
+        }
+    }
+}
+void oracle_refit(oracle_bvh *h)
+{
+    if (h->n_nodes > 0) refit_recursive(h, 0);
+}
+
+/* ----------------------------------------------------------------------------
+ * thread pool: static partition over nthreads (fcpw.inl:1547-1559)
+ * ------------------------------------------------------------------------- */
+typedef void (*range_fn)(void *ctx, int start, int end);
+typedef struct { range_fn fn; void *ctx; int start, end; } range_job;
+static void *range_thread(void *arg)
+{
+    range_job *j = (range_job *)arg;
+    j->fn(j->ctx, j->start, j->end);
+    return NULL;
+}
+static void parallel_for(int n, int nthreads, range_fn fn, void *ctx)
+{
+    if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads == 1 || n < 2 * nthreads) { fn(ctx, 0, n); return; }
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    range_job *jobs = (range_job *)malloc(sizeof(range_job) * nthreads);
+    int per = n / nthreads;
+    for (int i = 0; i < nthreads; i++) {
+        jobs[i].fn = fn; jobs[i].ctx = ctx;
+        jobs[i].start = i * per;
+        jobs[i].end = (i == nthreads - 1) ? n : (i + 1) * per;
+        pthread_create(&th[i], NULL, range_thread, &jobs[i]);
+    }
+    for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+    free(th);
+    free(jobs);
+}
+
+static void hit_default(oracle_hit *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->d = MAXF;
+    o->index = -1;
+}
+
+typedef struct { int node; float dist; } stack_entry;
+
+/* ----------------------------------------------------------------------------
+ * CPU closest point (bvh.inl:1290-1418 + Triangle/LineSegment::findClosestPoint)
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const oracle_bvh *h;
+    const float *q, *r2;
+    int record_normal;
+    oracle_hit *out;
+    int32_t *n_internal, *n_leaf_prims, *n_pops;
+} cp_ctx;
+
+static void cp_cpu_range(void *vctx, int start, int end)
+{
+    cp_ctx *c = (cp_ctx *)vctx;
+    const oracle_bvh *h = c->h;
+    int dim = h->dim;
+    stack_entry st[BVH_MAX_DEPTH];
+    for (int qi = start; qi < end; qi++) {
+        float x[3] = {0, 0, 0};
+        for (int d = 0; d < dim; d++) x[d] = c->q[(size_t)qi * dim + d];
+        float r2 = c->r2[qi];
+        oracle_hit best;
+        hit_default(&best);
+        int best_slot = -1;
+        float h0, h1, h2, h3;
+        box_sqdist_cpu(&h->nodes[0].box, x, dim, &h0, &h1);
+        if (h0 <= r2) {
+            r2 = smin(r2, h1);
+            int sp = 0;
+            st[0].node = 0;
+            st[0].dist = h0;
+            while (sp >= 0) {
+                int ni = st[sp].node;
+                float dist = st[sp].dist;
+                sp--;
+                if (dist > r2) continue;
+                const node_t *n = &h->nodes[ni];
+                if (n->nrefs > 0) {
+                    for (int p = 0; p < n->nrefs; p++) {
+                        int slot = n->offset + p;
+                        float pt[3], uv[2];
+                        float dd = prim_closest(h, slot, x, pt, uv, edot);
+                        if (dd * dd <= r2) {
+                            r2 = smin(r2, dd * dd);
+                            hit_default(&best);
+                            best.d = dd;
+                            memcpy(best.p, pt, sizeof(pt));
+                            memcpy(best.uv, uv, sizeof(uv));
+                            best.index = h->prim_orig[slot];
+                            best_slot = slot;
+                        }
+                    }
+                } else {
+                    int l = ni + 1, r = ni + n->offset;
+                    box_sqdist_cpu(&h->nodes[l].box, x, dim, &h0, &h1);
+                    int hit0 = h0 <= r2;
+                    r2 = smin(r2, h1);
+                    box_sqdist_cpu(&h->nodes[r].box, x, dim, &h2, &h3);
+                    int hit1 = h2 <= r2;
+                    r2 = smin(r2, h3);
+                    if (hit0 && hit1) {
+                        int closer = l, other = r;
+                        if (h0 == 0.0f && h2 == 0.0f) {
+                            if (h3 < h1) { closer = r; other = l; }
+                        } else if (h2 < h0) {
+                            float t = h0; h0 = h2; h2 = t;
+                            closer = r; other = l;
+                        }
+                        st[++sp].node = other; st[sp].dist = h2;
+                        st[++sp].node = closer; st[sp].dist = h0;
+                    } else if (hit0) {
+                        st[++sp].node = l; st[sp].dist = h0;
+                    } else if (hit1) {
+                        st[++sp].node = r; st[sp].dist = h2;
+                    }
+                }
+            }
+        }
+        if (best_slot >= 0 && c->record_normal) prim_normal_cpu(h, best_slot, best.n);
+        c->out[qi] = best;
+    }
+}
+
+void oracle_closest_points_cpu(const oracle_bvh *h, const float *queries, const float *r2, int n,
+                               int record_normal, int nthreads, oracle_hit *out)
+{
+    cp_ctx c = {h, queries, r2, record_normal, out, NULL, NULL, NULL};
+    parallel_for(n, nthreads, cp_cpu_range, &c);
+}
+
+/* ----------------------------------------------------------------------------
+ * GPU-order closest point (bvh.slang:395-502), with traversal counters
+ * ------------------------------------------------------------------------- */
+static void cp_gpu_range(void *vctx, int start, int end)
+{
+    cp_ctx *c = (cp_ctx *)vctx;
+    const oracle_bvh *h = c->h;
+    int dim = h->dim;
+    int st[BVH_MAX_DEPTH];
+    for (int qi = start; qi < end; qi++) {
+        float x[3] = {0, 0, 0};
+        for (int d = 0; d < dim; d++) x[d] = c->q[(size_t)qi * dim + d];
+        float r2 = c->r2[qi];
+        oracle_hit best;
+        hit_default(&best);
+        int best_slot = -1, n_int = 0, n_prim = 0, n_pop = 0;
+        float d0, d1, d2, d3;
+        box_sqdist_gpu(&h->nodes[0].box, x, dim, &d0, &d1);
+        if (d0 <= r2) {
+            r2 = fminf(r2, d1);
+            int sp = 0;
+            st[0] = 0;
+            while (sp >= 0) {
+                int ni = st[sp--];
+                const node_t *n = &h->nodes[ni];
+                n_pop++;
+                box_sqdist_gpu(&n->box, x, dim, &d0, &d1);
+                if (!(d0 <= r2)) continue;
+                if (n->nrefs > 0) {
+                    for (int p = 0; p < n->nrefs; p++) {
+                        int slot = n->offset + p;
+                        float pt[3], uv[2];
+                        n_prim++;
+                        float dd = prim_closest(h, slot, x, pt, uv, gdot);
+                        if (dd * dd <= r2) {
+                            r2 = fminf(r2, dd * dd);
+                            hit_default(&best);
+                            best.d = dd;
+                            memcpy(best.p, pt, sizeof(pt));
+                            memcpy(best.uv, uv, sizeof(uv));
+                            best.index = h->prim_orig[slot];
+                            best_slot = slot;
+                        }
+                    }
+                } else {
+                    n_int++;
+                    int l = ni + 1, r = ni + n->offset;
+                    box_sqdist_gpu(&h->nodes[l].box, x, dim, &d0, &d1);
+                    int ol = d0 <= r2;
+                    r2 = fminf(r2, d1);
+                    box_sqdist_gpu(&h->nodes[r].box, x, dim, &d2, &d3);
+                    int orr = d2 <= r2;
+                    r2 = fminf(r2, d3);
+                    if (ol && orr) {
+                        int closer = l, other = r;
+                        if (d2 < d0) { closer = r; other = l; }
+                        st[++sp] = other;
+                        st[++sp] = closer;
+                    } else if (ol) {
+                        st[++sp] = l;
+                    } else if (orr) {
+                        st[++sp] = r;
+                    }
+                }
+            }
+        }
+        if (best_slot >= 0 && c->record_normal) prim_normal_gpu(h, best_slot, best.n);
+        c->out[qi] = best;
+        if (c->n_internal) c->n_internal[qi] = n_int;
+        if (c->n_leaf_prims) c->n_leaf_prims[qi] = n_prim;
+        if (c->n_pops) c->n_pops[qi] = n_pop;
+    }
+}
+
+void oracle_closest_points_gpuorder(const oracle_bvh *h, const float *queries, const float *r2, int n,
+                                    int record_normal, int nthreads, oracle_hit *out, int32_t *n_internal,
+                                    int32_t *n_leaf_prims, int32_t *n_pops)
+{
+    cp_ctx c = {h, queries, r2, record_normal, out, n_internal, n_leaf_prims, n_pops};
+    parallel_for(n, nthreads, cp_gpu_range, &c);
+}
+
+/* ----------------------------------------------------------------------------
+ * Baseline aggregate (baseline.inl:342-384): scene order, FP32, Eigen order
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const oracle_bvh *h;
+    const float *q, *r2;
+    oracle_hit *out;
+    int32_t *slot_of; /* original prim -> leaf slot */
+} bf_ctx;
+
+static void bf_range(void *vctx, int start, int end)
+{
+    bf_ctx *c = (bf_ctx *)vctx;
+    const oracle_bvh *h = c->h;
+    int dim = h->dim;
+    for (int qi = start; qi < end; qi++) {
+        float x[3] = {0, 0, 0};
+        for (int d = 0; d < dim; d++) x[d] = c->q[(size_t)qi * dim + d];
+        float r2 = c->r2[qi];
+        oracle_hit best;
+        hit_default(&best);
+        for (int p = 0; p < h->n_prims; p++) {
+            int slot = c->slot_of[p];
+            float pt[3], uv[2];
+            float dd = prim_closest(h, slot, x, pt, uv, edot);
+            if (dd * dd <= r2) {
+                r2 = smin(r2, dd * dd);
+                best.d = dd;
+                memcpy(best.p, pt, sizeof(pt));
+                memcpy(best.uv, uv, sizeof(uv));
+                best.index = p;
+            }
+        }
+        c->out[qi] = best;
+    }
+}
+
+void oracle_brute_closest_f32(const oracle_bvh *h, const float *queries, const float *r2, int n, int nthreads,
+                              oracle_hit *out)
+{
+    int32_t *slot_of = (int32_t *)malloc((size_t)h->n_prims * sizeof(int32_t));
+    for (int s = 0; s < h->n_prims; s++) slot_of[h->prim_orig[s]] = s;
+    bf_ctx c = {h, queries, r2, out, slot_of};
+    parallel_for(n, nthreads, bf_range, &c);
+    free(slot_of);
+}
+
+/* ----------------------------------------------------------------------------
+ * rays: CPU order (bvh.inl:655-771) and GPU order (bvh.slang:144-254)
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const oracle_bvh *h;
+    const float *o, *d, *tmax;
+    int occlusion;
+    oracle_hit *out;
+    int32_t *n_internal, *n_leaf_prims;
+} ray_ctx;
+
+static void ray_cpu_range(void *vctx, int start, int end)
+{
+    ray_ctx *c = (ray_ctx *)vctx;
+    const oracle_bvh *h = c->h;
+    int dim = h->dim;
+    stack_entry st[BVH_MAX_DEPTH];
+    for (int qi = start; qi < end; qi++) {
+        float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, invd[3] = {0, 0, 0};
+        for (int k = 0; k < dim; k++) {
+            o[k] = c->o[(size_t)qi * dim + k];
+            d[k] = c->d[(size_t)qi * dim + k];
+            invd[k] = 1.0f / d[k]; /* Ray ctor: d.cwiseInverse() (ray.h:10-11) */
+        }
+        float tmax = c->tmax[qi];
+        oracle_hit best;
+        hit_default(&best);
+        float b0, b1, b2, b3;
+        if (box_ray_cpu(&h->nodes[0].box, o, invd, tmax, dim, &b0, &b1)) {
+            int sp = 0, done = 0;
+            st[0].node = 0;
+            st[0].dist = b0;
+            while (sp >= 0 && !done) {
+                int ni = st[sp].node;
+                float dist = st[sp].dist;
+                sp--;
+                if (dist > tmax) continue;
+                const node_t *n = &h->nodes[ni];
+                if (n->nrefs > 0) {
+                    for (int p = 0; p < n->nrefs; p++) {
+                        int slot = n->offset + p, occl = 0, hit;
+                        oracle_hit cand = best;
+                        if (dim == 3)
+                            hit = ray_triangle_cpu(vtx(h, slot, 0), vtx(h, slot, 1), vtx(h, slot, 2), o, d, tmax,
+                                                   c->occlusion, &cand, &occl);
+                        else
+                            hit = ray_segment(vtx(h, slot, 0), vtx(h, slot, 1), o, d, tmax, c->occlusion, 1, &cand,
+                                              &occl);
+                        if (hit) {
+                            if (c->occlusion) { done = 1; break; }
+                            cand.index = h->prim_orig[slot];
+                            best = cand;
+                            tmax = smin(tmax, best.d);
+                        }
+                    }
+                } else {
+                    int l = ni + 1, r = ni + n->offset;
+                    int hit0 = box_ray_cpu(&h->nodes[l].box, o, invd, tmax, dim, &b0, &b1);
+                    int hit1 = box_ray_cpu(&h->nodes[r].box, o, invd, tmax, dim, &b2, &b3);
+                    if (hit0 && hit1) {
+                        int closer = l, other = r;
+                        if (b2 < b0) {
+                            float t = b0; b0 = b2; b2 = t;
+                            t = b1; b1 = b3; b3 = t;
+                            closer = r; other = l;
+                        }
+                        st[++sp].node = other; st[sp].dist = b2;
+                        st[++sp].node = closer; st[sp].dist = b0;
+                    } else if (hit0) {
+                        st[++sp].node = l; st[sp].dist = b0;
+                    } else if (hit1) {
+                        st[++sp].node = r; st[sp].dist = b2;
+                    }
+                }
+            }
+        }
+        c->out[qi] = best;
+    }
+}
+
+void oracle_intersect_cpu(const oracle_bvh *h, const float *origins, const float *dirs, const float *tmax, int n,
+                          int occlusion, int nthreads, oracle_hit *out)
+{
+    ray_ctx c = {h, origins, dirs, tmax, occlusion, out, NULL, NULL};
+    parallel_for(n, nthreads, ray_cpu_range, &c);
+}
+
+static void ray_gpu_range(void *vctx, int start, int end)
+{
+    ray_ctx *c = (ray_ctx *)vctx;
+    const oracle_bvh *h = c->h;
+    int dim = h->dim;
+    stack_entry st[BVH_MAX_DEPTH];
+    for (int qi = start; qi < end; qi++) {
+        float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, dinv[3] = {0, 0, 0};
+        for (int k = 0; k < dim; k++) {
+            o[k] = c->o[(size_t)qi * dim + k];
+            d[k] = c->d[(size_t)qi * dim + k];
+            dinv[k] = 1.0f / d[k]; /* GPURay dInv (bvh_interop_structures.h:737-741) */
+        }
+        float tmax = c->tmax[qi];
+        oracle_hit best;
+        hit_default(&best);
+        int n_int = 0, n_prim = 0;
+        float b0, b1, b2, b3;
+        if (box_ray_gpu(&h->nodes[0].box, o, d, dinv, tmax, dim, &b0, &b1)) {
+            int sp = 0, done = 0;
+            st[0].node = 0;
+            st[0].dist = b0;
+            while (sp >= 0 && !done) {
+                int ni = st[sp].node;
+                float dist = st[sp].dist;
+                sp--;
+                if (dist > tmax) continue;
+                const node_t *n = &h->nodes[ni];
+                if (n->nrefs > 0) {
+                    for (int p = 0; p < n->nrefs; p++) {
+                        int slot = n->offset + p, hit;
+                        oracle_hit cand;
+                        hit_default(&cand);
+                        n_prim++;
+                        if (dim == 3)
+                            hit = ray_triangle_gpu(vtx(h, slot, 0), vtx(h, slot, 1), vtx(h, slot, 2), o, d, tmax,
+                                                   c->occlusion, &cand);
+                        else
+                            hit = ray_segment(vtx(h, slot, 0), vtx(h, slot, 1), o, d, tmax, c->occlusion, 0, &cand,
+                                              NULL);
+                        if (hit) {
+                            if (c->occlusion) {
+                                /* bvh.slang:183-187: only the index of the occluder is written */
+                                best.index = h->prim_orig[slot];
+                                done = 1;
+                                break;
+                            }
+                            cand.index = h->prim_orig[slot];
+                            tmax = fminf(tmax, cand.d);
+                            best = cand;
+                        }
+                    }
+                } else {
+                    n_int++;
+                    int l = ni + 1, r = ni + n->offset;
+                    int hl = box_ray_gpu(&h->nodes[l].box, o, d, dinv, tmax, dim, &b0, &b1);
+                    int hr = box_ray_gpu(&h->nodes[r].box, o, d, dinv, tmax, dim, &b2, &b3);
+                    if (hl && hr) {
+                        int closer = l, other = r;
+                        if (b2 < b0) {
+                            float t = b0; b0 = b2; b2 = t;
+                            closer = r; other = l;
+                        }
+                        st[++sp].node = other; st[sp].dist = b2;
+                        st[++sp].node = closer; st[sp].dist = b0;
+                    } else if (hl) {
+                        st[++sp].node = l; st[sp].dist = b0;
+                    } else if (hr) {
+                        st[++sp].node = r; st[sp].dist = b2;
+                    }
+                }
+            }
+        }
+        c->out[qi] = best;
+        if (c->n_internal) c->n_internal[qi] = n_int;
+        if (c->n_leaf_prims) c->n_leaf_prims[qi] = n_prim;
+    }
+}
+
+void oracle_intersect_gpuorder(const oracle_bvh *h, const float *origins, const float *dirs, const float *tmax,
+                               int n, int occlusion, int nthreads, oracle_hit *out, int32_t *n_internal,
+                               int32_t *n_leaf_prims)
+{
+    ray_ctx c = {h, origins, dirs, tmax, occlusion, out, n_internal, n_leaf_prims};
+    parallel_for(n, nthreads, ray_gpu_range, &c);
+}
+
+/* ----------------------------------------------------------------------------
+ * FP64 brute force, independent method: project onto the supporting line or
+ * plane, keep the projection when it falls inside the primitive, otherwise take
+ * the nearest boundary point (edge clamping).  No code is shared with the
+ * region test above, so an error in either shows up as a disagreement.
+ * ------------------------------------------------------------------------- */
+static double seg_dist_f64(const double *a, const double *b, const double *x, int dim, double *p, double *t_out)
+{
+    double ab[3] = {0, 0, 0}, ax[3] = {0, 0, 0};
+    double L = 0.0, s = 0.0;
+    for (int i = 0; i < dim; i++) { ab[i] = b[i] - a[i]; ax[i] = x[i] - a[i]; L += ab[i] * ab[i]; s += ab[i] * ax[i]; }
+    double t = L > 0.0 ? s / L : 0.0;
+    if (t < 0.0) t = 0.0;
+    if (t > 1.0) t = 1.0;
+    double dd = 0.0;
+    for (int i = 0; i < dim; i++) { p[i] = a[i] + t * ab[i]; double r = x[i] - p[i]; dd += r * r; }
+    if (t_out) *t_out = t;
+    return sqrt(dd);
+}
+
+static double tri_dist_f64(const double *a, const double *b, const double *c, const double *x, double *p)
+{
+    double e0[3], e1[3], n[3], ax[3];
+    for (int i = 0; i < 3; i++) { e0[i] = b[i] - a[i]; e1[i] = c[i] - a[i]; ax[i] = x[i] - a[i]; }
+    n[0] = e0[1] * e1[2] - e0[2] * e1[1];
+    n[1] = e0[2] * e1[0] - e0[0] * e1[2];
+    n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+    double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (nn > 0.0) {
+        double s = (ax[0] * n[0] + ax[1] * n[1] + ax[2] * n[2]) / nn;
+        double q[3];
+        for (int i = 0; i < 3; i++) q[i] = x[i] - s * n[i];
+        /* signed sub-triangle areas relative to n decide containment */
+        const double *V[3] = {a, b, c};
+        int inside = 1;
+        for (int k = 0; k < 3 && inside; k++) {
+            const double *u = V[k], *v = V[(k + 1) % 3];
+            double uv[3], uq[3], cr[3];
+            for (int i = 0; i < 3; i++) { uv[i] = v[i] - u[i]; uq[i] = q[i] - u[i]; }
+            cr[0] = uv[1] * uq[2] - uv[2] * uq[1];
+            cr[1] = uv[2] * uq[0] - uv[0] * uq[2];
+            cr[2] = uv[0] * uq[1] - uv[1] * uq[0];
+            if (cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2] < 0.0) inside = 0;
+        }
+        if (inside) {
+            double dd = 0.0;
+            for (int i = 0; i < 3; i++) { p[i] = q[i]; double r = x[i] - q[i]; dd += r * r; }
+            return sqrt(dd);
+        }
+    }
+    double best = INFINITY, pp[3];
+    const double *V[3] = {a, b, c};
+    for (int k = 0; k < 3; k++) {
+        double dk = seg_dist_f64(V[k], V[(k + 1) % 3], x, 3, pp, NULL);
+        if (dk < best) { best = dk; p[0] = pp[0]; p[1] = pp[1]; p[2] = pp[2]; }
+    }
+    return best;
+}
+
+typedef struct {
+    int dim;
+    const float *pos;
+    const int32_t *idx;
+    int np;
+    const float *q, *dirs;
+    double *d, *p;
+    int32_t *index;
+} f64_ctx;
+
+static void load_prim_f64(const f64_ctx *c, int prim, double v[3][3])
+{
+    for (int k = 0; k < c->dim; k++)
+        for (int i = 0; i < 3; i++)
+            v[k][i] = i < c->dim ? (double)c->pos[(size_t)c->idx[(size_t)prim * c->dim + k] * c->dim + i] : 0.0;
+}
+
+static void f64_cp_range(void *vctx, int start, int end)
+{
+    f64_ctx *c = (f64_ctx *)vctx;
+    for (int qi = start; qi < end; qi++) {
+        double x[3] = {0, 0, 0}, best = INFINITY, bp[3] = {0, 0, 0}, pp[3];
+        int bi = -1;
+        for (int i = 0; i < c->dim; i++) x[i] = c->q[(size_t)qi * c->dim + i];
+        for (int pr = 0; pr < c->np; pr++) {
+            double v[3][3];
+            load_prim_f64(c, pr, v);
+            double dd = c->dim == 3 ? tri_dist_f64(v[0], v[1], v[2], x, pp) : seg_dist_f64(v[0], v[1], x, 2, pp, NULL);
+            if (dd < best) { best = dd; bi = pr; bp[0] = pp[0]; bp[1] = pp[1]; bp[2] = c->dim == 3 ? pp[2] : 0.0; }
+        }
+        c->d[qi] = best;
+        for (int i = 0; i < 3; i++) c->p[(size_t)qi * 3 + i] = bp[i];
+        c->index[qi] = bi;
+    }
+}
+
+void oracle_brute_closest_f64(int dim, const float *positions, int n_vertices, const int32_t *indices, int n_prims,
+                              const float *queries, int n, int nthreads, double *d, double *p, int32_t *index)
+{
+    (void)n_vertices;
+    f64_ctx c = {dim, positions, indices, n_prims, queries, NULL, d, p, index};
+    parallel_for(n, nthreads, f64_cp_range, &c);
+}
+
+/* FP64 ray: exact plane/line intersection with an inside test by signed areas */
+static void f64_ray_range(void *vctx, int start, int end)
+{
+    f64_ctx *c = (f64_ctx *)vctx;
+    for (int qi = start; qi < end; qi++) {
+        double o[3] = {0, 0, 0}, dr[3] = {0, 0, 0}, best = INFINITY;
+        int bi = -1;
+        for (int i = 0; i < c->dim; i++) {
+            o[i] = c->q[(size_t)qi * c->dim + i];
+            dr[i] = c->dirs[(size_t)qi * c->dim + i];
+        }
+        for (int pr = 0; pr < c->np; pr++) {
+            double v[3][3] = {{0}};
+            load_prim_f64(c, pr, v);
+            double t = -1.0;
+            if (c->dim == 3) {
+                double e0[3], e1[3], n[3];
+                for (int i = 0; i < 3; i++) { e0[i] = v[1][i] - v[0][i]; e1[i] = v[2][i] - v[0][i]; }
+                n[0] = e0[1] * e1[2] - e0[2] * e1[1];
+                n[1] = e0[2] * e1[0] - e0[0] * e1[2];
+                n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+                double dn = dr[0] * n[0] + dr[1] * n[1] + dr[2] * n[2];
+                if (dn == 0.0) continue;
+                t = ((v[0][0] - o[0]) * n[0] + (v[0][1] - o[1]) * n[1] + (v[0][2] - o[2]) * n[2]) / dn;
+                if (t < 0.0) continue;
+                double q[3];
+                for (int i = 0; i < 3; i++) q[i] = o[i] + t * dr[i];
+                int inside = 1;
+                for (int k = 0; k < 3 && inside; k++) {
+                    const double *u = v[k], *w = v[(k + 1) % 3];
+                    double uw[3], uq[3], cr[3];
+                    for (int i = 0; i < 3; i++) { uw[i] = w[i] - u[i]; uq[i] = q[i] - u[i]; }
+                    cr[0] = uw[1] * uq[2] - uw[2] * uq[1];
+                    cr[1] = uw[2] * uq[0] - uw[0] * uq[2];
+                    cr[2] = uw[0] * uq[1] - uw[1] * uq[0];
+                    if (cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2] < 0.0) inside = 0;
+                }
+                if (!inside) continue;
+            } else {
+                double e[2] = {v[1][0] - v[0][0], v[1][1] - v[0][1]};
+                double den = dr[0] * e[1] - dr[1] * e[0];
+                if (den == 0.0) continue;
+                double ao[2] = {v[0][0] - o[0], v[0][1] - o[1]};
+                double s = (ao[0] * dr[1] - ao[1] * dr[0]) / den;
+                t = (ao[0] * e[1] - ao[1] * e[0]) / den;
+                if (s < 0.0 || s > 1.0 || t < 0.0) continue;
+            }
+            if (t < best) { best = t; bi = pr; }
+        }
+        c->d[qi] = best;
+        c->index[qi] = bi;
+    }
+}
+
+void oracle_brute_ray_f64(int dim, const float *positions, int n_vertices, const int32_t *indices, int n_prims,
+                          const float *origins, const float *dirs, int n, int nthreads, double *t, int32_t *index)
+{
+    (void)n_vertices;
+    f64_ctx c = {dim, positions, indices, n_prims, origins, dirs, t, NULL, index};
+    parallel_for(n, nthreads, f64_ray_range, &c);
+}
+
+void oracle_prim_dist_f64(int dim, const float *positions, const int32_t *indices, const float *queries,
+                          const int32_t *prims, int n, double *d)
+{
+    f64_ctx c = {dim, positions, indices, 0, NULL, NULL, NULL, NULL, NULL};
+    c.pos = positions;
+    c.idx = indices;
+    for (int qi = 0; qi < n; qi++) {
+        double x[3] = {0, 0, 0}, v[3][3], pp[3];
+        for (int i = 0; i < dim; i++) x[i] = queries[(size_t)qi * dim + i];
+        if (prims[qi] < 0) { d[qi] = INFINITY; continue; }
+        load_prim_f64(&c, prims[qi], v);
+        d[qi] = dim == 3 ? tri_dist_f64(v[0], v[1], v[2], x, pp) : seg_dist_f64(v[0], v[1], x, 2, pp, NULL);
+    }
+}

@@ -1,0 +1,102 @@
+/*
+ * fcpw_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference FCPW query path, used as the parity
+ * checker for the HIP kernels.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product (fcpw_amd) never does.
+ *
+ * What is restated (reference files relative to the fcpw source tree):
+ *   - binned SAH BVH build          include/fcpw/aggregates/bvh.inl:3-234,427-466
+ *   - leaf-order vertex renumbering include/fcpw/fcpw.inl:613-689 (indexMap)
+ *   - CPU closest-point traversal   include/fcpw/aggregates/bvh.inl:1290-1418
+ *   - CPU ray traversal             include/fcpw/aggregates/bvh.inl:655-771
+ *   - CPU refit                     include/fcpw/aggregates/bvh.inl:548-599
+ *   - brute-force aggregate         include/fcpw/aggregates/baseline.inl:342-384
+ *   - GPU traversal order           include/fcpw/gpu/bvh.slang:144-254,395-502
+ *   - primitive routines            include/fcpw/geometry/triangles.inl:12-32,222-260,374-487,
+ *                                   include/fcpw/geometry/line_segments.inl:11-28,140-175,200-255,
+ *                                   include/fcpw/gpu/geometry.slang:37-103,202-332
+ * plus an FP64 brute force written with an independent method (plane projection
+ * and edge clamping), which is the error reference for every FP32 path.
+ */
+#ifndef FCPW_ORACLE_H
+#define FCPW_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 40-byte record, the same layout as the reference GPUInteraction
+   (include/fcpw/gpu/bvh_interop_structures.h:760-774).  For the CPU-order
+   paths `index` is Interaction::primitiveIndex (-1 on a miss, whose bit
+   pattern equals FCPW_GPU_UINT_MAX). 2D results keep p.z = n.z = 0, uv.y = 0. */
+typedef struct {
+    float p[3];
+    float n[3];
+    float uv[2];
+    float d;
+    int32_t index;
+} oracle_hit;
+
+/* cost heuristics, numbered as fcpw::CostHeuristic (bvh.h:11-17) */
+enum { ORACLE_LONGEST_AXIS_CENTER = 0, ORACLE_SURFACE_AREA = 1, ORACLE_OVERLAP_SURFACE_AREA = 2,
+       ORACLE_VOLUME = 3, ORACLE_OVERLAP_VOLUME = 4 };
+
+typedef struct oracle_bvh oracle_bvh;
+
+/* dim = 2 (line segments, 2 indices per primitive) or 3 (triangles, 3 indices) */
+oracle_bvh *oracle_build(int dim, const float *positions, int n_vertices,
+                         const int32_t *indices, int n_prims,
+                         int heuristic, int leaf_size, int n_buckets);
+void oracle_free(oracle_bvh *h);
+
+int oracle_num_nodes(const oracle_bvh *h);
+int oracle_num_leaves(const oracle_bvh *h);
+int oracle_max_depth(const oracle_bvh *h);
+
+/* flat tree export: boxes padded to 3 floats (z = 0 in 2D), offset = reference
+   offset for leaves or secondChildOffset for internal nodes (nrefs == 0) */
+void oracle_export_nodes(const oracle_bvh *h, float *pmin, float *pmax, int32_t *nrefs, int32_t *offset);
+/* prim_order[j] = original primitive index stored at leaf slot j */
+void oracle_export_prim_order(const oracle_bvh *h, int32_t *prim_order);
+/* index_map[user vertex] = sorted vertex (fcpw.inl:652-689); -1 for unreferenced */
+void oracle_export_index_map(const oracle_bvh *h, int32_t *index_map);
+
+/* Scene::updateObjectVertices (fcpw.inl:1128-1143) followed by Scene::refit */
+void oracle_update_vertices(oracle_bvh *h, const float *positions_user_order, int n_vertices);
+void oracle_refit(oracle_bvh *h);
+
+/* nthreads <= 0: one thread per online core, static partition as fcpw.inl:1547-1559 */
+void oracle_closest_points_cpu(const oracle_bvh *h, const float *queries, const float *r2, int n,
+                               int record_normal, int nthreads, oracle_hit *out);
+/* reference GPU order; n_internal / n_leaf_prims (optional) receive per-query
+   counts of expanded internal nodes and primitive tests; n_pops the node pops */
+void oracle_closest_points_gpuorder(const oracle_bvh *h, const float *queries, const float *r2, int n,
+                                    int record_normal, int nthreads, oracle_hit *out,
+                                    int32_t *n_internal, int32_t *n_leaf_prims, int32_t *n_pops);
+void oracle_intersect_cpu(const oracle_bvh *h, const float *origins, const float *dirs,
+                          const float *tmax, int n, int occlusion, int nthreads, oracle_hit *out);
+void oracle_intersect_gpuorder(const oracle_bvh *h, const float *origins, const float *dirs,
+                               const float *tmax, int n, int occlusion, int nthreads, oracle_hit *out,
+                               int32_t *n_internal, int32_t *n_leaf_prims);
+/* Baseline aggregate: every primitive in scene order, FP32 (baseline.inl:342-384) */
+void oracle_brute_closest_f32(const oracle_bvh *h, const float *queries, const float *r2, int n,
+                              int nthreads, oracle_hit *out);
+
+/* FP64 brute force on raw geometry (positions in user order) */
+void oracle_brute_closest_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,
+                              int n_prims, const float *queries, int n, int nthreads,
+                              double *d, double *p, int32_t *index);
+void oracle_brute_ray_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,
+                          int n_prims, const float *origins, const float *dirs, int n, int nthreads,
+                          double *t, int32_t *index);
+/* FP64 distance from each query to one given primitive (tie classification) */
+void oracle_prim_dist_f64(int dim, const float *positions, const int32_t *indices,
+                          const float *queries, const int32_t *prims, int n, double *d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
