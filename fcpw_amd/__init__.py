@@ -1,0 +1,139 @@
+"""fcpw_amd -- MI355X query backend for FCPW.
+
+Drop-in for the reference's GPU Python surface (python/fcpw_bindings.cpp:493-673):
+``GPUScene2D/3D`` with ``transfer_to_gpu``, ``refit``, ``intersect`` and
+``find_closest_points``, the ``GPU*`` record types and ``GPUInteractionList``
+with its bulk ``get_*`` extraction, plus the ``Scene2D/3D`` subset that builds
+what the GPU consumes.  Queries run in hand-written HIP kernels for gfx950
+(``lib/libfcpw_hip.so``, C ABI in ``include/fcpw_hip.h``); there is no CPU
+fallback: importing fails if the native library is missing.
+
+Usage is the reference's::
+
+    import fcpw_amd as fcpw
+    scene = fcpw.Scene3D()
+    scene.set_object_count(1)
+    scene.set_object_vertices(V, 0)
+    scene.set_object_triangles(F, 0)
+    scene.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    gpu = fcpw.GPUScene3D(fcpw.slang_directory_path)
+    gpu.transfer_to_gpu(scene)
+    hits = fcpw.GPUInteractionList()
+    gpu.find_closest_points(Q, r2, hits)
+    d = hits.get_distances()
+"""
+import os as _os
+
+import numpy as _np
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+
+try:
+    from . import _fcpw_amd as _native
+except ImportError as e:  # no silent fallback: the HIP library is the product
+    raise ImportError(
+        f"fcpw_amd: the native module is not built ({e}). "
+        "Run `python -c 'import __graft_entry__ as g; g.build()'` or "
+        "`python fcpw_amd/_build.py` in the repository root.") from e
+
+PrimitiveType = _native.PrimitiveType
+AggregateType = _native.AggregateType
+Scene2D = _native.Scene2D
+Scene3D = _native.Scene3D
+GPUFloat2 = _native.GPUFloat2
+GPUFloat3 = _native.GPUFloat3
+GPURay = _native.GPURay
+GPUBoundingSphere = _native.GPUBoundingSphere
+GPUInteraction = _native.GPUInteraction
+GPUFloat3List = _native.GPUFloat3List
+GPURayList = _native.GPURayList
+GPUBoundingSphereList = _native.GPUBoundingSphereList
+GPUInteractionList = _native.GPUInteractionList
+UINT_MAX = _native.UINT_MAX
+
+# The reference exports the directory of its packaged Slang shaders
+# (python/fcpw/__init__.py:16-20) for GPUScene's first argument.  No shader is
+# compiled at run time here; the constant is kept so callers work unchanged.
+slang_directory_path = _HERE
+
+# numpy view of GPUInteraction / GPURay / GPUBoundingSphere records
+INTERACTION_DTYPE = _np.dtype([("p", "<f4", (3,)), ("n", "<f4", (3,)), ("uv", "<f4", (2,)),
+                               ("d", "<f4"), ("index", "<u4")])
+RAY_DTYPE = _np.dtype([("o", "<f4", (3,)), ("d", "<f4", (3,)), ("dinv", "<f4", (3,)), ("tmax", "<f4")])
+SPHERE_DTYPE = _np.dtype([("c", "<f4", (3,)), ("r2", "<f4")])
+
+
+def _is_list(x, cls):
+    return isinstance(x, cls)
+
+
+class _GPUSceneMixin:
+    """Reference signatures on top of the native GPUScene classes."""
+
+    def intersect(self, *args, **kwargs):
+        """intersect(ray_origins, ray_directions, ray_distance_bounds, interactions, check_for_occlusion=False)
+        intersect(rays, interactions, check_for_occlusion=False)
+        intersect(sphere_centers, sphere_squared_radii, rand_nums, interactions)   (not built: raises)
+        intersect(bounding_spheres, rand_nums, interactions)                      (not built: raises)"""
+        if "rays" in kwargs or (args and _is_list(args[0], GPURayList)):
+            return self._intersect_rays(*args, **kwargs)
+        if "bounding_spheres" in kwargs or (args and _is_list(args[0], GPUBoundingSphereList)):
+            return self._intersect_spheres(*args, **kwargs)
+        if "sphere_centers" in kwargs or "rand_nums" in kwargs:
+            raise RuntimeError("sphere intersection queries are not built by the MI355X backend")
+        if len(args) >= 3 and _np.ndim(args[1]) == 1 and _np.ndim(args[2]) == 2:
+            raise RuntimeError("sphere intersection queries are not built by the MI355X backend")
+        return self._intersect_arrays(*args, **kwargs)
+
+    def find_closest_points(self, *args, **kwargs):
+        """find_closest_points(query_points, squared_max_radii, interactions, record_normals=False)
+        find_closest_points(bounding_spheres, interactions, record_normals=False)"""
+        if "bounding_spheres" in kwargs or (args and _is_list(args[0], GPUBoundingSphereList)):
+            return self._find_closest_points_spheres(*args, **kwargs)
+        return self._find_closest_points_arrays(*args, **kwargs)
+
+    def find_closest_silhouette_points(self, *args, **kwargs):
+        raise RuntimeError("closest silhouette point queries are not built by the MI355X backend")
+
+    # -- device-resident queries (torch tensors) ---------------------------------
+    def find_closest_points_device(self, spheres, out, record_normals=False, stream=None):
+        """spheres: (n, 4) float32 CUDA tensor of GPUBoundingSphere records;
+        out: (n, 10) float32 CUDA tensor receiving GPUInteraction records."""
+        n = spheres.shape[0]
+        self._closest_points_device(spheres.data_ptr(), n, bool(record_normals), out.data_ptr(),
+                                    _stream_handle(spheres, stream))
+
+    def intersect_device(self, rays, out, check_for_occlusion=False, stream=None):
+        """rays: (n, 10) float32 CUDA tensor of GPURay records; out: (n, 10) float32."""
+        self._intersect_device(rays.data_ptr(), rays.shape[0], bool(check_for_occlusion), out.data_ptr(),
+                               _stream_handle(rays, stream))
+
+    def set_option(self, key, value):
+        self._set_option(key, int(value))
+
+    def get_option(self, key):
+        return self._get_option(key)
+
+    def info(self, key):
+        return self._info(key)
+
+
+def _stream_handle(t, stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(t.device)
+    return stream.cuda_stream
+
+
+class GPUScene2D(_GPUSceneMixin, _native.GPUScene2D):
+    DIM = 2
+
+
+class GPUScene3D(_GPUSceneMixin, _native.GPUScene3D):
+    DIM = 3
+
+
+__all__ = ["PrimitiveType", "AggregateType", "Scene2D", "Scene3D", "GPUFloat2", "GPUFloat3", "GPURay",
+           "GPUBoundingSphere", "GPUInteraction", "GPUFloat3List", "GPURayList", "GPUBoundingSphereList",
+           "GPUInteractionList", "GPUScene2D", "GPUScene3D", "slang_directory_path", "UINT_MAX",
+           "INTERACTION_DTYPE", "RAY_DTYPE", "SPHERE_DTYPE"]

@@ -1,0 +1,1203 @@
+// fcpw_hip.hip -- MI355X (gfx950) query backend for FCPW behind the C ABI in
+// include/fcpw_hip.h.
+//
+// Device layout (HBM), built on the host from the reference's flat DFS BVH:
+//   records  one per internal node, in DFS order.  A record holds the boxes of
+//            BOTH children plus a 32-bit link per child, so expanding a node is
+//            one 64-byte (3D) / 48-byte (2D) contiguous fetch:
+//              3D: float4 {loA.xyz, linkA} {hiA.xyz, 0} {loB.xyz, linkB} {hiB.xyz, 0}
+//              2D: float4 {loA.xy, hiA.xy} {loB.xy, hiB.xy} {linkA, linkB, 0, 0}
+//            link = internal record index, or LEAF_BIT | (count-1) << 27 | first leaf slot
+//   root     the root box and link in the same slot format (2 float4)
+//   prims    leaf-slot order; 3D: float4 {pa,0} {pb,0} {pc,0}; 2D: float4 {pa.xy, pb.xy}
+//   prim_index / prim_vids   original primitive index and vertex ids per leaf slot
+//   rec_parent, leaf_refs, counters   the bottom-up refit schedule
+//
+// Query kernels run one query per lane.  The traversal order is the reference
+// GPU order (include/fcpw/gpu/bvh.slang:144-254,395-502): test both children,
+// shrink r^2 by each child's farthest-corner distance, continue with the nearer
+// child and push the farther one.  Each stack entry carries the child's link and
+// its squared (or parametric) distance, so a pop is pruned without refetching
+// the node.  The top K entries of the stack live in LDS (K x 8 bytes per lane,
+// lane-interleaved so a wave's accesses hit distinct banks); entries beyond K
+// spill to a private array.  Depth <= 63 (leaf forced at depth 62,
+// bvh.inl:170), so 64 entries always suffice.
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../../include/fcpw_hip.h"
+#include "geometry.hpp"
+
+namespace fcpw_hip {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t CNT_SHIFT = 27;
+constexpr uint32_t CNT_MASK = 0xFu;
+constexpr uint32_t OFF_MASK = (1u << 27) - 1u;
+constexpr int MAX_LEAF_PRIMS = 16;
+constexpr int STACK_TOTAL = 64;
+
+struct DevScene {
+    const float4 *records;
+    const float4 *root;
+    const float4 *prims;
+    const int32_t *prim_index;
+    int *flags; // bit 0: traversal stack overflow
+};
+
+__device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
+
+// ---------------------------------------------------------------------------
+// stack: K entries in LDS, the rest private
+// ---------------------------------------------------------------------------
+template <int BLOCK, int K>
+struct Stack {
+    uint2 *lds; // this lane's column: entry s at lds[s * BLOCK]
+    uint2 spill[STACK_TOTAL - K];
+    int sp;
+    __device__ __forceinline__ bool push(uint32_t link, float d)
+    {
+        if (sp >= STACK_TOTAL) return false;
+        uint2 e = make_uint2(link, __float_as_uint(d));
+        if (sp < K) lds[sp * BLOCK] = e;
+        else spill[sp - K] = e;
+        sp++;
+        return true;
+    }
+    __device__ __forceinline__ void pop(uint32_t &link, float &d)
+    {
+        sp--;
+        uint2 e = (sp < K) ? lds[sp * BLOCK] : spill[sp - K];
+        link = e.x;
+        d = __uint_as_float(e.y);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// per-dimension record/primitive access
+// ---------------------------------------------------------------------------
+template <int DIM>
+struct Geo;
+
+template <>
+struct Geo<3> {
+    using vec = v3;
+    static constexpr int REC = 4;
+    static __device__ __forceinline__ void root(const DevScene &S, vec &lo, vec &hi, uint32_t &link)
+    {
+        float4 a = S.root[0], b = S.root[1];
+        lo = {a.x, a.y, a.z};
+        hi = {b.x, b.y, b.z};
+        link = link_of(a.w);
+    }
+    static __device__ __forceinline__ void record(const DevScene &S, uint32_t r, vec &loA, vec &hiA, uint32_t &lA,
+                                                  vec &loB, vec &hiB, uint32_t &lB)
+    {
+        const float4 *p = S.records + (size_t)r * REC;
+        float4 a0 = p[0], a1 = p[1], b0 = p[2], b1 = p[3];
+        loA = {a0.x, a0.y, a0.z};
+        hiA = {a1.x, a1.y, a1.z};
+        lA = link_of(a0.w);
+        loB = {b0.x, b0.y, b0.z};
+        hiB = {b1.x, b1.y, b1.z};
+        lB = link_of(b0.w);
+    }
+    static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
+                                                    float &u1)
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        return closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
+    }
+    static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        v3 m = cross({b.x - a.x, b.y - a.y, b.z - a.z}, {c.x - a.x, c.y - a.y, c.z - a.z});
+        float l = sqrtf(dot(m, m));
+        n[0] = m.x / l; n[1] = m.y / l; n[2] = m.z / l;
+    }
+    static __device__ __forceinline__ vec load_point(const float *q) { return {q[0], q[1], q[2]}; }
+};
+
+template <>
+struct Geo<2> {
+    using vec = v2;
+    static constexpr int REC = 3;
+    static __device__ __forceinline__ void root(const DevScene &S, vec &lo, vec &hi, uint32_t &link)
+    {
+        float4 a = S.root[0], b = S.root[1];
+        lo = {a.x, a.y};
+        hi = {a.z, a.w};
+        link = link_of(b.x);
+    }
+    static __device__ __forceinline__ void record(const DevScene &S, uint32_t r, vec &loA, vec &hiA, uint32_t &lA,
+                                                  vec &loB, vec &hiB, uint32_t &lB)
+    {
+        const float4 *p = S.records + (size_t)r * REC;
+        float4 a = p[0], b = p[1], l = p[2];
+        loA = {a.x, a.y};
+        hiA = {a.z, a.w};
+        loB = {b.x, b.y};
+        hiB = {b.z, b.w};
+        lA = link_of(l.x);
+        lB = link_of(l.y);
+    }
+    static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
+                                                    float &u1)
+    {
+        float4 s = S.prims[slot];
+        u1 = 0.0f;
+        return closest_point_segment({s.x, s.y}, {s.z, s.w}, x, p, u0);
+    }
+    static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
+    {
+        float4 s = S.prims[slot];
+        v2 m = {s.w - s.y, -(s.z - s.x)};
+        float l = sqrtf(dot(m, m));
+        n[0] = m.x / l; n[1] = m.y / l; n[2] = 0.0f;
+    }
+    static __device__ __forceinline__ vec load_point(const float *q) { return {q[0], q[1]}; }
+};
+
+__device__ __forceinline__ void store_p(v3 p, float o[3]) { o[0] = p.x; o[1] = p.y; o[2] = p.z; }
+__device__ __forceinline__ void store_p(v2 p, float o[3]) { o[0] = p.x; o[1] = p.y; o[2] = 0.0f; }
+
+// ---------------------------------------------------------------------------
+// closest point traversal (bvh.slang:395-502 order)
+// ---------------------------------------------------------------------------
+struct CpResult {
+    int32_t slot; // -1: not found
+    float d;
+    float p[3];
+    float uv[2];
+};
+
+template <int DIM, int BLOCK, int K>
+__device__ __forceinline__ void closest_point_query(const DevScene &S, typename Geo<DIM>::vec x, float r2,
+                                                    uint2 *lds_col, CpResult &res)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    res.slot = -1;
+    res.d = FLT_MAX;
+    res.p[0] = res.p[1] = res.p[2] = 0.0f;
+    res.uv[0] = res.uv[1] = 0.0f;
+
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float d2min, d2max;
+    box_sqdist(lo, hi, x, d2min, d2max);
+    if (!(d2min <= r2)) return;
+    r2 = fminf(r2, d2max);
+    float cur_d2 = d2min;
+
+    Stack<BLOCK, K> st;
+    st.lds = lds_col;
+    st.sp = 0;
+    vec bp = {};
+    while (true) {
+        if (cur_d2 <= r2) {
+            if (cur & LEAF_BIT) {
+                uint32_t off = cur & OFF_MASK;
+                uint32_t cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    vec p;
+                    float u0, u1;
+                    float d = G::closest(S, off + k, x, p, u0, u1);
+                    if (d * d <= r2) {
+                        r2 = fminf(r2, d * d);
+                        res.slot = (int32_t)(off + k);
+                        res.d = d;
+                        bp = p;
+                        res.uv[0] = u0;
+                        res.uv[1] = u1;
+                    }
+                }
+            } else {
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                float a0, a1, b0, b1;
+                box_sqdist(loA, hiA, x, a0, a1);
+                bool oA = a0 <= r2;
+                r2 = fminf(r2, a1);
+                box_sqdist(loB, hiB, x, b0, b1);
+                bool oB = b0 <= r2;
+                r2 = fminf(r2, b1);
+                if (oA && oB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearD = a0, farD = b0;
+                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                    if (!st.push(farL, farD)) { atomicOr(S.flags, 1); break; }
+                    cur = nearL;
+                    cur_d2 = nearD;
+                    continue;
+                } else if (oA) {
+                    cur = lA; cur_d2 = a0;
+                    continue;
+                } else if (oB) {
+                    cur = lB; cur_d2 = b0;
+                    continue;
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        st.pop(cur, cur_d2);
+    }
+    if (res.slot >= 0) store_p(bp, res.p);
+}
+
+// ---------------------------------------------------------------------------
+// ray traversal (bvh.slang:144-254 order)
+// ---------------------------------------------------------------------------
+struct RayResult {
+    int32_t slot;
+    float t;
+    float p[3];
+    float n[3];
+    float uv[2];
+};
+
+template <int DIM>
+struct RayGeo;
+
+template <>
+struct RayGeo<3> {
+    static __device__ __forceinline__ bool hit(const DevScene &S, uint32_t slot, v3 o, v3 d, float tmax,
+                                               RayResult &r, bool fill)
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        float tt, v, w;
+        v3 pa = {a.x, a.y, a.z}, pb = {b.x, b.y, b.z}, pc = {c.x, c.y, c.z};
+        if (!ray_triangle(pa, pb, pc, o, d, tmax, tt, v, w)) return false;
+        if (fill) {
+            v3 v1 = sub(pb, pa), v2 = sub(pc, pa);
+            r.p[0] = (pa.x + v1.x * v) + v2.x * w;
+            r.p[1] = (pa.y + v1.y * v) + v2.y * w;
+            r.p[2] = (pa.z + v1.z * v) + v2.z * w;
+            v3 n = cross(v1, v2);
+            float l = sqrtf(dot(n, n));
+            r.n[0] = n.x / l; r.n[1] = n.y / l; r.n[2] = n.z / l;
+            r.uv[0] = 1.0f - v - w;
+            r.uv[1] = v;
+            r.t = tt;
+        }
+        return true;
+    }
+};
+
+template <>
+struct RayGeo<2> {
+    static __device__ __forceinline__ bool hit(const DevScene &S, uint32_t slot, v2 o, v2 d, float tmax,
+                                               RayResult &r, bool fill)
+    {
+        float4 s4 = S.prims[slot];
+        v2 pa = {s4.x, s4.y}, pb = {s4.z, s4.w};
+        float tt, s;
+        if (!ray_segment(pa, pb, o, d, tmax, tt, s)) return false;
+        if (fill) {
+            v2 v = sub(pb, pa);
+            r.p[0] = pa.x + s * v.x;
+            r.p[1] = pa.y + s * v.y;
+            r.p[2] = 0.0f;
+            v2 n = {v.y, -v.x};
+            float l = sqrtf(dot(n, n));
+            r.n[0] = n.x / l; r.n[1] = n.y / l; r.n[2] = 0.0f;
+            r.uv[0] = s;
+            r.uv[1] = 0.0f;
+            r.t = tt;
+        }
+        return true;
+    }
+};
+
+template <int DIM, int BLOCK, int K>
+__device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::vec o, typename Geo<DIM>::vec d,
+                                          typename Geo<DIM>::vec dinv, float tmax, bool occlusion, uint2 *lds_col,
+                                          RayResult &res)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    res.slot = -1;
+    res.t = FLT_MAX;
+    for (int i = 0; i < 3; i++) { res.p[i] = 0.0f; res.n[i] = 0.0f; }
+    res.uv[0] = res.uv[1] = 0.0f;
+
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float cur_t;
+    if (!box_ray(lo, hi, o, d, dinv, tmax, cur_t)) return;
+
+    Stack<BLOCK, K> st;
+    st.lds = lds_col;
+    st.sp = 0;
+    while (true) {
+        if (!(cur_t > tmax)) {
+            if (cur & LEAF_BIT) {
+                uint32_t off = cur & OFF_MASK;
+                uint32_t cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                bool stop = false;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    if (RayGeo<DIM>::hit(S, off + k, o, d, tmax, res, !occlusion)) {
+                        res.slot = (int32_t)(off + k);
+                        if (occlusion) { stop = true; break; }
+                        tmax = fminf(tmax, res.t);
+                    }
+                }
+                if (stop) break;
+            } else {
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                float ta, tb;
+                bool hA = box_ray(loA, hiA, o, d, dinv, tmax, ta);
+                bool hB = box_ray(loB, hiB, o, d, dinv, tmax, tb);
+                if (hA && hB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearT = ta, farT = tb;
+                    if (tb < ta) { nearL = lB; farL = lA; nearT = tb; farT = ta; }
+                    if (!st.push(farL, farT)) { atomicOr(S.flags, 1); break; }
+                    cur = nearL;
+                    cur_t = nearT;
+                    continue;
+                } else if (hA) {
+                    cur = lA; cur_t = ta;
+                    continue;
+                } else if (hB) {
+                    cur = lB; cur_t = tb;
+                    continue;
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        st.pop(cur, cur_t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// interaction record written as five 8-byte stores (the record is 8-byte aligned)
+__device__ __forceinline__ void write_interaction(float2 *o, const float p[3], const float n[3], const float uv[2],
+                                                  float d, uint32_t index)
+{
+    o[0] = make_float2(p[0], p[1]);
+    o[1] = make_float2(p[2], n[0]);
+    o[2] = make_float2(n[1], n[2]);
+    o[3] = make_float2(uv[0], uv[1]);
+    o[4] = make_float2(d, __uint_as_float(index));
+}
+
+template <int DIM, int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void k_closest_points(DevScene S, const float4 *spheres, int64_t n,
+                                                          int record_normals, float2 *out)
+{
+    __shared__ uint2 lds[K * BLOCK];
+    uint2 *col = lds + threadIdx.x;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        float4 s = spheres[i];
+        float q[3] = {s.x, s.y, s.z};
+        CpResult r;
+        closest_point_query<DIM, BLOCK, K>(S, Geo<DIM>::load_point(q), s.w, col, r);
+        float nrm[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t index = FCPW_HIP_UINT_MAX;
+        if (r.slot >= 0) {
+            index = (uint32_t)S.prim_index[r.slot];
+            if (record_normals) Geo<DIM>::normal(S, (uint32_t)r.slot, nrm);
+        }
+        write_interaction(out + i * 5, r.p, nrm, r.uv, r.d, index);
+    }
+}
+
+template <int DIM, int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void k_closest_points_compact(DevScene S, const float *points, const float *r2in,
+                                                                  int64_t n, float *d_out, float *p_out,
+                                                                  int32_t *prim_out, int32_t *slot_out)
+{
+    __shared__ uint2 lds[K * BLOCK];
+    uint2 *col = lds + threadIdx.x;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const float *q = points + i * DIM;
+        float r2 = r2in ? r2in[i] : INFINITY;
+        CpResult r;
+        closest_point_query<DIM, BLOCK, K>(S, Geo<DIM>::load_point(q), r2, col, r);
+        if (d_out) d_out[i] = r.d;
+        if (p_out)
+            for (int k = 0; k < DIM; k++) p_out[i * DIM + k] = r.p[k];
+        if (prim_out) prim_out[i] = r.slot >= 0 ? S.prim_index[r.slot] : -1;
+        if (slot_out) slot_out[i] = r.slot;
+    }
+}
+
+template <int DIM, int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *rays, int64_t n, int occlusion,
+                                                     float2 *out)
+{
+    __shared__ uint2 lds[K * BLOCK];
+    uint2 *col = lds + threadIdx.x;
+    using vec = typename Geo<DIM>::vec;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
+        float2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
+        float o3[3] = {r0.x, r0.y, r1.x}, d3[3] = {r1.y, r2.x, r2.y}, di3[3] = {r3.x, r3.y, r4.x};
+        float tmax = r4.y;
+        vec o = Geo<DIM>::load_point(o3), d = Geo<DIM>::load_point(d3), di = Geo<DIM>::load_point(di3);
+        RayResult r;
+        ray_query<DIM, BLOCK, K>(S, o, d, di, tmax, occlusion != 0, col, r);
+        uint32_t index = r.slot >= 0 ? (uint32_t)S.prim_index[r.slot] : FCPW_HIP_UINT_MAX;
+        if (occlusion) {
+            // only the occluder's index is written (bvh.slang:183-187)
+            float z[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
+            write_interaction(out + i * 5, z, z, uv, FLT_MAX, index);
+        } else {
+            write_interaction(out + i * 5, r.p, r.n, r.uv, r.t, index);
+        }
+    }
+}
+
+// rebuild the primitive records from vertex positions (caller's vertex order)
+template <int DIM>
+__global__ void k_rebuild_prims(float4 *prims, const int32_t *vids, const float *V, int n_prims)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_prims) return;
+    if (DIM == 3) {
+        for (int k = 0; k < 3; k++) {
+            const float *v = V + (size_t)vids[(size_t)s * 3 + k] * 3;
+            prims[(size_t)s * 3 + k] = make_float4(v[0], v[1], v[2], 0.0f);
+        }
+    } else {
+        const float *a = V + (size_t)vids[(size_t)s * 2] * 2;
+        const float *b = V + (size_t)vids[(size_t)s * 2 + 1] * 2;
+        prims[s] = make_float4(a[0], a[1], b[0], b[1]);
+    }
+}
+
+// Bottom-up refit in one launch: one thread per leaf computes the leaf box,
+// writes it into its parent record's slot and counts the arrival; the second
+// arrival at a record unions the two slot boxes and climbs.  Box values only
+// involve min, max and the +-FLT_EPSILON inflation, so the result is bit-equal
+// to the reference refit (bvh.slang:27-141, bvh.inl:548-581) in any order.
+// The second arrival resets the counter, leaving all counters 0 for the next call.
+template <int DIM>
+__global__ void k_refit(float4 *records, float4 *root, const float4 *prims, const int32_t *leaf_refs,
+                        const uint32_t *leaf_links, const int32_t *rec_parent, unsigned int *counters, int n_leaves)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_leaves) return;
+    uint32_t link = leaf_links[j];
+    uint32_t off = link & OFF_MASK, cnt = ((link >> CNT_SHIFT) & CNT_MASK) + 1u;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t k = 0; k < cnt; k++) {
+        if (DIM == 3) {
+            const float4 *t = prims + (size_t)(off + k) * 3;
+            float4 a = t[0], b = t[1], c = t[2];
+            // Triangle.getBoundingBox (geometry.slang:342-346)
+            lo[0] = fminf(lo[0], fminf(fminf(a.x, b.x), c.x) - FLT_EPSILON);
+            lo[1] = fminf(lo[1], fminf(fminf(a.y, b.y), c.y) - FLT_EPSILON);
+            lo[2] = fminf(lo[2], fminf(fminf(a.z, b.z), c.z) - FLT_EPSILON);
+            hi[0] = fmaxf(hi[0], fmaxf(fmaxf(a.x, b.x), c.x) + FLT_EPSILON);
+            hi[1] = fmaxf(hi[1], fmaxf(fmaxf(a.y, b.y), c.y) + FLT_EPSILON);
+            hi[2] = fmaxf(hi[2], fmaxf(fmaxf(a.z, b.z), c.z) + FLT_EPSILON);
+        } else {
+            float4 s = prims[off + k];
+            // LineSegment.getBoundingBox (geometry.slang:112-116)
+            lo[0] = fminf(lo[0], fminf(s.x, s.z) - FLT_EPSILON);
+            lo[1] = fminf(lo[1], fminf(s.y, s.w) - FLT_EPSILON);
+            hi[0] = fmaxf(hi[0], fmaxf(s.x, s.z) + FLT_EPSILON);
+            hi[1] = fmaxf(hi[1], fmaxf(s.y, s.w) + FLT_EPSILON);
+        }
+    }
+    int32_t ref = leaf_refs[j];
+    while (true) {
+        if (ref < 0) { // the box belongs to the root
+            if (DIM == 3) {
+                float4 a = root[0];
+                root[0] = make_float4(lo[0], lo[1], lo[2], a.w);
+                root[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+            } else {
+                root[0] = make_float4(lo[0], lo[1], hi[0], hi[1]);
+            }
+            return;
+        }
+        int32_t rec = ref >> 1, slot = ref & 1;
+        if (DIM == 3) {
+            float4 *p = records + (size_t)rec * 4 + slot * 2;
+            float w = p[0].w;
+            p[0] = make_float4(lo[0], lo[1], lo[2], w);
+            p[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        } else {
+            records[(size_t)rec * 3 + slot] = make_float4(lo[0], lo[1], hi[0], hi[1]);
+        }
+        __threadfence();
+        unsigned int prev = atomicAdd(&counters[rec], 1u);
+        if (prev == 0) return;
+        counters[rec] = 0u;
+        __threadfence();
+        if (DIM == 3) {
+            const float4 *p = records + (size_t)rec * 4;
+            float4 a0 = p[0], a1 = p[1], b0 = p[2], b1 = p[3];
+            lo[0] = fminf(a0.x, b0.x); lo[1] = fminf(a0.y, b0.y); lo[2] = fminf(a0.z, b0.z);
+            hi[0] = fmaxf(a1.x, b1.x); hi[1] = fmaxf(a1.y, b1.y); hi[2] = fmaxf(a1.z, b1.z);
+        } else {
+            const float4 *p = records + (size_t)rec * 3;
+            float4 a = p[0], b = p[1];
+            lo[0] = fminf(a.x, b.x); lo[1] = fminf(a.y, b.y);
+            hi[0] = fmaxf(a.z, b.z); hi[1] = fmaxf(a.w, b.w);
+        }
+        ref = rec_parent[rec];
+    }
+}
+
+// Adjoint of d(x, mesh): recompute p and its barycentric weights from the saved
+// leaf slot, then d(d)/dx = (x - p)/d and d(d)/dV_k = -b_k (x - p)/d.
+template <int DIM>
+__global__ void k_distance_backward(const float4 *prims, const int32_t *prim_vids, const float *points,
+                                    const int32_t *slots, const float *grad_d, int64_t n, int n_prims,
+                                    float *grad_points, float *grad_vertices)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t s = slots[i];
+    if (s >= n_prims) s = -1; // not a slot of this scene
+    float g[DIM];
+    for (int k = 0; k < DIM; k++) g[k] = 0.0f;
+    float gd = grad_d[i];
+    if (s >= 0) {
+        float dist, w[3];
+        float diff[3] = {0.0f, 0.0f, 0.0f};
+        if (DIM == 3) {
+            const float4 *t = prims + (size_t)s * 3;
+            float4 a = t[0], b = t[1], c = t[2];
+            v3 x = {points[i * 3], points[i * 3 + 1], points[i * 3 + 2]}, p;
+            float t0, t1;
+            dist = closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, t0, t1);
+            w[0] = t0; w[1] = t1; w[2] = 1.0f - t0 - t1;
+            diff[0] = x.x - p.x; diff[1] = x.y - p.y; diff[2] = x.z - p.z;
+        } else {
+            float4 q = prims[s];
+            v2 x = {points[i * 2], points[i * 2 + 1]}, p;
+            float t;
+            dist = closest_point_segment({q.x, q.y}, {q.z, q.w}, x, p, t);
+            w[0] = 1.0f - t; w[1] = t;
+            diff[0] = x.x - p.x; diff[1] = x.y - p.y;
+        }
+        if (dist > 0.0f) {
+            float sc = gd / dist;
+            for (int k = 0; k < DIM; k++) g[k] = sc * diff[k];
+            if (grad_vertices) {
+                for (int v = 0; v < DIM; v++) {
+                    int32_t vid = prim_vids[(size_t)s * DIM + v];
+                    for (int k = 0; k < DIM; k++) atomicAdd(&grad_vertices[(size_t)vid * DIM + k], -w[v] * g[k]);
+                }
+            }
+        }
+    }
+    if (grad_points)
+        for (int k = 0; k < DIM; k++) grad_points[i * DIM + k] = g[k];
+}
+
+} // namespace fcpw_hip
+
+// ===========================================================================
+// host side
+// ===========================================================================
+using namespace fcpw_hip;
+
+static float bits_as_float(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, sizeof(f));
+    return f;
+}
+
+struct fcpw_hip_ctx {
+    int device = 0;
+    int dim = 0;
+    bool uploaded = false;
+    std::string err;
+    // device buffers
+    float4 *records = nullptr, *root = nullptr, *prims = nullptr;
+    int32_t *prim_index = nullptr, *prim_vids = nullptr;
+    float *V = nullptr;
+    int32_t *rec_parent = nullptr, *leaf_refs = nullptr;
+    uint32_t *leaf_links = nullptr;
+    unsigned int *counters = nullptr;
+    int *flags = nullptr;
+    // staging for the host-pointer entry points
+    void *stage_in = nullptr, *stage_out = nullptr;
+    size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    // facts
+    int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
+    // options
+    int block = 128, stack = 16, sort = 0, persistent = 0;
+    int n_cus = 256;
+};
+
+static std::string g_create_error;
+
+#define FAIL(ctx, code, ...)                                                                                   \
+    do {                                                                                                       \
+        char _b[512];                                                                                          \
+        snprintf(_b, sizeof(_b), __VA_ARGS__);                                                                 \
+        (ctx)->err = _b;                                                                                       \
+        return (code);                                                                                         \
+    } while (0)
+
+#define HIPCHECK(ctx, expr)                                                                                    \
+    do {                                                                                                       \
+        hipError_t _e = (expr);                                                                                \
+        if (_e != hipSuccess) {                                                                                \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                                   \
+            return _e == hipErrorOutOfMemory ? FCPW_HIP_ERR_OUT_OF_MEMORY : FCPW_HIP_ERR_HIP;                 \
+        }                                                                                                      \
+    } while (0)
+
+static void free_scene(fcpw_hip_ctx *c)
+{
+    void *bufs[] = {c->records, c->root, c->prims, c->prim_index, c->prim_vids, c->V,
+                    c->rec_parent, c->leaf_refs, c->leaf_links, c->counters};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->records = c->root = c->prims = nullptr;
+    c->prim_index = c->prim_vids = nullptr;
+    c->V = nullptr;
+    c->rec_parent = c->leaf_refs = nullptr;
+    c->leaf_links = nullptr;
+    c->counters = nullptr;
+    c->uploaded = false;
+}
+
+static DevScene dev_scene(fcpw_hip_ctx *c)
+{
+    return DevScene{c->records, c->root, c->prims, c->prim_index, c->flags};
+}
+
+static int grow(fcpw_hip_ctx *c, void **buf, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return FCPW_HIP_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    HIPCHECK(c, hipMalloc(buf, bytes));
+    *cap = bytes;
+    return FCPW_HIP_OK;
+}
+
+extern "C" {
+
+int fcpw_hip_create(int device, fcpw_hip_ctx **out)
+{
+    if (!out) {
+        g_create_error = "fcpw_hip_create: out is NULL";
+        return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        g_create_error = std::string("fcpw_hip_create: no HIP device (") + hipGetErrorString(e) + ")";
+        return FCPW_HIP_ERR_HIP;
+    }
+    if (device < 0 || device >= count) {
+        g_create_error = "fcpw_hip_create: device index out of range";
+        return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    }
+    fcpw_hip_ctx *c = new fcpw_hip_ctx();
+    c->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cus = prop.multiProcessorCount;
+        e = hipMalloc((void **)&c->flags, sizeof(int));
+    }
+    if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(int));
+    if (e != hipSuccess) {
+        g_create_error = std::string("fcpw_hip_create: ") + hipGetErrorString(e);
+        if (c->flags) (void)hipFree(c->flags);
+        delete c;
+        return FCPW_HIP_ERR_HIP;
+    }
+    *out = c;
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_destroy(fcpw_hip_ctx *c)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    free_scene(c);
+    if (c->flags) (void)hipFree(c->flags);
+    if (c->stage_in) (void)hipFree(c->stage_in);
+    if (c->stage_out) (void)hipFree(c->stage_out);
+    delete c;
+    return FCPW_HIP_OK;
+}
+
+const char *fcpw_hip_last_error(const fcpw_hip_ctx *c)
+{
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+static uint32_t make_link(bool leaf, int32_t offset, int32_t nrefs, const std::vector<int32_t> &rec_of, int node)
+{
+    if (!leaf) return (uint32_t)rec_of[node];
+    return LEAF_BIT | ((uint32_t)(nrefs - 1) << CNT_SHIFT) | (uint32_t)offset;
+}
+
+int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin, const float *pmax,
+                        const int32_t *nrefs, const int32_t *offset, int n_prims, const int32_t *prim_vertices,
+                        const int32_t *prim_index, int n_vertices, const float *positions)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    if (dim != 2 && dim != 3) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: dim must be 2 or 3, got %d", dim);
+    if (n_nodes <= 0 || n_prims <= 0 || n_vertices <= 0 || !pmin || !pmax || !nrefs || !offset || !prim_vertices ||
+        !prim_index || !positions)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: empty BVH or NULL array");
+    if ((size_t)n_prims > (size_t)OFF_MASK)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: %d primitives exceed the 2^27 leaf-slot range", n_prims);
+
+    // validate the reference layout and number internal nodes in DFS order
+    std::vector<int32_t> rec_of(n_nodes, -1);
+    int n_rec = 0, n_leaf = 0;
+    for (int i = 0; i < n_nodes; i++) {
+        if (nrefs[i] > 0) {
+            if (nrefs[i] > MAX_LEAF_PRIMS)
+                FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: leaf %d holds %d primitives (at most %d)", i, nrefs[i],
+                     MAX_LEAF_PRIMS);
+            if (offset[i] < 0 || (int64_t)offset[i] + nrefs[i] > n_prims)
+                FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: leaf %d primitive range out of bounds", i);
+            n_leaf++;
+        } else if (nrefs[i] == 0) {
+            if (offset[i] <= 1 || (int64_t)i + offset[i] >= n_nodes)
+                FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: internal node %d has a bad child offset %d", i,
+                     offset[i]);
+            rec_of[i] = n_rec++;
+        } else {
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: node %d has a negative primitive count", i);
+        }
+    }
+    if (n_leaf != n_rec + 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: not a binary tree");
+    for (int64_t k = 0; k < (int64_t)n_prims * dim; k++)
+        if (prim_vertices[k] < 0 || prim_vertices[k] >= n_vertices)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: vertex id out of range");
+
+    const int REC = dim == 3 ? 4 : 3;
+    std::vector<float4> rec((size_t)std::max(n_rec, 1) * REC, make_float4(0, 0, 0, 0));
+    std::vector<int32_t> parent((size_t)std::max(n_rec, 1), -1);
+    std::vector<int32_t> leaf_refs;
+    std::vector<uint32_t> leaf_links;
+    leaf_refs.reserve(n_leaf);
+    leaf_links.reserve(n_leaf);
+    std::vector<int32_t> depth(n_nodes, 0);
+    int max_depth = 0;
+
+    auto put_slot = [&](int r, int slot, int node, uint32_t link) {
+        const float *lo = pmin + (size_t)node * 3, *hi = pmax + (size_t)node * 3;
+        if (dim == 3) {
+            rec[(size_t)r * 4 + slot * 2] = make_float4(lo[0], lo[1], lo[2], bits_as_float(link));
+            rec[(size_t)r * 4 + slot * 2 + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        } else {
+            rec[(size_t)r * 3 + slot] = make_float4(lo[0], lo[1], hi[0], hi[1]);
+            if (slot == 0) rec[(size_t)r * 3 + 2].x = bits_as_float(link);
+            else rec[(size_t)r * 3 + 2].y = bits_as_float(link);
+        }
+    };
+    for (int i = 0; i < n_nodes; i++) {
+        if (nrefs[i] > 0) continue;
+        int r = rec_of[i];
+        int kids[2] = {i + 1, i + offset[i]};
+        for (int s = 0; s < 2; s++) {
+            int k = kids[s];
+            depth[k] = depth[i] + 1;
+            max_depth = std::max(max_depth, depth[k]);
+            bool leaf = nrefs[k] > 0;
+            uint32_t link = make_link(leaf, offset[k], nrefs[k], rec_of, k);
+            put_slot(r, s, k, link);
+            if (leaf) {
+                leaf_refs.push_back(r * 2 + s);
+                leaf_links.push_back(link);
+            } else {
+                parent[rec_of[k]] = r * 2 + s;
+            }
+        }
+    }
+    if (max_depth > STACK_TOTAL - 1)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: tree depth %d exceeds %d", max_depth, STACK_TOTAL - 1);
+
+    float4 root[2];
+    {
+        bool leaf = nrefs[0] > 0;
+        uint32_t link = make_link(leaf, offset[0], nrefs[0], rec_of, 0);
+        if (leaf) {
+            leaf_refs.push_back(-1);
+            leaf_links.push_back(link);
+        }
+        if (dim == 3) {
+            root[0] = make_float4(pmin[0], pmin[1], pmin[2], bits_as_float(link));
+            root[1] = make_float4(pmax[0], pmax[1], pmax[2], 0.0f);
+        } else {
+            root[0] = make_float4(pmin[0], pmin[1], pmax[0], pmax[1]);
+            root[1] = make_float4(bits_as_float(link), 0.0f, 0.0f, 0.0f);
+        }
+    }
+    if ((int)leaf_refs.size() != n_leaf) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: inconsistent leaves");
+
+    // primitive records in leaf-slot order
+    const int PR = dim == 3 ? 3 : 1;
+    std::vector<float4> prims((size_t)n_prims * PR);
+    for (int s = 0; s < n_prims; s++) {
+        if (dim == 3) {
+            for (int k = 0; k < 3; k++) {
+                const float *v = positions + (size_t)prim_vertices[(size_t)s * 3 + k] * 3;
+                prims[(size_t)s * 3 + k] = make_float4(v[0], v[1], v[2], 0.0f);
+            }
+        } else {
+            const float *a = positions + (size_t)prim_vertices[(size_t)s * 2] * 2;
+            const float *b = positions + (size_t)prim_vertices[(size_t)s * 2 + 1] * 2;
+            prims[s] = make_float4(a[0], a[1], b[0], b[1]);
+        }
+    }
+
+    HIPCHECK(c, hipSetDevice(c->device));
+    free_scene(c);
+    size_t rec_bytes = rec.size() * sizeof(float4);
+    HIPCHECK(c, hipMalloc((void **)&c->records, rec_bytes));
+    HIPCHECK(c, hipMalloc((void **)&c->root, 2 * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->prims, prims.size() * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->prim_index, (size_t)n_prims * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->prim_vids, (size_t)n_prims * dim * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->V, (size_t)n_vertices * dim * sizeof(float)));
+    HIPCHECK(c, hipMalloc((void **)&c->rec_parent, parent.size() * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->leaf_refs, leaf_refs.size() * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->leaf_links, leaf_links.size() * sizeof(uint32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->counters, parent.size() * sizeof(unsigned int)));
+    HIPCHECK(c, hipMemcpy(c->records, rec.data(), rec_bytes, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->root, root, sizeof(root), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->prims, prims.data(), prims.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->prim_index, prim_index, (size_t)n_prims * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->prim_vids, prim_vertices, (size_t)n_prims * dim * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->V, positions, (size_t)n_vertices * dim * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->rec_parent, parent.data(), parent.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->leaf_refs, leaf_refs.data(), leaf_refs.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->leaf_links, leaf_links.data(), leaf_links.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(c->counters, 0, parent.size() * sizeof(unsigned int)));
+    HIPCHECK(c, hipDeviceSynchronize());
+    c->dim = dim;
+    c->n_records = n_rec;
+    c->n_leaves = n_leaf;
+    c->n_prims = n_prims;
+    c->n_vertices = n_vertices;
+    c->max_depth = max_depth;
+    c->uploaded = true;
+    return FCPW_HIP_OK;
+}
+
+static int check_ready(fcpw_hip_ctx *c, const char *fn)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    if (!c->uploaded) FAIL(c, FCPW_HIP_ERR_NOT_UPLOADED, "%s: no BVH has been uploaded", fn);
+    return FCPW_HIP_OK;
+}
+
+static int launch_rebuild(fcpw_hip_ctx *c, hipStream_t s)
+{
+    int threads = 256, blocks = (c->n_prims + threads - 1) / threads;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_rebuild_prims<3>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, c->V,
+                           c->n_prims);
+    else
+        hipLaunchKernelGGL(k_rebuild_prims<2>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, c->V,
+                           c->n_prims);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_update_positions(fcpw_hip_ctx *c, const float *positions, int n_vertices)
+{
+    int st = check_ready(c, "update_positions");
+    if (st) return st;
+    if (!positions || n_vertices != c->n_vertices)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "update_positions: expected %d vertices, got %d", c->n_vertices,
+             n_vertices);
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipMemcpy(c->V, positions, (size_t)n_vertices * c->dim * sizeof(float), hipMemcpyHostToDevice));
+    st = launch_rebuild(c, 0);
+    if (st) return st;
+    HIPCHECK(c, hipDeviceSynchronize());
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_update_positions_device(fcpw_hip_ctx *c, const float *positions, int n_vertices, void *stream)
+{
+    int st = check_ready(c, "update_positions_device");
+    if (st) return st;
+    if (!positions || n_vertices != c->n_vertices)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "update_positions_device: expected %d vertices, got %d",
+             c->n_vertices, n_vertices);
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHECK(c, hipMemcpyAsync(c->V, positions, (size_t)n_vertices * c->dim * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+    return launch_rebuild(c, s);
+}
+
+int fcpw_hip_refit(fcpw_hip_ctx *c, void *stream)
+{
+    int st = check_ready(c, "refit");
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    int threads = 256, blocks = (c->n_leaves + threads - 1) / threads;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_refit<3>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->prims,
+                           c->leaf_refs, c->leaf_links, c->rec_parent, c->counters, c->n_leaves);
+    else
+        hipLaunchKernelGGL(k_refit<2>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->prims,
+                           c->leaf_refs, c->leaf_links, c->rec_parent, c->counters, c->n_leaves);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers: (block, stack) pairs are compiled in; the grid is capped and
+// the kernels stride over the remaining queries
+// ---------------------------------------------------------------------------
+static dim3 grid_for(fcpw_hip_ctx *c, int64_t n, int block)
+{
+    int64_t blocks = (n + block - 1) / block;
+    int64_t cap = (int64_t)c->n_cus * 64;
+    return dim3((unsigned)std::max<int64_t>(1, std::min(blocks, cap)));
+}
+
+#define FCPW_DISPATCH(c, KERNEL_CALL)                                                                          \
+    do {                                                                                                       \
+        const int _b = (c)->block, _k = (c)->stack;                                                           \
+        if (_b == 64 && _k == 8) { constexpr int B = 64, K = 8; KERNEL_CALL; }                               \
+        else if (_b == 64 && _k == 16) { constexpr int B = 64, K = 16; KERNEL_CALL; }                        \
+        else if (_b == 64 && _k == 24) { constexpr int B = 64, K = 24; KERNEL_CALL; }                        \
+        else if (_b == 128 && _k == 8) { constexpr int B = 128, K = 8; KERNEL_CALL; }                        \
+        else if (_b == 128 && _k == 16) { constexpr int B = 128, K = 16; KERNEL_CALL; }                      \
+        else if (_b == 128 && _k == 24) { constexpr int B = 128, K = 24; KERNEL_CALL; }                      \
+        else if (_b == 256 && _k == 8) { constexpr int B = 256, K = 8; KERNEL_CALL; }                        \
+        else if (_b == 256 && _k == 16) { constexpr int B = 256, K = 16; KERNEL_CALL; }                      \
+        else { constexpr int B = 256, K = 24; KERNEL_CALL; }                                                 \
+    } while (0)
+
+static int launch_closest(fcpw_hip_ctx *c, const void *spheres, int64_t n, int record_normals, void *out,
+                          hipStream_t s)
+{
+    DevScene S = dev_scene(c);
+    if (c->dim == 3)
+        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points<3, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+                                            (const float4 *)spheres, n, record_normals, (float2 *)out));
+    else
+        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points<2, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+                                            (const float4 *)spheres, n, record_normals, (float2 *)out));
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int occlusion, void *out, hipStream_t s)
+{
+    DevScene S = dev_scene(c);
+    if (c->dim == 3)
+        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_intersect<3, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+                                            (const float2 *)rays, n, occlusion, (float2 *)out));
+    else
+        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_intersect<2, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+                                            (const float2 *)rays, n, occlusion, (float2 *)out));
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+static int check_overflow(fcpw_hip_ctx *c)
+{
+    int flag = 0;
+    HIPCHECK(c, hipMemcpy(&flag, c->flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        HIPCHECK(c, hipMemset(c->flags, 0, sizeof(int)));
+        FAIL(c, FCPW_HIP_ERR_TRAVERSAL_OVERFLOW, "traversal stack exceeded %d entries", STACK_TOTAL);
+    }
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_closest_points_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                   int record_normals, fcpw_hip_interaction *out, void *stream)
+{
+    int st = check_ready(c, "closest_points_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: bad arguments");
+    if (((uintptr_t)spheres & 15) || ((uintptr_t)out & 7))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: spheres must be 16-byte and out 8-byte aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    return launch_closest(c, spheres, n, record_normals, out, (hipStream_t)stream);
+}
+
+int fcpw_hip_closest_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n, int record_normals,
+                            fcpw_hip_interaction *out)
+{
+    int st = check_ready(c, "closest_points");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, (size_t)n * sizeof(fcpw_hip_bounding_sphere));
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), hipMemcpyHostToDevice));
+    st = launch_closest(c, c->stage_in, n, record_normals, c->stage_out, 0);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    return check_overflow(c);
+}
+
+int fcpw_hip_intersect_device(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int occlusion,
+                              fcpw_hip_interaction *out, void *stream)
+{
+    int st = check_ready(c, "intersect_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!rays || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_device: bad arguments");
+    if (((uintptr_t)rays & 7) || ((uintptr_t)out & 7))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_device: rays and out must be 8-byte aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    return launch_intersect(c, rays, n, occlusion, out, (hipStream_t)stream);
+}
+
+int fcpw_hip_intersect(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int occlusion, fcpw_hip_interaction *out)
+{
+    int st = check_ready(c, "intersect");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!rays || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, (size_t)n * sizeof(fcpw_hip_ray));
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(c->stage_in, rays, (size_t)n * sizeof(fcpw_hip_ray), hipMemcpyHostToDevice));
+    st = launch_intersect(c, c->stage_in, n, occlusion, c->stage_out, 0);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    return check_overflow(c);
+}
+
+int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *, const fcpw_hip_float3 *, int64_t,
+                               fcpw_hip_interaction *)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "sphere intersection queries are not built by this backend");
+}
+
+int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *, const uint32_t *, int64_t,
+                                       float, float, fcpw_hip_interaction *)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "closest silhouette point queries are not built by this backend");
+}
+
+int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *c, const float *points, const float *r2, int64_t n,
+                                           float *d, float *p, int32_t *prim, int32_t *slot, void *stream)
+{
+    int st = check_ready(c, "closest_points_compact_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && !points)) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_compact_device: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    DevScene S = dev_scene(c);
+    hipStream_t s = (hipStream_t)stream;
+    if (c->dim == 3)
+        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points_compact<3, B, K>), grid_for(c, n, B), dim3(B), 0, s,
+                                            S, points, r2, n, d, p, prim, slot));
+    else
+        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points_compact<2, B, K>), grid_for(c, n, B), dim3(B), 0, s,
+                                            S, points, r2, n, d, p, prim, slot));
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_distance_backward_device(fcpw_hip_ctx *c, const float *points, const int32_t *slot, const float *grad_d,
+                                      int64_t n, float *grad_points, float *grad_vertices, void *stream)
+{
+    int st = check_ready(c, "distance_backward_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!points || !slot || !grad_d)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "distance_backward_device: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int threads = 256;
+    unsigned blocks = (unsigned)((n + threads - 1) / threads);
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_distance_backward<3>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, points,
+                           slot, grad_d, n, c->n_prims, grad_points, grad_vertices);
+    else
+        hipLaunchKernelGGL(k_distance_backward<2>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, points,
+                           slot, grad_d, n, c->n_prims, grad_points, grad_vertices);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
+{
+    if (!c || !key) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    std::string k(key);
+    if (k == "block") {
+        if (value != 64 && value != 128 && value != 256) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
+        c->block = value;
+    } else if (k == "stack") {
+        if (value != 8 && value != 16 && value != 24) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 8, 16 or 24");
+        c->stack = value;
+    } else if (k == "sort") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sort must be 0 or 1");
+        c->sort = value;
+    } else if (k == "persistent") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "persistent must be 0 or 1");
+        c->persistent = value;
+    } else {
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "unknown option '%s'", key);
+    }
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
+{
+    if (!c || !key || !value) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    std::string k(key);
+    if (k == "block") *value = c->block;
+    else if (k == "stack") *value = c->stack;
+    else if (k == "sort") *value = c->sort;
+    else if (k == "persistent") *value = c->persistent;
+    else return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_get_info(const fcpw_hip_ctx *c, const char *key, int64_t *value)
+{
+    if (!c || !key || !value) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    std::string k(key);
+    if (k == "n_records") *value = c->n_records;
+    else if (k == "n_leaves") *value = c->n_leaves;
+    else if (k == "n_prims") *value = c->n_prims;
+    else if (k == "n_vertices") *value = c->n_vertices;
+    else if (k == "dim") *value = c->dim;
+    else if (k == "max_depth") *value = c->max_depth;
+    else if (k == "n_cus") *value = c->n_cus;
+    else if (k == "uploaded") *value = c->uploaded ? 1 : 0;
+    else return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    return FCPW_HIP_OK;
+}
+
+} // extern "C"

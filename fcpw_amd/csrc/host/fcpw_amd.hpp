@@ -1,0 +1,407 @@
+// fcpw_amd.hpp -- C++ host API of the MI355X backend: the subset of FCPW's
+// Scene<DIM> (include/fcpw/fcpw.h:13-255) that builds what the GPU consumes,
+// and GPUScene<DIM> (include/fcpw/fcpw_gpu.h:9-93) implemented over the C ABI
+// of libfcpw_hip.so (include/fcpw_hip.h).
+//
+// Differences from the reference, all at the API edge:
+//  - Eigen matrices are replaced by row-major float/int arrays with a row count;
+//    the std::vector overloads keep the reference record types.
+//  - Errors throw std::runtime_error / std::invalid_argument instead of
+//    printing and calling exit(EXIT_FAILURE).
+//  - fcpwDirectoryPath is kept for signature compatibility and ignored: no
+//    shader source is compiled at run time.
+#pragma once
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/fcpw_hip.h"
+#include "bvh_build.hpp"
+
+namespace fcpw_amd {
+
+// scene_data.h:17-24
+enum class AggregateType {
+    Baseline = 0,
+    Bvh_LongestAxisCenter = 1,
+    Bvh_OverlapSurfaceArea = 2,
+    Bvh_SurfaceArea = 3,
+    Bvh_OverlapVolume = 4,
+    Bvh_Volume = 5
+};
+// fcpw.h:7-10
+enum class PrimitiveType { LineSegment, Triangle };
+
+// GPU records with the reference layouts (bvh_interop_structures.h:14-28,730-774)
+struct float2 {
+    float x, y;
+} __attribute__((aligned(8)));
+struct float3 {
+    float x, y, z;
+};
+struct GPURay {
+    GPURay() : o{0, 0, 0}, d{0, 0, 0}, dInv{0, 0, 0}, tMax(std::numeric_limits<float>::max()) {}
+    GPURay(const float3 &o_, const float3 &d_, float tMax_ = std::numeric_limits<float>::max())
+        : o(o_), d(d_), dInv{1.0f / d_.x, 1.0f / d_.y, 1.0f / d_.z}, tMax(tMax_) {}
+    float3 o, d, dInv;
+    float tMax;
+};
+struct GPUBoundingSphere {
+    GPUBoundingSphere() : c{0, 0, 0}, r2(0.0f) {}
+    GPUBoundingSphere(const float3 &c_, float r2_) : c(c_), r2(r2_) {}
+    float3 c;
+    float r2;
+};
+struct GPUInteraction {
+    GPUInteraction() : p{0, 0, 0}, n{0, 0, 0}, uv{0, 0}, d(std::numeric_limits<float>::max()), index(FCPW_HIP_UINT_MAX) {}
+    float3 p, n;
+    float2 uv;
+    float d;
+    uint32_t index;
+};
+static_assert(sizeof(GPURay) == sizeof(fcpw_hip_ray), "GPURay layout");
+static_assert(sizeof(GPUBoundingSphere) == sizeof(fcpw_hip_bounding_sphere), "GPUBoundingSphere layout");
+static_assert(sizeof(GPUInteraction) == sizeof(fcpw_hip_interaction) && sizeof(GPUInteraction) == 40,
+              "GPUInteraction layout");
+
+inline CostHeuristic heuristic_of(AggregateType t)
+{
+    switch (t) {
+    case AggregateType::Bvh_LongestAxisCenter: return CostHeuristic::LongestAxisCenter;
+    case AggregateType::Bvh_OverlapSurfaceArea: return CostHeuristic::OverlapSurfaceArea;
+    case AggregateType::Bvh_SurfaceArea: return CostHeuristic::SurfaceArea;
+    case AggregateType::Bvh_OverlapVolume: return CostHeuristic::OverlapVolume;
+    case AggregateType::Bvh_Volume: return CostHeuristic::Volume;
+    default: throw std::invalid_argument("AggregateType::Baseline has no BVH for the GPU path");
+    }
+}
+
+// static partition over hardware threads, as the reference's batched helpers
+// (fcpw.inl:1547-1559)
+template <typename F>
+inline void parallel_rows(int64_t n, F &&f)
+{
+    int nt = (int)std::thread::hardware_concurrency();
+    if (nt < 1 || n < 65536) { f((int64_t)0, n); return; }
+    std::vector<std::thread> th;
+    int64_t per = n / nt;
+    for (int i = 0; i < nt; i++) {
+        int64_t s = i * per, e = (i == nt - 1) ? n : (i + 1) * per;
+        th.emplace_back([&f, s, e]() { f(s, e); });
+    }
+    for (auto &t : th) t.join();
+}
+
+// ---------------------------------------------------------------------------
+// Scene<DIM>: geometry ingest, BVH build, vertex updates, CPU refit
+// ---------------------------------------------------------------------------
+template <int DIM>
+class Scene {
+public:
+    Scene() = default;
+
+    // fcpw.inl:206-216
+    void setObjectCount(int nObjects)
+    {
+        if (nObjects < 0) throw std::invalid_argument("setObjectCount: negative count");
+        objects_.assign((size_t)nObjects, Object());
+        bvh_.reset();
+    }
+    void setObjectVertexCount(int nVertices, int objectIndex)
+    {
+        obj(objectIndex).positions.assign((size_t)nVertices * DIM, 0.0f);
+    }
+    void setObjectVertex(const float *position, int vertexIndex, int objectIndex)
+    {
+        Object &o = obj(objectIndex);
+        check_vertex(o, vertexIndex);
+        for (int d = 0; d < DIM; d++) o.positions[(size_t)vertexIndex * DIM + d] = position[d];
+    }
+    // fcpw.inl:218-238 (row-major nVertices x DIM)
+    void setObjectVertices(const float *positions, int nVertices, int objectIndex)
+    {
+        Object &o = obj(objectIndex);
+        o.positions.assign(positions, positions + (size_t)nVertices * DIM);
+    }
+    void setObjectPrimitiveCount(int nPrims, int objectIndex)
+    {
+        Object &o = obj(objectIndex);
+        if (o.has_prims) throw std::runtime_error(DIM == 3 ? "Already set triangles!" : "Already set line segments!");
+        o.indices.assign((size_t)nPrims * DIM, -1);
+        o.has_prims = true;
+    }
+    void setObjectPrimitive(const int32_t *indices, int primIndex, int objectIndex)
+    {
+        Object &o = obj(objectIndex);
+        if (primIndex < 0 || (size_t)primIndex * DIM >= o.indices.size())
+            throw std::out_of_range("primitive index out of range");
+        for (int k = 0; k < DIM; k++) o.indices[(size_t)primIndex * DIM + k] = indices[k];
+    }
+    // fcpw.inl:240-390 (row-major nPrims x DIM); triangles for DIM 3, line segments for DIM 2
+    void setObjectPrimitives(const int32_t *indices, int nPrims, int objectIndex)
+    {
+        setObjectPrimitiveCount(nPrims, objectIndex);
+        Object &o = obj(objectIndex);
+        o.indices.assign(indices, indices + (size_t)nPrims * DIM);
+    }
+
+    // Scene::build (fcpw.inl:1052-1126) for a single object with a binary BVH.
+    // vectorize is accepted and ignored: the GPU path needs the binary BVH
+    // (fcpw_gpu.h:17-20), and no vectorized CPU traversal exists here.
+    void build(AggregateType aggregateType, bool vectorize, bool printStats = false, bool reduceMemoryFootprint = false)
+    {
+        (void)vectorize;
+        (void)reduceMemoryFootprint;
+        if (objects_.size() != 1)
+            throw std::runtime_error("the MI355X backend supports scenes with exactly one object (fcpw_gpu.h:17-20)");
+        Object &o = objects_[0];
+        if (!o.has_prims || o.indices.empty()) throw std::runtime_error("Cannot construct a BVH without primitives");
+        aggregate_ = aggregateType;
+        auto t0 = std::chrono::high_resolution_clock::now();
+        bvh_ = std::make_unique<FlatBvh<DIM>>(build_bvh<DIM>(o.positions.data(), nVertices(), o.indices.data(),
+                                                             nPrimitives(), heuristic_of(aggregateType)));
+        if (printStats) {
+            double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+            printf("BVH construction time: %g seconds\n", s);
+            printf("BVH stats: %zu nodes, %d leaves, %d max depth, %d primitives\n", bvh_->nodes.size(),
+                   bvh_->n_leaves, bvh_->max_depth, nPrimitives());
+        }
+        ++version_;
+    }
+
+    // fcpw.inl:1128-1143 (vertex ids in the caller's order)
+    void updateObjectVertex(const float *position, int vertexIndex, int objectIndex)
+    {
+        setObjectVertex(position, vertexIndex, objectIndex);
+    }
+    void updateObjectVertices(const float *positions, int nVertices, int objectIndex)
+    {
+        Object &o = obj(objectIndex);
+        if ((size_t)nVertices * DIM != o.positions.size())
+            throw std::invalid_argument("updateObjectVertices: vertex count differs from the scene's");
+        o.positions.assign(positions, positions + (size_t)nVertices * DIM);
+    }
+    // Scene::refit (fcpw.inl:1145-1158) on the host copy of the BVH
+    void refit(bool printStats = false)
+    {
+        require_bvh();
+        auto t0 = std::chrono::high_resolution_clock::now();
+        refit_bvh<DIM>(*bvh_, objects_[0].positions.data(), objects_[0].indices.data());
+        if (printStats) {
+            double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+            printf("refit time: %g seconds\n", s);
+        }
+    }
+
+    // accessors used by GPUScene
+    bool built() const { return (bool)bvh_; }
+    const FlatBvh<DIM> &bvh() const { require_bvh(); return *bvh_; }
+    const std::vector<float> &positions() const { return objects_.at(0).positions; }
+    const std::vector<int32_t> &indices() const { return objects_.at(0).indices; }
+    int nVertices() const { return (int)(objects_.at(0).positions.size() / DIM); }
+    int nPrimitives() const { return (int)(objects_.at(0).indices.size() / DIM); }
+    uint64_t version() const { return version_; }
+
+private:
+    struct Object {
+        std::vector<float> positions;
+        std::vector<int32_t> indices;
+        bool has_prims = false;
+    };
+    Object &obj(int i)
+    {
+        if (i < 0 || (size_t)i >= objects_.size()) throw std::out_of_range("object index out of range");
+        return objects_[i];
+    }
+    static void check_vertex(const Object &o, int v)
+    {
+        if (v < 0 || (size_t)v * DIM >= o.positions.size()) throw std::out_of_range("vertex index out of range");
+    }
+    void require_bvh() const
+    {
+        if (!bvh_) throw std::runtime_error("the scene has not been built");
+    }
+    std::vector<Object> objects_;
+    std::unique_ptr<FlatBvh<DIM>> bvh_;
+    AggregateType aggregate_ = AggregateType::Bvh_SurfaceArea;
+    uint64_t version_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// GPUScene<DIM> over the C ABI
+// ---------------------------------------------------------------------------
+inline void hip_check(fcpw_hip_ctx *ctx, int status, const char *what)
+{
+    if (status == FCPW_HIP_OK) return;
+    std::string msg = std::string(what) + ": " + fcpw_hip_last_error(ctx);
+    if (status == FCPW_HIP_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+template <int DIM>
+class GPUScene {
+public:
+    // fcpw_gpu.inl:10-16; the directory argument is ignored
+    GPUScene(const std::string &fcpwDirectoryPath, bool printLogs = false) : dir_(fcpwDirectoryPath), logs_(printLogs)
+    {
+        const char *dev = std::getenv("FCPW_HIP_DEVICE");
+        device_ = dev ? std::atoi(dev) : 0;
+    }
+    ~GPUScene()
+    {
+        if (ctx_) fcpw_hip_destroy(ctx_);
+    }
+    GPUScene(const GPUScene &) = delete;
+    GPUScene &operator=(const GPUScene &) = delete;
+
+    // fcpw_gpu.inl:18-62
+    void transferToGPU(Scene<DIM> &scene, const std::string &deviceBackend = "default")
+    {
+        (void)deviceBackend; // HIP is the only backend
+        if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
+        const FlatBvh<DIM> &b = scene.bvh();
+        size_t n = b.nodes.size();
+        std::vector<float> pmin(n * 3, 0.0f), pmax(n * 3, 0.0f);
+        std::vector<int32_t> nrefs(n), offset(n);
+        for (size_t i = 0; i < n; i++) {
+            for (int d = 0; d < DIM; d++) {
+                pmin[i * 3 + d] = b.nodes[i].box.lo[d];
+                pmax[i * 3 + d] = b.nodes[i].box.hi[d];
+            }
+            nrefs[i] = b.nodes[i].nrefs;
+            offset[i] = b.nodes[i].offset;
+        }
+        int np = scene.nPrimitives();
+        std::vector<int32_t> pv((size_t)np * DIM);
+        const auto &idx = scene.indices();
+        for (int s = 0; s < np; s++)
+            for (int k = 0; k < DIM; k++) pv[(size_t)s * DIM + k] = idx[(size_t)b.prim_order[s] * DIM + k];
+        hip_check(ctx_,
+                  fcpw_hip_upload_bvh(ctx_, DIM, (int)n, pmin.data(), pmax.data(), nrefs.data(), offset.data(), np,
+                                      pv.data(), b.prim_order.data(), scene.nVertices(), scene.positions().data()),
+                  "transferToGPU");
+        if (logs_) {
+            int64_t rec = 0, depth = 0;
+            fcpw_hip_get_info(ctx_, "n_records", &rec);
+            fcpw_hip_get_info(ctx_, "max_depth", &depth);
+            printf("transferToGPU: %zu nodes -> %lld records, depth %lld, %d primitives\n", n, (long long)rec,
+                   (long long)depth, np);
+        }
+    }
+
+    // fcpw_gpu.inl:64-80: with updateGeometry, upload the scene's current vertex
+    // positions first; then refit every box on the device
+    void refit(Scene<DIM> &scene, bool updateGeometry = true)
+    {
+        require();
+        if (updateGeometry)
+            hip_check(ctx_, fcpw_hip_update_positions(ctx_, scene.positions().data(), scene.nVertices()), "refit");
+        hip_check(ctx_, fcpw_hip_refit(ctx_, nullptr), "refit");
+    }
+
+    // fcpw_gpu.inl:94-135 (row-major n x DIM origins and directions)
+    void intersect(const float *rayOrigins, const float *rayDirections, const float *rayDistanceBounds, int64_t n,
+                   std::vector<GPUInteraction> &interactions, bool checkForOcclusion = false)
+    {
+        std::vector<GPURay> rays((size_t)n);
+        parallel_rows(n, [&](int64_t s, int64_t e) {
+            for (int64_t i = s; i < e; i++) {
+                const float *o = rayOrigins + i * DIM, *d = rayDirections + i * DIM;
+                float3 oo{o[0], o[1], DIM == 2 ? 0.0f : o[DIM - 1]};
+                float3 dd{d[0], d[1], DIM == 2 ? 0.0f : d[DIM - 1]};
+                rays[i] = GPURay(oo, dd, rayDistanceBounds[i]);
+            }
+        });
+        intersect(rays, interactions, checkForOcclusion);
+    }
+    // fcpw_gpu.inl:137-156
+    void intersect(const std::vector<GPURay> &rays, std::vector<GPUInteraction> &interactions,
+                   bool checkForOcclusion = false)
+    {
+        require();
+        interactions.assign(rays.size(), GPUInteraction());
+        hip_check(ctx_,
+                  fcpw_hip_intersect(ctx_, (const fcpw_hip_ray *)rays.data(), (int64_t)rays.size(),
+                                     checkForOcclusion ? 1 : 0, (fcpw_hip_interaction *)interactions.data()),
+                  "intersect");
+    }
+    // fcpw_gpu.inl:158-217: not built by this backend
+    void intersect(const std::vector<GPUBoundingSphere> &spheres, const std::vector<float3> &randNums,
+                   std::vector<GPUInteraction> &interactions)
+    {
+        require();
+        interactions.assign(spheres.size(), GPUInteraction());
+        hip_check(ctx_,
+                  fcpw_hip_intersect_spheres(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
+                                             (const fcpw_hip_float3 *)randNums.data(), (int64_t)spheres.size(),
+                                             (fcpw_hip_interaction *)interactions.data()),
+                  "intersect(spheres)");
+    }
+
+    // fcpw_gpu.inl:219-253 (row-major n x DIM points)
+    void findClosestPoints(const float *queryPoints, const float *squaredMaxRadii, int64_t n,
+                           std::vector<GPUInteraction> &interactions, bool recordNormals = false)
+    {
+        std::vector<GPUBoundingSphere> spheres((size_t)n);
+        parallel_rows(n, [&](int64_t s, int64_t e) {
+            for (int64_t i = s; i < e; i++) {
+                const float *q = queryPoints + i * DIM;
+                spheres[i] = GPUBoundingSphere(float3{q[0], q[1], DIM == 2 ? 0.0f : q[DIM - 1]}, squaredMaxRadii[i]);
+            }
+        });
+        findClosestPoints(spheres, interactions, recordNormals);
+    }
+    // fcpw_gpu.inl:255-274
+    void findClosestPoints(const std::vector<GPUBoundingSphere> &spheres, std::vector<GPUInteraction> &interactions,
+                           bool recordNormals = false)
+    {
+        require();
+        interactions.assign(spheres.size(), GPUInteraction());
+        hip_check(ctx_,
+                  fcpw_hip_closest_points(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
+                                          (int64_t)spheres.size(), recordNormals ? 1 : 0,
+                                          (fcpw_hip_interaction *)interactions.data()),
+                  "findClosestPoints");
+    }
+
+    // fcpw_gpu.inl:276-338: not built by this backend
+    void findClosestSilhouettePoints(const std::vector<GPUBoundingSphere> &spheres,
+                                     const std::vector<uint32_t> &flipNormalOrientation,
+                                     std::vector<GPUInteraction> &interactions, float squaredMinRadius = 0.0f,
+                                     float precision = 1e-3f)
+    {
+        require();
+        interactions.assign(spheres.size(), GPUInteraction());
+        hip_check(ctx_,
+                  fcpw_hip_closest_silhouette_points(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
+                                                     flipNormalOrientation.data(), (int64_t)spheres.size(),
+                                                     squaredMinRadius, precision,
+                                                     (fcpw_hip_interaction *)interactions.data()),
+                  "findClosestSilhouettePoints");
+    }
+
+    // the device context, for callers that pass device pointers directly
+    fcpw_hip_ctx *context()
+    {
+        require();
+        return ctx_;
+    }
+
+private:
+    void require()
+    {
+        if (!ctx_) throw std::runtime_error("the BVH must be transferred to the GPU first (call transferToGPU)");
+    }
+    std::string dir_;
+    bool logs_;
+    int device_ = 0;
+    fcpw_hip_ctx *ctx_ = nullptr;
+};
+
+} // namespace fcpw_amd

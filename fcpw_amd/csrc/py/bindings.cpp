@@ -1,0 +1,362 @@
+// bindings.cpp -- pybind11 module fcpw_amd._fcpw_amd: the Python surface of
+// the reference's GPU section (python/fcpw_bindings.cpp:493-673) and the Scene
+// subset that builds what the GPU consumes (python/fcpw_bindings.cpp:349-491).
+// Overloads that the reference separates by Eigen shapes are exposed under
+// distinct names here; fcpw_amd/__init__.py restores the reference signatures.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+#include <pybind11/stl_bind.h>
+
+#include <cstring>
+
+#include "../host/fcpw_amd.hpp"
+
+namespace py = pybind11;
+using namespace pybind11::literals;
+using namespace fcpw_amd;
+
+PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::float3>);
+PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::GPURay>);
+PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::GPUBoundingSphere>);
+PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::GPUInteraction>);
+
+using F32 = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using I32 = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+
+static const float *rows(const F32 &a, int64_t &n, int dim, const char *name)
+{
+    if (a.ndim() == 2 && a.shape(1) == dim) { n = a.shape(0); return a.data(); }
+    if (a.ndim() == 1 && a.shape(0) == 0) { n = 0; return a.data(); }
+    throw std::invalid_argument(std::string(name) + " must have shape (n, " + std::to_string(dim) + ")");
+}
+static const float *vec(const F32 &a, int64_t n, const char *name)
+{
+    if (a.ndim() != 1 || a.shape(0) != n)
+        throw std::invalid_argument(std::string(name) + " must have shape (" + std::to_string(n) + ",)");
+    return a.data();
+}
+
+template <int DIM>
+static void bind_scene(py::module_ &m, const char *name)
+{
+    using S = Scene<DIM>;
+    auto cls = py::class_<S>(m, name);
+    cls.def(py::init<>())
+        .def("set_object_count", &S::setObjectCount, "n_objects"_a)
+        .def("set_object_vertex_count", &S::setObjectVertexCount, "n_vertices"_a, "object_index"_a)
+        .def(
+            "set_object_vertex",
+            [](S &s, const F32 &p, int v, int o) {
+                if (p.size() != DIM) throw std::invalid_argument("position has the wrong length");
+                s.setObjectVertex(p.data(), v, o);
+            },
+            "position"_a, "vertex_index"_a, "object_index"_a)
+        .def(
+            "set_object_vertices",
+            [](S &s, const F32 &p, int o) {
+                int64_t n;
+                const float *d = rows(p, n, DIM, "positions");
+                s.setObjectVertices(d, (int)n, o);
+            },
+            "positions"_a, "object_index"_a)
+        .def(
+            "update_object_vertex",
+            [](S &s, const F32 &p, int v, int o) {
+                if (p.size() != DIM) throw std::invalid_argument("position has the wrong length");
+                s.updateObjectVertex(p.data(), v, o);
+            },
+            "position"_a, "vertex_index"_a, "object_index"_a)
+        .def(
+            "update_object_vertices",
+            [](S &s, const F32 &p, int o) {
+                int64_t n;
+                const float *d = rows(p, n, DIM, "positions");
+                s.updateObjectVertices(d, (int)n, o);
+            },
+            "positions"_a, "object_index"_a)
+        .def("build", &S::build, "aggregate_type"_a, "vectorize"_a, "print_stats"_a = false,
+             "reduce_memory_footprint"_a = false)
+        .def("refit", &S::refit, "print_stats"_a = false)
+        .def_property_readonly("n_vertices", &S::nVertices)
+        .def_property_readonly("n_primitives", &S::nPrimitives)
+        .def(
+            "_bvh_nodes",
+            [](const S &s) {
+                const auto &b = s.bvh();
+                size_t n = b.nodes.size();
+                py::array_t<float> lo({(py::ssize_t)n, (py::ssize_t)3}), hi({(py::ssize_t)n, (py::ssize_t)3});
+                py::array_t<int32_t> nrefs((py::ssize_t)n), off((py::ssize_t)n);
+                auto L = lo.mutable_unchecked<2>(), H = hi.mutable_unchecked<2>();
+                auto R = nrefs.mutable_unchecked<1>(), O = off.mutable_unchecked<1>();
+                for (size_t i = 0; i < n; i++) {
+                    for (int d = 0; d < 3; d++) {
+                        L(i, d) = d < DIM ? b.nodes[i].box.lo[d] : 0.0f;
+                        H(i, d) = d < DIM ? b.nodes[i].box.hi[d] : 0.0f;
+                    }
+                    R(i) = b.nodes[i].nrefs;
+                    O(i) = b.nodes[i].offset;
+                }
+                py::array_t<int32_t> order((py::ssize_t)b.prim_order.size());
+                std::memcpy(order.mutable_data(), b.prim_order.data(), b.prim_order.size() * sizeof(int32_t));
+                return py::make_tuple(lo, hi, nrefs, off, order, b.max_depth, b.n_leaves);
+            },
+            "flat BVH as (pmin, pmax, nrefs, offset, prim_order, max_depth, n_leaves)");
+    if (DIM == 3) {
+        cls.def(
+               "set_object_triangles",
+               [](S &s, const I32 &idx, int o) {
+                   if (idx.ndim() != 2 || idx.shape(1) != 3) throw std::invalid_argument("indices must be (n, 3)");
+                   s.setObjectPrimitives(idx.data(), (int)idx.shape(0), o);
+               },
+               "indices"_a, "object_index"_a)
+            .def("set_object_triangle_count", &S::setObjectPrimitiveCount, "n_triangles"_a, "object_index"_a)
+            .def(
+                "set_object_triangle",
+                [](S &s, const I32 &idx, int t, int o) {
+                    if (idx.size() != 3) throw std::invalid_argument("indices must have 3 entries");
+                    s.setObjectPrimitive(idx.data(), t, o);
+                },
+                "indices"_a, "triangle_index"_a, "object_index"_a);
+    } else {
+        cls.def(
+               "set_object_line_segments",
+               [](S &s, const I32 &idx, int o) {
+                   if (idx.ndim() != 2 || idx.shape(1) != 2) throw std::invalid_argument("indices must be (n, 2)");
+                   s.setObjectPrimitives(idx.data(), (int)idx.shape(0), o);
+               },
+               "indices"_a, "object_index"_a)
+            .def("set_object_line_segment_count", &S::setObjectPrimitiveCount, "n_line_segments"_a, "object_index"_a)
+            .def(
+                "set_object_line_segment",
+                [](S &s, const I32 &idx, int t, int o) {
+                    if (idx.size() != 2) throw std::invalid_argument("indices must have 2 entries");
+                    s.setObjectPrimitive(idx.data(), t, o);
+                },
+                "indices"_a, "line_segment_index"_a, "object_index"_a);
+    }
+}
+
+using InteractionList = std::vector<GPUInteraction>;
+
+template <int DIM>
+static void bind_gpu_scene(py::module_ &m, const char *name)
+{
+    using G = GPUScene<DIM>;
+    py::class_<G>(m, name)
+        .def(py::init<const std::string &, bool>(), "fcpw_directory_path"_a, "print_logs"_a = false)
+        .def("transfer_to_gpu", &G::transferToGPU, "scene"_a, "device_backend"_a = "default")
+        .def("refit", &G::refit, "scene"_a, "update_geometry"_a = true)
+        .def(
+            "_intersect_arrays",
+            [](G &g, const F32 &o, const F32 &d, const F32 &t, InteractionList &out, bool occl) {
+                int64_t n, n2;
+                const float *op = rows(o, n, DIM, "ray_origins");
+                const float *dp = rows(d, n2, DIM, "ray_directions");
+                if (n2 != n) throw std::invalid_argument("ray_directions and ray_origins differ in length");
+                const float *tp = vec(t, n, "ray_distance_bounds");
+                py::gil_scoped_release rel;
+                g.intersect(op, dp, tp, n, out, occl);
+            },
+            "ray_origins"_a, "ray_directions"_a, "ray_distance_bounds"_a, "interactions"_a,
+            "check_for_occlusion"_a = false)
+        .def(
+            "_intersect_rays",
+            [](G &g, const std::vector<GPURay> &rays, InteractionList &out, bool occl) {
+                py::gil_scoped_release rel;
+                g.intersect(rays, out, occl);
+            },
+            "rays"_a, "interactions"_a, "check_for_occlusion"_a = false)
+        .def(
+            "_intersect_spheres",
+            [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<float3> &r, InteractionList &out) {
+                g.intersect(s, r, out);
+            },
+            "bounding_spheres"_a, "rand_nums"_a, "interactions"_a)
+        .def(
+            "_find_closest_points_arrays",
+            [](G &g, const F32 &q, const F32 &r2, InteractionList &out, bool normals) {
+                int64_t n;
+                const float *qp = rows(q, n, DIM, "query_points");
+                const float *rp = vec(r2, n, "squared_max_radii");
+                py::gil_scoped_release rel;
+                g.findClosestPoints(qp, rp, n, out, normals);
+            },
+            "query_points"_a, "squared_max_radii"_a, "interactions"_a, "record_normals"_a = false)
+        .def(
+            "_find_closest_points_spheres",
+            [](G &g, const std::vector<GPUBoundingSphere> &s, InteractionList &out, bool normals) {
+                py::gil_scoped_release rel;
+                g.findClosestPoints(s, out, normals);
+            },
+            "bounding_spheres"_a, "interactions"_a, "record_normals"_a = false)
+        .def(
+            "_find_closest_silhouette_points",
+            [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<uint32_t> &flip, InteractionList &out,
+               float r2min, float precision) { g.findClosestSilhouettePoints(s, flip, out, r2min, precision); },
+            "bounding_spheres"_a, "flip_normal_orientation"_a, "interactions"_a, "squared_min_radius"_a = 0.0f,
+            "precision"_a = 1e-3f)
+        // device-pointer entry points (torch tensors pass data_ptr() and the stream handle)
+        .def(
+            "_closest_points_device",
+            [](G &g, uintptr_t spheres, int64_t n, bool normals, uintptr_t out, uintptr_t stream) {
+                hip_check(g.context(),
+                          fcpw_hip_closest_points_device(g.context(), (const fcpw_hip_bounding_sphere *)spheres, n,
+                                                         normals ? 1 : 0, (fcpw_hip_interaction *)out, (void *)stream),
+                          "closest_points_device");
+            })
+        .def(
+            "_intersect_device",
+            [](G &g, uintptr_t rays, int64_t n, bool occl, uintptr_t out, uintptr_t stream) {
+                hip_check(g.context(),
+                          fcpw_hip_intersect_device(g.context(), (const fcpw_hip_ray *)rays, n, occl ? 1 : 0,
+                                                    (fcpw_hip_interaction *)out, (void *)stream),
+                          "intersect_device");
+            })
+        .def(
+            "_closest_points_compact_device",
+            [](G &g, uintptr_t points, uintptr_t r2, int64_t n, uintptr_t d, uintptr_t p, uintptr_t prim,
+               uintptr_t slot, uintptr_t stream) {
+                hip_check(g.context(),
+                          fcpw_hip_closest_points_compact_device(g.context(), (const float *)points,
+                                                                 (const float *)r2, n, (float *)d, (float *)p,
+                                                                 (int32_t *)prim, (int32_t *)slot, (void *)stream),
+                          "closest_points_compact_device");
+            })
+        .def(
+            "_distance_backward_device",
+            [](G &g, uintptr_t points, uintptr_t slot, uintptr_t grad_d, int64_t n, uintptr_t gx, uintptr_t gv,
+               uintptr_t stream) {
+                hip_check(g.context(),
+                          fcpw_hip_distance_backward_device(g.context(), (const float *)points,
+                                                            (const int32_t *)slot, (const float *)grad_d, n,
+                                                            (float *)gx, (float *)gv, (void *)stream),
+                          "distance_backward_device");
+            })
+        .def(
+            "_update_positions_device",
+            [](G &g, uintptr_t pos, int n, uintptr_t stream) {
+                hip_check(g.context(),
+                          fcpw_hip_update_positions_device(g.context(), (const float *)pos, n, (void *)stream),
+                          "update_positions_device");
+            })
+        .def("_refit_device",
+             [](G &g, uintptr_t stream) { hip_check(g.context(), fcpw_hip_refit(g.context(), (void *)stream), "refit"); })
+        .def("_set_option",
+             [](G &g, const std::string &k, int v) {
+                 hip_check(g.context(), fcpw_hip_set_option(g.context(), k.c_str(), v), "set_option");
+             })
+        .def("_get_option",
+             [](G &g, const std::string &k) {
+                 int v = 0;
+                 hip_check(g.context(), fcpw_hip_get_option(g.context(), k.c_str(), &v), "get_option");
+                 return v;
+             })
+        .def("_info", [](G &g, const std::string &k) {
+            int64_t v = 0;
+            hip_check(g.context(), fcpw_hip_get_info(g.context(), k.c_str(), &v), "get_info");
+            return v;
+        });
+}
+
+template <typename T, size_t W>
+static py::array_t<float> gather(const InteractionList &v, size_t field_offset)
+{
+    py::array_t<float> out(W == 1 ? std::vector<py::ssize_t>{(py::ssize_t)v.size()}
+                                  : std::vector<py::ssize_t>{(py::ssize_t)v.size(), (py::ssize_t)W});
+    float *o = out.mutable_data();
+    for (size_t i = 0; i < v.size(); i++)
+        std::memcpy(o + i * W, reinterpret_cast<const char *>(&v[i]) + field_offset, W * sizeof(float));
+    return out;
+}
+
+PYBIND11_MODULE(_fcpw_amd, m)
+{
+    m.doc() = "MI355X query backend for FCPW (HIP/CDNA4)";
+
+    py::enum_<PrimitiveType>(m, "PrimitiveType")
+        .value("LineSegment", PrimitiveType::LineSegment)
+        .value("Triangle", PrimitiveType::Triangle);
+    py::enum_<AggregateType>(m, "AggregateType")
+        .value("Baseline", AggregateType::Baseline)
+        .value("Bvh_LongestAxisCenter", AggregateType::Bvh_LongestAxisCenter)
+        .value("Bvh_OverlapSurfaceArea", AggregateType::Bvh_OverlapSurfaceArea)
+        .value("Bvh_SurfaceArea", AggregateType::Bvh_SurfaceArea)
+        .value("Bvh_OverlapVolume", AggregateType::Bvh_OverlapVolume)
+        .value("Bvh_Volume", AggregateType::Bvh_Volume);
+
+    py::class_<float2>(m, "GPUFloat2")
+        .def(py::init<>([]() { return float2{0.0f, 0.0f}; }))
+        .def(py::init<>([](float x, float y) { return float2{x, y}; }), "x"_a, "y"_a)
+        .def_readwrite("x", &float2::x)
+        .def_readwrite("y", &float2::y);
+    py::class_<float3>(m, "GPUFloat3")
+        .def(py::init<>([]() { return float3{0.0f, 0.0f, 0.0f}; }))
+        .def(py::init<>([](float x, float y, float z) { return float3{x, y, z}; }), "x"_a, "y"_a, "z"_a)
+        .def_readwrite("x", &float3::x)
+        .def_readwrite("y", &float3::y)
+        .def_readwrite("z", &float3::z);
+    py::class_<GPURay>(m, "GPURay")
+        .def(py::init<>())
+        .def(py::init<const float3 &, const float3 &, float>(), "o"_a, "d"_a,
+             "tMax"_a = std::numeric_limits<float>::max())
+        .def_readwrite("o", &GPURay::o)
+        .def_readwrite("d", &GPURay::d)
+        .def_readwrite("inv_d", &GPURay::dInv)
+        .def_readwrite("t_max", &GPURay::tMax);
+    py::class_<GPUBoundingSphere>(m, "GPUBoundingSphere")
+        .def(py::init<>())
+        .def(py::init<const float3 &, float>(), "c"_a, "r2"_a)
+        .def_readwrite("c", &GPUBoundingSphere::c)
+        .def_readwrite("r2", &GPUBoundingSphere::r2);
+    py::class_<GPUInteraction>(m, "GPUInteraction")
+        .def(py::init<>())
+        .def_readwrite("p", &GPUInteraction::p)
+        .def_readwrite("n", &GPUInteraction::n)
+        .def_readwrite("uv", &GPUInteraction::uv)
+        .def_readwrite("d", &GPUInteraction::d)
+        .def_readwrite("index", &GPUInteraction::index);
+
+    py::bind_vector<std::vector<float3>>(m, "GPUFloat3List");
+    py::bind_vector<std::vector<GPURay>>(m, "GPURayList");
+    py::bind_vector<std::vector<GPUBoundingSphere>>(m, "GPUBoundingSphereList");
+    auto il = py::bind_vector<InteractionList>(m, "GPUInteractionList");
+    // bulk extraction (fcpw_bindings.cpp:545-585)
+    il.def("get_distances", [](const InteractionList &v) { return gather<float, 1>(v, offsetof(GPUInteraction, d)); },
+           "Extract all distances as a NumPy array");
+    il.def("get_positions", [](const InteractionList &v) { return gather<float, 3>(v, offsetof(GPUInteraction, p)); },
+           "Extract all positions as a NumPy array of shape (n, 3)");
+    il.def("get_normals", [](const InteractionList &v) { return gather<float, 3>(v, offsetof(GPUInteraction, n)); },
+           "Extract all normals as a NumPy array of shape (n, 3)");
+    il.def("get_uvs", [](const InteractionList &v) { return gather<float, 2>(v, offsetof(GPUInteraction, uv)); },
+           "Extract all UV coordinates as a NumPy array of shape (n, 2)");
+    il.def(
+        "get_indices",
+        [](const InteractionList &v) {
+            py::array_t<uint32_t> out((py::ssize_t)v.size());
+            uint32_t *o = out.mutable_data();
+            for (size_t i = 0; i < v.size(); i++) o[i] = v[i].index;
+            return out;
+        },
+        "Extract all primitive indices as a NumPy array");
+    il.def(
+        "_from_buffer",
+        [](InteractionList &v, py::buffer b) {
+            py::buffer_info info = b.request();
+            size_t bytes = (size_t)info.size * info.itemsize;
+            if (bytes % sizeof(GPUInteraction)) throw std::invalid_argument("buffer is not a whole number of records");
+            v.resize(bytes / sizeof(GPUInteraction));
+            std::memcpy(v.data(), info.ptr, bytes);
+        },
+        "replace the contents with raw 40-byte records");
+    il.def("_as_bytes", [](const InteractionList &v) {
+        return py::bytes(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(GPUInteraction));
+    });
+
+    bind_scene<2>(m, "Scene2D");
+    bind_scene<3>(m, "Scene3D");
+    bind_gpu_scene<2>(m, "GPUScene2D");
+    bind_gpu_scene<3>(m, "GPUScene3D");
+
+    m.attr("UINT_MAX") = (uint32_t)FCPW_HIP_UINT_MAX;
+}

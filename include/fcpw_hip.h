@@ -1,0 +1,174 @@
+/*
+ * fcpw_hip.h -- C ABI of the MI355X query backend for FCPW (libfcpw_hip.so).
+ *
+ * The backend replaces the Slang/slang-rhi GPU path of FCPW's GPUScene<DIM>
+ * (include/fcpw/fcpw_gpu.h:9-93, include/fcpw/fcpw_gpu.inl,
+ *  include/fcpw/gpu/bvh.cs.slang, include/fcpw/gpu/bvh.slang).
+ * Plain pointers and sizes only; every function returns an fcpw_hip_status
+ * and records a message readable through fcpw_hip_last_error().
+ *
+ * Conventions
+ *  - The caller owns every pointer it passes; the context owns what it uploads.
+ *  - "_device" entry points take device pointers and a hipStream_t (passed as
+ *    void*, NULL = default stream); they are stream-ordered and never
+ *    synchronize.  The other query entry points take host pointers and return
+ *    after the results are back on the host (as the reference's submit +
+ *    waitOnHost, include/fcpw/gpu/slang_rhi_utils.h:582-583).
+ *  - One context per thread: like the reference (fcpw_gpu.h:30), a context is
+ *    not thread-safe.
+ *  - Record layouts are those of include/fcpw/gpu/bvh_interop_structures.h:730-774.
+ */
+#ifndef FCPW_HIP_H
+#define FCPW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCPW_HIP_UINT_MAX 4294967295u /* FCPW_GPU_UINT_MAX, bvh_interop_structures.h:10 */
+
+typedef enum {
+    FCPW_HIP_OK = 0,
+    FCPW_HIP_ERR_INVALID_ARGUMENT = 1,
+    FCPW_HIP_ERR_NOT_UPLOADED = 2,   /* query or refit before fcpw_hip_upload_bvh */
+    FCPW_HIP_ERR_HIP = 3,            /* a HIP runtime call failed */
+    FCPW_HIP_ERR_OUT_OF_MEMORY = 4,
+    FCPW_HIP_ERR_UNSUPPORTED = 5,    /* a reference feature this backend does not build */
+    FCPW_HIP_ERR_TRAVERSAL_OVERFLOW = 6
+} fcpw_hip_status;
+
+/* float3 / float2 as bvh_interop_structures.h:14-20 (float2 is 8-byte aligned) */
+typedef struct { float x, y, z; } fcpw_hip_float3;
+typedef struct { float x, y; } __attribute__((aligned(8))) fcpw_hip_float2;
+
+/* GPUBoundingSphere (bvh_interop_structures.h:749-758), 16 bytes */
+typedef struct { fcpw_hip_float3 c; float r2; } fcpw_hip_bounding_sphere;
+
+/* GPURay (bvh_interop_structures.h:730-747), 40 bytes; dinv = 1/d per component */
+typedef struct { fcpw_hip_float3 o, d, dinv; float tmax; } fcpw_hip_ray;
+
+/* GPUInteraction (bvh_interop_structures.h:760-774), 40 bytes; a miss keeps
+   p = n = uv = 0, d = FLT_MAX, index = FCPW_HIP_UINT_MAX */
+typedef struct {
+    fcpw_hip_float3 p, n;
+    fcpw_hip_float2 uv;
+    float d;
+    uint32_t index;
+} fcpw_hip_interaction;
+
+typedef struct fcpw_hip_ctx fcpw_hip_ctx;
+
+/* Stands in for GPUContext::initDevice (slang_rhi_utils.h:23-62) at
+   GPUScene construction: binds the context to a HIP device. */
+int fcpw_hip_create(int device, fcpw_hip_ctx **out);
+int fcpw_hip_destroy(fcpw_hip_ctx *ctx);
+/* Last error message of ctx, or of the failed create when ctx is NULL. */
+const char *fcpw_hip_last_error(const fcpw_hip_ctx *ctx);
+
+/*
+ * Replaces GPUBvhBuffers::allocate<DIM> (bvh_interop_structures.h:656-728),
+ * called from GPUScene::transferToGPU (fcpw_gpu.inl:18-62).
+ * The BVH arrives as the reference's flat DFS array (bvh.h:19-31, the data
+ * extractBvhNodes reads, bvh_interop_structures.h:172-190): node i has box
+ * [pmin, pmax] (3 floats each, z = 0 in 2D), nrefs > 0 for a leaf whose
+ * primitives occupy leaf slots [offset, offset + nrefs), nrefs == 0 for an
+ * internal node whose children are i + 1 and i + offset.
+ * Primitives arrive in leaf-slot order (extractTriangles / extractLineSegments,
+ * bvh_interop_structures.h:220-275): prim_vertices holds dim vertex ids per
+ * slot into positions (n_vertices x dim floats, the caller's vertex order) and
+ * prim_index the original primitive index reported in results.
+ * dim = 3: triangles; dim = 2: line segments.  Replaces any previous upload.
+ */
+int fcpw_hip_upload_bvh(fcpw_hip_ctx *ctx, int dim, int n_nodes, const float *node_pmin,
+                        const float *node_pmax, const int32_t *node_nrefs, const int32_t *node_offset,
+                        int n_prims, const int32_t *prim_vertices, const int32_t *prim_index,
+                        int n_vertices, const float *positions);
+
+/*
+ * Replaces the geometry re-upload of GPUScene::refit(scene, updateGeometry=true)
+ * (fcpw_gpu.inl:64-80 -> bvh_interop_structures.h:582-603): new vertex
+ * positions in the caller's vertex order; the primitive records are rebuilt on
+ * the device.  Call fcpw_hip_refit afterwards to update the boxes.
+ */
+int fcpw_hip_update_positions(fcpw_hip_ctx *ctx, const float *positions, int n_vertices);
+int fcpw_hip_update_positions_device(fcpw_hip_ctx *ctx, const float *positions, int n_vertices,
+                                     void *stream);
+
+/* Replaces runBvhUpdate + the refit entry point (bvh_interop_structures.h:891-934,
+   bvh.cs.slang:135-149): one bottom-up launch recomputes every box. */
+int fcpw_hip_refit(fcpw_hip_ctx *ctx, void *stream);
+
+/*
+ * Replaces GPUScene::findClosestPoints (fcpw_gpu.inl:255-274) and the
+ * closestPoint entry point (bvh.cs.slang:84-105).  record_normals != 0 writes
+ * the face normal of the closest primitive (bvh.slang:496-499).
+ */
+int fcpw_hip_closest_points(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                            int record_normals, fcpw_hip_interaction *out);
+int fcpw_hip_closest_points_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                   int record_normals, fcpw_hip_interaction *out, void *stream);
+
+/*
+ * Replaces GPUScene::intersect(rays) (fcpw_gpu.inl:137-156) and the
+ * rayIntersection entry point (bvh.cs.slang:36-57).  With check_for_occlusion
+ * the first hit ends the query and only its index is written (bvh.slang:183-187).
+ */
+int fcpw_hip_intersect(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int64_t n, int check_for_occlusion,
+                       fcpw_hip_interaction *out);
+int fcpw_hip_intersect_device(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int64_t n, int check_for_occlusion,
+                              fcpw_hip_interaction *out, void *stream);
+
+/*
+ * The sphere-intersection and closest-silhouette queries of the reference
+ * (fcpw_gpu.inl:158-217,276-338; bvh.cs.slang:59-82,107-133) are not built by
+ * this backend; these return FCPW_HIP_ERR_UNSUPPORTED.
+ */
+int fcpw_hip_intersect_spheres(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
+                               const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out);
+int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
+                                       const uint32_t *flip_normal_orientation, int64_t n,
+                                       float squared_min_radius, float precision, fcpw_hip_interaction *out);
+
+/*
+ * Compact closest-point query for differentiable use (no reference
+ * counterpart; the reference has no gradients).  points: n x dim floats;
+ * r2: n floats or NULL for an unbounded search.  Writes d (n), p (n x dim),
+ * prim (n, original primitive index or -1) and slot (n, leaf slot or -1, the
+ * handle fcpw_hip_distance_backward_device recomputes the primitive from).
+ * Any output pointer may be NULL.
+ */
+int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *ctx, const float *points, const float *r2, int64_t n,
+                                           float *d, float *p, int32_t *prim, int32_t *slot, void *stream);
+
+/*
+ * Adjoint of the unsigned distance d(x, mesh) (no reference counterpart).
+ * From x, the saved leaf slot and grad_d = dL/dd it recomputes the closest
+ * point p and its barycentric weights b, and writes
+ *   grad_points[i]  = grad_d[i] * (x - p) / d                 (n x dim, overwritten)
+ *   grad_vertices[v] += -grad_d[i] * b_k * (x - p) / d        (n_vertices x dim, FP32 atomics)
+ * in the caller's vertex order.  Queries with d == 0 or slot < 0 contribute 0.
+ * Either output may be NULL.
+ */
+int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, const int32_t *slot,
+                                      const float *grad_d, int64_t n, float *grad_points, float *grad_vertices,
+                                      void *stream);
+
+/* Tuning knobs read by the kernels (values chosen by fcpw_amd.autotune):
+     "block"        threads per block for the query kernels (64, 128, 256)
+     "stack"        LDS stack entries per lane (8, 12, 16, 24)
+     "sort"         1 = reorder queries along a Morton curve before traversal
+     "persistent"   1 = persistent waves refilled from a work counter
+   Unknown keys or values return FCPW_HIP_ERR_INVALID_ARGUMENT. */
+int fcpw_hip_set_option(fcpw_hip_ctx *ctx, const char *key, int value);
+int fcpw_hip_get_option(const fcpw_hip_ctx *ctx, const char *key, int *value);
+
+/* Uploaded BVH facts: n_records (internal nodes), n_leaves, n_prims, dim, max_depth. */
+int fcpw_hip_get_info(const fcpw_hip_ctx *ctx, const char *key, int64_t *value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCPW_HIP_H */
