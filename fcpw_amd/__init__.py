@@ -28,6 +28,16 @@ import numpy as _np
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 
+# One HIP runtime per process: torch's libtorch_hip needs the unversioned
+# libamdhip64.so from its own directory, while libfcpw_hip.so needs
+# libamdhip64.so.7.  Loading torch first lets the dynamic loader satisfy
+# libfcpw_hip.so with the runtime torch already mapped (same soname); the other
+# order maps two runtimes, and the one initialised second finds no device.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:
+    _torch = None
+
 try:
     from . import _fcpw_amd as _native
 except ImportError as e:  # no silent fallback: the HIP library is the product

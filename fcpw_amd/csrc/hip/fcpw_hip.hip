@@ -701,18 +701,18 @@ extern "C" {
 int fcpw_hip_create(int device, fcpw_hip_ctx **out)
 {
     if (!out) {
-        g_create_error = "fcpw_hip_create: out is NULL";
+        g_create_error = "out is NULL";
         return FCPW_HIP_ERR_INVALID_ARGUMENT;
     }
     *out = nullptr;
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count == 0) {
-        g_create_error = std::string("fcpw_hip_create: no HIP device (") + hipGetErrorString(e) + ")";
+        g_create_error = std::string("no HIP device (") + hipGetErrorString(e) + ")";
         return FCPW_HIP_ERR_HIP;
     }
     if (device < 0 || device >= count) {
-        g_create_error = "fcpw_hip_create: device index out of range";
+        g_create_error = "device index out of range";
         return FCPW_HIP_ERR_INVALID_ARGUMENT;
     }
     fcpw_hip_ctx *c = new fcpw_hip_ctx();
@@ -725,7 +725,7 @@ int fcpw_hip_create(int device, fcpw_hip_ctx **out)
     }
     if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(int));
     if (e != hipSuccess) {
-        g_create_error = std::string("fcpw_hip_create: ") + hipGetErrorString(e);
+        g_create_error = hipGetErrorString(e);
         if (c->flags) (void)hipFree(c->flags);
         delete c;
         return FCPW_HIP_ERR_HIP;

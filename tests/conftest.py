@@ -3,6 +3,12 @@ import sys
 
 import pytest
 
+# map torch's HIP runtime before anything loads libfcpw_hip.so (one runtime per process)
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -47,13 +47,23 @@ def assert_bit_equal(rec, ref, fields=("p", "n", "uv", "d")):
         assert eq.all(), f"field {f}: {np.count_nonzero(~eq)} mismatches"
 
 
+def abs_floor(V, q):
+    """FP32 floor of any distance computation: x - p cancels, leaving a few ulp
+    of the coordinate magnitude regardless of d (4 ulp of the largest coordinate)"""
+    scale = max(float(np.abs(V).max()), float(np.abs(q).max()))
+    return 4.0 * float(np.finfo(np.float32).eps) * scale
+
+
 def check_vs_cpu_path(V, F, q, rec, cpu):
-    """reference CPU order: same hit set, indices equal except at ties, d within 1e-5 rel"""
+    """reference CPU order: same hit set, indices equal except at ties,
+    |d_gpu - d_cpu| <= 1e-5 d + floor (the two paths sum their FP32 dot products
+    in different orders, so near d -> 0 only the absolute floor is attainable)"""
     idx = rec["index"].view(np.int32)
     np.testing.assert_array_equal(idx >= 0, cpu["index"] >= 0)
     found = idx >= 0
-    rel = np.abs(rec["d"][found] - cpu["d"][found]) / np.maximum(cpu["d"][found], 1e-30)
-    assert rel.max(initial=0.0) <= REL_TOL
+    err = np.abs(rec["d"][found].astype(np.float64) - cpu["d"][found])
+    tol = REL_TOL * cpu["d"][found].astype(np.float64) + abs_floor(V, q)
+    assert (err <= tol).all(), f"max excess {np.max(err - tol)}"
     m = idx != cpu["index"]
     if m.any():
         tie = oracle.classify_index_mismatch(V, F, q[m], idx[m], cpu["index"][m])
