@@ -56,30 +56,42 @@ struct DevScene {
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
 
 // ---------------------------------------------------------------------------
-// stack: K entries in LDS, the rest private
+// traversal stack: entries [0, K) in LDS, lane-interleaved (entry s of lane t
+// at lds[s * BLOCK + t], so a wave's accesses fall in distinct banks), entries
+// [K, 64) in a private spill array.  The stack pointer stays in a register and
+// the LDS array is declared in the traversal function itself, so the accesses
+// compile to ds_read/ds_write rather than generic (flat) loads.
 // ---------------------------------------------------------------------------
-template <int BLOCK, int K>
-struct Stack {
-    uint2 *lds; // this lane's column: entry s at lds[s * BLOCK]
-    uint2 spill[STACK_TOTAL - K];
-    int sp;
-    __device__ __forceinline__ bool push(uint32_t link, float d)
-    {
-        if (sp >= STACK_TOTAL) return false;
-        uint2 e = make_uint2(link, __float_as_uint(d));
-        if (sp < K) lds[sp * BLOCK] = e;
-        else spill[sp - K] = e;
-        sp++;
-        return true;
+#define FCPW_STACK_DECL(BLOCK, K)                                                                              \
+    __shared__ uint64_t lds_stack[(K) * (BLOCK)];                                                            \
+    uint64_t stack_spill[STACK_TOTAL - (K)];                                                                 \
+    const int stack_lane = threadIdx.x;                                                                       \
+    int sp = 0
+
+// An entry packs (link, distance bits) into 64 bits.  The cold spill path uses
+// nontemporal accesses: besides keeping the scratch lines out of the caches,
+// that stops the compiler from merging the LDS and the spill access into one
+// generic (flat) access, which would send every LDS push and pop down the slower
+// vector-memory path.
+#define FCPW_PUSH(BLOCK, K, LINK, D, ON_FULL)                                                                  \
+    if (sp >= STACK_TOTAL) {                                                                                  \
+        ON_FULL;                                                                                              \
+    } else {                                                                                                  \
+        uint64_t e_ = (uint64_t)(LINK) | ((uint64_t)__float_as_uint(D) << 32);                                \
+        if (sp < (K)) lds_stack[sp * (BLOCK) + stack_lane] = e_;                                              \
+        else __builtin_nontemporal_store(e_, &stack_spill[sp - (K)]);                                         \
+        sp++;                                                                                                 \
     }
-    __device__ __forceinline__ void pop(uint32_t &link, float &d)
-    {
-        sp--;
-        uint2 e = (sp < K) ? lds[sp * BLOCK] : spill[sp - K];
-        link = e.x;
-        d = __uint_as_float(e.y);
-    }
-};
+
+#define FCPW_POP(BLOCK, K, LINK, D)                                                                            \
+    do {                                                                                                      \
+        sp--;                                                                                                 \
+        uint64_t e_;                                                                                          \
+        if (sp < (K)) e_ = lds_stack[sp * (BLOCK) + stack_lane];                                              \
+        else e_ = *(volatile uint64_t *)&stack_spill[sp - (K)];                                              \
+        (LINK) = (uint32_t)e_;                                                                                \
+        (D) = __uint_as_float((uint32_t)(e_ >> 32));                                                          \
+    } while (0)
 
 // ---------------------------------------------------------------------------
 // per-dimension record/primitive access
@@ -110,11 +122,13 @@ struct Geo<3> {
         hiB = {b1.x, b1.y, b1.z};
         lB = link_of(b0.w);
     }
+    template <bool SEL = false>
     static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
                                                     float &u1)
     {
         const float4 *t = S.prims + (size_t)slot * 3;
         float4 a = t[0], b = t[1], c = t[2];
+        if (SEL) return closest_point_triangle_sel({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
         return closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
     }
     static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
@@ -151,6 +165,7 @@ struct Geo<2> {
         lA = link_of(l.x);
         lB = link_of(l.y);
     }
+    template <bool SEL = false>
     static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
                                                     float &u1)
     {
@@ -181,9 +196,9 @@ struct CpResult {
     float uv[2];
 };
 
-template <int DIM, int BLOCK, int K>
+template <int DIM, int BLOCK, int K, bool SEL>
 __device__ __forceinline__ void closest_point_query(const DevScene &S, typename Geo<DIM>::vec x, float r2,
-                                                    uint2 *lds_col, CpResult &res)
+                                                    CpResult &res)
 {
     using G = Geo<DIM>;
     using vec = typename G::vec;
@@ -201,9 +216,7 @@ __device__ __forceinline__ void closest_point_query(const DevScene &S, typename 
     r2 = fminf(r2, d2max);
     float cur_d2 = d2min;
 
-    Stack<BLOCK, K> st;
-    st.lds = lds_col;
-    st.sp = 0;
+    FCPW_STACK_DECL(BLOCK, K);
     vec bp = {};
     while (true) {
         if (cur_d2 <= r2) {
@@ -213,7 +226,7 @@ __device__ __forceinline__ void closest_point_query(const DevScene &S, typename 
                 for (uint32_t k = 0; k < cnt; k++) {
                     vec p;
                     float u0, u1;
-                    float d = G::closest(S, off + k, x, p, u0, u1);
+                    float d = G::template closest<SEL>(S, off + k, x, p, u0, u1);
                     if (d * d <= r2) {
                         r2 = fminf(r2, d * d);
                         res.slot = (int32_t)(off + k);
@@ -238,7 +251,7 @@ __device__ __forceinline__ void closest_point_query(const DevScene &S, typename 
                     uint32_t nearL = lA, farL = lB;
                     float nearD = a0, farD = b0;
                     if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
-                    if (!st.push(farL, farD)) { atomicOr(S.flags, 1); break; }
+                    FCPW_PUSH(BLOCK, K, farL, farD, { atomicOr(S.flags, 1); break; });
                     cur = nearL;
                     cur_d2 = nearD;
                     continue;
@@ -251,11 +264,12 @@ __device__ __forceinline__ void closest_point_query(const DevScene &S, typename 
                 }
             }
         }
-        if (st.sp == 0) break;
-        st.pop(cur, cur_d2);
+        if (sp == 0) break;
+        FCPW_POP(BLOCK, K, cur, cur_d2);
     }
     if (res.slot >= 0) store_p(bp, res.p);
 }
+
 
 // ---------------------------------------------------------------------------
 // ray traversal (bvh.slang:144-254 order)
@@ -324,7 +338,7 @@ struct RayGeo<2> {
 
 template <int DIM, int BLOCK, int K>
 __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::vec o, typename Geo<DIM>::vec d,
-                                          typename Geo<DIM>::vec dinv, float tmax, bool occlusion, uint2 *lds_col,
+                                          typename Geo<DIM>::vec dinv, float tmax, bool occlusion,
                                           RayResult &res)
 {
     using G = Geo<DIM>;
@@ -340,9 +354,7 @@ __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::
     float cur_t;
     if (!box_ray(lo, hi, o, d, dinv, tmax, cur_t)) return;
 
-    Stack<BLOCK, K> st;
-    st.lds = lds_col;
-    st.sp = 0;
+    FCPW_STACK_DECL(BLOCK, K);
     while (true) {
         if (!(cur_t > tmax)) {
             if (cur & LEAF_BIT) {
@@ -368,7 +380,7 @@ __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::
                     uint32_t nearL = lA, farL = lB;
                     float nearT = ta, farT = tb;
                     if (tb < ta) { nearL = lB; farL = lA; nearT = tb; farT = ta; }
-                    if (!st.push(farL, farT)) { atomicOr(S.flags, 1); break; }
+                    FCPW_PUSH(BLOCK, K, farL, farT, { atomicOr(S.flags, 1); break; });
                     cur = nearL;
                     cur_t = nearT;
                     continue;
@@ -381,8 +393,8 @@ __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::
                 }
             }
         }
-        if (st.sp == 0) break;
-        st.pop(cur, cur_t);
+        if (sp == 0) break;
+        FCPW_POP(BLOCK, K, cur, cur_t);
     }
 }
 
@@ -400,62 +412,269 @@ __device__ __forceinline__ void write_interaction(float2 *o, const float p[3], c
     o[4] = make_float2(d, __uint_as_float(index));
 }
 
-template <int DIM, int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void k_closest_points(DevScene S, const float4 *spheres, int64_t n,
-                                                          int record_normals, float2 *out)
+// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD
+// b mod xcds).  Renumbering blocks so that XCD k owns one contiguous range of
+// tiles keeps each XCD's L2 on one region of the (Morton-ordered) queries.
+// The tile is formed in 32 bits and widened once at the end: a version that
+// returned a 64-bit value from two branches was compiled (ROCm 7.2 clang) with
+// the high word of the pass-through branch left holding a stale quotient, so
+// the blocks past the last full group of xcds computed out-of-range tiles.
+__device__ __forceinline__ int64_t xcd_tile(int xcds)
 {
-    __shared__ uint2 lds[K * BLOCK];
-    uint2 *col = lds + threadIdx.x;
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        float4 s = spheres[i];
-        float q[3] = {s.x, s.y, s.z};
-        CpResult r;
-        closest_point_query<DIM, BLOCK, K>(S, Geo<DIM>::load_point(q), s.w, col, r);
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    const uint32_t x = xcds > 1 ? (uint32_t)xcds : 1u;
+    const uint32_t full = (G / x) * x;
+    const uint32_t t = b < full ? (b % x) * (full / x) + b / x : b;
+    return (int64_t)(uint64_t)t;
+}
+
+// query input / output policies shared by the closest-point kernels
+struct CpIO {
+    const float *in;     // spheres (4 floats: c, r2) or points (DIM floats)
+    const float *r2;     // compact input: per-query r^2 or nullptr (unbounded)
+    int64_t n;
+    const uint32_t *order;
+    // interaction output
+    float2 *out;
+    int record_normals;
+    // compact output (any may be nullptr)
+    float *d, *p;
+    int32_t *prim, *slot;
+};
+enum { CP_OUT_INTERACTION = 0, CP_OUT_COMPACT = 1 };
+
+template <int DIM, int OUT>
+__device__ __forceinline__ void cp_load(const CpIO &io, int64_t i, float q[3], float &r2)
+{
+    if (OUT == CP_OUT_INTERACTION) {
+        float4 s = reinterpret_cast<const float4 *>(io.in)[i];
+        q[0] = s.x; q[1] = s.y; q[2] = s.z;
+        r2 = s.w;
+    } else {
+        const float *x = io.in + i * DIM;
+        q[0] = x[0]; q[1] = x[1]; q[2] = DIM == 3 ? x[DIM - 1] : 0.0f;
+        r2 = io.r2 ? io.r2[i] : INFINITY;
+    }
+}
+
+template <int DIM, int OUT>
+__device__ __forceinline__ void cp_store(const DevScene &S, const CpIO &io, int64_t i, const CpResult &r)
+{
+    if (OUT == CP_OUT_INTERACTION) {
         float nrm[3] = {0.0f, 0.0f, 0.0f};
         uint32_t index = FCPW_HIP_UINT_MAX;
         if (r.slot >= 0) {
             index = (uint32_t)S.prim_index[r.slot];
-            if (record_normals) Geo<DIM>::normal(S, (uint32_t)r.slot, nrm);
+            if (io.record_normals) Geo<DIM>::normal(S, (uint32_t)r.slot, nrm);
         }
-        write_interaction(out + i * 5, r.p, nrm, r.uv, r.d, index);
+        write_interaction(io.out + i * 5, r.p, nrm, r.uv, r.d, index);
+    } else {
+        if (io.d) io.d[i] = r.d;
+        if (io.p)
+            for (int k = 0; k < DIM; k++) io.p[i * DIM + k] = r.p[k];
+        if (io.prim) io.prim[i] = r.slot >= 0 ? S.prim_index[r.slot] : -1;
+        if (io.slot) io.slot[i] = r.slot;
     }
 }
 
-template <int DIM, int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void k_closest_points_compact(DevScene S, const float *points, const float *r2in,
-                                                                  int64_t n, float *d_out, float *p_out,
-                                                                  int32_t *prim_out, int32_t *slot_out)
+// one query per lane, the reference loop
+template <int DIM, int BLOCK, int K, int OUT>
+__global__ __launch_bounds__(BLOCK) void k_closest_points(DevScene S, CpIO io, int xcds)
 {
-    __shared__ uint2 lds[K * BLOCK];
-    uint2 *col = lds + threadIdx.x;
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        const float *q = points + i * DIM;
-        float r2 = r2in ? r2in[i] : INFINITY;
-        CpResult r;
-        closest_point_query<DIM, BLOCK, K>(S, Geo<DIM>::load_point(q), r2, col, r);
-        if (d_out) d_out[i] = r.d;
-        if (p_out)
-            for (int k = 0; k < DIM; k++) p_out[i * DIM + k] = r.p[k];
-        if (prim_out) prim_out[i] = r.slot >= 0 ? S.prim_index[r.slot] : -1;
-        if (slot_out) slot_out[i] = r.slot;
+    int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    if (k >= io.n) return;
+    int64_t i = io.order ? (int64_t)io.order[k] : k;
+    float q[3], r2;
+    cp_load<DIM, OUT>(io, i, q, r2);
+    CpResult r;
+    closest_point_query<DIM, BLOCK, K, false>(S, Geo<DIM>::load_point(q), r2, r);
+    cp_store<DIM, OUT>(S, io, i, r);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-cooperative leaf processing (one wave per workgroup).  Lanes descend
+// the tree independently; a lane that reaches a leaf holds it.  Once at least
+// `hold_min` lanes hold a leaf (or every lane still working does), the wave
+// lays all (lane, primitive) pairs of the held leaves out in LDS and all 64
+// lanes -- including lanes whose own query has finished -- evaluate up to 64
+// primitive distances per pass.  Each owner then replays the reference's
+// acceptance test (d*d <= r2, r2 = d*d, later equal distances win) over its
+// own primitives in leaf order.  A distance does not depend on r2, so the
+// replay gives the reference's result bit for bit; the closest point and uv
+// of the winning primitive are recomputed once at the end.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int DIM, int K, bool SEL>
+__device__ __forceinline__ void closest_point_query_coop(const DevScene &S, typename Geo<DIM>::vec x, float r2,
+                                                         bool valid, int hold_min, CpResult &res)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    constexpr int BLOCK = 64;
+    __shared__ float coop_q[64 * 3];
+    __shared__ uint32_t coop_off[64];
+    __shared__ uint16_t coop_item[256];
+    __shared__ float coop_d[256];
+    const int lane = threadIdx.x;
+    float xs[3];
+    store_p(x, xs);
+    coop_q[lane * 3 + 0] = xs[0];
+    coop_q[lane * 3 + 1] = xs[1];
+    coop_q[lane * 3 + 2] = xs[2];
+    res.slot = -1;
+    res.d = FLT_MAX;
+    res.p[0] = res.p[1] = res.p[2] = 0.0f;
+    res.uv[0] = res.uv[1] = 0.0f;
+
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float d2min, d2max, cur_d2 = 0.0f;
+    box_sqdist(lo, hi, x, d2min, d2max);
+    bool active = valid && (d2min <= r2);
+    if (active) {
+        r2 = fminf(r2, d2max);
+        cur_d2 = d2min;
     }
+    bool live = active, holding = false, overflow = false;
+    uint32_t loff = 0, lcnt = 0;
+    int32_t best = -1;
+    FCPW_STACK_DECL(BLOCK, K);
+    __syncthreads();
+    while (true) {
+        if (active && !holding) {
+            if (!(live && cur_d2 <= r2)) {
+                live = false;
+                while (sp > 0) {
+                    FCPW_POP(BLOCK, K, cur, cur_d2);
+                    if (cur_d2 <= r2) { live = true; break; }
+                }
+            }
+            if (!live) {
+                active = false;
+            } else if (cur & LEAF_BIT) {
+                holding = true;
+                live = false;
+                loff = cur & OFF_MASK;
+                lcnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+            } else {
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                float a0, a1, b0, b1;
+                box_sqdist(loA, hiA, x, a0, a1);
+                bool oA = a0 <= r2;
+                r2 = fminf(r2, a1);
+                box_sqdist(loB, hiB, x, b0, b1);
+                bool oB = b0 <= r2;
+                r2 = fminf(r2, b1);
+                live = oA || oB;
+                if (oA && oB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearD = a0, farD = b0;
+                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                    FCPW_PUSH(BLOCK, K, farL, farD, { overflow = true; active = false; live = false; });
+                    cur = nearL; cur_d2 = nearD;
+                } else if (oA) {
+                    cur = lA; cur_d2 = a0;
+                } else if (oB) {
+                    cur = lB; cur_d2 = b0;
+                }
+            }
+        }
+        uint64_t act = __ballot(active);
+        if (act == 0) break;
+        uint64_t hold = __ballot(holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+
+        coop_off[lane] = loff;
+        for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
+            bool h0 = holding && lcnt > j0, h1 = holding && lcnt > j0 + 1;
+            bool h2 = holding && lcnt > j0 + 2, h3 = holding && lcnt > j0 + 3;
+            uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            if (m0 == 0) break;
+            uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+            uint32_t total = c0 + c1 + c2 + __popcll(m3);
+            uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                     i3 = c0 + c1 + c2 + lanes_below(m3);
+            if (h0) coop_item[i0] = (uint16_t)((lane << 4) | j0);
+            if (h1) coop_item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
+            if (h2) coop_item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
+            if (h3) coop_item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
+            __syncthreads();
+            for (uint32_t pass = 0; pass < total; pass += 64) {
+                uint32_t k = pass + (uint32_t)lane;
+                if (k < total) {
+                    uint32_t it = coop_item[k];
+                    uint32_t o = it >> 4;
+                    float xo[3] = {coop_q[o * 3], coop_q[o * 3 + 1], coop_q[o * 3 + 2]};
+                    vec pp;
+                    float u0, u1;
+                    coop_d[k] = G::template closest<SEL>(S, coop_off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
+                }
+            }
+            __syncthreads();
+            if (holding) {
+                const uint32_t idx[4] = {i0, i1, i2, i3};
+                for (uint32_t jj = 0; jj < 4; jj++) {
+                    if (lcnt > j0 + jj) {
+                        float d = coop_d[idx[jj]];
+                        if (d * d <= r2) {
+                            r2 = fminf(r2, d * d);
+                            best = (int32_t)(loff + j0 + jj);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (m3 == 0) break;
+        }
+        holding = false;
+    }
+    if (overflow) atomicOr(S.flags, 1);
+    if (best >= 0) {
+        vec p;
+        float u0, u1;
+        res.d = G::template closest<SEL>(S, (uint32_t)best, x, p, u0, u1);
+        res.slot = best;
+        res.uv[0] = u0;
+        res.uv[1] = u1;
+        store_p(p, res.p);
+    }
+}
+
+template <int DIM, int K, int OUT>
+__global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io, int xcds, int hold_min)
+{
+    int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
+    bool valid = k < io.n;
+    int64_t i = valid ? (io.order ? (int64_t)io.order[k] : k) : 0;
+    float q[3] = {0.0f, 0.0f, 0.0f}, r2 = -1.0f;
+    if (valid) cp_load<DIM, OUT>(io, i, q, r2);
+    CpResult r;
+    closest_point_query_coop<DIM, K, false>(S, Geo<DIM>::load_point(q), r2, valid, hold_min, r);
+    if (valid) cp_store<DIM, OUT>(S, io, i, r);
 }
 
 template <int DIM, int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *rays, int64_t n, int occlusion,
-                                                     float2 *out)
+                                                     float2 *out, const uint32_t *order, int xcds)
 {
-    __shared__ uint2 lds[K * BLOCK];
-    uint2 *col = lds + threadIdx.x;
     using vec = typename Geo<DIM>::vec;
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    if (k < n) {
+        int64_t i = order ? (int64_t)order[k] : k;
         const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
         float2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
         float o3[3] = {r0.x, r0.y, r1.x}, d3[3] = {r1.y, r2.x, r2.y}, di3[3] = {r3.x, r3.y, r4.x};
         float tmax = r4.y;
         vec o = Geo<DIM>::load_point(o3), d = Geo<DIM>::load_point(d3), di = Geo<DIM>::load_point(di3);
         RayResult r;
-        ray_query<DIM, BLOCK, K>(S, o, d, di, tmax, occlusion != 0, col, r);
+        ray_query<DIM, BLOCK, K>(S, o, d, di, tmax, occlusion != 0, r);
         uint32_t index = r.slot >= 0 ? (uint32_t)S.prim_index[r.slot] : FCPW_HIP_UINT_MAX;
         if (occlusion) {
             // only the occluder's index is written (bvh.slang:183-187)
@@ -465,6 +684,95 @@ __global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *r
             write_interaction(out + i * 5, r.p, r.n, r.uv, r.t, index);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Morton ordering of a query batch: a counting sort of the queries over a
+// 2^18-cell grid (6 bits per axis in 3D, 9 in 2D) laid along a Z-order curve.
+// Neighbouring lanes then traverse neighbouring parts of the tree, which keeps
+// a wave's branches together and its node fetches in L1/L2.  The order inside
+// a cell is whatever the atomics produce; results do not depend on it.
+// ---------------------------------------------------------------------------
+constexpr int MORTON_BITS = 18;
+constexpr uint32_t MORTON_CELLS = 1u << MORTON_BITS;
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) // 6 bits -> every third bit
+{
+    v &= 0x3Fu;
+    v = (v | (v << 8)) & 0x0000F00Fu;
+    v = (v | (v << 4)) & 0x000C30C3u;
+    v = (v | (v << 2)) & 0x00249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) // 9 bits -> every second bit
+{
+    v &= 0x1FFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quant(float x, float lo, float inv, uint32_t cells)
+{
+    float t = (x - lo) * inv; // NaN and out-of-range values clamp to the border cells
+    int c = (t >= 0.0f) ? (int)fminf(t * (float)cells, (float)(cells - 1)) : 0;
+    return (uint32_t)c;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int DIM>
+__global__ void k_morton_hist(const float *pts, int stride, int64_t n, float4 lo, float4 inv, uint32_t *codes,
+                              uint32_t *hist)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *q = pts + i * stride;
+    uint32_t code;
+    if (DIM == 3)
+        code = spread3(quant(q[0], lo.x, inv.x, 64)) | (spread3(quant(q[1], lo.y, inv.y, 64)) << 1) |
+               (spread3(quant(q[2], lo.z, inv.z, 64)) << 2);
+    else
+        code = spread2(quant(q[0], lo.x, inv.x, 512)) | (spread2(quant(q[1], lo.y, inv.y, 512)) << 1);
+    codes[i] = code;
+    atomicAdd(&hist[code], 1u);
+}
+
+// exclusive scan of MORTON_CELLS counters by one 1024-thread block
+__global__ __launch_bounds__(1024) void k_scan_cells(uint32_t *hist)
+{
+    constexpr int PER = MORTON_CELLS / 1024;
+    __shared__ uint32_t part[1024];
+    uint32_t *h = hist + (size_t)threadIdx.x * PER;
+    uint32_t sum = 0;
+    for (int k = 0; k < PER; k++) sum += h[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) { // Hillis-Steele inclusive scan
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (int k = 0; k < PER; k++) {
+        uint32_t c = h[k];
+        h[k] = run;
+        run += c;
+    }
+}
+
+__global__ void k_morton_scatter(const uint32_t *codes, int64_t n, uint32_t *offsets, uint32_t *perm)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pos = atomicAdd(&offsets[codes[i]], 1u);
+    perm[pos] = (uint32_t)i;
 }
 
 // rebuild the primitive records from vertex positions (caller's vertex order)
@@ -639,10 +947,14 @@ struct fcpw_hip_ctx {
     // staging for the host-pointer entry points
     void *stage_in = nullptr, *stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    // query ordering workspace (grown on demand, kept for the context's lifetime)
+    void *ws_codes = nullptr, *ws_perm = nullptr, *ws_hist = nullptr;
+    size_t ws_codes_bytes = 0, ws_perm_bytes = 0, ws_hist_bytes = 0;
+    float qlo[3] = {0, 0, 0}, qinv[3] = {1, 1, 1}; // Morton grid: 1.5x the scene box
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
-    int block = 128, stack = 16, sort = 0, persistent = 0;
+    int block = 128, stack = 16, sort = 1, xcds = 8, coop = 1, hold_min = 2;
     int n_cus = 256;
 };
 
@@ -740,8 +1052,9 @@ int fcpw_hip_destroy(fcpw_hip_ctx *c)
     (void)hipSetDevice(c->device);
     free_scene(c);
     if (c->flags) (void)hipFree(c->flags);
-    if (c->stage_in) (void)hipFree(c->stage_in);
-    if (c->stage_out) (void)hipFree(c->stage_out);
+    void *ws[] = {c->stage_in, c->stage_out, c->ws_codes, c->ws_perm, c->ws_hist};
+    for (void *b : ws)
+        if (b) (void)hipFree(b);
     delete c;
     return FCPW_HIP_OK;
 }
@@ -898,6 +1211,12 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
                           hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemset(c->counters, 0, parent.size() * sizeof(unsigned int)));
     HIPCHECK(c, hipDeviceSynchronize());
+    for (int d = 0; d < 3; d++) {
+        float lo = d < dim ? pmin[d] : 0.0f, hi = d < dim ? pmax[d] : 0.0f;
+        float ext = std::max(hi - lo, 1e-6f);
+        c->qlo[d] = 0.5f * (lo + hi) - 0.75f * ext;
+        c->qinv[d] = 1.0f / (1.5f * ext);
+    }
     c->dim = dim;
     c->n_records = n_rec;
     c->n_leaves = n_leaf;
@@ -976,50 +1295,112 @@ int fcpw_hip_refit(fcpw_hip_ctx *c, void *stream)
 // launch helpers: (block, stack) pairs are compiled in; the grid is capped and
 // the kernels stride over the remaining queries
 // ---------------------------------------------------------------------------
-static dim3 grid_for(fcpw_hip_ctx *c, int64_t n, int block)
+static dim3 grid_for(fcpw_hip_ctx *, int64_t n, int block)
 {
-    int64_t blocks = (n + block - 1) / block;
-    int64_t cap = (int64_t)c->n_cus * 64;
-    return dim3((unsigned)std::max<int64_t>(1, std::min(blocks, cap)));
+    return dim3((unsigned)std::max<int64_t>(1, (n + block - 1) / block));
 }
 
-#define FCPW_DISPATCH(c, KERNEL_CALL)                                                                          \
+// Morton order of n query points (pts + i * stride floats); returns the
+// permutation, or nullptr when ordering is off or the batch is small
+static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n, hipStream_t s,
+                        const uint32_t **perm_out)
+{
+    *perm_out = nullptr;
+    if (!c->sort || n < 4096) return FCPW_HIP_OK;
+    if (n > (int64_t)UINT32_MAX) FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "query batches above 2^32 are not supported");
+    int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 4);
+    if (!st) st = grow(c, &c->ws_perm, &c->ws_perm_bytes, (size_t)n * 4);
+    if (!st) st = grow(c, &c->ws_hist, &c->ws_hist_bytes, (size_t)MORTON_CELLS * 4);
+    if (st) return st;
+    uint32_t *codes = (uint32_t *)c->ws_codes, *perm = (uint32_t *)c->ws_perm, *hist = (uint32_t *)c->ws_hist;
+    unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(MORTON_CELLS / 256), dim3(256), 0, s, hist, 0u, (int64_t)MORTON_CELLS);
+    float4 lo = make_float4(c->qlo[0], c->qlo[1], c->qlo[2], 0.0f);
+    float4 inv = make_float4(c->qinv[0], c->qinv[1], c->qinv[2], 0.0f);
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_morton_hist<3>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, codes, hist);
+    else
+        hipLaunchKernelGGL(k_morton_hist<2>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, codes, hist);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, hist);
+    hipLaunchKernelGGL(k_morton_scatter, dim3(nb), dim3(256), 0, s, codes, n, hist, perm);
+    HIPCHECK(c, hipGetLastError());
+    *perm_out = perm;
+    return FCPW_HIP_OK;
+}
+
+#define FCPW_DISPATCH_BK(c, V, ...)                                                                            \
     do {                                                                                                       \
         const int _b = (c)->block, _k = (c)->stack;                                                           \
-        if (_b == 64 && _k == 8) { constexpr int B = 64, K = 8; KERNEL_CALL; }                               \
-        else if (_b == 64 && _k == 16) { constexpr int B = 64, K = 16; KERNEL_CALL; }                        \
-        else if (_b == 64 && _k == 24) { constexpr int B = 64, K = 24; KERNEL_CALL; }                        \
-        else if (_b == 128 && _k == 8) { constexpr int B = 128, K = 8; KERNEL_CALL; }                        \
-        else if (_b == 128 && _k == 16) { constexpr int B = 128, K = 16; KERNEL_CALL; }                      \
-        else if (_b == 128 && _k == 24) { constexpr int B = 128, K = 24; KERNEL_CALL; }                      \
-        else if (_b == 256 && _k == 8) { constexpr int B = 256, K = 8; KERNEL_CALL; }                        \
-        else if (_b == 256 && _k == 16) { constexpr int B = 256, K = 16; KERNEL_CALL; }                      \
-        else { constexpr int B = 256, K = 24; KERNEL_CALL; }                                                 \
+        if (_b == 64 && _k == 8) { constexpr int B = 64, K = 8; __VA_ARGS__; }                               \
+        else if (_b == 64 && _k == 16) { constexpr int B = 64, K = 16; __VA_ARGS__; }                        \
+        else if (_b == 64 && _k == 24) { constexpr int B = 64, K = 24; __VA_ARGS__; }                        \
+        else if (_b == 128 && _k == 8) { constexpr int B = 128, K = 8; __VA_ARGS__; }                        \
+        else if (_b == 128 && _k == 16) { constexpr int B = 128, K = 16; __VA_ARGS__; }                      \
+        else if (_b == 128 && _k == 24) { constexpr int B = 128, K = 24; __VA_ARGS__; }                      \
+        else if (_b == 256 && _k == 8) { constexpr int B = 256, K = 8; __VA_ARGS__; }                        \
+        else if (_b == 256 && _k == 16) { constexpr int B = 256, K = 16; __VA_ARGS__; }                      \
+        else { constexpr int B = 256, K = 24; __VA_ARGS__; }                                                 \
     } while (0)
 
-static int launch_closest(fcpw_hip_ctx *c, const void *spheres, int64_t n, int record_normals, void *out,
-                          hipStream_t s)
+
+
+} // extern "C"
+
+// closest-point launch for either output policy; io.in holds spheres (4
+// floats per query) for interaction records, points (dim floats) otherwise
+template <int OUT>
+static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
 {
     DevScene S = dev_scene(c);
-    if (c->dim == 3)
-        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points<3, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
-                                            (const float4 *)spheres, n, record_normals, (float2 *)out));
-    else
-        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points<2, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
-                                            (const float4 *)spheres, n, record_normals, (float2 *)out));
+    int st = morton_order(c, io.in, OUT == CP_OUT_INTERACTION ? 4 : c->dim, io.n, s, &io.order);
+    if (st) return st;
+    const int64_t n = io.n;
+    if (c->coop) {
+        dim3 g = grid_for(c, n, 64);
+        if (c->dim == 3 && c->stack <= 8)
+            hipLaunchKernelGGL((k_closest_points_coop<3, 8, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+        else if (c->dim == 3)
+            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+        else if (c->stack <= 8)
+            hipLaunchKernelGGL((k_closest_points_coop<2, 8, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+        else
+            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+    } else if (c->dim == 3) {
+        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_closest_points<3, B, K, OUT>), grid_for(c, n, B), dim3(B), 0, s,
+                                                  S, io, c->xcds));
+    } else {
+        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_closest_points<2, B, K, OUT>), grid_for(c, n, B), dim3(B), 0, s,
+                                                  S, io, c->xcds));
+    }
     HIPCHECK(c, hipGetLastError());
     return FCPW_HIP_OK;
 }
 
+static int launch_closest_records(fcpw_hip_ctx *c, const void *spheres, int64_t n, int record_normals, void *out,
+                                  hipStream_t s)
+{
+    CpIO io = {};
+    io.in = (const float *)spheres;
+    io.n = n;
+    io.out = (float2 *)out;
+    io.record_normals = record_normals;
+    return launch_closest<CP_OUT_INTERACTION>(c, io, s);
+}
+
+extern "C" {
+
 static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int occlusion, void *out, hipStream_t s)
 {
     DevScene S = dev_scene(c);
+    const uint32_t *order;
+    int st = morton_order(c, (const float *)rays, 10, n, s, &order); // ordered by origin
+    if (st) return st;
     if (c->dim == 3)
-        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_intersect<3, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
-                                            (const float2 *)rays, n, occlusion, (float2 *)out));
+        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<3, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+                                            (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds));
     else
-        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_intersect<2, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
-                                            (const float2 *)rays, n, occlusion, (float2 *)out));
+        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<2, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+                                            (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds));
     HIPCHECK(c, hipGetLastError());
     return FCPW_HIP_OK;
 }
@@ -1044,7 +1425,7 @@ int fcpw_hip_closest_points_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphe
     if (((uintptr_t)spheres & 15) || ((uintptr_t)out & 7))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: spheres must be 16-byte and out 8-byte aligned");
     if (n == 0) return FCPW_HIP_OK;
-    return launch_closest(c, spheres, n, record_normals, out, (hipStream_t)stream);
+    return launch_closest_records(c, spheres, n, record_normals, out, (hipStream_t)stream);
 }
 
 int fcpw_hip_closest_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n, int record_normals,
@@ -1060,7 +1441,7 @@ int fcpw_hip_closest_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *sph
     st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
     if (st) return st;
     HIPCHECK(c, hipMemcpy(c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), hipMemcpyHostToDevice));
-    st = launch_closest(c, c->stage_in, n, record_normals, c->stage_out, 0);
+    st = launch_closest_records(c, c->stage_in, n, record_normals, c->stage_out, 0);
     if (st) return st;
     HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
     return check_overflow(c);
@@ -1117,16 +1498,15 @@ int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *c, const float *points,
     if (st) return st;
     if (n < 0 || (n > 0 && !points)) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_compact_device: bad arguments");
     if (n == 0) return FCPW_HIP_OK;
-    DevScene S = dev_scene(c);
-    hipStream_t s = (hipStream_t)stream;
-    if (c->dim == 3)
-        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points_compact<3, B, K>), grid_for(c, n, B), dim3(B), 0, s,
-                                            S, points, r2, n, d, p, prim, slot));
-    else
-        FCPW_DISPATCH(c, hipLaunchKernelGGL((k_closest_points_compact<2, B, K>), grid_for(c, n, B), dim3(B), 0, s,
-                                            S, points, r2, n, d, p, prim, slot));
-    HIPCHECK(c, hipGetLastError());
-    return FCPW_HIP_OK;
+    CpIO io = {};
+    io.in = points;
+    io.r2 = r2;
+    io.n = n;
+    io.d = d;
+    io.p = p;
+    io.prim = prim;
+    io.slot = slot;
+    return launch_closest<CP_OUT_COMPACT>(c, io, (hipStream_t)stream);
 }
 
 int fcpw_hip_distance_backward_device(fcpw_hip_ctx *c, const float *points, const int32_t *slot, const float *grad_d,
@@ -1163,9 +1543,15 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "sort") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sort must be 0 or 1");
         c->sort = value;
-    } else if (k == "persistent") {
-        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "persistent must be 0 or 1");
-        c->persistent = value;
+    } else if (k == "coop") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "coop must be 0 or 1");
+        c->coop = value;
+    } else if (k == "hold_min") {
+        if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "hold_min must be in [1, 64]");
+        c->hold_min = value;
+    } else if (k == "xcds") {
+        if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcds must be in [1, 64]");
+        c->xcds = value;
     } else {
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "unknown option '%s'", key);
     }
@@ -1179,7 +1565,9 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     if (k == "block") *value = c->block;
     else if (k == "stack") *value = c->stack;
     else if (k == "sort") *value = c->sort;
-    else if (k == "persistent") *value = c->persistent;
+    else if (k == "coop") *value = c->coop;
+    else if (k == "xcds") *value = c->xcds;
+    else if (k == "hold_min") *value = c->hold_min;
     else return FCPW_HIP_ERR_INVALID_ARGUMENT;
     return FCPW_HIP_OK;
 }

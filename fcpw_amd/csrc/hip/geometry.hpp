@@ -22,18 +22,18 @@ struct v2 {
     float x, y;
 };
 
-__device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ v2 sub(v2 a, v2 b) { return {a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ float dot(v2 a, v2 b) { return a.x * b.x + a.y * b.y; }
-__device__ __forceinline__ v3 cross(v3 a, v3 b)
+__host__ __device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ __forceinline__ v2 sub(v2 a, v2 b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__host__ __device__ __forceinline__ float dot(v2 a, v2 b) { return a.x * b.x + a.y * b.y; }
+__host__ __device__ __forceinline__ v3 cross(v3 a, v3 b)
 {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
 // BoundingBox.overlap (bounding-volumes.slang:43-53): squared distance to the
 // box (0 inside) and to its farthest corner
-__device__ __forceinline__ void box_sqdist(v3 lo, v3 hi, v3 c, float &d2min, float &d2max)
+__host__ __device__ __forceinline__ void box_sqdist(v3 lo, v3 hi, v3 c, float &d2min, float &d2max)
 {
     v3 u = sub(lo, c), v = sub(c, hi);
     v3 a = {fmaxf(fmaxf(u.x, v.x), 0.0f), fmaxf(fmaxf(u.y, v.y), 0.0f), fmaxf(fmaxf(u.z, v.z), 0.0f)};
@@ -41,7 +41,7 @@ __device__ __forceinline__ void box_sqdist(v3 lo, v3 hi, v3 c, float &d2min, flo
     d2min = dot(a, a);
     d2max = dot(b, b);
 }
-__device__ __forceinline__ void box_sqdist(v2 lo, v2 hi, v2 c, float &d2min, float &d2max)
+__host__ __device__ __forceinline__ void box_sqdist(v2 lo, v2 hi, v2 c, float &d2min, float &d2max)
 {
     v2 u = sub(lo, c), v = sub(c, hi);
     v2 a = {fmaxf(fmaxf(u.x, v.x), 0.0f), fmaxf(fmaxf(u.y, v.y), 0.0f)};
@@ -52,7 +52,7 @@ __device__ __forceinline__ void box_sqdist(v2 lo, v2 hi, v2 c, float &d2min, flo
 
 // BoundingBox.intersect (bounding-volumes.slang:67-106), slab test with the
 // parallel-slab fix; a 2D ray has an unconstrained z slab, so 2D skips z
-__device__ __forceinline__ bool slab1(float lo, float hi, float o, float d, float dinv, float &tn, float &tf)
+__host__ __device__ __forceinline__ bool slab1(float lo, float hi, float o, float d, float dinv, float &tn, float &tf)
 {
     float t0 = (lo - o) * dinv;
     float t1 = (hi - o) * dinv;
@@ -62,7 +62,7 @@ __device__ __forceinline__ bool slab1(float lo, float hi, float o, float d, floa
     tf = parallel ? FLT_MAX : fmaxf(t0, t1);
     return !(parallel && !within);
 }
-__device__ __forceinline__ bool box_ray(v3 lo, v3 hi, v3 o, v3 d, v3 dinv, float tmax, float &tmin_out)
+__host__ __device__ __forceinline__ bool box_ray(v3 lo, v3 hi, v3 o, v3 d, v3 dinv, float tmax, float &tmin_out)
 {
     float n0, f0, n1, f1, n2, f2;
     bool ok = slab1(lo.x, hi.x, o.x, d.x, dinv.x, n0, f0);
@@ -73,7 +73,7 @@ __device__ __forceinline__ bool box_ray(v3 lo, v3 hi, v3 o, v3 d, v3 dinv, float
     tmin_out = tNearMax;
     return ok && !(tNearMax > tFarMin);
 }
-__device__ __forceinline__ bool box_ray(v2 lo, v2 hi, v2 o, v2 d, v2 dinv, float tmax, float &tmin_out)
+__host__ __device__ __forceinline__ bool box_ray(v2 lo, v2 hi, v2 o, v2 d, v2 dinv, float tmax, float &tmin_out)
 {
     float n0, f0, n1, f1;
     bool ok = slab1(lo.x, hi.x, o.x, d.x, dinv.x, n0, f0);
@@ -87,7 +87,7 @@ __device__ __forceinline__ bool box_ray(v2 lo, v2 hi, v2 o, v2 d, v2 dinv, float
 
 // findClosestPointTriangle (geometry.slang:251-332): Ericson's region test;
 // t = barycentric weights of (pa, pb)
-__device__ __forceinline__ float closest_point_triangle(v3 pa, v3 pb, v3 pc, v3 x, v3 &p, float &t0, float &t1)
+__host__ __device__ __forceinline__ float closest_point_triangle(v3 pa, v3 pb, v3 pc, v3 x, v3 &p, float &t0, float &t1)
 {
     v3 ab = sub(pb, pa), ac = sub(pc, pa), ax = sub(x, pa);
     float d1 = dot(ab, ax), d2 = dot(ac, ax);
@@ -143,8 +143,51 @@ __device__ __forceinline__ float closest_point_triangle(v3 pa, v3 pb, v3 pc, v3 
     return sqrtf(dot(r, r));
 }
 
+
+// The same region test without divergent branches: every region's formula
+// needs at most one IEEE division, so the (numerator, denominator) pair of the
+// region is selected and divided once; p and t are then assembled with selects.
+// Per region the arithmetic is the reference's (p = base + e1*s1 [+ e2*s2]), so
+// the result equals closest_point_triangle bit for bit (up to the sign of a zero
+// coordinate of p, which does not change any distance).
+__host__ __device__ __forceinline__ float closest_point_triangle_sel(v3 pa, v3 pb, v3 pc, v3 x, v3 &p, float &t0, float &t1)
+{
+    v3 ab = sub(pb, pa), ac = sub(pc, pa), ax = sub(x, pa);
+    float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    v3 bx = sub(x, pb);
+    float d3 = dot(ab, bx), d4 = dot(ac, bx);
+    v3 cx = sub(x, pc);
+    float d5 = dot(ab, cx), d6 = dot(ac, cx);
+    float vc = d1 * d4 - d3 * d2;
+    float vb = d5 * d2 - d1 * d6;
+    float va = d3 * d6 - d5 * d4;
+    float e43 = d4 - d3, e56 = d5 - d6;
+    bool rA = d1 <= 0.0f && d2 <= 0.0f;
+    bool rB = !rA && d3 >= 0.0f && d4 <= d3;
+    bool rC = !rA && !rB && d6 >= 0.0f && d5 <= d6;
+    bool vtx = rA || rB || rC;
+    bool rAB = !vtx && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f;
+    bool rAC = !vtx && !rAB && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
+    bool rBC = !vtx && !rAB && !rAC && va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f;
+    bool face = !vtx && !rAB && !rAC && !rBC;
+    float num = rAB ? d1 : (rAC ? d2 : (rBC ? e43 : 1.0f));
+    float den = rAB ? (d1 - d3) : (rAC ? (d2 - d6) : (rBC ? (e43 + e56) : (face ? (va + vb + vc) : 1.0f)));
+    float q = num / den;
+    float v = face ? vb * q : q;
+    float w = face ? vc * q : 0.0f;
+    v3 base = rB ? pb : (rC ? pc : (rBC ? pb : pa));
+    v3 bc = sub(pc, pb);
+    v3 e1 = rAC ? ac : (rBC ? bc : ab);
+    float s1 = vtx ? 0.0f : v;
+    p = {(base.x + e1.x * s1) + ac.x * w, (base.y + e1.y * s1) + ac.y * w, (base.z + e1.z * s1) + ac.z * w};
+    t0 = rA ? 1.0f : ((rB || rC || rBC) ? 0.0f : (rAB ? 1.0f - q : (rAC ? 1.0f - q : (1.0f - v) - w)));
+    t1 = rB ? 1.0f : ((rA || rC || rAC) ? 0.0f : (rAB ? q : (rBC ? 1.0f - q : v)));
+    v3 r = sub(x, p);
+    return sqrtf(dot(r, r));
+}
+
 // findClosestPointLineSegment (geometry.slang:79-103)
-__device__ __forceinline__ float closest_point_segment(v2 pa, v2 pb, v2 x, v2 &p, float &t)
+__host__ __device__ __forceinline__ float closest_point_segment(v2 pa, v2 pb, v2 x, v2 &p, float &t)
 {
     v2 u = sub(pb, pa), v = sub(x, pa);
     float c1 = dot(u, v);
@@ -164,7 +207,7 @@ __device__ __forceinline__ float closest_point_segment(v2 pa, v2 pb, v2 x, v2 &p
 }
 
 // intersectTriangle (geometry.slang:202-249), Moller-Trumbore
-__device__ __forceinline__ bool ray_triangle(v3 pa, v3 pb, v3 pc, v3 o, v3 d, float tmax, float &t_out, float &v_out,
+__host__ __device__ __forceinline__ bool ray_triangle(v3 pa, v3 pb, v3 pc, v3 o, v3 d, float tmax, float &t_out, float &v_out,
                                              float &w_out)
 {
     v3 v1 = sub(pb, pa), v2 = sub(pc, pa);
@@ -187,7 +230,7 @@ __device__ __forceinline__ bool ray_triangle(v3 pa, v3 pb, v3 pc, v3 o, v3 d, fl
 }
 
 // intersectLineSegment (geometry.slang:37-77)
-__device__ __forceinline__ bool ray_segment(v2 pa, v2 pb, v2 o, v2 d, float tmax, float &t_out, float &s_out)
+__host__ __device__ __forceinline__ bool ray_segment(v2 pa, v2 pb, v2 o, v2 d, float tmax, float &t_out, float &s_out)
 {
     v2 u = sub(pa, o), v = sub(pb, pa);
     float dv = d.x * v.y - d.y * v.x;

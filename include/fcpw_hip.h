@@ -156,11 +156,16 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, co
                                       const float *grad_d, int64_t n, float *grad_points, float *grad_vertices,
                                       void *stream);
 
-/* Tuning knobs read by the kernels (values chosen by fcpw_amd.autotune):
-     "block"        threads per block for the query kernels (64, 128, 256)
-     "stack"        LDS stack entries per lane (8, 12, 16, 24)
-     "sort"         1 = reorder queries along a Morton curve before traversal
-     "persistent"   1 = persistent waves refilled from a work counter
+/* Tuning knobs read by the kernels (values chosen by fcpw_amd.autotune).  None
+   of them changes a result bit: every setting visits nodes in the same order.
+     "coop"      1 = wave-cooperative leaf processing (default), 0 = one query per
+                 lane with the leaf loop inline
+     "hold_min"  coop: lanes that must hold a leaf before the wave processes
+                 leaves together (1..64, default 2)
+     "stack"     LDS stack entries per lane (coop: 8 or 16; else 8, 16, 24)
+     "block"     threads per block when coop = 0 (64, 128, 256)
+     "sort"      1 = order queries along a Morton curve before traversal (default)
+     "xcds"      number of XCDs the block -> tile mapping groups by (1..64, default 8)
    Unknown keys or values return FCPW_HIP_ERR_INVALID_ARGUMENT. */
 int fcpw_hip_set_option(fcpw_hip_ctx *ctx, const char *key, int value);
 int fcpw_hip_get_option(const fcpw_hip_ctx *ctx, const char *key, int *value);

@@ -201,23 +201,60 @@ def test_empty_query_batch(gpu, cache):
 
 
 def test_options_do_not_change_results(gpu, cache):
+    """every tuning choice gives the same bits (the traversal order never changes)"""
     V, F, scene, g, ob = get(cache, "bunny")
     q = meshes.queries_around(V, 20000, seed=5)
     r2 = np.full(len(q), np.inf, np.float32)
+    configs = [dict(coop=0, block=b, stack=k, sort=srt) for b in (64, 128, 256) for k in (8, 16, 24) for srt in (0, 1)]
+    configs += [dict(coop=1, stack=k, hold_min=h, sort=srt, xcds=x) for k in (8, 16) for h in (1, 2, 8, 64)
+                for srt in (0, 1) for x in (1, 8)]
     base = None
-    for block in (64, 128, 256):
-        for stack in (8, 16, 24):
-            g.set_option("block", block)
-            g.set_option("stack", stack)
-            hits = fcpw.GPUInteractionList()
-            g.find_closest_points(q, r2, hits)
-            rec = as_records(hits).copy()
-            if base is None:
-                base = rec
-            else:
-                np.testing.assert_array_equal(rec, base)
-    g.set_option("block", 128)
-    g.set_option("stack", 16)
+    for cfg in configs:
+        for key, val in cfg.items():
+            g.set_option(key, val)
+        hits = fcpw.GPUInteractionList()
+        g.find_closest_points(q, r2, hits)
+        rec = as_records(hits).copy()
+        if base is None:
+            base = rec
+            assert_bit_equal(rec, ob.closest_points_gpuorder(q, r2))
+        else:
+            np.testing.assert_array_equal(rec, base, err_msg=str(cfg))
+    for key, val in dict(coop=1, stack=16, hold_min=2, sort=1, xcds=8, block=128).items():
+        g.set_option(key, val)
+
+
+@pytest.mark.parametrize("n", [4097, 4096 + 64 * 8 - 1, 50001, 64 * 8 * 13 + 65])
+def test_every_query_of_a_ragged_batch_is_written(gpu, cache, n):
+    """batch sizes that leave a partial group of workgroups after the XCD
+    remap: every output slot is written, by every kernel family"""
+    import torch
+    V, F, scene, g, ob = get(cache, "icosphere")
+    q = meshes.queries_around(V, n, seed=11)
+    sph = np.zeros((n, 4), np.float32)
+    sph[:, :3] = q
+    sph[:, 3] = np.inf
+    ref = ob.closest_points_gpuorder(q)
+    stream = torch.cuda.current_stream().cuda_stream
+    for cfg in (dict(coop=1, sort=1), dict(coop=1, sort=0), dict(coop=0, sort=1, block=128),
+                dict(coop=0, sort=0, block=256)):
+        for key, val in cfg.items():
+            g.set_option(key, val)
+        out = torch.full((n, 10), -7.0, dtype=torch.float32, device="cuda")
+        g.find_closest_points_device(torch.from_numpy(sph).cuda(), out)
+        d = torch.full((n,), -7.0, device="cuda")
+        x = torch.from_numpy(q).cuda()
+        g._closest_points_compact_device(x.data_ptr(), 0, n, d.data_ptr(), 0, 0, 0, stream)
+        torch.cuda.synchronize()
+        rec = out.cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel()
+        assert_bit_equal(rec, ref)
+        np.testing.assert_array_equal(d.cpu().numpy(), ref["d"], err_msg=str(cfg))
+    for key, val in dict(coop=1, stack=16, hold_min=2, sort=1, xcds=8, block=128).items():
+        g.set_option(key, val)
+    o, dr = meshes.rays_from(V, n, seed=12)
+    hits = fcpw.GPUInteractionList()
+    g.intersect(o, dr, np.full(n, np.inf, np.float32), hits)
+    assert_bit_equal(as_records(hits), ob.intersect_gpuorder(o, dr))
 
 
 def test_device_entry_points_match_host(gpu, cache):
