@@ -8,18 +8,27 @@ Queries are resident in HBM as GPUBoundingSphere records before the timed
 region; results are written as GPUInteraction records (the drop-in layout).
 
 The printed JSON line carries
-  value        queries/s of the forward closest-point query (whole job, 1 GPU)
-  roofline     algorithmic bytes per query (from the reference GPU traversal's
-               node and primitive counts, counted by the oracle on a 65,536-query
-               sample) x queries/s against MI355X HBM bandwidth; kernel time from
-               HIP events on the launch stream
-  cpu_baseline the reference CPU path (oracle restatement of Scene<3>::
-               findClosestPoints, std::thread-style static partition) timed on
-               this host's cores over a bounded sample of the same queries
+  value        forward closest-point queries/s (whole job, one GPU)
+  fwd_bwd      the differentiable step on the same queries: compact forward
+               (d, p, primitive, leaf slot) + the adjoint kernel writing dL/dx and
+               dL/dV (dL/dd = 1); queries/s and its ratio to the forward rate
+  rays         16,777,216 rays (origins uniform in the AABB, directions uniform
+               on S^2, tMax = inf) through the ray-intersection kernel
+  roofline     algorithmic bytes per query x queries per launch / the traversal
+               kernel's mean launch duration (HIP events on the launch stream),
+               against MI355X HBM bandwidth.  Bytes per query
+               B = 16 + 40 + 32 (1 + 2 I) + 40 P, with I (records expanded) and P
+               (primitives tested) counted by the kernel's own counting variant
+               over the whole batch.  traffic = measured HBM bytes per launch from
+               the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (null if absent)
+  cpu_baseline the reference CPU path (restatement of Scene<3>::findClosestPoints,
+               binary SAH BVH, static thread partition) timed on this host's cores
+               over a bounded sample of the same queries
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "closest-point queries/s, 1M-tri mesh, fwd and fwd+bwd; % HBM roofline vs CPU"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "closest_points_coop__torus_1M_tris_16M_queries__traffic.txt")
 
 
 def log(*a):
@@ -45,9 +55,17 @@ def parse():
     p.add_argument("--grid", type=int, nargs=2, default=[1024, 512], help="torus U V (2UV triangles)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=1 << 19, help="queries timed on the CPU path")
-    p.add_argument("--block", type=int, default=None)
-    p.add_argument("--stack", type=int, default=None)
-    p.add_argument("--sort", type=int, default=None)
+    p.add_argument("--no-extra", action="store_true", help="forward only (skip fwd+bwd and rays)")
+    p.add_argument("--calib-copy-mib", type=int, default=0,
+                   help="after the timed work, copy a buffer of this size 3x (counter calibration)")
+    for k in ("block", "stack", "sort", "coop", "hold_min", "xcds"):
+        p.add_argument("--" + k.replace("_", "-"), type=int, default=None)
+    p.add_argument("--summarize-kernels", metavar="DB", help="write a kernel-trace summary and exit")
+    p.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                   help="write the HBM traffic summary from two counter runs and exit")
+    p.add_argument("--calib-bytes", type=int, default=0, help="bytes of the calibration copy (summaries)")
+    p.add_argument("--command", action="append", default=[], help="command lines recorded in a summary")
+    p.add_argument("--out", default=None, help="summary file")
     return p.parse_args()
 
 
@@ -58,16 +76,25 @@ def algorithmic_bytes(n_internal, n_prims, sphere_in=16, record_out=40, node_byt
     return sphere_in + record_out + node_bytes * (1.0 + 2.0 * n_internal) + prim_bytes * n_prims
 
 
-def cpu_baseline(V, F, q, r2, sample, counters_sample=65536):
+def read_traffic():
+    """HBM bytes per launch of the forward kernel from the committed rocprofv3 summary."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                m = re.match(r"\s*traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", line)
+                if m:
+                    return float(m.group(1))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(V, F, q, r2, sample):
     """Reference CPU path on this host (oracle = restatement of bvh.inl)."""
     import oracle
     t0 = time.time()
     ob = oracle.Bvh(V, F)
     log(f"[cpu] oracle BVH built in {time.time() - t0:.1f}s")
-    # node/primitive counts of the reference GPU traversal order, for the byte model
-    idx = np.random.default_rng(2024).choice(len(q), size=min(counters_sample, len(q)), replace=False)
-    _, ni, npr, _ = ob.closest_points_gpuorder(q[idx], r2[idx], counters=True)
-    I, P = float(ni.mean()), float(npr.mean())
     cores = os.cpu_count() or 1
     n = min(sample, len(q))
     qs, rs = q[:n], r2[:n]
@@ -79,11 +106,107 @@ def cpu_baseline(V, F, q, r2, sample, counters_sample=65536):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     return {"value": n / t, "unit": "queries/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} of the same queries, median of 3 runs, {cores} threads, BVH build excluded"}, I, P
+            "sample": f"first {n} of the same queries, median of 3 runs, {cores} threads, BVH build excluded"}
+
+
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries (no GPU needed): turn the SQLite output of a kernel-trace
+# run or of FETCH_SIZE / WRITE_SIZE counter runs into the text kept in profiles/
+# ---------------------------------------------------------------------------
+MAIN_KERNEL = "k_closest_points_coop<3, 16, 0, false>"
+
+
+def _db(path):
+    import glob
+    import sqlite3
+    if os.path.isdir(path):
+        found = sorted(glob.glob(os.path.join(path, "**", "*.db"), recursive=True))
+        if not found:
+            raise SystemExit(f"no rocprofv3 database under {path}")
+        path = found[0]
+    return sqlite3.connect(path), path
+
+
+def summarize_kernels(db, command, out):
+    con, path = _db(db)
+    rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
+                       "from kernels group by name order by sum(duration) desc").fetchall()
+    total = sum(r[5] for r in rows) or 1
+    lines = [f"# rocprofv3 kernel trace summary ({os.path.basename(path)})", f"# command: {command}", "",
+             f"{'calls':>6} {'avg_us':>12} {'min_us':>12} {'max_us':>12} {'total_%':>8}  kernel"]
+    for name, n, avg, mn, mx, tot in rows:
+        lines.append(f"{n:6d} {avg / 1e3:12.3f} {mn / 1e3:12.3f} {mx / 1e3:12.3f} {100 * tot / total:8.2f}  {name}")
+    _emit(lines, out)
+
+
+def _counter(con, counter, pattern, pick_largest_grid=False):
+    rows = con.execute("select dispatch_id, kernel_name, grid_size, sum(value) from counters_collection "
+                       "where counter_name = ? group by dispatch_id", (counter,)).fetchall()
+    rows = [r for r in rows if re.search(pattern, r[1])]
+    if pick_largest_grid and rows:
+        g = max(r[2] for r in rows)
+        rows = [r for r in rows if r[2] == g]
+    if not rows:
+        raise SystemExit(f"no dispatch of /{pattern}/ with counter {counter}")
+    return float(np.mean([r[3] for r in rows])), len(rows), rows[0][1]
+
+
+def summarize_traffic(fetch_db, write_db, calib_bytes, commands, out):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the forward kernel, each
+    scaled by calib_bytes / (the counter's reading for a copy of calib_bytes)."""
+    fcon, fpath = _db(fetch_db)
+    wcon, wpath = _db(write_db)
+    pat = re.escape(MAIN_KERNEL)
+    calib = r"copyBuffer|neg_kernel|direct_copy_kernel"  # read N bytes, write N bytes
+    kf, nf, name = _counter(fcon, "FETCH_SIZE", pat)
+    kw, nw, _ = _counter(wcon, "WRITE_SIZE", pat)
+    cf, ncf, cname = _counter(fcon, "FETCH_SIZE", calib, pick_largest_grid=True)
+    cw, ncw, _ = _counter(wcon, "WRITE_SIZE", calib, pick_largest_grid=True)
+    ff, fw = calib_bytes / (cf * 1024.0), calib_bytes / (cw * 1024.0)
+    fetch_b, write_b = kf * 1024.0 * ff, kw * 1024.0 * fw
+    lines = ["# HBM traffic per launch of the forward closest-point kernel",
+             "# workload: torus 1,048,576 tris, 16,777,216 queries (bench.py defaults)"]
+    lines += [f"# command: {c}" for c in commands]
+    lines += ["", f"kernel: {name}",
+              f"dispatches: {nf} (FETCH_SIZE pass), {nw} (WRITE_SIZE pass)",
+              f"FETCH_SIZE_kib_per_dispatch: {kf:.1f}",
+              f"WRITE_SIZE_kib_per_dispatch: {kw:.1f}",
+              f"calibration: {cname[:90]}, {ncf}+{ncw} dispatches copying {calib_bytes} bytes",
+              f"calibration_FETCH_SIZE_kib: {cf:.1f}  -> fetch factor {ff:.4f}",
+              f"calibration_WRITE_SIZE_kib: {cw:.1f}  -> write factor {fw:.4f}",
+              f"fetch_bytes_per_launch: {fetch_b:.4e}",
+              f"write_bytes_per_launch: {write_b:.4e}",
+              f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}"]
+    _emit(lines, out)
+
+
+def _emit(lines, out):
+    text = "\n".join(lines) + "\n"
+    if out:
+        with open(out, "w") as fh:
+            fh.write(text)
+    print(text, end="")
+
+
+def timed(torch, stream, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
 def main():
     args = parse()
+    if args.summarize_kernels:
+        summarize_kernels(args.summarize_kernels, " && ".join(args.command), args.out)
+        return
+    if args.summarize_traffic:
+        summarize_traffic(*args.summarize_traffic, args.calib_bytes, args.command, args.out)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
@@ -105,7 +228,8 @@ def main():
     q = synth.f32(np.concatenate([synth.uniform_in_box(n_u, lo, hi, 1.2, rq),
                                   synth.near_surface(args.queries - n_u, v, f, 0.05, rq)]))
     r2 = np.full(len(q), np.inf, np.float32)
-    log(f"[setup] {len(F)} triangles, {len(q)} queries generated in {time.time() - t0:.1f}s")
+    nq = len(q)
+    log(f"[setup] {len(F)} triangles, {nq} queries generated in {time.time() - t0:.1f}s")
 
     t0 = time.time()
     scene = fcpw.Scene3D()
@@ -115,70 +239,124 @@ def main():
     scene.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
     gpu = fcpw.GPUScene3D(fcpw.slang_directory_path)
     gpu.transfer_to_gpu(scene)
-    for k, val in (("block", args.block), ("stack", args.stack), ("sort", args.sort)):
+    for k in ("block", "stack", "sort", "coop", "hold_min", "xcds"):
+        val = getattr(args, k)
         if val is not None:
             gpu.set_option(k, val)
+    opts = {k: gpu.get_option(k) for k in ("coop", "stack", "hold_min", "sort", "xcds", "block")}
     log(f"[setup] BVH built + uploaded in {time.time() - t0:.1f}s: {gpu.info('n_records')} records, "
-        f"depth {gpu.info('max_depth')}, block {gpu.get_option('block')}, stack {gpu.get_option('stack')}")
+        f"depth {gpu.info('max_depth')}, options {opts}")
 
-    sph = np.empty((len(q), 4), np.float32)
-    sph[:, :3] = q
-    sph[:, 3] = r2
-    d_in = torch.from_numpy(sph).to(dev)
-    d_out = torch.empty((len(q), 10), dtype=torch.float32, device=dev)
+    sph = torch.empty((nq, 4), dtype=torch.float32, device=dev)
+    sph[:, :3] = torch.from_numpy(q).to(dev)
+    sph[:, 3] = float("inf")
+    d_out = torch.empty((nq, 10), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    def step():
-        gpu.find_closest_points_device(d_in, d_out, stream=stream)
+    def fwd():
+        gpu.find_closest_points_device(sph, d_out, stream=stream)
 
+    # ---- forward: the timed region (device-resident inputs, one launch per step)
     for _ in range(args.warmup):
-        step()
+        fwd()
     torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    gpu.set_option("timing", 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
+    for _ in range(args.steps):
+        fwd()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    kernel_s = gpu.info("kernel_ns") / 1e9 / gpu.info("kernel_launches")
+    gpu.set_option("timing", 0)
     ms_step = wall * 1e3 / args.steps
-    qps = len(q) / (wall / args.steps)
-    log(f"[gpu] {ms_step:.3f} ms/step wall, kernel median {np.median(kernel_ms):.3f} ms, {qps:.3e} queries/s")
+    qps = nq / (wall / args.steps)
+    log(f"[fwd] {ms_step:.3f} ms/step wall, traversal kernel {kernel_s * 1e3:.3f} ms, {qps:.4e} queries/s")
 
-    # spot check against the oracle's GPU-order restatement on a sample
-    out = d_out[:4096].cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel()
-    import oracle  # checker only
-    ref = oracle.Bvh(V, F).closest_points_gpuorder(q[:4096], r2[:4096]) if args.no_cpu_baseline else None
-
-    cpu, I, P = (None, None, None)
-    if not args.no_cpu_baseline:
-        cpu, I, P = cpu_baseline(V, F, q, r2, args.cpu_sample)
-        log(f"[cpu] {cpu['value']:.3e} queries/s on {cpu['cores']} threads; GPU-order counts I={I:.1f} P={P:.1f}")
-    if ref is not None:
-        assert np.array_equal(out["d"], ref["d"]), "GPU result differs from the oracle"
-
-    roofline = None
-    if I is not None:
-        B = algorithmic_bytes(I, P)
-        kern_s = float(np.median(kernel_ms)) / 1e3
-        achieved = B * len(q) / kern_s / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "bytes_per_query": round(B, 1), "internal_nodes_per_query": round(I, 2),
-                    "prim_tests_per_query": round(P, 2), "kernel_ms": round(float(np.median(kernel_ms)), 4)}
+    # ---- traversal counts for the byte model (counting variant, same order)
+    gpu.set_option("stats", 1)
+    fwd()
+    I = gpu.info("stat_internal_nodes") / gpu.info("stat_queries")
+    P = gpu.info("stat_prim_tests") / gpu.info("stat_queries")
+    gpu.set_option("stats", 0)
+    B = algorithmic_bytes(I, P)
+    achieved = B * nq / kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                "bytes_per_query": round(B, 1), "internal_nodes_per_query": round(I, 3),
+                "prim_tests_per_query": round(P, 3), "kernel_ms": round(kernel_s * 1e3, 4),
+                "queries_per_launch": nq}
+    log(f"[fwd] I={I:.2f} P={P:.2f} B={B:.0f} B/query -> {achieved:.0f} GB/s algorithmic")
 
     res = {"metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"torus {len(F)} tris, {len(q)} closest-point queries (half uniform in 1.2x AABB, "
-                                  f"half within 0.05 of the surface), r2=inf, fwd",
-                      "triangles": len(F), "queries": len(q), "block": gpu.get_option("block"),
-                      "stack": gpu.get_option("stack")},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "config": {"workload": f"torus {len(F)} tris, {nq} closest-point queries (half uniform in 1.2x AABB, "
+                                  f"half within 0.05 of the surface), r2=inf, forward",
+                      "triangles": len(F), "queries": nq, **opts}}
+
+    if not args.no_extra:
+        # ---- forward + backward (compact forward, adjoint kernel for dL/dx and dL/dV)
+        x = sph[:, :3].contiguous()
+        d = torch.empty(nq, dtype=torch.float32, device=dev)
+        p = torch.empty((nq, 3), dtype=torch.float32, device=dev)
+        prim = torch.empty(nq, dtype=torch.int32, device=dev)
+        slot = torch.empty(nq, dtype=torch.int32, device=dev)
+        gd = torch.ones(nq, dtype=torch.float32, device=dev)
+        gx = torch.empty_like(x)
+        gV = torch.zeros((len(V), 3), dtype=torch.float32, device=dev)
+        h = stream.cuda_stream
+
+        def fwd_bwd():
+            gpu._closest_points_compact_device(x.data_ptr(), 0, nq, d.data_ptr(), p.data_ptr(), prim.data_ptr(),
+                                               slot.data_ptr(), h)
+            gV.zero_()
+            gpu._distance_backward_device(x.data_ptr(), slot.data_ptr(), gd.data_ptr(), nq, gx.data_ptr(),
+                                          gV.data_ptr(), h)
+        t_fb = timed(torch, stream, fwd_bwd, args.steps, args.warmup)
+        res["fwd_bwd"] = {"value": nq / t_fb, "unit": "queries/s", "ms_per_step": t_fb * 1e3,
+                          "ratio_to_fwd": t_fb * 1e3 / ms_step}
+        log(f"[fwd+bwd] {t_fb * 1e3:.3f} ms/step, {nq / t_fb:.4e} queries/s ({t_fb * 1e3 / ms_step:.2f}x fwd)")
+        # same closest points as the record-layout forward
+        rec = d_out.view(-1)[: 4096 * 10].view(4096, 10)
+        assert torch.equal(rec[:, 8], d[:4096]), "compact and record forward passes differ"
+        del x, d, p, prim, slot, gd, gx, gV
+
+        # ---- rays
+        rr = synth.Rng(synth.seed(2, 3))
+        o = synth.f32(synth.uniform_in_box(nq, lo, hi, 1.0, rr))
+        dr = synth.f32(rr.sphere(nq, 3))
+        rays = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+        rays[:, 0:3] = torch.from_numpy(o).to(dev)
+        rays[:, 3:6] = torch.from_numpy(dr).to(dev)
+        rays[:, 6:9] = 1.0 / rays[:, 3:6]
+        rays[:, 9] = float("inf")
+        ray_out = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+        del sph
+
+        def ray_step():
+            gpu.intersect_device(rays, ray_out, stream=stream)
+        t_r = timed(torch, stream, ray_step, args.steps, args.warmup)
+        res["rays"] = {"value": nq / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
+                       "workload": f"{nq} rays, origins uniform in the AABB, directions uniform on S^2, tMax=inf"}
+        log(f"[rays] {t_r * 1e3:.3f} ms/step, {nq / t_r:.4e} rays/s")
+        del rays, ray_out
+
+    if args.calib_copy_mib > 0:
+        nbytes = args.calib_copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log(f"[calib] copied {nbytes} bytes 3x")
+
+    res["roofline"] = roofline
+    res["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(V, F, q, r2, args.cpu_sample)
+        log(f"[cpu] {res['cpu_baseline']['value']:.3e} queries/s on {res['cpu_baseline']['cores']} threads")
     print(json.dumps(res), flush=True)
 
 

@@ -50,7 +50,8 @@ struct DevScene {
     const float4 *root;
     const float4 *prims;
     const int32_t *prim_index;
-    int *flags; // bit 0: traversal stack overflow
+    int *flags;                // bit 0: traversal stack overflow
+    unsigned long long *stats; // traversal counters: internal nodes, primitive tests, queries
 };
 
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
@@ -64,33 +65,40 @@ __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f)
 // ---------------------------------------------------------------------------
 #define FCPW_STACK_DECL(BLOCK, K)                                                                              \
     __shared__ uint64_t lds_stack[(K) * (BLOCK)];                                                            \
-    uint64_t stack_spill[STACK_TOTAL - (K)];                                                                 \
+    uint32_t stack_spill[2 * (STACK_TOTAL - (K))];                                                           \
     const int stack_lane = threadIdx.x;                                                                       \
     int sp = 0
 
-// An entry packs (link, distance bits) into 64 bits.  The cold spill path uses
-// nontemporal accesses: besides keeping the scratch lines out of the caches,
-// that stops the compiler from merging the LDS and the spill access into one
-// generic (flat) access, which would send every LDS push and pop down the slower
-// vector-memory path.
+// An entry packs (link, distance bits) into 64 bits.  The cold spill path moves
+// the entry as two 32-bit halves: an access of a different type from the LDS
+// one cannot be merged with it into a single generic (flat) access, which would
+// send every LDS push and pop down the slower vector-memory path.  Spilled
+// entries are ordinary cached scratch accesses.
 #define FCPW_PUSH(BLOCK, K, LINK, D, ON_FULL)                                                                  \
     if (sp >= STACK_TOTAL) {                                                                                  \
         ON_FULL;                                                                                              \
     } else {                                                                                                  \
-        uint64_t e_ = (uint64_t)(LINK) | ((uint64_t)__float_as_uint(D) << 32);                                \
-        if (sp < (K)) lds_stack[sp * (BLOCK) + stack_lane] = e_;                                              \
-        else __builtin_nontemporal_store(e_, &stack_spill[sp - (K)]);                                         \
+        const uint32_t l_ = (uint32_t)(LINK), d_ = __float_as_uint(D);                                        \
+        if (sp < (K)) {                                                                                       \
+            lds_stack[sp * (BLOCK) + stack_lane] = (uint64_t)l_ | ((uint64_t)d_ << 32);                       \
+        } else {                                                                                              \
+            stack_spill[2 * (sp - (K))] = l_;                                                                 \
+            stack_spill[2 * (sp - (K)) + 1] = d_;                                                             \
+        }                                                                                                     \
         sp++;                                                                                                 \
     }
 
 #define FCPW_POP(BLOCK, K, LINK, D)                                                                            \
     do {                                                                                                      \
         sp--;                                                                                                 \
-        uint64_t e_;                                                                                          \
-        if (sp < (K)) e_ = lds_stack[sp * (BLOCK) + stack_lane];                                              \
-        else e_ = *(volatile uint64_t *)&stack_spill[sp - (K)];                                              \
-        (LINK) = (uint32_t)e_;                                                                                \
-        (D) = __uint_as_float((uint32_t)(e_ >> 32));                                                          \
+        if (sp < (K)) {                                                                                       \
+            const uint64_t e_ = lds_stack[sp * (BLOCK) + stack_lane];                                         \
+            (LINK) = (uint32_t)e_;                                                                            \
+            (D) = __uint_as_float((uint32_t)(e_ >> 32));                                                      \
+        } else {                                                                                              \
+            (LINK) = stack_spill[2 * (sp - (K))];                                                             \
+            (D) = __uint_as_float(stack_spill[2 * (sp - (K)) + 1]);                                           \
+        }                                                                                                     \
     } while (0)
 
 // ---------------------------------------------------------------------------
@@ -194,6 +202,7 @@ struct CpResult {
     float d;
     float p[3];
     float uv[2];
+    uint32_t n_internal, n_prims; // records expanded and primitives tested (coop kernel)
 };
 
 template <int DIM, int BLOCK, int K, bool SEL>
@@ -280,6 +289,7 @@ struct RayResult {
     float p[3];
     float n[3];
     float uv[2];
+    uint32_t n_internal, n_prims; // records expanded and primitives tested
 };
 
 template <int DIM>
@@ -347,6 +357,7 @@ __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::
     res.t = FLT_MAX;
     for (int i = 0; i < 3; i++) { res.p[i] = 0.0f; res.n[i] = 0.0f; }
     res.uv[0] = res.uv[1] = 0.0f;
+    res.n_internal = res.n_prims = 0;
 
     vec lo, hi;
     uint32_t cur;
@@ -362,6 +373,7 @@ __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::
                 uint32_t cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
                 bool stop = false;
                 for (uint32_t k = 0; k < cnt; k++) {
+                    res.n_prims++;
                     if (RayGeo<DIM>::hit(S, off + k, o, d, tmax, res, !occlusion)) {
                         res.slot = (int32_t)(off + k);
                         if (occlusion) { stop = true; break; }
@@ -373,6 +385,7 @@ __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::
                 vec loA, hiA, loB, hiB;
                 uint32_t lA, lB;
                 G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                res.n_internal++;
                 float ta, tb;
                 bool hA = box_ray(loA, hiA, o, d, dinv, tmax, ta);
                 bool hB = box_ray(loB, hiB, o, d, dinv, tmax, tb);
@@ -542,6 +555,7 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
     }
     bool live = active, holding = false, overflow = false;
     uint32_t loff = 0, lcnt = 0;
+    uint32_t n_internal = 0, n_prims = 0;
     int32_t best = -1;
     FCPW_STACK_DECL(BLOCK, K);
     __syncthreads();
@@ -565,6 +579,7 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
                 vec loA, hiA, loB, hiB;
                 uint32_t lA, lB;
                 G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                n_internal++;
                 float a0, a1, b0, b1;
                 box_sqdist(loA, hiA, x, a0, a1);
                 bool oA = a0 <= r2;
@@ -622,6 +637,7 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
                 const uint32_t idx[4] = {i0, i1, i2, i3};
                 for (uint32_t jj = 0; jj < 4; jj++) {
                     if (lcnt > j0 + jj) {
+                        n_prims++;
                         float d = coop_d[idx[jj]];
                         if (d * d <= r2) {
                             r2 = fminf(r2, d * d);
@@ -636,6 +652,8 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
         holding = false;
     }
     if (overflow) atomicOr(S.flags, 1);
+    res.n_internal = n_internal;
+    res.n_prims = n_prims;
     if (best >= 0) {
         vec p;
         float u0, u1;
@@ -647,7 +665,23 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
     }
 }
 
-template <int DIM, int K, int OUT>
+// Adds a wave's traversal counts to S.stats (called by every lane of the wave).
+__device__ __forceinline__ void wave_stats_add(const DevScene &S, uint32_t n_internal, uint32_t n_prims,
+                                               uint32_t n_queries)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        n_internal += __shfl_xor(n_internal, off, 64);
+        n_prims += __shfl_xor(n_prims, off, 64);
+        n_queries += __shfl_xor(n_queries, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&S.stats[0], (unsigned long long)n_internal);
+        atomicAdd(&S.stats[1], (unsigned long long)n_prims);
+        atomicAdd(&S.stats[2], (unsigned long long)n_queries);
+    }
+}
+
+template <int DIM, int K, int OUT, bool STATS>
 __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io, int xcds, int hold_min)
 {
     int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
@@ -658,14 +692,16 @@ __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io,
     CpResult r;
     closest_point_query_coop<DIM, K, false>(S, Geo<DIM>::load_point(q), r2, valid, hold_min, r);
     if (valid) cp_store<DIM, OUT>(S, io, i, r);
+    if (STATS) wave_stats_add(S, r.n_internal, r.n_prims, valid ? 1u : 0u);
 }
 
-template <int DIM, int BLOCK, int K>
+template <int DIM, int BLOCK, int K, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *rays, int64_t n, int occlusion,
                                                      float2 *out, const uint32_t *order, int xcds)
 {
     using vec = typename Geo<DIM>::vec;
     int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    uint32_t n_internal = 0, n_prims = 0;
     if (k < n) {
         int64_t i = order ? (int64_t)order[k] : k;
         const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
@@ -675,6 +711,8 @@ __global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *r
         vec o = Geo<DIM>::load_point(o3), d = Geo<DIM>::load_point(d3), di = Geo<DIM>::load_point(di3);
         RayResult r;
         ray_query<DIM, BLOCK, K>(S, o, d, di, tmax, occlusion != 0, r);
+        n_internal = r.n_internal;
+        n_prims = r.n_prims;
         uint32_t index = r.slot >= 0 ? (uint32_t)S.prim_index[r.slot] : FCPW_HIP_UINT_MAX;
         if (occlusion) {
             // only the occluder's index is written (bvh.slang:183-187)
@@ -684,6 +722,7 @@ __global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *r
             write_interaction(out + i * 5, r.p, r.n, r.uv, r.t, index);
         }
     }
+    if (STATS) wave_stats_add(S, n_internal, n_prims, k < n ? 1u : 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -944,6 +983,7 @@ struct fcpw_hip_ctx {
     uint32_t *leaf_links = nullptr;
     unsigned int *counters = nullptr;
     int *flags = nullptr;
+    unsigned long long *stats = nullptr; // internal nodes, primitive tests, queries (option "stats")
     // staging for the host-pointer entry points
     void *stage_in = nullptr, *stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
@@ -955,7 +995,12 @@ struct fcpw_hip_ctx {
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
     int block = 128, stack = 16, sort = 1, xcds = 8, coop = 1, hold_min = 2;
+    int stats_on = 0, timing = 0;
     int n_cus = 256;
+    // HIP event pairs around each traversal launch while "timing" is on
+    static constexpr int MAX_TIMED = 1024;
+    std::vector<hipEvent_t> ev_begin, ev_end;
+    int ev_used = 0;
 };
 
 static std::string g_create_error;
@@ -994,7 +1039,7 @@ static void free_scene(fcpw_hip_ctx *c)
 
 static DevScene dev_scene(fcpw_hip_ctx *c)
 {
-    return DevScene{c->records, c->root, c->prims, c->prim_index, c->flags};
+    return DevScene{c->records, c->root, c->prims, c->prim_index, c->flags, c->stats};
 }
 
 static int grow(fcpw_hip_ctx *c, void **buf, size_t *cap, size_t bytes)
@@ -1036,9 +1081,12 @@ int fcpw_hip_create(int device, fcpw_hip_ctx **out)
         e = hipMalloc((void **)&c->flags, sizeof(int));
     }
     if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->stats, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->stats, 0, 4 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         g_create_error = hipGetErrorString(e);
         if (c->flags) (void)hipFree(c->flags);
+        if (c->stats) (void)hipFree(c->stats);
         delete c;
         return FCPW_HIP_ERR_HIP;
     }
@@ -1052,9 +1100,12 @@ int fcpw_hip_destroy(fcpw_hip_ctx *c)
     (void)hipSetDevice(c->device);
     free_scene(c);
     if (c->flags) (void)hipFree(c->flags);
+    if (c->stats) (void)hipFree(c->stats);
     void *ws[] = {c->stage_in, c->stage_out, c->ws_codes, c->ws_perm, c->ws_hist};
     for (void *b : ws)
         if (b) (void)hipFree(b);
+    for (hipEvent_t e : c->ev_begin) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_end) (void)hipEventDestroy(e);
     delete c;
     return FCPW_HIP_OK;
 }
@@ -1300,6 +1351,31 @@ static dim3 grid_for(fcpw_hip_ctx *, int64_t n, int block)
     return dim3((unsigned)std::max<int64_t>(1, (n + block - 1) / block));
 }
 
+// "timing": a HIP event pair on the launch stream around each traversal launch
+static int timing_begin(fcpw_hip_ctx *c, hipStream_t s)
+{
+    if (!c->timing || c->ev_used >= fcpw_hip_ctx::MAX_TIMED) return FCPW_HIP_OK;
+    if (c->ev_used == (int)c->ev_begin.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        HIPCHECK(c, hipEventCreate(&a));
+        hipError_t e = hipEventCreate(&b);
+        if (e != hipSuccess) (void)hipEventDestroy(a);
+        HIPCHECK(c, e);
+        c->ev_begin.push_back(a);
+        c->ev_end.push_back(b);
+    }
+    HIPCHECK(c, hipEventRecord(c->ev_begin[c->ev_used], s));
+    return FCPW_HIP_OK;
+}
+
+static int timing_end(fcpw_hip_ctx *c, hipStream_t s)
+{
+    if (!c->timing || c->ev_used >= fcpw_hip_ctx::MAX_TIMED) return FCPW_HIP_OK;
+    HIPCHECK(c, hipEventRecord(c->ev_end[c->ev_used], s));
+    c->ev_used++;
+    return FCPW_HIP_OK;
+}
+
 // Morton order of n query points (pts + i * stride floats); returns the
 // permutation, or nullptr when ordering is off or the batch is small
 static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n, hipStream_t s,
@@ -1355,16 +1431,30 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
     int st = morton_order(c, io.in, OUT == CP_OUT_INTERACTION ? 4 : c->dim, io.n, s, &io.order);
     if (st) return st;
     const int64_t n = io.n;
-    if (c->coop) {
+    if ((st = timing_begin(c, s))) return st;
+    if (c->stats_on) {
+        // counting pass: the coop kernel with per-wave counters (same traversal order)
+        dim3 g = grid_for(c, n, 64);
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT, true>), g, dim3(64), 0, s, S, io, c->xcds,
+                               c->hold_min);
+        else
+            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT, true>), g, dim3(64), 0, s, S, io, c->xcds,
+                               c->hold_min);
+    } else if (c->coop) {
         dim3 g = grid_for(c, n, 64);
         if (c->dim == 3 && c->stack <= 8)
-            hipLaunchKernelGGL((k_closest_points_coop<3, 8, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+            hipLaunchKernelGGL((k_closest_points_coop<3, 8, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
+                               c->hold_min);
         else if (c->dim == 3)
-            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
+                               c->hold_min);
         else if (c->stack <= 8)
-            hipLaunchKernelGGL((k_closest_points_coop<2, 8, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+            hipLaunchKernelGGL((k_closest_points_coop<2, 8, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
+                               c->hold_min);
         else
-            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min);
+            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
+                               c->hold_min);
     } else if (c->dim == 3) {
         FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_closest_points<3, B, K, OUT>), grid_for(c, n, B), dim3(B), 0, s,
                                                   S, io, c->xcds));
@@ -1373,7 +1463,7 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
                                                   S, io, c->xcds));
     }
     HIPCHECK(c, hipGetLastError());
-    return FCPW_HIP_OK;
+    return timing_end(c, s);
 }
 
 static int launch_closest_records(fcpw_hip_ctx *c, const void *spheres, int64_t n, int record_normals, void *out,
@@ -1395,14 +1485,21 @@ static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int oc
     const uint32_t *order;
     int st = morton_order(c, (const float *)rays, 10, n, s, &order); // ordered by origin
     if (st) return st;
-    if (c->dim == 3)
-        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<3, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+    if ((st = timing_begin(c, s))) return st;
+    if (c->stats_on && c->dim == 3)
+        hipLaunchKernelGGL((k_intersect<3, 128, 16, true>), grid_for(c, n, 128), dim3(128), 0, s, S,
+                           (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds);
+    else if (c->stats_on)
+        hipLaunchKernelGGL((k_intersect<2, 128, 16, true>), grid_for(c, n, 128), dim3(128), 0, s, S,
+                           (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds);
+    else if (c->dim == 3)
+        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<3, B, K, false>), grid_for(c, n, B), dim3(B), 0, s, S,
                                             (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds));
     else
-        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<2, B, K>), grid_for(c, n, B), dim3(B), 0, s, S,
+        FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<2, B, K, false>), grid_for(c, n, B), dim3(B), 0, s, S,
                                             (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds));
     HIPCHECK(c, hipGetLastError());
-    return FCPW_HIP_OK;
+    return timing_end(c, s);
 }
 
 static int check_overflow(fcpw_hip_ctx *c)
@@ -1552,6 +1649,18 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "xcds") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcds must be in [1, 64]");
         c->xcds = value;
+    } else if (k == "stats") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stats must be 0 or 1");
+        if (value) {
+            HIPCHECK(c, hipSetDevice(c->device));
+            HIPCHECK(c, hipDeviceSynchronize());
+            HIPCHECK(c, hipMemset(c->stats, 0, 4 * sizeof(unsigned long long)));
+        }
+        c->stats_on = value;
+    } else if (k == "timing") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "timing must be 0 or 1");
+        if (value) c->ev_used = 0;
+        c->timing = value;
     } else {
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "unknown option '%s'", key);
     }
@@ -1568,6 +1677,8 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "coop") *value = c->coop;
     else if (k == "xcds") *value = c->xcds;
     else if (k == "hold_min") *value = c->hold_min;
+    else if (k == "stats") *value = c->stats_on;
+    else if (k == "timing") *value = c->timing;
     else return FCPW_HIP_ERR_INVALID_ARGUMENT;
     return FCPW_HIP_OK;
 }
@@ -1584,7 +1695,26 @@ int fcpw_hip_get_info(const fcpw_hip_ctx *c, const char *key, int64_t *value)
     else if (k == "max_depth") *value = c->max_depth;
     else if (k == "n_cus") *value = c->n_cus;
     else if (k == "uploaded") *value = c->uploaded ? 1 : 0;
-    else return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    else if (k == "stat_internal_nodes" || k == "stat_prim_tests" || k == "stat_queries") {
+        fcpw_hip_ctx *m = const_cast<fcpw_hip_ctx *>(c);
+        unsigned long long v[4];
+        HIPCHECK(m, hipSetDevice(c->device));
+        HIPCHECK(m, hipDeviceSynchronize());
+        HIPCHECK(m, hipMemcpy(v, c->stats, sizeof(v), hipMemcpyDeviceToHost));
+        *value = (int64_t)(k == "stat_internal_nodes" ? v[0] : k == "stat_prim_tests" ? v[1] : v[2]);
+    } else if (k == "kernel_launches") {
+        *value = c->ev_used;
+    } else if (k == "kernel_ns") {
+        fcpw_hip_ctx *m = const_cast<fcpw_hip_ctx *>(c);
+        double total = 0.0;
+        if (c->ev_used > 0) HIPCHECK(m, hipEventSynchronize(c->ev_end[c->ev_used - 1]));
+        for (int i = 0; i < c->ev_used; i++) {
+            float ms = 0.0f;
+            HIPCHECK(m, hipEventElapsedTime(&ms, c->ev_begin[i], c->ev_end[i]));
+            total += ms;
+        }
+        *value = (int64_t)(total * 1e6);
+    } else return FCPW_HIP_ERR_INVALID_ARGUMENT;
     return FCPW_HIP_OK;
 }
 

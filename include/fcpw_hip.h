@@ -166,11 +166,21 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, co
      "block"     threads per block when coop = 0 (64, 128, 256)
      "sort"      1 = order queries along a Morton curve before traversal (default)
      "xcds"      number of XCDs the block -> tile mapping groups by (1..64, default 8)
+   Measurement switches (off by default; results are unchanged):
+     "stats"     1 = run the counting variant of the traversal kernels, which adds
+                 the internal nodes expanded, primitives tested and queries to
+                 counters read back through fcpw_hip_get_info; setting 1 clears them
+     "timing"    1 = record a HIP event pair on the launch stream around every
+                 traversal kernel (up to 1024 launches); setting 1 starts a new series
    Unknown keys or values return FCPW_HIP_ERR_INVALID_ARGUMENT. */
 int fcpw_hip_set_option(fcpw_hip_ctx *ctx, const char *key, int value);
 int fcpw_hip_get_option(const fcpw_hip_ctx *ctx, const char *key, int *value);
 
-/* Uploaded BVH facts: n_records (internal nodes), n_leaves, n_prims, dim, max_depth. */
+/* Uploaded BVH facts: n_records (internal nodes), n_leaves, n_prims, n_vertices,
+   dim, max_depth, n_cus.  Measurement: stat_internal_nodes, stat_prim_tests,
+   stat_queries (totals since "stats" was set to 1; synchronizes the device),
+   kernel_launches and kernel_ns (count and summed duration of the traversal
+   launches timed since "timing" was set to 1; waits for the last one). */
 int fcpw_hip_get_info(const fcpw_hip_ctx *ctx, const char *key, int64_t *value);
 
 #ifdef __cplusplus

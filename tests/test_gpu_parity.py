@@ -257,6 +257,49 @@ def test_every_query_of_a_ragged_batch_is_written(gpu, cache, n):
     assert_bit_equal(as_records(hits), ob.intersect_gpuorder(o, dr))
 
 
+@pytest.mark.parametrize("name", ["torus", "bunny", "star2d"])
+def test_traversal_counters_equal_reference_order_counts(gpu, cache, name):
+    """the counting kernels expand exactly the internal nodes and test exactly
+    the primitives of the reference GPU traversal order (the byte model's I, P)"""
+    V, F, scene, g, ob = get(cache, name)
+    q = meshes.queries_around(V, 20000, seed=21)
+    r2 = np.full(len(q), np.inf, np.float32)
+    g.set_option("stats", 1)
+    hits = fcpw.GPUInteractionList()
+    g.find_closest_points(q, r2, hits)
+    got = (g.info("stat_internal_nodes"), g.info("stat_prim_tests"), g.info("stat_queries"))
+    ref, ni, npr, _ = ob.closest_points_gpuorder(q, r2, counters=True)
+    assert_bit_equal(as_records(hits), ref)
+    assert got == (int(ni.sum()), int(npr.sum()), len(q))
+    g.set_option("stats", 1)
+    o, d = meshes.rays_from(V, 20000, seed=22)
+    g.intersect(o, d, np.full(len(o), np.inf, np.float32), hits)
+    got = (g.info("stat_internal_nodes"), g.info("stat_prim_tests"), g.info("stat_queries"))
+    ref, ni, npr = ob.intersect_gpuorder(o, d, counters=True)
+    assert_bit_equal(as_records(hits), ref)
+    assert got == (int(ni.sum()), int(npr.sum()), len(o))
+    g.set_option("stats", 0)
+
+
+def test_timing_events_bracket_each_traversal_launch(gpu, cache):
+    import torch
+    V, F, scene, g, ob = get(cache, "torus")
+    q = meshes.queries_around(V, 50000, seed=23)
+    sph = torch.zeros((len(q), 4), dtype=torch.float32, device="cuda")
+    sph[:, :3] = torch.from_numpy(q).cuda()
+    sph[:, 3] = float("inf")
+    out = torch.empty((len(q), 10), dtype=torch.float32, device="cuda")
+    g.set_option("timing", 1)
+    for _ in range(3):
+        g.find_closest_points_device(sph, out)
+    assert g.info("kernel_launches") == 3
+    ns = g.info("kernel_ns")
+    assert 0 < ns < 10 ** 9
+    g.set_option("timing", 0)
+    g.find_closest_points_device(sph, out)
+    assert g.info("kernel_launches") == 3
+
+
 def test_device_entry_points_match_host(gpu, cache):
     import torch
     V, F, scene, g, ob = get(cache, "icosphere")
