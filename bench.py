@@ -113,7 +113,7 @@ def cpu_baseline(V, F, q, r2, sample):
 # rocprofv3 summaries (no GPU needed): turn the SQLite output of a kernel-trace
 # run or of FETCH_SIZE / WRITE_SIZE counter runs into the text kept in profiles/
 # ---------------------------------------------------------------------------
-MAIN_KERNEL = "k_closest_points_coop<3, 16, 0, false>"
+MAIN_KERNEL = "k_closest_points_coop<3, 8, 0, false>"
 
 
 def _db(path):

@@ -64,7 +64,7 @@ __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f)
 // compile to ds_read/ds_write rather than generic (flat) loads.
 // ---------------------------------------------------------------------------
 #define FCPW_STACK_DECL(BLOCK, K)                                                                              \
-    __shared__ uint64_t lds_stack[(K) * (BLOCK)];                                                            \
+    __shared__ uint64_t lds_stack[(K) > 0 ? (K) * (BLOCK) : 1];                                              \
     uint32_t stack_spill[2 * (STACK_TOTAL - (K))];                                                           \
     const int stack_lane = threadIdx.x;                                                                       \
     int sp = 0
@@ -130,13 +130,11 @@ struct Geo<3> {
         hiB = {b1.x, b1.y, b1.z};
         lB = link_of(b0.w);
     }
-    template <bool SEL = false>
     static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
                                                     float &u1)
     {
         const float4 *t = S.prims + (size_t)slot * 3;
         float4 a = t[0], b = t[1], c = t[2];
-        if (SEL) return closest_point_triangle_sel({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
         return closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
     }
     static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
@@ -173,7 +171,6 @@ struct Geo<2> {
         lA = link_of(l.x);
         lB = link_of(l.y);
     }
-    template <bool SEL = false>
     static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
                                                     float &u1)
     {
@@ -205,7 +202,7 @@ struct CpResult {
     uint32_t n_internal, n_prims; // records expanded and primitives tested (coop kernel)
 };
 
-template <int DIM, int BLOCK, int K, bool SEL>
+template <int DIM, int BLOCK, int K>
 __device__ __forceinline__ void closest_point_query(const DevScene &S, typename Geo<DIM>::vec x, float r2,
                                                     CpResult &res)
 {
@@ -235,7 +232,7 @@ __device__ __forceinline__ void closest_point_query(const DevScene &S, typename 
                 for (uint32_t k = 0; k < cnt; k++) {
                     vec p;
                     float u0, u1;
-                    float d = G::template closest<SEL>(S, off + k, x, p, u0, u1);
+                    float d = G::closest(S, off + k, x, p, u0, u1);
                     if (d * d <= r2) {
                         r2 = fminf(r2, d * d);
                         res.slot = (int32_t)(off + k);
@@ -500,7 +497,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest_points(DevScene S, CpIO io, i
     float q[3], r2;
     cp_load<DIM, OUT>(io, i, q, r2);
     CpResult r;
-    closest_point_query<DIM, BLOCK, K, false>(S, Geo<DIM>::load_point(q), r2, r);
+    closest_point_query<DIM, BLOCK, K>(S, Geo<DIM>::load_point(q), r2, r);
     cp_store<DIM, OUT>(S, io, i, r);
 }
 
@@ -521,7 +518,7 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-template <int DIM, int K, bool SEL>
+template <int DIM, int K>
 __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, typename Geo<DIM>::vec x, float r2,
                                                          bool valid, int hold_min, CpResult &res)
 {
@@ -629,7 +626,7 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
                     float xo[3] = {coop_q[o * 3], coop_q[o * 3 + 1], coop_q[o * 3 + 2]};
                     vec pp;
                     float u0, u1;
-                    coop_d[k] = G::template closest<SEL>(S, coop_off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
+                    coop_d[k] = G::closest(S, coop_off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
                 }
             }
             __syncthreads();
@@ -657,7 +654,7 @@ __device__ __forceinline__ void closest_point_query_coop(const DevScene &S, type
     if (best >= 0) {
         vec p;
         float u0, u1;
-        res.d = G::template closest<SEL>(S, (uint32_t)best, x, p, u0, u1);
+        res.d = G::closest(S, (uint32_t)best, x, p, u0, u1);
         res.slot = best;
         res.uv[0] = u0;
         res.uv[1] = u1;
@@ -690,7 +687,7 @@ __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io,
     float q[3] = {0.0f, 0.0f, 0.0f}, r2 = -1.0f;
     if (valid) cp_load<DIM, OUT>(io, i, q, r2);
     CpResult r;
-    closest_point_query_coop<DIM, K, false>(S, Geo<DIM>::load_point(q), r2, valid, hold_min, r);
+    closest_point_query_coop<DIM, K>(S, Geo<DIM>::load_point(q), r2, valid, hold_min, r);
     if (valid) cp_store<DIM, OUT>(S, io, i, r);
     if (STATS) wave_stats_add(S, r.n_internal, r.n_prims, valid ? 1u : 0u);
 }
@@ -994,7 +991,7 @@ struct fcpw_hip_ctx {
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
-    int block = 128, stack = 16, sort = 1, xcds = 8, coop = 1, hold_min = 2;
+    int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1;
     int stats_on = 0, timing = 0;
     int n_cus = 256;
     // HIP event pairs around each traversal launch while "timing" is on
@@ -1406,7 +1403,7 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
 
 #define FCPW_DISPATCH_BK(c, V, ...)                                                                            \
     do {                                                                                                       \
-        const int _b = (c)->block, _k = (c)->stack;                                                           \
+        const int _b = (c)->block, _k = (c)->stack < 8 ? 8 : (c)->stack;                                      \
         if (_b == 64 && _k == 8) { constexpr int B = 64, K = 8; __VA_ARGS__; }                               \
         else if (_b == 64 && _k == 16) { constexpr int B = 64, K = 16; __VA_ARGS__; }                        \
         else if (_b == 64 && _k == 24) { constexpr int B = 64, K = 24; __VA_ARGS__; }                        \
@@ -1443,18 +1440,21 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
                                c->hold_min);
     } else if (c->coop) {
         dim3 g = grid_for(c, n, 64);
-        if (c->dim == 3 && c->stack <= 8)
-            hipLaunchKernelGGL((k_closest_points_coop<3, 8, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
-                               c->hold_min);
-        else if (c->dim == 3)
-            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
-                               c->hold_min);
-        else if (c->stack <= 8)
-            hipLaunchKernelGGL((k_closest_points_coop<2, 8, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
-                               c->hold_min);
-        else
-            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds,
-                               c->hold_min);
+#define FCPW_COOP(D, K) \
+    hipLaunchKernelGGL((k_closest_points_coop<D, K, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min)
+        const int k = c->stack >= 16 ? 16 : c->stack; // 0, 4, 8 or 16 LDS entries
+        if (c->dim == 3) {
+            if (k == 0) FCPW_COOP(3, 0);
+            else if (k == 4) FCPW_COOP(3, 4);
+            else if (k == 8) FCPW_COOP(3, 8);
+            else FCPW_COOP(3, 16);
+        } else {
+            if (k == 0) FCPW_COOP(2, 0);
+            else if (k == 4) FCPW_COOP(2, 4);
+            else if (k == 8) FCPW_COOP(2, 8);
+            else FCPW_COOP(2, 16);
+        }
+#undef FCPW_COOP
     } else if (c->dim == 3) {
         FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_closest_points<3, B, K, OUT>), grid_for(c, n, B), dim3(B), 0, s,
                                                   S, io, c->xcds));
@@ -1635,7 +1635,8 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
         if (value != 64 && value != 128 && value != 256) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
         c->block = value;
     } else if (k == "stack") {
-        if (value != 8 && value != 16 && value != 24) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 8, 16 or 24");
+        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 24)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 8, 16 or 24");
         c->stack = value;
     } else if (k == "sort") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sort must be 0 or 1");

@@ -144,48 +144,6 @@ __host__ __device__ __forceinline__ float closest_point_triangle(v3 pa, v3 pb, v
 }
 
 
-// The same region test without divergent branches: every region's formula
-// needs at most one IEEE division, so the (numerator, denominator) pair of the
-// region is selected and divided once; p and t are then assembled with selects.
-// Per region the arithmetic is the reference's (p = base + e1*s1 [+ e2*s2]), so
-// the result equals closest_point_triangle bit for bit (up to the sign of a zero
-// coordinate of p, which does not change any distance).
-__host__ __device__ __forceinline__ float closest_point_triangle_sel(v3 pa, v3 pb, v3 pc, v3 x, v3 &p, float &t0, float &t1)
-{
-    v3 ab = sub(pb, pa), ac = sub(pc, pa), ax = sub(x, pa);
-    float d1 = dot(ab, ax), d2 = dot(ac, ax);
-    v3 bx = sub(x, pb);
-    float d3 = dot(ab, bx), d4 = dot(ac, bx);
-    v3 cx = sub(x, pc);
-    float d5 = dot(ab, cx), d6 = dot(ac, cx);
-    float vc = d1 * d4 - d3 * d2;
-    float vb = d5 * d2 - d1 * d6;
-    float va = d3 * d6 - d5 * d4;
-    float e43 = d4 - d3, e56 = d5 - d6;
-    bool rA = d1 <= 0.0f && d2 <= 0.0f;
-    bool rB = !rA && d3 >= 0.0f && d4 <= d3;
-    bool rC = !rA && !rB && d6 >= 0.0f && d5 <= d6;
-    bool vtx = rA || rB || rC;
-    bool rAB = !vtx && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f;
-    bool rAC = !vtx && !rAB && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f;
-    bool rBC = !vtx && !rAB && !rAC && va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f;
-    bool face = !vtx && !rAB && !rAC && !rBC;
-    float num = rAB ? d1 : (rAC ? d2 : (rBC ? e43 : 1.0f));
-    float den = rAB ? (d1 - d3) : (rAC ? (d2 - d6) : (rBC ? (e43 + e56) : (face ? (va + vb + vc) : 1.0f)));
-    float q = num / den;
-    float v = face ? vb * q : q;
-    float w = face ? vc * q : 0.0f;
-    v3 base = rB ? pb : (rC ? pc : (rBC ? pb : pa));
-    v3 bc = sub(pc, pb);
-    v3 e1 = rAC ? ac : (rBC ? bc : ab);
-    float s1 = vtx ? 0.0f : v;
-    p = {(base.x + e1.x * s1) + ac.x * w, (base.y + e1.y * s1) + ac.y * w, (base.z + e1.z * s1) + ac.z * w};
-    t0 = rA ? 1.0f : ((rB || rC || rBC) ? 0.0f : (rAB ? 1.0f - q : (rAC ? 1.0f - q : (1.0f - v) - w)));
-    t1 = rB ? 1.0f : ((rA || rC || rAC) ? 0.0f : (rAB ? q : (rBC ? 1.0f - q : v)));
-    v3 r = sub(x, p);
-    return sqrtf(dot(r, r));
-}
-
 // findClosestPointLineSegment (geometry.slang:79-103)
 __host__ __device__ __forceinline__ float closest_point_segment(v2 pa, v2 pb, v2 x, v2 &p, float &t)
 {

@@ -161,8 +161,10 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, co
      "coop"      1 = wave-cooperative leaf processing (default), 0 = one query per
                  lane with the leaf loop inline
      "hold_min"  coop: lanes that must hold a leaf before the wave processes
-                 leaves together (1..64, default 2)
-     "stack"     LDS stack entries per lane (coop: 8 or 16; else 8, 16, 24)
+                 leaves together (1..64, default 1)
+     "stack"     traversal-stack entries per lane kept in LDS (default 8), the rest in cached
+                 scratch: 0, 4, 8, 16 or 24 (the coop kernel uses 16 for 24, the
+                 one-query-per-lane kernel 8 for 0 and 4)
      "block"     threads per block when coop = 0 (64, 128, 256)
      "sort"      1 = order queries along a Morton curve before traversal (default)
      "xcds"      number of XCDs the block -> tile mapping groups by (1..64, default 8)
