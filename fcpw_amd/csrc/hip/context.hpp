@@ -1,0 +1,97 @@
+// context.hpp -- host-side state of one uploaded scene (fcpw_hip_ctx) and the
+// error-reporting helpers shared by the entry points.  Included once, by
+// fcpw_hip.hip, after device.hpp.
+#pragma once
+using namespace fcpw_hip;
+
+static float bits_as_float(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, sizeof(f));
+    return f;
+}
+
+struct fcpw_hip_ctx {
+    int device = 0;
+    int dim = 0;
+    bool uploaded = false;
+    uint64_t generation = 0; // bumped by every upload (batches re-read the buffers)
+    std::string err;
+    // device buffers
+    float4 *records = nullptr, *root = nullptr, *prims = nullptr;
+    int32_t *prim_index = nullptr, *prim_vids = nullptr;
+    float *V = nullptr;
+    int32_t *rec_parent = nullptr, *leaf_refs = nullptr;
+    uint32_t *leaf_links = nullptr;
+    unsigned int *counters = nullptr;
+    int *flags = nullptr;
+    unsigned long long *stats = nullptr; // internal nodes, primitive tests, queries (option "stats")
+    // staging for the host-pointer entry points
+    void *stage_in = nullptr, *stage_out = nullptr;
+    size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    // query ordering workspace (grown on demand, kept for the context's lifetime)
+    void *ws_codes = nullptr, *ws_perm = nullptr, *ws_hist = nullptr;
+    size_t ws_codes_bytes = 0, ws_perm_bytes = 0, ws_hist_bytes = 0;
+    float qlo[3] = {0, 0, 0}, qinv[3] = {1, 1, 1}; // Morton grid: 1.5x the scene box
+    // facts
+    int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
+    // options
+    int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1;
+    int stats_on = 0, timing = 0;
+    int n_cus = 256;
+    // HIP event pairs around each traversal launch while "timing" is on
+    static constexpr int MAX_TIMED = 1024;
+    std::vector<hipEvent_t> ev_begin, ev_end;
+    int ev_used = 0;
+};
+
+static std::string g_create_error;
+
+#define FAIL(ctx, code, ...)                                                                                   \
+    do {                                                                                                       \
+        char _b[512];                                                                                          \
+        snprintf(_b, sizeof(_b), __VA_ARGS__);                                                                 \
+        (ctx)->err = _b;                                                                                       \
+        return (code);                                                                                         \
+    } while (0)
+
+#define HIPCHECK(ctx, expr)                                                                                    \
+    do {                                                                                                       \
+        hipError_t _e = (expr);                                                                                \
+        if (_e != hipSuccess) {                                                                                \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                                   \
+            return _e == hipErrorOutOfMemory ? FCPW_HIP_ERR_OUT_OF_MEMORY : FCPW_HIP_ERR_HIP;                 \
+        }                                                                                                      \
+    } while (0)
+
+static void free_scene(fcpw_hip_ctx *c)
+{
+    void *bufs[] = {c->records, c->root, c->prims, c->prim_index, c->prim_vids, c->V,
+                    c->rec_parent, c->leaf_refs, c->leaf_links, c->counters};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->records = c->root = c->prims = nullptr;
+    c->prim_index = c->prim_vids = nullptr;
+    c->V = nullptr;
+    c->rec_parent = c->leaf_refs = nullptr;
+    c->leaf_links = nullptr;
+    c->counters = nullptr;
+    c->uploaded = false;
+}
+
+static DevScene dev_scene(fcpw_hip_ctx *c)
+{
+    return DevScene{c->records, c->root, c->prims, c->prim_index, c->flags, c->stats};
+}
+
+static int grow(fcpw_hip_ctx *c, void **buf, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return FCPW_HIP_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    HIPCHECK(c, hipMalloc(buf, bytes));
+    *cap = bytes;
+    return FCPW_HIP_OK;
+}
+

@@ -1,0 +1,969 @@
+#pragma once
+// device.hpp -- device code of the MI355X (gfx950) query backend: record and
+// primitive access, traversal stacks, the closest-point / ray / refit / adjoint
+// kernels and the query-ordering kernels.  Included once, by fcpw_hip.hip.
+//
+// Device layout (HBM), built on the host from the reference's flat DFS BVH:
+//   records  one per internal node, in DFS order.  A record holds the boxes of
+//            BOTH children plus a 32-bit link per child, so expanding a node is
+//            one 64-byte (3D) / 48-byte (2D) contiguous fetch:
+//              3D: float4 {loA.xyz, linkA} {hiA.xyz, 0} {loB.xyz, linkB} {hiB.xyz, 0}
+//              2D: float4 {loA.xy, hiA.xy} {loB.xy, hiB.xy} {linkA, linkB, 0, 0}
+//            link = internal record index, or LEAF_BIT | (count-1) << 27 | first leaf slot
+//   root     the root box and link in the same slot format (2 float4)
+//   prims    leaf-slot order; 3D: float4 {pa,0} {pb,0} {pc,0}; 2D: float4 {pa.xy, pb.xy}
+//   prim_index / prim_vids   original primitive index and vertex ids per leaf slot
+//   rec_parent, leaf_refs, counters   the bottom-up refit schedule
+//
+// Query kernels run one query per lane.  The traversal order is the reference
+// GPU order (include/fcpw/gpu/bvh.slang:144-254,395-502): test both children,
+// shrink r^2 by each child's farthest-corner distance, continue with the nearer
+// child and push the farther one.  Each stack entry carries the child's link and
+// its squared (or parametric) distance, so a pop is pruned without refetching
+// the node.  The top K entries of the stack live in LDS (K x 8 bytes per lane,
+// lane-interleaved so a wave's accesses hit distinct banks); entries beyond K
+// spill to a private array.  Depth <= 63 (leaf forced at depth 62,
+// bvh.inl:170), so 64 entries always suffice.
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../../include/fcpw_hip.h"
+#include "geometry.hpp"
+
+namespace fcpw_hip {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t CNT_SHIFT = 27;
+constexpr uint32_t CNT_MASK = 0xFu;
+constexpr uint32_t OFF_MASK = (1u << 27) - 1u;
+constexpr int MAX_LEAF_PRIMS = 16;
+constexpr int STACK_TOTAL = 64;
+
+struct DevScene {
+    const float4 *records;
+    const float4 *root;
+    const float4 *prims;
+    const int32_t *prim_index;
+    int *flags;                // bit 0: traversal stack overflow
+    unsigned long long *stats; // traversal counters: internal nodes, primitive tests, queries
+};
+
+__device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
+
+// ---------------------------------------------------------------------------
+// traversal stack: entries [0, K) in LDS, lane-interleaved (entry s of lane t
+// at lds[s * BLOCK + t], so a wave's accesses fall in distinct banks), entries
+// [K, 64) in a private spill array.  The stack pointer stays in a register and
+// the LDS array is declared in the traversal function itself, so the accesses
+// compile to ds_read/ds_write rather than generic (flat) loads.
+// ---------------------------------------------------------------------------
+#define FCPW_STACK_DECL(BLOCK, K)                                                                              \
+    __shared__ uint64_t lds_stack[(K) > 0 ? (K) * (BLOCK) : 1];                                              \
+    uint32_t stack_spill[2 * (STACK_TOTAL - (K))];                                                           \
+    const int stack_lane = threadIdx.x;                                                                       \
+    int sp = 0
+
+// An entry packs (link, distance bits) into 64 bits.  The cold spill path moves
+// the entry as two 32-bit halves: an access of a different type from the LDS
+// one cannot be merged with it into a single generic (flat) access, which would
+// send every LDS push and pop down the slower vector-memory path.  Spilled
+// entries are ordinary cached scratch accesses.
+#define FCPW_PUSH(BLOCK, K, LINK, D, ON_FULL)                                                                  \
+    if (sp >= STACK_TOTAL) {                                                                                  \
+        ON_FULL;                                                                                              \
+    } else {                                                                                                  \
+        const uint32_t l_ = (uint32_t)(LINK), d_ = __float_as_uint(D);                                        \
+        if (sp < (K)) {                                                                                       \
+            lds_stack[sp * (BLOCK) + stack_lane] = (uint64_t)l_ | ((uint64_t)d_ << 32);                       \
+        } else {                                                                                              \
+            stack_spill[2 * (sp - (K))] = l_;                                                                 \
+            stack_spill[2 * (sp - (K)) + 1] = d_;                                                             \
+        }                                                                                                     \
+        sp++;                                                                                                 \
+    }
+
+#define FCPW_POP(BLOCK, K, LINK, D)                                                                            \
+    do {                                                                                                      \
+        sp--;                                                                                                 \
+        if (sp < (K)) {                                                                                       \
+            const uint64_t e_ = lds_stack[sp * (BLOCK) + stack_lane];                                         \
+            (LINK) = (uint32_t)e_;                                                                            \
+            (D) = __uint_as_float((uint32_t)(e_ >> 32));                                                      \
+        } else {                                                                                              \
+            (LINK) = stack_spill[2 * (sp - (K))];                                                             \
+            (D) = __uint_as_float(stack_spill[2 * (sp - (K)) + 1]);                                           \
+        }                                                                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// per-dimension record/primitive access
+// ---------------------------------------------------------------------------
+template <int DIM>
+struct Geo;
+
+template <>
+struct Geo<3> {
+    using vec = v3;
+    static constexpr int REC = 4;
+    static __device__ __forceinline__ void root(const DevScene &S, vec &lo, vec &hi, uint32_t &link)
+    {
+        float4 a = S.root[0], b = S.root[1];
+        lo = {a.x, a.y, a.z};
+        hi = {b.x, b.y, b.z};
+        link = link_of(a.w);
+    }
+    static __device__ __forceinline__ void record(const DevScene &S, uint32_t r, vec &loA, vec &hiA, uint32_t &lA,
+                                                  vec &loB, vec &hiB, uint32_t &lB)
+    {
+        const float4 *p = S.records + (size_t)r * REC;
+        float4 a0 = p[0], a1 = p[1], b0 = p[2], b1 = p[3];
+        loA = {a0.x, a0.y, a0.z};
+        hiA = {a1.x, a1.y, a1.z};
+        lA = link_of(a0.w);
+        loB = {b0.x, b0.y, b0.z};
+        hiB = {b1.x, b1.y, b1.z};
+        lB = link_of(b0.w);
+    }
+    static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
+                                                    float &u1)
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        return closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
+    }
+    static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        v3 m = cross({b.x - a.x, b.y - a.y, b.z - a.z}, {c.x - a.x, c.y - a.y, c.z - a.z});
+        float l = sqrtf(dot(m, m));
+        n[0] = m.x / l; n[1] = m.y / l; n[2] = m.z / l;
+    }
+    static __device__ __forceinline__ vec load_point(const float *q) { return {q[0], q[1], q[2]}; }
+};
+
+template <>
+struct Geo<2> {
+    using vec = v2;
+    static constexpr int REC = 3;
+    static __device__ __forceinline__ void root(const DevScene &S, vec &lo, vec &hi, uint32_t &link)
+    {
+        float4 a = S.root[0], b = S.root[1];
+        lo = {a.x, a.y};
+        hi = {a.z, a.w};
+        link = link_of(b.x);
+    }
+    static __device__ __forceinline__ void record(const DevScene &S, uint32_t r, vec &loA, vec &hiA, uint32_t &lA,
+                                                  vec &loB, vec &hiB, uint32_t &lB)
+    {
+        const float4 *p = S.records + (size_t)r * REC;
+        float4 a = p[0], b = p[1], l = p[2];
+        loA = {a.x, a.y};
+        hiA = {a.z, a.w};
+        loB = {b.x, b.y};
+        hiB = {b.z, b.w};
+        lA = link_of(l.x);
+        lB = link_of(l.y);
+    }
+    static __device__ __forceinline__ float closest(const DevScene &S, uint32_t slot, vec x, vec &p, float &u0,
+                                                    float &u1)
+    {
+        float4 s = S.prims[slot];
+        u1 = 0.0f;
+        return closest_point_segment({s.x, s.y}, {s.z, s.w}, x, p, u0);
+    }
+    static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
+    {
+        float4 s = S.prims[slot];
+        v2 m = {s.w - s.y, -(s.z - s.x)};
+        float l = sqrtf(dot(m, m));
+        n[0] = m.x / l; n[1] = m.y / l; n[2] = 0.0f;
+    }
+    static __device__ __forceinline__ vec load_point(const float *q) { return {q[0], q[1]}; }
+};
+
+__device__ __forceinline__ void store_p(v3 p, float o[3]) { o[0] = p.x; o[1] = p.y; o[2] = p.z; }
+__device__ __forceinline__ void store_p(v2 p, float o[3]) { o[0] = p.x; o[1] = p.y; o[2] = 0.0f; }
+
+// ---------------------------------------------------------------------------
+// closest point traversal (bvh.slang:395-502 order)
+// ---------------------------------------------------------------------------
+struct CpResult {
+    int32_t slot; // -1: not found
+    float d;
+    float p[3];
+    float uv[2];
+    uint32_t n_internal, n_prims; // records expanded and primitives tested (coop kernel)
+};
+
+template <int DIM, int BLOCK, int K>
+__device__ __forceinline__ void closest_point_query(const DevScene &S, typename Geo<DIM>::vec x, float r2,
+                                                    CpResult &res)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    res.slot = -1;
+    res.d = FLT_MAX;
+    res.p[0] = res.p[1] = res.p[2] = 0.0f;
+    res.uv[0] = res.uv[1] = 0.0f;
+
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float d2min, d2max;
+    box_sqdist(lo, hi, x, d2min, d2max);
+    if (!(d2min <= r2)) return;
+    r2 = fminf(r2, d2max);
+    float cur_d2 = d2min;
+
+    FCPW_STACK_DECL(BLOCK, K);
+    vec bp = {};
+    while (true) {
+        if (cur_d2 <= r2) {
+            if (cur & LEAF_BIT) {
+                uint32_t off = cur & OFF_MASK;
+                uint32_t cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    vec p;
+                    float u0, u1;
+                    float d = G::closest(S, off + k, x, p, u0, u1);
+                    if (d * d <= r2) {
+                        r2 = fminf(r2, d * d);
+                        res.slot = (int32_t)(off + k);
+                        res.d = d;
+                        bp = p;
+                        res.uv[0] = u0;
+                        res.uv[1] = u1;
+                    }
+                }
+            } else {
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                float a0, a1, b0, b1;
+                box_sqdist(loA, hiA, x, a0, a1);
+                bool oA = a0 <= r2;
+                r2 = fminf(r2, a1);
+                box_sqdist(loB, hiB, x, b0, b1);
+                bool oB = b0 <= r2;
+                r2 = fminf(r2, b1);
+                if (oA && oB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearD = a0, farD = b0;
+                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                    FCPW_PUSH(BLOCK, K, farL, farD, { atomicOr(S.flags, 1); break; });
+                    cur = nearL;
+                    cur_d2 = nearD;
+                    continue;
+                } else if (oA) {
+                    cur = lA; cur_d2 = a0;
+                    continue;
+                } else if (oB) {
+                    cur = lB; cur_d2 = b0;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        FCPW_POP(BLOCK, K, cur, cur_d2);
+    }
+    if (res.slot >= 0) store_p(bp, res.p);
+}
+
+
+// ---------------------------------------------------------------------------
+// ray traversal (bvh.slang:144-254 order)
+// ---------------------------------------------------------------------------
+struct RayResult {
+    int32_t slot;
+    float t;
+    float p[3];
+    float n[3];
+    float uv[2];
+    uint32_t n_internal, n_prims; // records expanded and primitives tested
+};
+
+template <int DIM>
+struct RayGeo;
+
+template <>
+struct RayGeo<3> {
+    static __device__ __forceinline__ bool hit(const DevScene &S, uint32_t slot, v3 o, v3 d, float tmax,
+                                               RayResult &r, bool fill)
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        float tt, v, w;
+        v3 pa = {a.x, a.y, a.z}, pb = {b.x, b.y, b.z}, pc = {c.x, c.y, c.z};
+        if (!ray_triangle(pa, pb, pc, o, d, tmax, tt, v, w)) return false;
+        if (fill) {
+            v3 v1 = sub(pb, pa), v2 = sub(pc, pa);
+            r.p[0] = (pa.x + v1.x * v) + v2.x * w;
+            r.p[1] = (pa.y + v1.y * v) + v2.y * w;
+            r.p[2] = (pa.z + v1.z * v) + v2.z * w;
+            v3 n = cross(v1, v2);
+            float l = sqrtf(dot(n, n));
+            r.n[0] = n.x / l; r.n[1] = n.y / l; r.n[2] = n.z / l;
+            r.uv[0] = 1.0f - v - w;
+            r.uv[1] = v;
+            r.t = tt;
+        }
+        return true;
+    }
+};
+
+template <>
+struct RayGeo<2> {
+    static __device__ __forceinline__ bool hit(const DevScene &S, uint32_t slot, v2 o, v2 d, float tmax,
+                                               RayResult &r, bool fill)
+    {
+        float4 s4 = S.prims[slot];
+        v2 pa = {s4.x, s4.y}, pb = {s4.z, s4.w};
+        float tt, s;
+        if (!ray_segment(pa, pb, o, d, tmax, tt, s)) return false;
+        if (fill) {
+            v2 v = sub(pb, pa);
+            r.p[0] = pa.x + s * v.x;
+            r.p[1] = pa.y + s * v.y;
+            r.p[2] = 0.0f;
+            v2 n = {v.y, -v.x};
+            float l = sqrtf(dot(n, n));
+            r.n[0] = n.x / l; r.n[1] = n.y / l; r.n[2] = 0.0f;
+            r.uv[0] = s;
+            r.uv[1] = 0.0f;
+            r.t = tt;
+        }
+        return true;
+    }
+};
+
+template <int DIM, int BLOCK, int K>
+__device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::vec o, typename Geo<DIM>::vec d,
+                                          typename Geo<DIM>::vec dinv, float tmax, bool occlusion,
+                                          RayResult &res)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    res.slot = -1;
+    res.t = FLT_MAX;
+    for (int i = 0; i < 3; i++) { res.p[i] = 0.0f; res.n[i] = 0.0f; }
+    res.uv[0] = res.uv[1] = 0.0f;
+    res.n_internal = res.n_prims = 0;
+
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float cur_t;
+    if (!box_ray(lo, hi, o, d, dinv, tmax, cur_t)) return;
+
+    FCPW_STACK_DECL(BLOCK, K);
+    while (true) {
+        if (!(cur_t > tmax)) {
+            if (cur & LEAF_BIT) {
+                uint32_t off = cur & OFF_MASK;
+                uint32_t cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                bool stop = false;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    res.n_prims++;
+                    if (RayGeo<DIM>::hit(S, off + k, o, d, tmax, res, !occlusion)) {
+                        res.slot = (int32_t)(off + k);
+                        if (occlusion) { stop = true; break; }
+                        tmax = fminf(tmax, res.t);
+                    }
+                }
+                if (stop) break;
+            } else {
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                res.n_internal++;
+                float ta, tb;
+                bool hA = box_ray(loA, hiA, o, d, dinv, tmax, ta);
+                bool hB = box_ray(loB, hiB, o, d, dinv, tmax, tb);
+                if (hA && hB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearT = ta, farT = tb;
+                    if (tb < ta) { nearL = lB; farL = lA; nearT = tb; farT = ta; }
+                    FCPW_PUSH(BLOCK, K, farL, farT, { atomicOr(S.flags, 1); break; });
+                    cur = nearL;
+                    cur_t = nearT;
+                    continue;
+                } else if (hA) {
+                    cur = lA; cur_t = ta;
+                    continue;
+                } else if (hB) {
+                    cur = lB; cur_t = tb;
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        FCPW_POP(BLOCK, K, cur, cur_t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// interaction record written as five 8-byte stores (the record is 8-byte aligned)
+__device__ __forceinline__ void write_interaction(float2 *o, const float p[3], const float n[3], const float uv[2],
+                                                  float d, uint32_t index)
+{
+    o[0] = make_float2(p[0], p[1]);
+    o[1] = make_float2(p[2], n[0]);
+    o[2] = make_float2(n[1], n[2]);
+    o[3] = make_float2(uv[0], uv[1]);
+    o[4] = make_float2(d, __uint_as_float(index));
+}
+
+// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD
+// b mod xcds).  Renumbering blocks so that XCD k owns one contiguous range of
+// tiles keeps each XCD's L2 on one region of the (Morton-ordered) queries.
+// The tile is formed in 32 bits and widened once at the end: a version that
+// returned a 64-bit value from two branches was compiled (ROCm 7.2 clang) with
+// the high word of the pass-through branch left holding a stale quotient, so
+// the blocks past the last full group of xcds computed out-of-range tiles.
+__device__ __forceinline__ int64_t xcd_tile(int xcds)
+{
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    const uint32_t x = xcds > 1 ? (uint32_t)xcds : 1u;
+    const uint32_t full = (G / x) * x;
+    const uint32_t t = b < full ? (b % x) * (full / x) + b / x : b;
+    return (int64_t)(uint64_t)t;
+}
+
+// query input / output policies shared by the closest-point kernels
+struct CpIO {
+    const float *in;     // spheres (4 floats: c, r2) or points (DIM floats)
+    const float *r2;     // compact input: per-query r^2 or nullptr (unbounded)
+    int64_t n;
+    const uint32_t *order;
+    // interaction output
+    float2 *out;
+    int record_normals;
+    // compact output (any may be nullptr)
+    float *d, *p;
+    int32_t *prim, *slot;
+};
+enum { CP_OUT_INTERACTION = 0, CP_OUT_COMPACT = 1 };
+
+template <int DIM, int OUT>
+__device__ __forceinline__ void cp_load(const CpIO &io, int64_t i, float q[3], float &r2)
+{
+    if (OUT == CP_OUT_INTERACTION) {
+        float4 s = reinterpret_cast<const float4 *>(io.in)[i];
+        q[0] = s.x; q[1] = s.y; q[2] = s.z;
+        r2 = s.w;
+    } else {
+        const float *x = io.in + i * DIM;
+        q[0] = x[0]; q[1] = x[1]; q[2] = DIM == 3 ? x[DIM - 1] : 0.0f;
+        r2 = io.r2 ? io.r2[i] : INFINITY;
+    }
+}
+
+template <int DIM, int OUT>
+__device__ __forceinline__ void cp_store(const DevScene &S, const CpIO &io, int64_t i, const CpResult &r)
+{
+    if (OUT == CP_OUT_INTERACTION) {
+        float nrm[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t index = FCPW_HIP_UINT_MAX;
+        if (r.slot >= 0) {
+            index = (uint32_t)S.prim_index[r.slot];
+            if (io.record_normals) Geo<DIM>::normal(S, (uint32_t)r.slot, nrm);
+        }
+        write_interaction(io.out + i * 5, r.p, nrm, r.uv, r.d, index);
+    } else {
+        if (io.d) io.d[i] = r.d;
+        if (io.p)
+            for (int k = 0; k < DIM; k++) io.p[i * DIM + k] = r.p[k];
+        if (io.prim) io.prim[i] = r.slot >= 0 ? S.prim_index[r.slot] : -1;
+        if (io.slot) io.slot[i] = r.slot;
+    }
+}
+
+// one query per lane, the reference loop
+template <int DIM, int BLOCK, int K, int OUT>
+__global__ __launch_bounds__(BLOCK) void k_closest_points(DevScene S, CpIO io, int xcds)
+{
+    int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    if (k >= io.n) return;
+    int64_t i = io.order ? (int64_t)io.order[k] : k;
+    float q[3], r2;
+    cp_load<DIM, OUT>(io, i, q, r2);
+    CpResult r;
+    closest_point_query<DIM, BLOCK, K>(S, Geo<DIM>::load_point(q), r2, r);
+    cp_store<DIM, OUT>(S, io, i, r);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-cooperative leaf processing (one wave per workgroup).  Lanes descend
+// the tree independently; a lane that reaches a leaf holds it.  Once at least
+// `hold_min` lanes hold a leaf (or every lane still working does), the wave
+// lays all (lane, primitive) pairs of the held leaves out in LDS and all 64
+// lanes -- including lanes whose own query has finished -- evaluate up to 64
+// primitive distances per pass.  Each owner then replays the reference's
+// acceptance test (d*d <= r2, r2 = d*d, later equal distances win) over its
+// own primitives in leaf order.  A distance does not depend on r2, so the
+// replay gives the reference's result bit for bit; the closest point and uv
+// of the winning primitive are recomputed once at the end.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int DIM, int K>
+__device__ __forceinline__ void closest_point_query_coop(const DevScene &S, typename Geo<DIM>::vec x, float r2,
+                                                         bool valid, int hold_min, CpResult &res)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    constexpr int BLOCK = 64;
+    __shared__ float coop_q[64 * 3];
+    __shared__ uint32_t coop_off[64];
+    __shared__ uint16_t coop_item[256];
+    __shared__ float coop_d[256];
+    const int lane = threadIdx.x;
+    float xs[3];
+    store_p(x, xs);
+    coop_q[lane * 3 + 0] = xs[0];
+    coop_q[lane * 3 + 1] = xs[1];
+    coop_q[lane * 3 + 2] = xs[2];
+    res.slot = -1;
+    res.d = FLT_MAX;
+    res.p[0] = res.p[1] = res.p[2] = 0.0f;
+    res.uv[0] = res.uv[1] = 0.0f;
+
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float d2min, d2max, cur_d2 = 0.0f;
+    box_sqdist(lo, hi, x, d2min, d2max);
+    bool active = valid && (d2min <= r2);
+    if (active) {
+        r2 = fminf(r2, d2max);
+        cur_d2 = d2min;
+    }
+    bool live = active, holding = false, overflow = false;
+    uint32_t loff = 0, lcnt = 0;
+    uint32_t n_internal = 0, n_prims = 0;
+    int32_t best = -1;
+    FCPW_STACK_DECL(BLOCK, K);
+    __syncthreads();
+    while (true) {
+        if (active && !holding) {
+            if (!(live && cur_d2 <= r2)) {
+                live = false;
+                while (sp > 0) {
+                    FCPW_POP(BLOCK, K, cur, cur_d2);
+                    if (cur_d2 <= r2) { live = true; break; }
+                }
+            }
+            if (!live) {
+                active = false;
+            } else if (cur & LEAF_BIT) {
+                holding = true;
+                live = false;
+                loff = cur & OFF_MASK;
+                lcnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+            } else {
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+                n_internal++;
+                float a0, a1, b0, b1;
+                box_sqdist(loA, hiA, x, a0, a1);
+                bool oA = a0 <= r2;
+                r2 = fminf(r2, a1);
+                box_sqdist(loB, hiB, x, b0, b1);
+                bool oB = b0 <= r2;
+                r2 = fminf(r2, b1);
+                live = oA || oB;
+                if (oA && oB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearD = a0, farD = b0;
+                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                    FCPW_PUSH(BLOCK, K, farL, farD, { overflow = true; active = false; live = false; });
+                    cur = nearL; cur_d2 = nearD;
+                } else if (oA) {
+                    cur = lA; cur_d2 = a0;
+                } else if (oB) {
+                    cur = lB; cur_d2 = b0;
+                }
+            }
+        }
+        uint64_t act = __ballot(active);
+        if (act == 0) break;
+        uint64_t hold = __ballot(holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+
+        coop_off[lane] = loff;
+        for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
+            bool h0 = holding && lcnt > j0, h1 = holding && lcnt > j0 + 1;
+            bool h2 = holding && lcnt > j0 + 2, h3 = holding && lcnt > j0 + 3;
+            uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            if (m0 == 0) break;
+            uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+            uint32_t total = c0 + c1 + c2 + __popcll(m3);
+            uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                     i3 = c0 + c1 + c2 + lanes_below(m3);
+            if (h0) coop_item[i0] = (uint16_t)((lane << 4) | j0);
+            if (h1) coop_item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
+            if (h2) coop_item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
+            if (h3) coop_item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
+            __syncthreads();
+            for (uint32_t pass = 0; pass < total; pass += 64) {
+                uint32_t k = pass + (uint32_t)lane;
+                if (k < total) {
+                    uint32_t it = coop_item[k];
+                    uint32_t o = it >> 4;
+                    float xo[3] = {coop_q[o * 3], coop_q[o * 3 + 1], coop_q[o * 3 + 2]};
+                    vec pp;
+                    float u0, u1;
+                    coop_d[k] = G::closest(S, coop_off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
+                }
+            }
+            __syncthreads();
+            if (holding) {
+                const uint32_t idx[4] = {i0, i1, i2, i3};
+                for (uint32_t jj = 0; jj < 4; jj++) {
+                    if (lcnt > j0 + jj) {
+                        n_prims++;
+                        float d = coop_d[idx[jj]];
+                        if (d * d <= r2) {
+                            r2 = fminf(r2, d * d);
+                            best = (int32_t)(loff + j0 + jj);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (m3 == 0) break;
+        }
+        holding = false;
+    }
+    if (overflow) atomicOr(S.flags, 1);
+    res.n_internal = n_internal;
+    res.n_prims = n_prims;
+    if (best >= 0) {
+        vec p;
+        float u0, u1;
+        res.d = G::closest(S, (uint32_t)best, x, p, u0, u1);
+        res.slot = best;
+        res.uv[0] = u0;
+        res.uv[1] = u1;
+        store_p(p, res.p);
+    }
+}
+
+// Adds a wave's traversal counts to S.stats (called by every lane of the wave).
+__device__ __forceinline__ void wave_stats_add(const DevScene &S, uint32_t n_internal, uint32_t n_prims,
+                                               uint32_t n_queries)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        n_internal += __shfl_xor(n_internal, off, 64);
+        n_prims += __shfl_xor(n_prims, off, 64);
+        n_queries += __shfl_xor(n_queries, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&S.stats[0], (unsigned long long)n_internal);
+        atomicAdd(&S.stats[1], (unsigned long long)n_prims);
+        atomicAdd(&S.stats[2], (unsigned long long)n_queries);
+    }
+}
+
+template <int DIM, int K, int OUT, bool STATS>
+__global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io, int xcds, int hold_min)
+{
+    int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
+    bool valid = k < io.n;
+    int64_t i = valid ? (io.order ? (int64_t)io.order[k] : k) : 0;
+    float q[3] = {0.0f, 0.0f, 0.0f}, r2 = -1.0f;
+    if (valid) cp_load<DIM, OUT>(io, i, q, r2);
+    CpResult r;
+    closest_point_query_coop<DIM, K>(S, Geo<DIM>::load_point(q), r2, valid, hold_min, r);
+    if (valid) cp_store<DIM, OUT>(S, io, i, r);
+    if (STATS) wave_stats_add(S, r.n_internal, r.n_prims, valid ? 1u : 0u);
+}
+
+template <int DIM, int BLOCK, int K, bool STATS>
+__global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *rays, int64_t n, int occlusion,
+                                                     float2 *out, const uint32_t *order, int xcds)
+{
+    using vec = typename Geo<DIM>::vec;
+    int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    uint32_t n_internal = 0, n_prims = 0;
+    if (k < n) {
+        int64_t i = order ? (int64_t)order[k] : k;
+        const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
+        float2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
+        float o3[3] = {r0.x, r0.y, r1.x}, d3[3] = {r1.y, r2.x, r2.y}, di3[3] = {r3.x, r3.y, r4.x};
+        float tmax = r4.y;
+        vec o = Geo<DIM>::load_point(o3), d = Geo<DIM>::load_point(d3), di = Geo<DIM>::load_point(di3);
+        RayResult r;
+        ray_query<DIM, BLOCK, K>(S, o, d, di, tmax, occlusion != 0, r);
+        n_internal = r.n_internal;
+        n_prims = r.n_prims;
+        uint32_t index = r.slot >= 0 ? (uint32_t)S.prim_index[r.slot] : FCPW_HIP_UINT_MAX;
+        if (occlusion) {
+            // only the occluder's index is written (bvh.slang:183-187)
+            float z[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
+            write_interaction(out + i * 5, z, z, uv, FLT_MAX, index);
+        } else {
+            write_interaction(out + i * 5, r.p, r.n, r.uv, r.t, index);
+        }
+    }
+    if (STATS) wave_stats_add(S, n_internal, n_prims, k < n ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// Morton ordering of a query batch: a counting sort of the queries over a
+// 2^18-cell grid (6 bits per axis in 3D, 9 in 2D) laid along a Z-order curve.
+// Neighbouring lanes then traverse neighbouring parts of the tree, which keeps
+// a wave's branches together and its node fetches in L1/L2.  The order inside
+// a cell is whatever the atomics produce; results do not depend on it.
+// ---------------------------------------------------------------------------
+constexpr int MORTON_BITS = 18;
+constexpr uint32_t MORTON_CELLS = 1u << MORTON_BITS;
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) // 6 bits -> every third bit
+{
+    v &= 0x3Fu;
+    v = (v | (v << 8)) & 0x0000F00Fu;
+    v = (v | (v << 4)) & 0x000C30C3u;
+    v = (v | (v << 2)) & 0x00249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) // 9 bits -> every second bit
+{
+    v &= 0x1FFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quant(float x, float lo, float inv, uint32_t cells)
+{
+    float t = (x - lo) * inv; // NaN and out-of-range values clamp to the border cells
+    int c = (t >= 0.0f) ? (int)fminf(t * (float)cells, (float)(cells - 1)) : 0;
+    return (uint32_t)c;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int DIM>
+__global__ void k_morton_hist(const float *pts, int stride, int64_t n, float4 lo, float4 inv, uint32_t *codes,
+                              uint32_t *hist)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *q = pts + i * stride;
+    uint32_t code;
+    if (DIM == 3)
+        code = spread3(quant(q[0], lo.x, inv.x, 64)) | (spread3(quant(q[1], lo.y, inv.y, 64)) << 1) |
+               (spread3(quant(q[2], lo.z, inv.z, 64)) << 2);
+    else
+        code = spread2(quant(q[0], lo.x, inv.x, 512)) | (spread2(quant(q[1], lo.y, inv.y, 512)) << 1);
+    codes[i] = code;
+    atomicAdd(&hist[code], 1u);
+}
+
+// exclusive scan of MORTON_CELLS counters by one 1024-thread block
+__global__ __launch_bounds__(1024) void k_scan_cells(uint32_t *hist)
+{
+    constexpr int PER = MORTON_CELLS / 1024;
+    __shared__ uint32_t part[1024];
+    uint32_t *h = hist + (size_t)threadIdx.x * PER;
+    uint32_t sum = 0;
+    for (int k = 0; k < PER; k++) sum += h[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) { // Hillis-Steele inclusive scan
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (int k = 0; k < PER; k++) {
+        uint32_t c = h[k];
+        h[k] = run;
+        run += c;
+    }
+}
+
+__global__ void k_morton_scatter(const uint32_t *codes, int64_t n, uint32_t *offsets, uint32_t *perm)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pos = atomicAdd(&offsets[codes[i]], 1u);
+    perm[pos] = (uint32_t)i;
+}
+
+// rebuild the primitive records from vertex positions (caller's vertex order)
+template <int DIM>
+__global__ void k_rebuild_prims(float4 *prims, const int32_t *vids, const float *V, int n_prims)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_prims) return;
+    if (DIM == 3) {
+        for (int k = 0; k < 3; k++) {
+            const float *v = V + (size_t)vids[(size_t)s * 3 + k] * 3;
+            prims[(size_t)s * 3 + k] = make_float4(v[0], v[1], v[2], 0.0f);
+        }
+    } else {
+        const float *a = V + (size_t)vids[(size_t)s * 2] * 2;
+        const float *b = V + (size_t)vids[(size_t)s * 2 + 1] * 2;
+        prims[s] = make_float4(a[0], a[1], b[0], b[1]);
+    }
+}
+
+// Bottom-up refit in one launch: one thread per leaf computes the leaf box,
+// writes it into its parent record's slot and counts the arrival; the second
+// arrival at a record unions the two slot boxes and climbs.  Box values only
+// involve min, max and the +-FLT_EPSILON inflation, so the result is bit-equal
+// to the reference refit (bvh.slang:27-141, bvh.inl:548-581) in any order.
+// The second arrival resets the counter, leaving all counters 0 for the next call.
+template <int DIM>
+__global__ void k_refit(float4 *records, float4 *root, const float4 *prims, const int32_t *leaf_refs,
+                        const uint32_t *leaf_links, const int32_t *rec_parent, unsigned int *counters, int n_leaves)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_leaves) return;
+    uint32_t link = leaf_links[j];
+    uint32_t off = link & OFF_MASK, cnt = ((link >> CNT_SHIFT) & CNT_MASK) + 1u;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t k = 0; k < cnt; k++) {
+        if (DIM == 3) {
+            const float4 *t = prims + (size_t)(off + k) * 3;
+            float4 a = t[0], b = t[1], c = t[2];
+            // Triangle.getBoundingBox (geometry.slang:342-346)
+            lo[0] = fminf(lo[0], fminf(fminf(a.x, b.x), c.x) - FLT_EPSILON);
+            lo[1] = fminf(lo[1], fminf(fminf(a.y, b.y), c.y) - FLT_EPSILON);
+            lo[2] = fminf(lo[2], fminf(fminf(a.z, b.z), c.z) - FLT_EPSILON);
+            hi[0] = fmaxf(hi[0], fmaxf(fmaxf(a.x, b.x), c.x) + FLT_EPSILON);
+            hi[1] = fmaxf(hi[1], fmaxf(fmaxf(a.y, b.y), c.y) + FLT_EPSILON);
+            hi[2] = fmaxf(hi[2], fmaxf(fmaxf(a.z, b.z), c.z) + FLT_EPSILON);
+        } else {
+            float4 s = prims[off + k];
+            // LineSegment.getBoundingBox (geometry.slang:112-116)
+            lo[0] = fminf(lo[0], fminf(s.x, s.z) - FLT_EPSILON);
+            lo[1] = fminf(lo[1], fminf(s.y, s.w) - FLT_EPSILON);
+            hi[0] = fmaxf(hi[0], fmaxf(s.x, s.z) + FLT_EPSILON);
+            hi[1] = fmaxf(hi[1], fmaxf(s.y, s.w) + FLT_EPSILON);
+        }
+    }
+    int32_t ref = leaf_refs[j];
+    while (true) {
+        if (ref < 0) { // the box belongs to the root
+            if (DIM == 3) {
+                float4 a = root[0];
+                root[0] = make_float4(lo[0], lo[1], lo[2], a.w);
+                root[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+            } else {
+                root[0] = make_float4(lo[0], lo[1], hi[0], hi[1]);
+            }
+            return;
+        }
+        int32_t rec = ref >> 1, slot = ref & 1;
+        if (DIM == 3) {
+            float4 *p = records + (size_t)rec * 4 + slot * 2;
+            float w = p[0].w;
+            p[0] = make_float4(lo[0], lo[1], lo[2], w);
+            p[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        } else {
+            records[(size_t)rec * 3 + slot] = make_float4(lo[0], lo[1], hi[0], hi[1]);
+        }
+        __threadfence();
+        unsigned int prev = atomicAdd(&counters[rec], 1u);
+        if (prev == 0) return;
+        counters[rec] = 0u;
+        __threadfence();
+        if (DIM == 3) {
+            const float4 *p = records + (size_t)rec * 4;
+            float4 a0 = p[0], a1 = p[1], b0 = p[2], b1 = p[3];
+            lo[0] = fminf(a0.x, b0.x); lo[1] = fminf(a0.y, b0.y); lo[2] = fminf(a0.z, b0.z);
+            hi[0] = fmaxf(a1.x, b1.x); hi[1] = fmaxf(a1.y, b1.y); hi[2] = fmaxf(a1.z, b1.z);
+        } else {
+            const float4 *p = records + (size_t)rec * 3;
+            float4 a = p[0], b = p[1];
+            lo[0] = fminf(a.x, b.x); lo[1] = fminf(a.y, b.y);
+            hi[0] = fmaxf(a.z, b.z); hi[1] = fmaxf(a.w, b.w);
+        }
+        ref = rec_parent[rec];
+    }
+}
+
+// Adjoint of d(x, mesh): recompute p and its barycentric weights from the saved
+// leaf slot, then d(d)/dx = (x - p)/d and d(d)/dV_k = -b_k (x - p)/d.
+// one query of the adjoint (shared by the single-scene and batched kernels)
+template <int DIM>
+__device__ __forceinline__ void distance_backward_one(const float4 *prims, const int32_t *prim_vids, int n_prims,
+                                                      const float *points, const int32_t *slots,
+                                                      const float *grad_d, int64_t i, float *grad_points,
+                                                      float *grad_vertices)
+{
+    int32_t s = slots[i];
+    if (s >= n_prims) s = -1; // not a slot of this scene
+    float g[DIM];
+    for (int k = 0; k < DIM; k++) g[k] = 0.0f;
+    float gd = grad_d[i];
+    if (s >= 0) {
+        float dist, w[3];
+        float diff[3] = {0.0f, 0.0f, 0.0f};
+        if (DIM == 3) {
+            const float4 *t = prims + (size_t)s * 3;
+            float4 a = t[0], b = t[1], c = t[2];
+            v3 x = {points[i * 3], points[i * 3 + 1], points[i * 3 + 2]}, p;
+            float t0, t1;
+            dist = closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, t0, t1);
+            w[0] = t0; w[1] = t1; w[2] = 1.0f - t0 - t1;
+            diff[0] = x.x - p.x; diff[1] = x.y - p.y; diff[2] = x.z - p.z;
+        } else {
+            float4 q = prims[s];
+            v2 x = {points[i * 2], points[i * 2 + 1]}, p;
+            float t;
+            dist = closest_point_segment({q.x, q.y}, {q.z, q.w}, x, p, t);
+            w[0] = 1.0f - t; w[1] = t;
+            diff[0] = x.x - p.x; diff[1] = x.y - p.y;
+        }
+        if (dist > 0.0f) {
+            float sc = gd / dist;
+            for (int k = 0; k < DIM; k++) g[k] = sc * diff[k];
+            if (grad_vertices) {
+                for (int v = 0; v < DIM; v++) {
+                    int32_t vid = prim_vids[(size_t)s * DIM + v];
+                    for (int k = 0; k < DIM; k++) atomicAdd(&grad_vertices[(size_t)vid * DIM + k], -w[v] * g[k]);
+                }
+            }
+        }
+    }
+    if (grad_points)
+        for (int k = 0; k < DIM; k++) grad_points[i * DIM + k] = g[k];
+}
+
+template <int DIM>
+__global__ void k_distance_backward(const float4 *prims, const int32_t *prim_vids, const float *points,
+                                    const int32_t *slots, const float *grad_d, int64_t n, int n_prims,
+                                    float *grad_points, float *grad_vertices)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    distance_backward_one<DIM>(prims, prim_vids, n_prims, points, slots, grad_d, i, grad_points, grad_vertices);
+}
+
+} // namespace fcpw_hip

@@ -404,4 +404,70 @@ private:
     fcpw_hip_ctx *ctx_ = nullptr;
 };
 
+// Several GPUScenes queried in one persistent launch (no reference counterpart:
+// the reference GPU path serves one object per GPUScene, fcpw_gpu.h:17-20).
+// The scenes must outlive the batch.  Queries are concatenated in scene order,
+// counts[s] for scene s; see fcpw_hip_batch_* in include/fcpw_hip.h.
+template <int DIM>
+class GPUSceneBatch {
+public:
+    explicit GPUSceneBatch(const std::vector<GPUScene<DIM> *> &scenes)
+    {
+        std::vector<fcpw_hip_ctx *> ctxs;
+        for (GPUScene<DIM> *g : scenes) ctxs.push_back(g->context());
+        check(fcpw_hip_batch_create(ctxs.data(), (int)ctxs.size(), &b_), "GPUSceneBatch");
+        n_scenes_ = (int)scenes.size();
+    }
+    ~GPUSceneBatch()
+    {
+        if (b_) fcpw_hip_batch_destroy(b_);
+    }
+    GPUSceneBatch(const GPUSceneBatch &) = delete;
+    GPUSceneBatch &operator=(const GPUSceneBatch &) = delete;
+
+    int nScenes() const { return n_scenes_; }
+    void setQueries(const std::vector<int64_t> &counts)
+    {
+        if ((int)counts.size() != n_scenes_) throw std::invalid_argument("setQueries: one count per scene");
+        check(fcpw_hip_batch_set_queries(b_, counts.data()), "setQueries");
+    }
+    void findClosestPointsDevice(const GPUBoundingSphere *spheres, GPUInteraction *out, bool recordNormals,
+                                 void *stream)
+    {
+        check(fcpw_hip_batch_closest_points_device(b_, (const fcpw_hip_bounding_sphere *)spheres,
+                                                   recordNormals ? 1 : 0, (fcpw_hip_interaction *)out, stream),
+              "findClosestPointsDevice");
+    }
+    void closestPointsCompactDevice(const float *points, const float *r2, float *d, float *p, int32_t *prim,
+                                    int32_t *slot, void *stream)
+    {
+        check(fcpw_hip_batch_closest_points_compact_device(b_, points, r2, d, p, prim, slot, stream),
+              "closestPointsCompactDevice");
+    }
+    void distanceBackwardDevice(const float *points, const int32_t *slot, const float *gradD, float *gradPoints,
+                                float *gradVertices, void *stream)
+    {
+        check(fcpw_hip_batch_distance_backward_device(b_, points, slot, gradD, gradPoints, gradVertices, stream),
+              "distanceBackwardDevice");
+    }
+    void setOption(const std::string &key, int value) { check(fcpw_hip_batch_set_option(b_, key.c_str(), value), "setOption"); }
+    int getOption(const std::string &key) const
+    {
+        int v = 0;
+        check(fcpw_hip_batch_get_option(b_, key.c_str(), &v), "getOption");
+        return v;
+    }
+
+private:
+    void check(int status, const char *what) const
+    {
+        if (status == FCPW_HIP_OK) return;
+        std::string msg = std::string(what) + ": " + fcpw_hip_batch_last_error(b_);
+        if (status == FCPW_HIP_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+        throw std::runtime_error(msg);
+    }
+    fcpw_hip_batch *b_ = nullptr;
+    int n_scenes_ = 0;
+};
+
 } // namespace fcpw_amd

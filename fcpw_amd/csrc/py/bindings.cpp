@@ -259,6 +259,37 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
         });
 }
 
+template <int DIM>
+static void bind_gpu_scene_batch(py::module_ &m, const char *name)
+{
+    using B = GPUSceneBatch<DIM>;
+    using G = GPUScene<DIM>;
+    py::class_<B>(m, name)
+        .def(py::init([](const std::vector<G *> &scenes) { return new B(scenes); }), "gpu_scenes"_a,
+             py::keep_alive<1, 2>())
+        .def_property_readonly("n_scenes", &B::nScenes)
+        .def("set_queries", &B::setQueries, "counts"_a)
+        .def("_closest_points_device",
+             [](B &b, uintptr_t spheres, bool normals, uintptr_t out, uintptr_t stream) {
+                 b.findClosestPointsDevice((const GPUBoundingSphere *)spheres, (GPUInteraction *)out, normals,
+                                           (void *)stream);
+             })
+        .def("_closest_points_compact_device",
+             [](B &b, uintptr_t points, uintptr_t r2, uintptr_t d, uintptr_t p, uintptr_t prim, uintptr_t slot,
+                uintptr_t stream) {
+                 b.closestPointsCompactDevice((const float *)points, (const float *)r2, (float *)d, (float *)p,
+                                              (int32_t *)prim, (int32_t *)slot, (void *)stream);
+             })
+        .def("_distance_backward_device",
+             [](B &b, uintptr_t points, uintptr_t slot, uintptr_t grad_d, uintptr_t gx, uintptr_t gv,
+                uintptr_t stream) {
+                 b.distanceBackwardDevice((const float *)points, (const int32_t *)slot, (const float *)grad_d,
+                                          (float *)gx, (float *)gv, (void *)stream);
+             })
+        .def("_set_option", &B::setOption)
+        .def("_get_option", &B::getOption);
+}
+
 template <typename T, size_t W>
 static py::array_t<float> gather(const InteractionList &v, size_t field_offset)
 {
@@ -357,6 +388,8 @@ PYBIND11_MODULE(_fcpw_amd, m)
     bind_scene<3>(m, "Scene3D");
     bind_gpu_scene<2>(m, "GPUScene2D");
     bind_gpu_scene<3>(m, "GPUScene3D");
+    bind_gpu_scene_batch<2>(m, "GPUSceneBatch2D");
+    bind_gpu_scene_batch<3>(m, "GPUSceneBatch3D");
 
     m.attr("UINT_MAX") = (uint32_t)FCPW_HIP_UINT_MAX;
 }

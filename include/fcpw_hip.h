@@ -156,6 +156,37 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, co
                                       const float *grad_d, int64_t n, float *grad_points, float *grad_vertices,
                                       void *stream);
 
+/*
+ * Batched queries over several uploaded contexts in one persistent launch (no
+ * reference counterpart: the reference GPU path serves one object per
+ * GPUScene, fcpw_gpu.h:17-20).  The scenes must share dim and device and stay
+ * alive while the batch is used; a re-uploaded member is picked up by the next
+ * call.  Queries are concatenated in scene order: counts[s] queries for scene
+ * s (fcpw_hip_batch_set_queries, which also sizes every workspace).  After
+ * that, the query entry points allocate nothing and only enqueue kernels on
+ * the stream, so a caller may capture them in a HIP graph.  Results index each
+ * scene's own primitives; grad_vertices is the concatenation of the scenes'
+ * vertex gradients (each scene's vertices in its caller order).
+ */
+typedef struct fcpw_hip_batch fcpw_hip_batch;
+int fcpw_hip_batch_create(fcpw_hip_ctx *const *scenes, int n_scenes, fcpw_hip_batch **out);
+int fcpw_hip_batch_destroy(fcpw_hip_batch *batch);
+/* Last error of batch, or of the failed create when batch is NULL. */
+const char *fcpw_hip_batch_last_error(const fcpw_hip_batch *batch);
+int fcpw_hip_batch_set_queries(fcpw_hip_batch *batch, const int64_t *counts);
+int fcpw_hip_batch_closest_points_device(fcpw_hip_batch *batch, const fcpw_hip_bounding_sphere *spheres,
+                                         int record_normals, fcpw_hip_interaction *out, void *stream);
+int fcpw_hip_batch_closest_points_compact_device(fcpw_hip_batch *batch, const float *points, const float *r2,
+                                                 float *d, float *p, int32_t *prim, int32_t *slot, void *stream);
+int fcpw_hip_batch_distance_backward_device(fcpw_hip_batch *batch, const float *points, const int32_t *slot,
+                                            const float *grad_d, float *grad_points, float *grad_vertices,
+                                            void *stream);
+/* Batch knobs (none changes a result bit): "stack" (0, 4, 8, 16), "hold_min"
+   (1..64), "chunk" (queries per work item: 64, 128, 256, 512), "sort" (0, 1),
+   "waves_per_cu" (0 = as many as fit, else 1..32). */
+int fcpw_hip_batch_set_option(fcpw_hip_batch *batch, const char *key, int value);
+int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int *value);
+
 /* Tuning knobs read by the kernels (values chosen by fcpw_amd.autotune).  None
    of them changes a result bit: every setting visits nodes in the same order.
      "coop"      1 = wave-cooperative leaf processing (default), 0 = one query per
