@@ -49,6 +49,9 @@ def log(*a):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--workload", choices=("torus", "batch", "star2d", "diff"), default="torus",
+                   help="torus: 1M-tri torus, 16M queries (the headline line); batch: 256 uneven meshes; "
+                        "star2d: 262,144-segment polygon with refit; diff: 200k-tri differentiable distance")
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--queries", type=int, default=1 << 24)
@@ -212,11 +215,18 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    runners = {"torus": run_torus, "batch": run_batch, "star2d": run_star2d, "diff": run_diff}
+    res = runners[args.workload](args, torch)
+    print(json.dumps(res), flush=True)
 
+
+def run_torus(args, torch):
+    """The 1,048,576-triangle torus with 16M closest-point queries (plus the
+    fwd+bwd and ray rates and the CPU baseline on the same queries)."""
     import fcpw_amd as fcpw
     from fcpw_amd import synth
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     U, Vg = args.grid
     t0 = time.time()
@@ -357,7 +367,223 @@ def main():
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(V, F, q, r2, args.cpu_sample)
         log(f"[cpu] {res['cpu_baseline']['value']:.3e} queries/s on {res['cpu_baseline']['cores']} threads")
-    print(json.dumps(res), flush=True)
+    return res
+
+
+def _scene(fcpw, V, F):
+    dim = V.shape[1]
+    s = fcpw.Scene3D() if dim == 3 else fcpw.Scene2D()
+    s.set_object_count(1)
+    s.set_object_vertices(V, 0)
+    (s.set_object_triangles if dim == 3 else s.set_object_line_segments)(F, 0)
+    s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    g = fcpw.GPUScene3D("") if dim == 3 else fcpw.GPUScene2D("")
+    g.transfer_to_gpu(s)
+    return s, g
+
+
+def _spheres(torch, q, dev):
+    sph = torch.empty((len(q), 4), dtype=torch.float32, device=dev)
+    sph[:, :3] = 0.0
+    sph[:, :q.shape[1]] = torch.from_numpy(q).to(dev)
+    sph[:, 3] = float("inf")
+    return sph
+
+
+def _rays(torch, o, d, dev):
+    dim = o.shape[1]
+    rays = torch.zeros((len(o), 10), dtype=torch.float32, device=dev)
+    rays[:, 0:dim] = torch.from_numpy(o).to(dev)
+    rays[:, 3:3 + dim] = torch.from_numpy(d).to(dev)
+    rays[:, 6:9] = 1.0 / rays[:, 3:6]
+    rays[:, 9] = float("inf")
+    return rays
+
+
+def run_batch(args, torch):
+    """256 independent meshes (alternately tori and geodesic spheres, triangle
+    counts log-uniform in 1k-500k) with log-uniform 1k-1M queries each (half
+    uniform in 1.2x the mesh box, half within 0.05 of the surface): one
+    persistent launch over all of them, replayed from a HIP graph, against one
+    plain launch per mesh; forward, and compact forward + adjoint."""
+    import fcpw_amd as fcpw
+    from fcpw_amd import autotune, synth
+    from fcpw_amd.batch import BatchedScenes
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    meshes = synth.workload5()
+    gpus = []
+    for m in meshes:
+        gpus.append(_scene(fcpw, m["positions"], m["indices"])[1])
+    counts = [len(m["queries"]) for m in meshes]
+    q_all = np.concatenate([m["queries"] for m in meshes])
+    n = len(q_all)
+    n_tris = sum(len(m["indices"]) for m in meshes)
+    log(f"[setup] {len(meshes)} meshes, {n_tris} triangles, {n} queries in {time.time() - t0:.1f}s")
+    sph = _spheres(torch, q_all, dev)
+    out = torch.empty((n, 10), dtype=torch.float32, device=dev)
+    out_sep = torch.empty_like(out)
+    batch = BatchedScenes(gpus)
+    batch.set_queries(counts)
+    t0 = time.time()
+    best, _ = autotune.tune_batch(batch, sph, out, reps=1)
+    log(f"[tune] {best} in {time.time() - t0:.1f}s")
+
+    def fwd():
+        batch.find_closest_points_device(sph, out)
+
+    def sep():
+        batch.separate_find_closest_points_device(sph, out_sep)
+    t_eager = timed(torch, None, fwd, args.steps, args.warmup)
+    graph = batch.capture(fwd)
+    t_graph = timed(torch, None, graph.replay, args.steps, args.warmup)
+    t_sep = timed(torch, None, sep, args.steps, args.warmup)
+    same = bool(torch.equal(out.view(torch.int32), out_sep.view(torch.int32)))
+    log(f"[fwd] graph {t_graph * 1e3:.2f} ms, eager {t_eager * 1e3:.2f} ms, separate {t_sep * 1e3:.2f} ms, "
+        f"identical {same}")
+    del out_sep
+
+    x = sph[:, :3].contiguous()
+    d = torch.empty(n, dtype=torch.float32, device=dev)
+    slot = torch.empty(n, dtype=torch.int32, device=dev)
+    gd = torch.ones(n, dtype=torch.float32, device=dev)
+    gx = torch.empty_like(x)
+    gV = torch.zeros((int(batch.vertex_offsets[-1]), 3), dtype=torch.float32, device=dev)
+
+    def fwd_bwd():
+        batch.closest_points_compact_device(x, d, slot=slot)
+        gV.zero_()
+        batch.distance_backward_device(x, slot, gd, gx, gV)
+    graph_fb = batch.capture(fwd_bwd)
+    t_fb = timed(torch, None, graph_fb.replay, args.steps, args.warmup)
+    st = torch.cuda.current_stream().cuda_stream
+
+    def fwd_bwd_sep():
+        gV.zero_()
+        for s_, g in enumerate(gpus):
+            a, b = int(batch.offsets[s_]), int(batch.offsets[s_ + 1])
+            v0 = int(batch.vertex_offsets[s_])
+            if b > a:
+                g._closest_points_compact_device(x[a:].data_ptr(), 0, b - a, d[a:].data_ptr(), 0, 0,
+                                                 slot[a:].data_ptr(), st)
+                g._distance_backward_device(x[a:].data_ptr(), slot[a:].data_ptr(), gd[a:].data_ptr(), b - a,
+                                            gx[a:].data_ptr(), gV[v0:].data_ptr(), st)
+    t_fb_sep = timed(torch, None, fwd_bwd_sep, args.steps, args.warmup)
+    log(f"[fwd+bwd] graph {t_fb * 1e3:.2f} ms, separate {t_fb_sep * 1e3:.2f} ms")
+    return {"metric": "closest-point queries/s, batched uneven workload (one persistent launch)", "value": n / t_graph,
+            "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": t_graph * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{len(meshes)} meshes (tori / geodesic spheres, 1k-500k triangles log-uniform), "
+                                   f"1k-1M queries each, {n_tris} triangles and {n} queries in total",
+                       "meshes": len(meshes), "triangles": n_tris, "queries": n, **batch.options()},
+            "eager": {"value": n / t_eager, "ms_per_step": t_eager * 1e3},
+            "separate_launches": {"value": n / t_sep, "ms_per_step": t_sep * 1e3, "launches_per_step": len(meshes),
+                                  "identical_results": same},
+            "speedup_vs_separate": t_sep / t_graph,
+            "fwd_bwd": {"value": n / t_fb, "ms_per_step": t_fb * 1e3, "ratio_to_fwd": t_fb / t_graph,
+                        "separate_ms_per_step": t_fb_sep * 1e3}}
+
+
+def run_star2d(args, torch):
+    """262,144-segment star polygon, 8M queries uniform in [-1.5, 1.5]^2:
+    closest points and rays (directions uniform on S^1); then 10 steps that
+    each move a random 10% of the vertices radially by U(-0.01, 0.01), upload
+    them and refit the BVH on the device, with the queries rerun after each."""
+    import fcpw_amd as fcpw
+    from fcpw_amd import synth
+
+    dev = torch.device("cuda:0")
+    w = synth.workload3()
+    V, F = w["positions"], w["indices"]
+    scene, gpu = _scene(fcpw, V, F)
+    n = len(w["queries"])
+    sph = _spheres(torch, w["queries"], dev)
+    out = torch.empty((n, 10), dtype=torch.float32, device=dev)
+    rays = _rays(torch, w["ray_o"], w["ray_d"], dev)
+    rout = torch.empty((n, 10), dtype=torch.float32, device=dev)
+    t_cp = timed(torch, None, lambda: gpu.find_closest_points_device(sph, out), args.steps, args.warmup)
+    t_ray = timed(torch, None, lambda: gpu.intersect_device(rays, rout), args.steps, args.warmup)
+    st = torch.cuda.current_stream().cuda_stream
+    refit_ms, cp_ms = [], []
+    Vd = torch.empty((len(V), 2), dtype=torch.float32, device=dev)
+    Vk = V
+    for step in range(10):
+        Vk = synth.perturb_star(Vk, step)
+        Vd.copy_(torch.from_numpy(Vk))
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        gpu._update_positions_device(Vd.data_ptr(), len(V), st)
+        gpu._refit_device(st)
+        b.record()
+        b.synchronize()
+        refit_ms.append(a.elapsed_time(b))
+        cp_ms.append(timed(torch, None, lambda: gpu.find_closest_points_device(sph, out), 3, 0) * 1e3)
+    log(f"[star2d] closest {t_cp * 1e3:.2f} ms, rays {t_ray * 1e3:.2f} ms, refit {np.median(refit_ms):.3f} ms")
+    return {"metric": "closest-point queries/s, 2D segment scene with refit", "value": n / t_cp, "unit": "queries/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_cp * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"star polygon {len(F)} segments, {n} queries uniform in [-1.5,1.5]^2, r2=inf",
+                       "segments": len(F), "queries": n, **{k: gpu.get_option(k) for k in ("coop", "stack", "hold_min")}},
+            "rays": {"value": n / t_ray, "unit": "rays/s", "ms_per_step": t_ray * 1e3},
+            "refit": {"ms_median": float(np.median(refit_ms)), "ms_max": float(np.max(refit_ms)),
+                      "what": "vertex upload (device gather) + one bottom-up refit launch, 10 perturbation steps"},
+            "closest_after_refit_ms": [round(v, 3) for v in cp_ms]}
+
+
+def run_diff(args, torch):
+    """Differentiable distance: class-I geodesic sphere of frequency 100
+    (200,000 triangles, radial noise 0.01), 4M query points within 0.1 of the
+    surface, loss = mean d^2 minimised over the vertex positions with Adam for
+    100 steps; each step uploads the vertices, refits the BVH on the device,
+    runs the compact forward and the adjoint (dL/dx and dL/dV)."""
+    from fcpw_amd import synth
+    from fcpw_amd.torch_ops import closest_point_distance
+    import fcpw_amd as fcpw
+
+    dev = torch.device("cuda:0")
+    w = synth.workload4()
+    V, F = w["positions"], w["indices"]
+    scene, gpu = _scene(fcpw, V, F)
+    n = len(w["queries"])
+    x = torch.from_numpy(w["queries"]).to(dev).requires_grad_(True)
+    Vt = torch.nn.Parameter(torch.from_numpy(V).to(dev))
+    opt = torch.optim.Adam([Vt], lr=1e-4)
+    losses = []
+
+    def step():
+        opt.zero_grad(set_to_none=False)
+        d, p, prim = closest_point_distance(gpu, x, Vt)
+        loss = (d * d).mean()
+        loss.backward()
+        opt.step()
+        return loss
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(100):
+        loss = step()
+        if k % 10 == 0:
+            losses.append(float(loss.detach()))
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / 100
+    # the forward alone on the same queries, for the fwd+bwd ratio
+    with torch.no_grad():
+        t_fwd = timed(torch, None, lambda: closest_point_distance(gpu, x.detach()), args.steps, args.warmup)
+    log(f"[diff] step {t_step * 1e3:.2f} ms (forward alone {t_fwd * 1e3:.2f} ms), loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    return {"metric": "differentiable closest-point queries/s (forward + adjoint + refit + Adam step)",
+            "value": n / t_step, "unit": "queries/s", "n_gpus": 1, "steps": 100, "warmup": args.warmup,
+            "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"geodesic sphere {len(F)} tris, {n} queries within 0.1 of the surface, "
+                                   "loss mean d^2, Adam lr 1e-4, 100 steps with device refit",
+                       "triangles": len(F), "queries": n},
+            "forward_only": {"value": n / t_fwd, "ms_per_step": t_fwd * 1e3},
+            "ratio_step_to_forward": t_step / t_fwd, "loss_every_10_steps": losses}
 
 
 if __name__ == "__main__":
