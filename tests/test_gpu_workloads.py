@@ -1,0 +1,136 @@
+"""The benchmark configurations at their full mesh sizes, on subsamples of
+their queries: bit-exact parity with the oracle's reference-order traversal,
+and the error against an FP64 brute force stated per configuration.
+
+Error statistic (reported on stdout as one JSON line per configuration):
+  rel = |d32 - d64| / max(d64, floor),   floor = 4 eps32 max|coordinate| / 1e-5
+An FP32 distance carries an absolute error of a few ulp of the coordinates
+(x - p cancels), so 1e-5 relative is attainable only where d >= floor; below
+it the bar max rel <= 1e-5 reads as |d32 - d64| <= 4 eps32 max|coordinate|.
+Second bar: no index differs from the FP64 winner except at FP64-classified
+ties.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import oracle
+from fcpw_amd import synth
+from tests import meshes
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+fcpw = pytest.importorskip("fcpw_amd")
+
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def transfer(V, F):
+    dim = V.shape[1]
+    s = fcpw.Scene3D() if dim == 3 else fcpw.Scene2D()
+    s.set_object_count(1)
+    s.set_object_vertices(V, 0)
+    (s.set_object_triangles if dim == 3 else s.set_object_line_segments)(F, 0)
+    s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    g = fcpw.GPUScene3D("") if dim == 3 else fcpw.GPUScene2D("")
+    g.transfer_to_gpu(s)
+    return s, g
+
+
+def gpu_closest(g, q):
+    dim = q.shape[1]
+    sph = torch.zeros((len(q), 4), dtype=torch.float32, device="cuda")
+    sph[:, :dim] = torch.from_numpy(q).cuda()
+    sph[:, 3] = float("inf")
+    out = torch.empty((len(q), 10), dtype=torch.float32, device="cuda")
+    g.find_closest_points_device(sph, out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel()
+
+
+def fp64_report(name, V, F, q, rec):
+    d64, p64, idx64 = oracle.brute_closest_f64(V, F, q)
+    floor = 4 * EPS32 * max(float(np.abs(V).max()), float(np.abs(q).max())) / 1e-5
+    d32 = rec["d"].astype(np.float64)
+    rel = np.abs(d32 - d64) / np.maximum(d64, floor)
+    idx = rec["index"].view(np.int32)
+    mism = idx != idx64
+    ties = oracle.classify_index_mismatch(V, F, q[mism], idx[mism], idx64[mism]) if mism.any() else np.zeros(0, bool)
+    above = d64 >= floor
+    report = dict(config=name, queries=len(q), max_rel=float(rel.max()), p999_rel=float(np.quantile(rel, 0.999)),
+                  floor=floor, share_above_floor=float(above.mean()),
+                  max_rel_above_floor=float(rel[above].max()) if above.any() else None,
+                  max_abs_below_floor=float(np.abs(d32 - d64)[~above].max()) if (~above).any() else None,
+                  index_mismatch=int(mism.sum()), non_tie=int((~ties).sum()))
+    print(json.dumps(report))
+    assert rel.max() <= 1e-5, report
+    assert report["non_tie"] == 0, report
+    return report
+
+
+def test_config1_icosphere_20k(gpu):
+    w = synth.workload1(65536)
+    V, F, q = w["positions"], w["indices"], w["queries"]
+    assert len(F) == 20480
+    s, g = transfer(V, F)
+    rec = gpu_closest(g, q)
+    ref = oracle.Bvh(V, F).closest_points_gpuorder(q)
+    np.testing.assert_array_equal(rec["d"], ref["d"])
+    np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
+    fp64_report("1: icosphere 20,480 tris, uniform in 1.5x box", V, F, q[:8192], rec[:8192])
+
+
+def test_config2_torus_1m(gpu):
+    v, f = synth.torus_mesh(1024, 512)
+    V, F = synth.f32(v), f.astype(np.int32)
+    lo, hi = V.min(0).astype(np.float64), V.max(0).astype(np.float64)
+    rq = synth.Rng(synth.seed(2, 2))
+    q = synth.f32(np.concatenate([synth.uniform_in_box(16384, lo, hi, 1.2, rq),
+                                  synth.near_surface(16384, v, f, 0.05, rq)]))
+    s, g = transfer(V, F)
+    rec = gpu_closest(g, q)
+    ob = oracle.Bvh(V, F)
+    ref = ob.closest_points_gpuorder(q)
+    np.testing.assert_array_equal(rec["d"], ref["d"])
+    np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
+    sub = np.r_[0:1024, 16384:16384 + 1024]
+    fp64_report("2: torus 1,048,576 tris, half uniform / half near-surface", V, F, q[sub], rec[sub])
+
+
+def test_config3_star_refit_steps(gpu):
+    w = synth.workload3(n_queries=1 << 16)
+    V, F, q = w["positions"], w["indices"], w["queries"]
+    s, g = transfer(V, F)
+    ob = oracle.Bvh(V, F)
+    o, d = w["ray_o"], w["ray_d"]
+    Vk = V
+    for step in range(11):
+        if step:
+            Vk = synth.perturb_star(Vk, step - 1)
+            s.update_object_vertices(Vk, 0)
+            g.refit(s)
+            ob.update_vertices(Vk)
+            ob.refit()
+        rec = gpu_closest(g, q)
+        ref = ob.closest_points_gpuorder(q)
+        np.testing.assert_array_equal(rec["d"], ref["d"], err_msg=f"step {step}")
+        np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
+        if step in (0, 10):
+            hits = fcpw.GPUInteractionList()
+            g.intersect(o[:16384], d[:16384], np.full(16384, np.inf, np.float32), hits)
+            rr = np.frombuffer(hits._as_bytes(), dtype=fcpw.INTERACTION_DTYPE)
+            np.testing.assert_array_equal(rr["d"], ob.intersect_gpuorder(o[:16384], d[:16384])["d"])
+    fp64_report("3: star polygon 262,144 segments after 10 refit steps", Vk, F, q[:8192], rec[:8192])
+
+
+def test_config4_geodesic_200k(gpu):
+    w = synth.workload4(n_queries=1 << 15)
+    V, F, q = w["positions"], w["indices"], w["queries"]
+    assert len(F) == 200000
+    s, g = transfer(V, F)
+    rec = gpu_closest(g, q)
+    ref = oracle.Bvh(V, F).closest_points_gpuorder(q)
+    np.testing.assert_array_equal(rec["d"], ref["d"])
+    np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
+    fp64_report("4: geodesic sphere 200,000 tris, within 0.1 of the surface", V, F, q[:4096], rec[:4096])
