@@ -253,7 +253,7 @@ def run_torus(args, torch):
         val = getattr(args, k)
         if val is not None:
             gpu.set_option(k, val)
-    opts = {k: gpu.get_option(k) for k in ("coop", "stack", "hold_min", "sort", "xcds", "block")}
+    opts = {k: gpu.get_option(k) for k in ("coop", "stack", "hold_min", "sort", "morton_bits", "xcds", "block")}
     log(f"[setup] BVH built + uploaded in {time.time() - t0:.1f}s: {gpu.info('n_records')} records, "
         f"depth {gpu.info('max_depth')}, options {opts}")
 

@@ -74,84 +74,10 @@ __global__ __launch_bounds__(64) void k_batch_morton_hist(const BatchScene *T, i
     int64_t i = B.q_begin + local;
     const float *q = pts + i * stride;
     uint32_t code;
-    if (DIM == 3)
-        code = spread3(quant(q[0], B.qlo.x, B.qinv.x, 32)) | (spread3(quant(q[1], B.qlo.y, B.qinv.y, 32)) << 1) |
-               (spread3(quant(q[2], B.qlo.z, B.qinv.z, 32)) << 2);
-    else
-        code = spread2(quant(q[0], B.qlo.x, B.qinv.x, 128)) | (spread2(quant(q[1], B.qlo.y, B.qinv.y, 128)) << 1);
+    code = morton_cell<DIM>(q, B.qlo, B.qinv, DIM == 3 ? BATCH_CELL_BITS / 3 : BATCH_CELL_BITS / 2);
     code += (uint32_t)t << BATCH_CELL_BITS;
     codes[i] = code;
     atomicAdd(&hist[code], 1u);
-}
-
-// exclusive scan of n counters in place, three phases: per-segment sums, a
-// one-block scan of the sums, then each segment rescanned from its base
-constexpr int SCAN_THREADS = 256, SCAN_PER = 32, SCAN_SEG = SCAN_THREADS * SCAN_PER;
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_sums(const uint32_t *v, int64_t n, uint32_t *sums)
-{
-    __shared__ uint32_t part[SCAN_THREADS];
-    int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * SCAN_PER;
-    uint32_t s = 0;
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) s += v[base + k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = SCAN_THREADS / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
-}
-
-// exclusive scan of m <= 1024 * 16 segment sums by one block
-__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t *sums, int m)
-{
-    constexpr int PER = 16;
-    __shared__ uint32_t part[1024];
-    const int base = threadIdx.x * PER;
-    uint32_t s = 0;
-    for (int k = 0; k < PER; k++)
-        if (base + k < m) s += sums[base + k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
-    for (int k = 0; k < PER; k++)
-        if (base + k < m) {
-            uint32_t c = sums[base + k];
-            sums[base + k] = run;
-            run += c;
-        }
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_apply(uint32_t *v, int64_t n, const uint32_t *sums)
-{
-    __shared__ uint32_t part[SCAN_THREADS];
-    int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * SCAN_PER;
-    uint32_t s = 0;
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) s += v[base + k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        uint32_t x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += x;
-        __syncthreads();
-    }
-    uint32_t run = sums[blockIdx.x] + part[threadIdx.x] - s;
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) {
-            uint32_t c = v[base + k];
-            v[base + k] = run;
-            run += c;
-        }
 }
 
 // Persistent closest-point launch: each wave pops work items until none are

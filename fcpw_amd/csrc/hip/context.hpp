@@ -30,13 +30,13 @@ struct fcpw_hip_ctx {
     void *stage_in = nullptr, *stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
     // query ordering workspace (grown on demand, kept for the context's lifetime)
-    void *ws_codes = nullptr, *ws_perm = nullptr, *ws_hist = nullptr;
-    size_t ws_codes_bytes = 0, ws_perm_bytes = 0, ws_hist_bytes = 0;
+    void *ws_codes = nullptr, *ws_perm = nullptr, *ws_hist = nullptr, *ws_sums = nullptr;
+    size_t ws_codes_bytes = 0, ws_perm_bytes = 0, ws_hist_bytes = 0, ws_sums_bytes = 0;
     float qlo[3] = {0, 0, 0}, qinv[3] = {1, 1, 1}; // Morton grid: 1.5x the scene box
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
-    int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1;
+    int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1, morton_bits = 24;
     int stats_on = 0, timing = 0;
     int n_cus = 256;
     // HIP event pairs around each traversal launch while "timing" is on

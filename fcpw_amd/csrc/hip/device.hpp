@@ -54,7 +54,8 @@ struct DevScene {
     const float4 *prims;
     const int32_t *prim_index;
     int *flags;                // bit 0: traversal stack overflow
-    unsigned long long *stats; // traversal counters: internal nodes, primitive tests, queries
+    unsigned long long *stats; // traversal counters: internal nodes, primitive tests, queries, and per wave:
+                               // loop iterations, leaf passes, 64-item evaluation groups
 };
 
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
@@ -181,6 +182,7 @@ struct Geo<2> {
         u1 = 0.0f;
         return closest_point_segment({s.x, s.y}, {s.z, s.w}, x, p, u0);
     }
+
     static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
     {
         float4 s = S.prims[slot];
@@ -203,6 +205,7 @@ struct CpResult {
     float p[3];
     float uv[2];
     uint32_t n_internal, n_prims; // records expanded and primitives tested (coop kernel)
+    uint32_t n_iter, n_pass, n_group; // wave loop iterations, leaf passes, 64-item evaluation groups (wave-uniform)
 };
 
 template <int DIM, int BLOCK, int K>
@@ -521,148 +524,211 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-template <int DIM, int K>
-__device__ __forceinline__ void closest_point_query_coop(const DevScene &S, typename Geo<DIM>::vec x, float r2,
-                                                         bool valid, int hold_min, CpResult &res)
+// Per-lane state of a cooperative traversal.
+template <int DIM>
+struct CoopLane {
+    typename Geo<DIM>::vec x;
+    float r2, cur_d2;
+    uint32_t cur, loff, lcnt;
+    int32_t best;
+    bool active, live, holding;
+    uint32_t n_internal, n_prims;
+};
+
+// LDS shared by the wave for its leaf passes
+struct CoopShared {
+    float q[64 * 3];      // each lane's query point
+    uint32_t off[64];     // each lane's held leaf (first slot)
+    uint16_t item[256];   // (lane, primitive) pairs of one pass
+    float d[256];         // their distances
+};
+
+// Starts lane state L on query x with squared radius r2 (valid = the lane has a query).
+template <int DIM>
+__device__ __forceinline__ void coop_begin(const DevScene &S, CoopShared &sh, CoopLane<DIM> &L,
+                                           typename Geo<DIM>::vec x, float r2, bool valid)
 {
     using G = Geo<DIM>;
-    using vec = typename G::vec;
-    constexpr int BLOCK = 64;
-    __shared__ float coop_q[64 * 3];
-    __shared__ uint32_t coop_off[64];
-    __shared__ uint16_t coop_item[256];
-    __shared__ float coop_d[256];
-    const int lane = threadIdx.x;
     float xs[3];
     store_p(x, xs);
-    coop_q[lane * 3 + 0] = xs[0];
-    coop_q[lane * 3 + 1] = xs[1];
-    coop_q[lane * 3 + 2] = xs[2];
+    const int lane = threadIdx.x & 63;
+    sh.q[lane * 3 + 0] = xs[0];
+    sh.q[lane * 3 + 1] = xs[1];
+    sh.q[lane * 3 + 2] = xs[2];
+    typename G::vec lo, hi;
+    G::root(S, lo, hi, L.cur);
+    float d2min, d2max;
+    box_sqdist(lo, hi, x, d2min, d2max);
+    L.x = x;
+    L.active = valid && (d2min <= r2);
+    L.r2 = L.active ? fminf(r2, d2max) : r2;
+    L.cur_d2 = L.active ? d2min : 0.0f;
+    L.live = L.active;
+    L.holding = false;
+    L.loff = L.lcnt = 0;
+    L.best = -1;
+    L.n_internal = L.n_prims = 0;
+}
+
+// One traversal step of an active lane that holds no leaf: pop until a node
+// survives the r^2 test, then either hold it (leaf) or expand it (internal:
+// test both children, shrink r^2 by their d2Max, continue with the nearer,
+// push the farther).  The stack variables are those FCPW_STACK_DECL declares.
+#define FCPW_COOP_STEP(DIM, BLOCK, K, S, L, OVERFLOW)                                                        \
+    do {                                                                                                      \
+        using G_ = Geo<DIM>;                                                                                  \
+        using vec_ = typename G_::vec;                                                                        \
+        if ((L).active && !(L).holding) {                                                                     \
+            if (!((L).live && (L).cur_d2 <= (L).r2)) {                                                       \
+                (L).live = false;                                                                             \
+                while (sp > 0) {                                                                              \
+                    FCPW_POP(BLOCK, K, (L).cur, (L).cur_d2);                                                  \
+                    if ((L).cur_d2 <= (L).r2) { (L).live = true; break; }                                     \
+                }                                                                                             \
+            }                                                                                                 \
+            if (!(L).live) {                                                                                  \
+                (L).active = false;                                                                           \
+            } else if ((L).cur & LEAF_BIT) {                                                                  \
+                (L).holding = true;                                                                           \
+                (L).live = false;                                                                             \
+                (L).loff = (L).cur & OFF_MASK;                                                                \
+                (L).lcnt = (((L).cur >> CNT_SHIFT) & CNT_MASK) + 1u;                                          \
+            } else {                                                                                          \
+                vec_ loA, hiA, loB, hiB;                                                                      \
+                uint32_t lA, lB;                                                                              \
+                G_::record(S, (L).cur, loA, hiA, lA, loB, hiB, lB);                                           \
+                (L).n_internal++;                                                                             \
+                float a0, a1, b0, b1;                                                                         \
+                box_sqdist(loA, hiA, (L).x, a0, a1);                                                          \
+                bool oA = a0 <= (L).r2;                                                                       \
+                (L).r2 = fminf((L).r2, a1);                                                                   \
+                box_sqdist(loB, hiB, (L).x, b0, b1);                                                          \
+                bool oB = b0 <= (L).r2;                                                                       \
+                (L).r2 = fminf((L).r2, b1);                                                                   \
+                (L).live = oA || oB;                                                                          \
+                if (oA && oB) {                                                                               \
+                    uint32_t nearL = lA, farL = lB;                                                           \
+                    float nearD = a0, farD = b0;                                                              \
+                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }                           \
+                    FCPW_PUSH(BLOCK, K, farL, farD, { OVERFLOW = true; (L).active = false; (L).live = false; }); \
+                    (L).cur = nearL;                                                                          \
+                    (L).cur_d2 = nearD;                                                                       \
+                } else if (oA) {                                                                              \
+                    (L).cur = lA;                                                                             \
+                    (L).cur_d2 = a0;                                                                          \
+                } else if (oB) {                                                                              \
+                    (L).cur = lB;                                                                             \
+                    (L).cur_d2 = b0;                                                                          \
+                }                                                                                             \
+            }                                                                                                 \
+        }                                                                                                     \
+    } while (0)
+
+// The leaf pass, run by the whole wave: all (lane, primitive) pairs of the held
+// leaves are evaluated by all 64 lanes, then each owner replays the reference's
+// acceptance test (d*d <= r2, later equal distances win) in leaf order.
+template <int DIM>
+__device__ __forceinline__ uint32_t coop_leaf_pass(const DevScene &S, CoopShared &sh, CoopLane<DIM> &L)
+{
+    uint32_t groups = 0;
+    using G = Geo<DIM>;
+    const int lane = threadIdx.x & 63;
+    sh.off[lane] = L.loff;
+    for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
+        bool h0 = L.holding && L.lcnt > j0, h1 = L.holding && L.lcnt > j0 + 1;
+        bool h2 = L.holding && L.lcnt > j0 + 2, h3 = L.holding && L.lcnt > j0 + 3;
+        uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+        if (m0 == 0) break;
+        uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+        uint32_t total = c0 + c1 + c2 + __popcll(m3);
+        uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                 i3 = c0 + c1 + c2 + lanes_below(m3);
+        if (h0) sh.item[i0] = (uint16_t)((lane << 4) | j0);
+        if (h1) sh.item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
+        if (h2) sh.item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
+        if (h3) sh.item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
+        __syncthreads();
+        for (uint32_t pass = 0; pass < total; pass += 64) {
+            groups++;
+            uint32_t k = pass + (uint32_t)lane;
+            if (k < total) {
+                uint32_t it = sh.item[k];
+                uint32_t o = it >> 4;
+                float xo[3] = {sh.q[o * 3], sh.q[o * 3 + 1], sh.q[o * 3 + 2]};
+                typename G::vec pp;
+                float u0, u1;
+                sh.d[k] = G::closest(S, sh.off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
+            }
+        }
+        __syncthreads();
+        if (L.holding) {
+            const uint32_t idx[4] = {i0, i1, i2, i3};
+            for (uint32_t jj = 0; jj < 4; jj++) {
+                if (L.lcnt > j0 + jj) {
+                    L.n_prims++;
+                    float d = sh.d[idx[jj]];
+                    if (d * d <= L.r2) {
+                        L.r2 = fminf(L.r2, d * d);
+                        L.best = (int32_t)(L.loff + j0 + jj);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (m3 == 0) break;
+    }
+    L.holding = false;
+    return groups;
+}
+
+// The lane's result: the winning primitive's closest point and uv recomputed.
+template <int DIM>
+__device__ __forceinline__ void coop_result(const DevScene &S, const CoopLane<DIM> &L, CpResult &res)
+{
     res.slot = -1;
     res.d = FLT_MAX;
     res.p[0] = res.p[1] = res.p[2] = 0.0f;
     res.uv[0] = res.uv[1] = 0.0f;
-
-    vec lo, hi;
-    uint32_t cur;
-    G::root(S, lo, hi, cur);
-    float d2min, d2max, cur_d2 = 0.0f;
-    box_sqdist(lo, hi, x, d2min, d2max);
-    bool active = valid && (d2min <= r2);
-    if (active) {
-        r2 = fminf(r2, d2max);
-        cur_d2 = d2min;
-    }
-    bool live = active, holding = false, overflow = false;
-    uint32_t loff = 0, lcnt = 0;
-    uint32_t n_internal = 0, n_prims = 0;
-    int32_t best = -1;
-    FCPW_STACK_DECL(BLOCK, K);
-    __syncthreads();
-    while (true) {
-        if (active && !holding) {
-            if (!(live && cur_d2 <= r2)) {
-                live = false;
-                while (sp > 0) {
-                    FCPW_POP(BLOCK, K, cur, cur_d2);
-                    if (cur_d2 <= r2) { live = true; break; }
-                }
-            }
-            if (!live) {
-                active = false;
-            } else if (cur & LEAF_BIT) {
-                holding = true;
-                live = false;
-                loff = cur & OFF_MASK;
-                lcnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
-            } else {
-                vec loA, hiA, loB, hiB;
-                uint32_t lA, lB;
-                G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
-                n_internal++;
-                float a0, a1, b0, b1;
-                box_sqdist(loA, hiA, x, a0, a1);
-                bool oA = a0 <= r2;
-                r2 = fminf(r2, a1);
-                box_sqdist(loB, hiB, x, b0, b1);
-                bool oB = b0 <= r2;
-                r2 = fminf(r2, b1);
-                live = oA || oB;
-                if (oA && oB) {
-                    uint32_t nearL = lA, farL = lB;
-                    float nearD = a0, farD = b0;
-                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
-                    FCPW_PUSH(BLOCK, K, farL, farD, { overflow = true; active = false; live = false; });
-                    cur = nearL; cur_d2 = nearD;
-                } else if (oA) {
-                    cur = lA; cur_d2 = a0;
-                } else if (oB) {
-                    cur = lB; cur_d2 = b0;
-                }
-            }
-        }
-        uint64_t act = __ballot(active);
-        if (act == 0) break;
-        uint64_t hold = __ballot(holding);
-        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
-
-        coop_off[lane] = loff;
-        for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
-            bool h0 = holding && lcnt > j0, h1 = holding && lcnt > j0 + 1;
-            bool h2 = holding && lcnt > j0 + 2, h3 = holding && lcnt > j0 + 3;
-            uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
-            if (m0 == 0) break;
-            uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
-            uint32_t total = c0 + c1 + c2 + __popcll(m3);
-            uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
-                     i3 = c0 + c1 + c2 + lanes_below(m3);
-            if (h0) coop_item[i0] = (uint16_t)((lane << 4) | j0);
-            if (h1) coop_item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
-            if (h2) coop_item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
-            if (h3) coop_item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
-            __syncthreads();
-            for (uint32_t pass = 0; pass < total; pass += 64) {
-                uint32_t k = pass + (uint32_t)lane;
-                if (k < total) {
-                    uint32_t it = coop_item[k];
-                    uint32_t o = it >> 4;
-                    float xo[3] = {coop_q[o * 3], coop_q[o * 3 + 1], coop_q[o * 3 + 2]};
-                    vec pp;
-                    float u0, u1;
-                    coop_d[k] = G::closest(S, coop_off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
-                }
-            }
-            __syncthreads();
-            if (holding) {
-                const uint32_t idx[4] = {i0, i1, i2, i3};
-                for (uint32_t jj = 0; jj < 4; jj++) {
-                    if (lcnt > j0 + jj) {
-                        n_prims++;
-                        float d = coop_d[idx[jj]];
-                        if (d * d <= r2) {
-                            r2 = fminf(r2, d * d);
-                            best = (int32_t)(loff + j0 + jj);
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            if (m3 == 0) break;
-        }
-        holding = false;
-    }
-    if (overflow) atomicOr(S.flags, 1);
-    res.n_internal = n_internal;
-    res.n_prims = n_prims;
-    if (best >= 0) {
-        vec p;
+    res.n_internal = L.n_internal;
+    res.n_prims = L.n_prims;
+    if (L.best >= 0) {
+        typename Geo<DIM>::vec p;
         float u0, u1;
-        res.d = G::closest(S, (uint32_t)best, x, p, u0, u1);
-        res.slot = best;
+        res.d = Geo<DIM>::closest(S, (uint32_t)L.best, L.x, p, u0, u1);
+        res.slot = L.best;
         res.uv[0] = u0;
         res.uv[1] = u1;
         store_p(p, res.p);
     }
+}
+
+template <int DIM, int K>
+__device__ __forceinline__ void closest_point_query_coop(const DevScene &S, typename Geo<DIM>::vec x, float r2,
+                                                         bool valid, int hold_min, CpResult &res)
+{
+    constexpr int BLOCK = 64;
+    __shared__ CoopShared sh;
+    CoopLane<DIM> L;
+    coop_begin<DIM>(S, sh, L, x, r2, valid);
+    bool overflow = false;
+    uint32_t n_iter = 0, n_pass = 0, n_group = 0;
+    FCPW_STACK_DECL(BLOCK, K);
+    __syncthreads();
+    while (true) {
+        FCPW_COOP_STEP(DIM, BLOCK, K, S, L, overflow);
+        uint64_t act = __ballot(L.active);
+        if (act == 0) break;
+        n_iter++;
+        uint64_t hold = __ballot(L.holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+        n_pass++;
+        n_group += coop_leaf_pass<DIM>(S, sh, L);
+    }
+    if (overflow) atomicOr(S.flags, 1);
+    coop_result<DIM>(S, L, res);
+    res.n_iter = n_iter;
+    res.n_pass = n_pass;
+    res.n_group = n_group;
 }
 
 // Adds a wave's traversal counts to S.stats (called by every lane of the wave).
@@ -681,6 +747,18 @@ __device__ __forceinline__ void wave_stats_add(const DevScene &S, uint32_t n_int
     }
 }
 
+// Adds a wave's loop counts (wave-uniform values) to S.stats.
+__device__ __forceinline__ void wave_loop_stats_add(const DevScene &S, uint32_t n_iter, uint32_t n_pass,
+                                                    uint32_t n_group)
+{
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&S.stats[3], (unsigned long long)n_iter);
+        atomicAdd(&S.stats[4], (unsigned long long)n_pass);
+        atomicAdd(&S.stats[5], (unsigned long long)n_group);
+        atomicAdd(&S.stats[6], 1ull);
+    }
+}
+
 template <int DIM, int K, int OUT, bool STATS>
 __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io, int xcds, int hold_min)
 {
@@ -692,7 +770,10 @@ __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io,
     CpResult r;
     closest_point_query_coop<DIM, K>(S, Geo<DIM>::load_point(q), r2, valid, hold_min, r);
     if (valid) cp_store<DIM, OUT>(S, io, i, r);
-    if (STATS) wave_stats_add(S, r.n_internal, r.n_prims, valid ? 1u : 0u);
+    if (STATS) {
+        wave_stats_add(S, r.n_internal, r.n_prims, valid ? 1u : 0u);
+        wave_loop_stats_add(S, r.n_iter, r.n_pass, r.n_group);
+    }
 }
 
 template <int DIM, int BLOCK, int K, bool STATS>
@@ -726,26 +807,25 @@ __global__ __launch_bounds__(BLOCK) void k_intersect(DevScene S, const float2 *r
 }
 
 // ---------------------------------------------------------------------------
-// Morton ordering of a query batch: a counting sort of the queries over a
-// 2^18-cell grid (6 bits per axis in 3D, 9 in 2D) laid along a Z-order curve.
+// Morton ordering of a query batch: a counting sort of the queries over a grid
+// of 2^18, 2^21 or 2^24 cells (6, 7 or 8 bits per axis in 3D; 9, 10 or 12 in
+// 2D) laid along a Z-order curve, with a three-phase multi-block scan.
 // Neighbouring lanes then traverse neighbouring parts of the tree, which keeps
 // a wave's branches together and its node fetches in L1/L2.  The order inside
 // a cell is whatever the atomics produce; results do not depend on it.
 // ---------------------------------------------------------------------------
-constexpr int MORTON_BITS = 18;
-constexpr uint32_t MORTON_CELLS = 1u << MORTON_BITS;
-
-__device__ __forceinline__ uint32_t spread3(uint32_t v) // 6 bits -> every third bit
+__device__ __forceinline__ uint32_t spread3(uint32_t v) // 10 bits -> every third bit
 {
-    v &= 0x3Fu;
-    v = (v | (v << 8)) & 0x0000F00Fu;
-    v = (v | (v << 4)) & 0x000C30C3u;
-    v = (v | (v << 2)) & 0x00249249u;
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
     return v;
 }
-__device__ __forceinline__ uint32_t spread2(uint32_t v) // 9 bits -> every second bit
+__device__ __forceinline__ uint32_t spread2(uint32_t v) // 16 bits -> every second bit
 {
-    v &= 0x1FFu;
+    v &= 0xFFFFu;
     v = (v | (v << 8)) & 0x00FF00FFu;
     v = (v | (v << 4)) & 0x0F0F0F0Fu;
     v = (v | (v << 2)) & 0x33333333u;
@@ -765,45 +845,96 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n)
     if (i < n) p[i] = v;
 }
 
+// cell of a point on a grid of 2^axis_bits cells per axis, Z-order
 template <int DIM>
-__global__ void k_morton_hist(const float *pts, int stride, int64_t n, float4 lo, float4 inv, uint32_t *codes,
-                              uint32_t *hist)
+__device__ __forceinline__ uint32_t morton_cell(const float *q, float4 lo, float4 inv, int axis_bits)
+{
+    const uint32_t cells = 1u << axis_bits;
+    if (DIM == 3)
+        return spread3(quant(q[0], lo.x, inv.x, cells)) | (spread3(quant(q[1], lo.y, inv.y, cells)) << 1) |
+               (spread3(quant(q[2], lo.z, inv.z, cells)) << 2);
+    return spread2(quant(q[0], lo.x, inv.x, cells)) | (spread2(quant(q[1], lo.y, inv.y, cells)) << 1);
+}
+
+template <int DIM>
+__global__ void k_morton_hist(const float *pts, int stride, int64_t n, float4 lo, float4 inv, int axis_bits,
+                              uint32_t *codes, uint32_t *hist)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float *q = pts + i * stride;
-    uint32_t code;
-    if (DIM == 3)
-        code = spread3(quant(q[0], lo.x, inv.x, 64)) | (spread3(quant(q[1], lo.y, inv.y, 64)) << 1) |
-               (spread3(quant(q[2], lo.z, inv.z, 64)) << 2);
-    else
-        code = spread2(quant(q[0], lo.x, inv.x, 512)) | (spread2(quant(q[1], lo.y, inv.y, 512)) << 1);
+    uint32_t code = morton_cell<DIM>(pts + i * stride, lo, inv, axis_bits);
     codes[i] = code;
     atomicAdd(&hist[code], 1u);
 }
 
-// exclusive scan of MORTON_CELLS counters by one 1024-thread block
-__global__ __launch_bounds__(1024) void k_scan_cells(uint32_t *hist)
+// exclusive scan of n counters in place, three phases: per-segment sums, a
+// one-block scan of the sums, then each segment rescanned from its base
+constexpr int SCAN_THREADS = 256, SCAN_PER = 32, SCAN_SEG = SCAN_THREADS * SCAN_PER;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_sums(const uint32_t *v, int64_t n, uint32_t *sums)
 {
-    constexpr int PER = MORTON_CELLS / 1024;
-    __shared__ uint32_t part[1024];
-    uint32_t *h = hist + (size_t)threadIdx.x * PER;
-    uint32_t sum = 0;
-    for (int k = 0; k < PER; k++) sum += h[k];
-    part[threadIdx.x] = sum;
+    __shared__ uint32_t part[SCAN_THREADS];
+    int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * SCAN_PER;
+    uint32_t s = 0;
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) s += v[base + k];
+    part[threadIdx.x] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) { // Hillis-Steele inclusive scan
+    for (int off = SCAN_THREADS / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
+}
+
+// exclusive scan of m <= 1024 * 16 segment sums by one block
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t *sums, int m)
+{
+    constexpr int PER = 16;
+    __shared__ uint32_t part[1024];
+    const int base = threadIdx.x * PER;
+    uint32_t s = 0;
+    for (int k = 0; k < PER; k++)
+        if (base + k < m) s += sums[base + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
         uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
         __syncthreads();
         part[threadIdx.x] += v;
         __syncthreads();
     }
-    uint32_t run = part[threadIdx.x] - sum;
-    for (int k = 0; k < PER; k++) {
-        uint32_t c = h[k];
-        h[k] = run;
-        run += c;
+    uint32_t run = part[threadIdx.x] - s;
+    for (int k = 0; k < PER; k++)
+        if (base + k < m) {
+            uint32_t c = sums[base + k];
+            sums[base + k] = run;
+            run += c;
+        }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_apply(uint32_t *v, int64_t n, const uint32_t *sums)
+{
+    __shared__ uint32_t part[SCAN_THREADS];
+    int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * SCAN_PER;
+    uint32_t s = 0;
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) s += v[base + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        uint32_t x = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
     }
+    uint32_t run = sums[blockIdx.x] + part[threadIdx.x] - s;
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) {
+            uint32_t c = v[base + k];
+            v[base + k] = run;
+            run += c;
+        }
 }
 
 __global__ void k_morton_scatter(const uint32_t *codes, int64_t n, uint32_t *offsets, uint32_t *perm)

@@ -33,8 +33,8 @@ int fcpw_hip_create(int device, fcpw_hip_ctx **out)
         e = hipMalloc((void **)&c->flags, sizeof(int));
     }
     if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc((void **)&c->stats, 4 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->stats, 0, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->stats, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->stats, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         g_create_error = hipGetErrorString(e);
         if (c->flags) (void)hipFree(c->flags);
@@ -53,7 +53,7 @@ int fcpw_hip_destroy(fcpw_hip_ctx *c)
     free_scene(c);
     if (c->flags) (void)hipFree(c->flags);
     if (c->stats) (void)hipFree(c->stats);
-    void *ws[] = {c->stage_in, c->stage_out, c->ws_codes, c->ws_perm, c->ws_hist};
+    void *ws[] = {c->stage_in, c->stage_out, c->ws_codes, c->ws_perm, c->ws_hist, c->ws_sums};
     for (void *b : ws)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->ev_begin) (void)hipEventDestroy(e);
@@ -337,20 +337,29 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     *perm_out = nullptr;
     if (!c->sort || n < 4096) return FCPW_HIP_OK;
     if (n > (int64_t)UINT32_MAX) FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "query batches above 2^32 are not supported");
+    const int axis_bits = c->morton_bits / c->dim;
+    const int64_t cells = (int64_t)1 << (axis_bits * c->dim);
+    const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
     int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 4);
     if (!st) st = grow(c, &c->ws_perm, &c->ws_perm_bytes, (size_t)n * 4);
-    if (!st) st = grow(c, &c->ws_hist, &c->ws_hist_bytes, (size_t)MORTON_CELLS * 4);
+    if (!st) st = grow(c, &c->ws_hist, &c->ws_hist_bytes, (size_t)cells * 4);
+    if (!st) st = grow(c, &c->ws_sums, &c->ws_sums_bytes, (size_t)segs * 4);
     if (st) return st;
     uint32_t *codes = (uint32_t *)c->ws_codes, *perm = (uint32_t *)c->ws_perm, *hist = (uint32_t *)c->ws_hist;
+    uint32_t *sums = (uint32_t *)c->ws_sums;
     unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_fill_u32, dim3(MORTON_CELLS / 256), dim3(256), 0, s, hist, 0u, (int64_t)MORTON_CELLS);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, hist, 0u, cells);
     float4 lo = make_float4(c->qlo[0], c->qlo[1], c->qlo[2], 0.0f);
     float4 inv = make_float4(c->qinv[0], c->qinv[1], c->qinv[2], 0.0f);
     if (c->dim == 3)
-        hipLaunchKernelGGL(k_morton_hist<3>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, codes, hist);
+        hipLaunchKernelGGL(k_morton_hist<3>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, axis_bits, codes,
+                           hist);
     else
-        hipLaunchKernelGGL(k_morton_hist<2>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, codes, hist);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, hist);
+        hipLaunchKernelGGL(k_morton_hist<2>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, axis_bits, codes,
+                           hist);
+    hipLaunchKernelGGL(k_scan_segment_sums, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)segs);
+    hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
     hipLaunchKernelGGL(k_morton_scatter, dim3(nb), dim3(256), 0, s, codes, n, hist, perm);
     HIPCHECK(c, hipGetLastError());
     *perm_out = perm;
@@ -603,15 +612,20 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "hold_min") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "hold_min must be in [1, 64]");
         c->hold_min = value;
+
     } else if (k == "xcds") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcds must be in [1, 64]");
         c->xcds = value;
+    } else if (k == "morton_bits") {
+        if (value != 18 && value != 21 && value != 24)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "morton_bits must be 18, 21 or 24");
+        c->morton_bits = value;
     } else if (k == "stats") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stats must be 0 or 1");
         if (value) {
             HIPCHECK(c, hipSetDevice(c->device));
             HIPCHECK(c, hipDeviceSynchronize());
-            HIPCHECK(c, hipMemset(c->stats, 0, 4 * sizeof(unsigned long long)));
+            HIPCHECK(c, hipMemset(c->stats, 0, 8 * sizeof(unsigned long long)));
         }
         c->stats_on = value;
     } else if (k == "timing") {
@@ -633,7 +647,9 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "sort") *value = c->sort;
     else if (k == "coop") *value = c->coop;
     else if (k == "xcds") *value = c->xcds;
+    else if (k == "morton_bits") *value = c->morton_bits;
     else if (k == "hold_min") *value = c->hold_min;
+
     else if (k == "stats") *value = c->stats_on;
     else if (k == "timing") *value = c->timing;
     else return FCPW_HIP_ERR_INVALID_ARGUMENT;
@@ -652,13 +668,19 @@ int fcpw_hip_get_info(const fcpw_hip_ctx *c, const char *key, int64_t *value)
     else if (k == "max_depth") *value = c->max_depth;
     else if (k == "n_cus") *value = c->n_cus;
     else if (k == "uploaded") *value = c->uploaded ? 1 : 0;
-    else if (k == "stat_internal_nodes" || k == "stat_prim_tests" || k == "stat_queries") {
+    else if (k.rfind("stat_", 0) == 0) {
+        static const char *names[7] = {"stat_internal_nodes", "stat_prim_tests", "stat_queries", "stat_wave_iterations",
+                                       "stat_leaf_passes", "stat_leaf_groups", "stat_waves"};
+        int which = -1;
+        for (int j = 0; j < 7; j++)
+            if (k == names[j]) which = j;
+        if (which < 0) return FCPW_HIP_ERR_INVALID_ARGUMENT;
         fcpw_hip_ctx *m = const_cast<fcpw_hip_ctx *>(c);
-        unsigned long long v[4];
+        unsigned long long v[8];
         HIPCHECK(m, hipSetDevice(c->device));
         HIPCHECK(m, hipDeviceSynchronize());
         HIPCHECK(m, hipMemcpy(v, c->stats, sizeof(v), hipMemcpyDeviceToHost));
-        *value = (int64_t)(k == "stat_internal_nodes" ? v[0] : k == "stat_prim_tests" ? v[1] : v[2]);
+        *value = (int64_t)v[which];
     } else if (k == "kernel_launches") {
         *value = c->ev_used;
     } else if (k == "kernel_ns") {

@@ -198,6 +198,7 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
                  one-query-per-lane kernel 8 for 0 and 4)
      "block"     threads per block when coop = 0 (64, 128, 256)
      "sort"      1 = order queries along a Morton curve before traversal (default)
+     "morton_bits"  cells of that ordering grid, 2^18, 2^21 or 2^24 (default 24)
      "xcds"      number of XCDs the block -> tile mapping groups by (1..64, default 8)
    Measurement switches (off by default; results are unchanged):
      "stats"     1 = run the counting variant of the traversal kernels, which adds
@@ -211,7 +212,9 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *ctx, const char *key, int *value);
 
 /* Uploaded BVH facts: n_records (internal nodes), n_leaves, n_prims, n_vertices,
    dim, max_depth, n_cus.  Measurement: stat_internal_nodes, stat_prim_tests,
-   stat_queries (totals since "stats" was set to 1; synchronizes the device),
+   stat_queries and, for closest-point launches, stat_waves, stat_wave_iterations,
+   stat_leaf_passes, stat_leaf_groups (cooperative leaf passes and their 64-item
+   evaluation groups); totals since "stats" was set to 1; synchronizes the device;
    kernel_launches and kernel_ns (count and summed duration of the traversal
    launches timed since "timing" was set to 1; waits for the last one). */
 int fcpw_hip_get_info(const fcpw_hip_ctx *ctx, const char *key, int64_t *value);
