@@ -535,10 +535,9 @@ struct CoopLane {
     uint32_t n_internal, n_prims;
 };
 
-// LDS shared by the wave for its leaf passes
+// LDS shared by the wave for its leaf passes (a lane's query point and held
+// leaf are read from its registers with cross-lane permutes)
 struct CoopShared {
-    float q[64 * 3];      // each lane's query point
-    uint32_t off[64];     // each lane's held leaf (first slot)
     uint16_t item[256];   // (lane, primitive) pairs of one pass
     float d[256];         // their distances
 };
@@ -549,12 +548,7 @@ __device__ __forceinline__ void coop_begin(const DevScene &S, CoopShared &sh, Co
                                            typename Geo<DIM>::vec x, float r2, bool valid)
 {
     using G = Geo<DIM>;
-    float xs[3];
-    store_p(x, xs);
-    const int lane = threadIdx.x & 63;
-    sh.q[lane * 3 + 0] = xs[0];
-    sh.q[lane * 3 + 1] = xs[1];
-    sh.q[lane * 3 + 2] = xs[2];
+    (void)sh;
     typename G::vec lo, hi;
     G::root(S, lo, hi, L.cur);
     float d2min, d2max;
@@ -633,7 +627,8 @@ __device__ __forceinline__ uint32_t coop_leaf_pass(const DevScene &S, CoopShared
     uint32_t groups = 0;
     using G = Geo<DIM>;
     const int lane = threadIdx.x & 63;
-    sh.off[lane] = L.loff;
+    float xs[3];
+    store_p(L.x, xs);
     for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
         bool h0 = L.holding && L.lcnt > j0, h1 = L.holding && L.lcnt > j0 + 1;
         bool h2 = L.holding && L.lcnt > j0 + 2, h3 = L.holding && L.lcnt > j0 + 3;
@@ -650,14 +645,19 @@ __device__ __forceinline__ uint32_t coop_leaf_pass(const DevScene &S, CoopShared
         __syncthreads();
         for (uint32_t pass = 0; pass < total; pass += 64) {
             groups++;
-            uint32_t k = pass + (uint32_t)lane;
+            const uint32_t k = pass + (uint32_t)lane;
+            // every lane takes part in the permutes; lanes past the end read item 0
+            const uint32_t it = sh.item[k < total ? k : 0u];
+            const int o = (int)(it >> 4);
+            float xo[3];
+            xo[0] = __shfl(xs[0], o, 64);
+            xo[1] = __shfl(xs[1], o, 64);
+            xo[2] = DIM == 3 ? __shfl(xs[2], o, 64) : 0.0f;
+            const uint32_t off = __shfl(L.loff, o, 64);
             if (k < total) {
-                uint32_t it = sh.item[k];
-                uint32_t o = it >> 4;
-                float xo[3] = {sh.q[o * 3], sh.q[o * 3 + 1], sh.q[o * 3 + 2]};
                 typename G::vec pp;
                 float u0, u1;
-                sh.d[k] = G::closest(S, sh.off[o] + (it & 15u), G::load_point(xo), pp, u0, u1);
+                sh.d[k] = G::closest(S, off + (it & 15u), G::load_point(xo), pp, u0, u1);
             }
         }
         __syncthreads();
