@@ -83,16 +83,15 @@ class _GPUSceneMixin:
     def intersect(self, *args, **kwargs):
         """intersect(ray_origins, ray_directions, ray_distance_bounds, interactions, check_for_occlusion=False)
         intersect(rays, interactions, check_for_occlusion=False)
-        intersect(sphere_centers, sphere_squared_radii, rand_nums, interactions)   (not built: raises)
-        intersect(bounding_spheres, rand_nums, interactions)                      (not built: raises)"""
+        intersect(sphere_centers, sphere_squared_radii, rand_nums, interactions)
+        intersect(bounding_spheres, rand_nums, interactions)
+        (python/fcpw_bindings.cpp:596-611 overloads)"""
         if "rays" in kwargs or (args and _is_list(args[0], GPURayList)):
             return self._intersect_rays(*args, **kwargs)
         if "bounding_spheres" in kwargs or (args and _is_list(args[0], GPUBoundingSphereList)):
             return self._intersect_spheres(*args, **kwargs)
-        if "sphere_centers" in kwargs or "rand_nums" in kwargs:
-            raise RuntimeError("sphere intersection queries are not built by the MI355X backend")
-        if len(args) >= 3 and _np.ndim(args[1]) == 1 and _np.ndim(args[2]) == 2:
-            raise RuntimeError("sphere intersection queries are not built by the MI355X backend")
+        if "sphere_centers" in kwargs or (len(args) >= 3 and _np.ndim(args[1]) == 1 and _np.ndim(args[2]) == 2):
+            return self._intersect_spheres_arrays(*args, **kwargs)
         return self._intersect_arrays(*args, **kwargs)
 
     def find_closest_points(self, *args, **kwargs):
@@ -117,6 +116,15 @@ class _GPUSceneMixin:
         """rays: (n, 10) float32 CUDA tensor of GPURay records; out: (n, 10) float32."""
         self._intersect_device(rays.data_ptr(), rays.shape[0], bool(check_for_occlusion), out.data_ptr(),
                                _stream_handle(rays, stream))
+
+    def intersect_spheres_device(self, spheres, rand_nums, out, stream=None):
+        """spheres: (n, 4) GPUBoundingSphere records, rand_nums: (n, 3) float32 in [0, 1),
+        out: (n, 10) GPUInteraction records (CUDA tensors)."""
+        n = spheres.shape[0]
+        if rand_nums.shape[0] != n or rand_nums.shape[-1] != 3 or not rand_nums.is_contiguous():
+            raise ValueError("rand_nums must be a contiguous (n, 3) tensor")
+        self._intersect_spheres_device(spheres.data_ptr(), rand_nums.data_ptr(), n, out.data_ptr(),
+                                       _stream_handle(spheres, stream))
 
     def set_option(self, key, value):
         self._set_option(key, int(value))

@@ -141,6 +141,31 @@ struct Geo<3> {
         float4 a = t[0], b = t[1], c = t[2];
         return closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, u0, u1);
     }
+    // Triangle.getSurfaceArea (geometry.slang:356-359)
+    static __device__ __forceinline__ float area(const DevScene &S, uint32_t slot)
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        v3 m = cross({b.x - a.x, b.y - a.y, b.z - a.z}, {c.x - a.x, c.y - a.y, c.z - a.z});
+        return 0.5f * sqrtf(dot(m, m));
+    }
+    // Triangle.samplePoint (geometry.slang:409-423): uniform point, returns its pdf
+    static __device__ __forceinline__ float sample(const DevScene &S, uint32_t slot, float r1, float r2, float uv[2],
+                                                   float p[3], float n[3])
+    {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        v3 m = cross({b.x - a.x, b.y - a.y, b.z - a.z}, {c.x - a.x, c.y - a.y, c.z - a.z});
+        float ar = sqrtf(dot(m, m));
+        float u1 = sqrtf(r1), u = 1.0f - u1, v = r2 * u1, w = (1.0f - u) - v;
+        uv[0] = u;
+        uv[1] = v;
+        p[0] = (a.x * u + b.x * v) + c.x * w;
+        p[1] = (a.y * u + b.y * v) + c.y * w;
+        p[2] = (a.z * u + b.z * v) + c.z * w;
+        n[0] = m.x / ar; n[1] = m.y / ar; n[2] = m.z / ar;
+        return 2.0f / ar;
+    }
     static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
     {
         const float4 *t = S.prims + (size_t)slot * 3;
@@ -181,6 +206,29 @@ struct Geo<2> {
         float4 s = S.prims[slot];
         u1 = 0.0f;
         return closest_point_segment({s.x, s.y}, {s.z, s.w}, x, p, u0);
+    }
+    // LineSegment.getSurfaceArea (geometry.slang:133-136)
+    static __device__ __forceinline__ float area(const DevScene &S, uint32_t slot)
+    {
+        float4 s = S.prims[slot];
+        v2 m = {s.z - s.x, s.w - s.y};
+        return sqrtf(dot(m, m));
+    }
+    // LineSegment.samplePoint (geometry.slang:180-191)
+    static __device__ __forceinline__ float sample(const DevScene &S, uint32_t slot, float r1, float r2, float uv[2],
+                                                   float p[3], float n[3])
+    {
+        (void)r2;
+        float4 s = S.prims[slot];
+        v2 m = {s.z - s.x, s.w - s.y};
+        float ar = sqrtf(dot(m, m));
+        uv[0] = r1;
+        uv[1] = 0.0f;
+        p[0] = s.x + r1 * m.x;
+        p[1] = s.y + r1 * m.y;
+        p[2] = 0.0f;
+        n[0] = m.y / ar; n[1] = -m.x / ar; n[2] = 0.0f;
+        return 1.0f / ar;
     }
 
     static __device__ __forceinline__ void normal(const DevScene &S, uint32_t slot, float n[3])
@@ -774,6 +822,97 @@ __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io,
         wave_stats_add(S, r.n_internal, r.n_prims, valid ? 1u : 0u);
         wave_loop_stats_add(S, r.n_iter, r.n_pass, r.n_group);
     }
+}
+
+// Sphere intersection (bvh.slang:257-392) with the constant branch weight of
+// the sphereIntersection entry point (bvh.cs.slang:59-82): one random walk
+// from the root that picks, with equal probability, one of the children
+// overlapping the sphere (no stack), then an area-weighted reservoir choice
+// among the leaf's primitives inside the sphere (all of them when the leaf box
+// lies inside), then a uniform point on the chosen primitive.  d = the pdf of
+// that point.  Random numbers: rnd[3i] drives the walk and the choice,
+// rnd[3i+1], rnd[3i+2] the point.
+template <int DIM>
+__global__ __launch_bounds__(128) void k_intersect_spheres(DevScene S, const float4 *spheres, const float *rnd,
+                                                           int64_t n, float2 *out)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (i >= n) return;
+    const float4 s4 = spheres[i];
+    const float q[3] = {s4.x, s4.y, s4.z};
+    const float r2 = s4.w;
+    const vec x = G::load_point(q);
+    float p[3] = {0.0f, 0.0f, 0.0f}, nrm[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
+    float dsel = FLT_MAX;
+    int32_t selected = -1;
+    vec lo, hi;
+    uint32_t cur;
+    G::root(S, lo, hi, cur);
+    float d0, d1;
+    box_sqdist(lo, hi, x, d0, d1);
+    if (d0 <= r2) {
+        float maxd = d1, pdf = 1.0f, u = rnd[i * 3];
+        while (true) {
+            if (cur & LEAF_BIT) {
+                const uint32_t off = cur & OFF_MASK, cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                float total = 0.0f;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    bool hitp = maxd <= r2;
+                    if (!hitp) {
+                        vec pp;
+                        float u0, u1;
+                        float dd = G::closest(S, off + k, x, pp, u0, u1);
+                        hitp = dd * dd <= r2;
+                    }
+                    if (hitp) {
+                        const float a = G::area(S, off + k);
+                        total += a;
+                        const float prob = a / total;
+                        if (u < prob) {
+                            u = u / prob;
+                            dsel = a * pdf;
+                            selected = (int32_t)(off + k);
+                        } else {
+                            u = (u - prob) / (1.0f - prob);
+                        }
+                    }
+                }
+                if (total > 0.0f) dsel = dsel / total;
+                break;
+            }
+            vec loA, hiA, loB, hiB;
+            uint32_t lA, lB;
+            G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+            float a0, a1, b0, b1;
+            box_sqdist(loA, hiA, x, a0, a1);
+            box_sqdist(loB, hiB, x, b0, b1);
+            const float wl = a0 <= r2 ? 1.0f : 0.0f, wr = b0 <= r2 ? 1.0f : 0.0f;
+            const float tot = wl + wr;
+            if (!(tot > 0.0f)) break;
+            const float pl = wl / tot, pr = 1.0f - pl;
+            if (u < pl) {
+                u = u / pl;
+                cur = lA;
+                pdf *= pl;
+                maxd = a1;
+            } else {
+                u = (u - pl) / pr;
+                cur = lB;
+                pdf *= pr;
+                maxd = b1;
+            }
+        }
+    }
+    uint32_t index = FCPW_HIP_UINT_MAX;
+    if (selected >= 0) {
+        dsel *= G::sample(S, (uint32_t)selected, rnd[i * 3 + 1], rnd[i * 3 + 2], uv, p, nrm);
+        index = (uint32_t)S.prim_index[selected];
+    } else {
+        dsel = FLT_MAX;
+    }
+    write_interaction(out + i * 5, p, nrm, uv, dsel, index);
 }
 
 template <int DIM, int BLOCK, int K, bool STATS>

@@ -539,11 +539,55 @@ int fcpw_hip_intersect(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int
     return check_overflow(c);
 }
 
-int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *, const fcpw_hip_float3 *, int64_t,
-                               fcpw_hip_interaction *)
+static int launch_spheres(fcpw_hip_ctx *c, const void *spheres, const float *rnd, int64_t n, void *out, hipStream_t s)
 {
-    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
-    FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "sphere intersection queries are not built by this backend");
+    DevScene S = dev_scene(c);
+    const unsigned blocks = (unsigned)((n + 127) / 128);
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_intersect_spheres<3>, dim3(blocks), dim3(128), 0, s, S, (const float4 *)spheres, rnd, n,
+                           (float2 *)out);
+    else
+        hipLaunchKernelGGL(k_intersect_spheres<2>, dim3(blocks), dim3(128), 0, s, S, (const float4 *)spheres, rnd, n,
+                           (float2 *)out);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
+                                      const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
+                                      void *stream)
+{
+    int st = check_ready(c, "intersect_spheres_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres_device: bad arguments");
+    if (((uintptr_t)spheres & 15) || ((uintptr_t)rand_nums & 3) || ((uintptr_t)out & 7))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "intersect_spheres_device: spheres must be 16-byte, rand_nums 4-byte and out 8-byte aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    return launch_spheres(c, spheres, (const float *)rand_nums, n, out, (hipStream_t)stream);
+}
+
+int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
+                               const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out)
+{
+    int st = check_ready(c, "intersect_spheres");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t sb = (size_t)n * sizeof(fcpw_hip_bounding_sphere), rb = (size_t)n * sizeof(fcpw_hip_float3);
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, sb + rb);
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, sb, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy((char *)c->stage_in + sb, rand_nums, rb, hipMemcpyHostToDevice));
+    st = launch_spheres(c, c->stage_in, (const float *)((char *)c->stage_in + sb), n, c->stage_out, 0);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    return FCPW_HIP_OK;
 }
 
 int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *, const uint32_t *, int64_t,

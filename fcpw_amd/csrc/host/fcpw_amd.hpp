@@ -331,10 +331,29 @@ public:
                                      checkForOcclusion ? 1 : 0, (fcpw_hip_interaction *)interactions.data()),
                   "intersect");
     }
-    // fcpw_gpu.inl:158-217: not built by this backend
+    // fcpw_gpu.inl:158-196 (row-major n x DIM centers, n x 3 random numbers in [0, 1)); with raw
+    // pointers this overload would collide with the ray one, hence the distinct name
+    void intersectSpheres(const float *sphereCenters, const float *sphereSquaredRadii, const float *randNums,
+                          int64_t n, std::vector<GPUInteraction> &interactions)
+    {
+        std::vector<GPUBoundingSphere> spheres((size_t)n);
+        std::vector<float3> rnd((size_t)n);
+        parallel_rows(n, [&](int64_t s, int64_t e) {
+            for (int64_t i = s; i < e; i++) {
+                const float *c = sphereCenters + i * DIM;
+                spheres[i] = GPUBoundingSphere(float3{c[0], c[1], DIM == 2 ? 0.0f : c[DIM - 1]}, sphereSquaredRadii[i]);
+                const float *r = randNums + i * 3;
+                rnd[i] = float3{r[0], r[1], DIM == 2 ? 0.0f : r[2]};
+            }
+        });
+        intersect(spheres, rnd, interactions);
+    }
+    // fcpw_gpu.inl:198-217
     void intersect(const std::vector<GPUBoundingSphere> &spheres, const std::vector<float3> &randNums,
                    std::vector<GPUInteraction> &interactions)
     {
+        if (randNums.size() != spheres.size())
+            throw std::invalid_argument("intersect(spheres): one rand_nums entry per sphere");
         require();
         interactions.assign(spheres.size(), GPUInteraction());
         hip_check(ctx_,

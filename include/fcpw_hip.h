@@ -122,12 +122,25 @@ int fcpw_hip_intersect_device(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int64
                               fcpw_hip_interaction *out, void *stream);
 
 /*
- * The sphere-intersection and closest-silhouette queries of the reference
- * (fcpw_gpu.inl:158-217,276-338; bvh.cs.slang:59-82,107-133) are not built by
- * this backend; these return FCPW_HIP_ERR_UNSUPPORTED.
+ * Replaces GPUScene::intersect(spheres, randNums) (fcpw_gpu.inl:158-217) and the
+ * sphereIntersection entry point (bvh.cs.slang:59-82, bvh.slang:257-392): per
+ * sphere, one primitive inside it is chosen at random (one random walk down the
+ * overlapping children with equal probability, then area-weighted among the
+ * leaf's primitives) and a uniform point on it is sampled.  out: p, n, uv of the
+ * point, d = its pdf, index = the primitive; a sphere with no primitive keeps
+ * the default record.  rand_nums: three numbers in [0, 1) per sphere (the first
+ * drives the choice, the other two the point; the third is unused in 2D).
  */
 int fcpw_hip_intersect_spheres(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
                                const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out);
+int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
+                                      const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
+                                      void *stream);
+
+/*
+ * The closest-silhouette query of the reference (fcpw_gpu.inl:276-338,
+ * bvh.cs.slang:107-133) is not built by this backend: FCPW_HIP_ERR_UNSUPPORTED.
+ */
 int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
                                        const uint32_t *flip_normal_orientation, int64_t n,
                                        float squared_min_radius, float precision, fcpw_hip_interaction *out);

@@ -96,6 +96,7 @@ def _declare(L):
     L.oracle_intersect_gpuorder.argtypes = [c_void, _f32p, _f32p, _f32p, c_int, c_int, c_int, _hitp,
                                             oi32, oi32]
     L.oracle_brute_closest_f32.argtypes = [c_void, _f32p, _f32p, c_int, c_int, _hitp]
+    L.oracle_intersect_spheres.argtypes = [c_void, _f32p, _f32p, c_int, c_int, _hitp]
     L.oracle_brute_closest_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, c_int, c_int,
                                            _f64p, _f64p, _i32p]
     L.oracle_brute_ray_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, _f32p, c_int, c_int,
@@ -202,6 +203,18 @@ class Bvh:
         npr = np.zeros(len(o), np.int32) if counters else None
         lib().oracle_intersect_gpuorder(self._h, o, d, tmax, len(o), int(occlusion), nthreads, out, ni, npr)
         return (out, ni, npr) if counters else out
+
+    def intersect_spheres(self, centers, r2, rand_nums, nthreads=0):
+        """GPU sphere intersection (bvh.slang:257-392): centers (n, dim), r2 (n,),
+        rand_nums (n, 3) in [0, 1)."""
+        c = self._q(centers)
+        sph = np.zeros((len(c), 4), np.float32)
+        sph[:, :c.shape[1]] = c
+        sph[:, 3] = _f32(r2).ravel()
+        rn = np.ascontiguousarray(rand_nums, dtype=np.float32).reshape(len(c), 3)
+        out = np.zeros(len(c), HIT_DTYPE)
+        lib().oracle_intersect_spheres(self._h, sph, rn, len(c), nthreads, out)
+        return out
 
     def brute_closest_f32(self, q, r2=None, nthreads=0):
         q = self._q(q)

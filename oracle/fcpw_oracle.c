@@ -1203,6 +1203,167 @@ void oracle_intersect_gpuorder(const oracle_bvh *h, const float *origins, const 
 }
 
 /* ----------------------------------------------------------------------------
+ * Sphere intersection, GPU semantics (bvh.slang:257-392) with the constant
+ * branch weight of the sphereIntersection entry point (bvh.cs.slang:59-82,
+ * aggregate.slang:13-20): one random walk from the root, choosing among the
+ * children that overlap the sphere with equal probability, then an
+ * area-weighted reservoir choice among the leaf's primitives inside the
+ * sphere, then a uniform point on the chosen primitive.  Primitive area,
+ * sphere test and point sampling follow geometry.slang (LineSegment
+ * 133-136,152-162,180-191; Triangle 356-359,380-391,409-423).
+ * ------------------------------------------------------------------------- */
+static float prim_area_gpu(const oracle_bvh *h, int slot)
+{
+    if (h->dim == 3) {
+        const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1), *pc = vtx(h, slot, 2);
+        float a[3], b[3], n[3];
+        for (int i = 0; i < 3; i++) { a[i] = pb[i] - pa[i]; b[i] = pc[i] - pa[i]; }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        return 0.5f * sqrtf(gdot(n, n, 3));
+    }
+    const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1);
+    float sv[2] = {pb[0] - pa[0], pb[1] - pa[1]};
+    return sqrtf(gdot(sv, sv, 2));
+}
+
+/* samplePoint: writes uv, p, n and returns the sampling pdf */
+static float prim_sample_gpu(const oracle_bvh *h, int slot, float r1, float r2, float *uv, float *p, float *n)
+{
+    if (h->dim == 3) {
+        const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1), *pc = vtx(h, slot, 2);
+        float a[3], b[3];
+        for (int i = 0; i < 3; i++) { a[i] = pb[i] - pa[i]; b[i] = pc[i] - pa[i]; }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        float area = sqrtf(gdot(n, n, 3));
+        float u1 = sqrtf(r1), u = 1.0f - u1, v = r2 * u1, w = (1.0f - u) - v;
+        uv[0] = u;
+        uv[1] = v;
+        for (int i = 0; i < 3; i++) {
+            p[i] = (pa[i] * u + pb[i] * v) + pc[i] * w;
+            n[i] = n[i] / area;
+        }
+        return 2.0f / area;
+    }
+    const float *pa = vtx(h, slot, 0), *pb = vtx(h, slot, 1);
+    float sv[2] = {pb[0] - pa[0], pb[1] - pa[1]};
+    float area = sqrtf(gdot(sv, sv, 2));
+    uv[0] = r1;
+    uv[1] = 0.0f;
+    p[0] = pa[0] + r1 * sv[0];
+    p[1] = pa[1] + r1 * sv[1];
+    p[2] = 0.0f;
+    n[0] = sv[1] / area;
+    n[1] = -sv[0] / area;
+    n[2] = 0.0f;
+    return 1.0f / area;
+}
+
+typedef struct {
+    const oracle_bvh *h;
+    const float *spheres, *rnd; /* n x 4 (c, r2) and n x 3 */
+    oracle_hit *out;
+} sph_ctx;
+
+static void sph_range(void *vctx, int start, int end)
+{
+    sph_ctx *c = (sph_ctx *)vctx;
+    const oracle_bvh *h = c->h;
+    int dim = h->dim;
+    for (int qi = start; qi < end; qi++) {
+        float x[3] = {c->spheres[(size_t)qi * 4], c->spheres[(size_t)qi * 4 + 1], 0.0f};
+        if (dim == 3) x[2] = c->spheres[(size_t)qi * 4 + 2];
+        float r2 = c->spheres[(size_t)qi * 4 + 3];
+        const float *rn = c->rnd + (size_t)qi * 3;
+        oracle_hit it;
+        hit_default(&it);
+        int selected = -1, did = 0;
+        float d0, d1, d2, d3;
+        box_sqdist_gpu(&h->nodes[0].box, x, dim, &d0, &d1);
+        if (d0 <= r2) {
+            float maxd = d1, pdf = 1.0f, u = rn[0];
+            int cur = 0;
+            for (;;) {
+                const node_t *nd = &h->nodes[cur];
+                if (nd->nrefs > 0) {
+                    float total = 0.0f;
+                    for (int k = 0; k < nd->nrefs; k++) {
+                        int slot = nd->offset + k, hitp = 0;
+                        oracle_hit cand;
+                        hit_default(&cand);
+                        if (maxd <= r2) {
+                            hitp = 1;
+                        } else {
+                            float pt[3], uv[2];
+                            float dd = prim_closest(h, slot, x, pt, uv, gdot);
+                            if (dd * dd <= r2) {
+                                hitp = 1;
+                                memcpy(cand.p, pt, sizeof(pt));
+                                memcpy(cand.uv, uv, sizeof(uv));
+                            }
+                        }
+                        if (hitp) {
+                            cand.d = prim_area_gpu(h, slot);
+                            cand.index = h->prim_orig[slot];
+                            did = 1;
+                            total += cand.d;
+                            float prob = cand.d / total;
+                            if (u < prob) {
+                                u = u / prob;
+                                it = cand;
+                                it.d *= pdf;
+                                selected = slot;
+                            } else {
+                                u = (u - prob) / (1.0f - prob);
+                            }
+                        }
+                    }
+                    if (total > 0.0f) it.d /= total;
+                    break;
+                }
+                int l = cur + 1, r = cur + nd->offset;
+                box_sqdist_gpu(&h->nodes[l].box, x, dim, &d0, &d1);
+                float wl = (d0 <= r2) ? 1.0f : 0.0f;
+                box_sqdist_gpu(&h->nodes[r].box, x, dim, &d2, &d3);
+                float wr = (d2 <= r2) ? 1.0f : 0.0f;
+                float tot = wl + wr;
+                if (!(tot > 0.0f)) break;
+                float pl = wl / tot, pr = 1.0f - pl;
+                if (u < pl) {
+                    u = u / pl;
+                    cur = l;
+                    pdf *= pl;
+                    maxd = d1;
+                } else {
+                    u = (u - pl) / pr;
+                    cur = r;
+                    pdf *= pr;
+                    maxd = d3;
+                }
+            }
+        }
+        if (did && (it.index == -1 || selected < 0)) did = 0;
+        if (did) {
+            float spdf = prim_sample_gpu(h, selected, rn[1], rn[2], it.uv, it.p, it.n);
+            it.d *= spdf;
+        } else {
+            hit_default(&it);
+        }
+        c->out[qi] = it;
+    }
+}
+
+void oracle_intersect_spheres(const oracle_bvh *h, const float *spheres, const float *rand_nums, int n,
+                              int nthreads, oracle_hit *out)
+{
+    sph_ctx c = {h, spheres, rand_nums, out};
+    parallel_for(n, nthreads, sph_range, &c);
+}
+
+/* ----------------------------------------------------------------------------
  * FP64 brute force, independent method: project onto the supporting line or
  * plane, keep the projection when it falls inside the primitive, otherwise take
  * the nearest boundary point (edge clamping).  No code is shared with the

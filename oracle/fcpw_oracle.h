@@ -13,6 +13,7 @@
  *   - CPU refit                     include/fcpw/aggregates/bvh.inl:548-599
  *   - brute-force aggregate         include/fcpw/aggregates/baseline.inl:342-384
  *   - GPU traversal order           include/fcpw/gpu/bvh.slang:144-254,395-502
+ *   - GPU sphere intersection       include/fcpw/gpu/bvh.slang:257-392
  *   - primitive routines            include/fcpw/geometry/triangles.inl:12-32,222-260,374-487,
  *                                   include/fcpw/geometry/line_segments.inl:11-28,140-175,200-255,
  *                                   include/fcpw/gpu/geometry.slang:37-103,202-332
@@ -82,6 +83,10 @@ void oracle_intersect_gpuorder(const oracle_bvh *h, const float *origins, const 
                                const float *tmax, int n, int occlusion, int nthreads, oracle_hit *out,
                                int32_t *n_internal, int32_t *n_leaf_prims);
 /* Baseline aggregate: every primitive in scene order, FP32 (baseline.inl:342-384) */
+/* GPU sphere intersection (bvh.slang:257-392, constant branch weight):
+   spheres n x 4 (center, r^2), rand_nums n x 3 */
+void oracle_intersect_spheres(const oracle_bvh *h, const float *spheres, const float *rand_nums, int n,
+                              int nthreads, oracle_hit *out);
 void oracle_brute_closest_f32(const oracle_bvh *h, const float *queries, const float *r2, int n,
                               int nthreads, oracle_hit *out);
 
