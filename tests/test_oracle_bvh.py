@@ -167,3 +167,66 @@ def test_refit_matches_boxes_recomputed_on_same_topology(name):
     # refit answers match a brute force on the moved geometry
     q = meshes.queries_around(V2, 1000)
     _assert_matches_brute(V2, F, b.closest_points_cpu(q), b.brute_closest_f32(q), q)
+
+
+def _sizes64(V, F):
+    V64 = V.astype(np.float64)
+    if V.shape[1] == 3:
+        a, b, c = V64[F[:, 0]], V64[F[:, 1]], V64[F[:, 2]]
+        return 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1), (a + b + c) / 3
+    a, b = V64[F[:, 0]], V64[F[:, 1]]
+    return np.linalg.norm(b - a, axis=1), (a + b) / 2
+
+
+def test_sphere_sampling_single_primitive():
+    """One triangle (one segment) inside the sphere: the walk is trivial, so
+    pdf = 1/size exactly and the sampled points are uniform over the
+    primitive: their mean is its centroid."""
+    rng = np.random.default_rng(11)
+    for V, F in ((np.array([[0, 0, 0], [2, 0, 0], [0.5, 1.5, 0.3]], np.float32), np.array([[0, 1, 2]], np.int32)),
+                 (np.array([[-1, 0.5], [2, 1]], np.float32), np.array([[0, 1]], np.int32))):
+        n = 200000
+        dim = V.shape[1]
+        u = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+        c = np.zeros((n, dim), np.float32)
+        hits = oracle.Bvh(V, F).intersect_spheres(c, np.full(n, 100.0, np.float32), u)
+        size, cen = _sizes64(V, F)
+        assert (hits["index"] == 0).all()
+        np.testing.assert_allclose(hits["d"], 1.0 / size[0], rtol=1e-6)
+        p = hits["p"][:, :dim].astype(np.float64)
+        se = p.std(0) / np.sqrt(n)
+        assert (np.abs(p.mean(0) - cen[0]) <= 5 * se + 1e-7).all()
+        assert oracle.prim_dist_f64(V, F, p.astype(np.float32), np.zeros(n, np.int32)).max() <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["icosphere", "star2d"])
+def test_sphere_sampling_is_an_unbiased_area_estimator(name):
+    """Law of the stochastic sphere intersection (bvh.slang:257-392): over the
+    random numbers, E[1/pdf] is the total size of the primitives that overlap
+    the sphere and E[p/pdf] is their size-weighted centroid sum (uniform
+    sampling within a primitive), both computed here in FP64 from the mesh."""
+    V, F = CASES[name]
+    dim = V.shape[1]
+    size, cen = _sizes64(V, F)
+    b = oracle.Bvh(V, F)
+    rng = np.random.default_rng(12)
+    n = 100000
+    for k, radius in enumerate([0.2, 0.5, 1.0, 10.0]):
+        centre = (V[rng.integers(len(V))] + rng.normal(0, 0.05, dim)).astype(np.float32)
+        hits = b.intersect_spheres(np.repeat(centre[None], n, 0), np.full(n, radius * radius, np.float32),
+                                   rng.uniform(0, 1, (n, 3)).astype(np.float32))
+        hit = hits["index"] >= 0
+        w = np.where(hit, 1.0 / hits["d"].astype(np.float64), 0.0)
+        d64 = oracle.prim_dist_f64(V, F, np.repeat(centre[None], len(F), 0), np.arange(len(F), dtype=np.int32))
+        lo = size[d64 <= radius * (1 - 1e-5)].sum()
+        hi = size[d64 <= radius * (1 + 1e-5)].sum()
+        se = w.std() / np.sqrt(n)
+        assert lo - 5 * se <= w.mean() <= hi + 5 * se, (k, w.mean(), lo, hi, se)
+        if radius == 10.0:  # every primitive is inside: check the point law too
+            assert lo == size.sum()
+            wp = w[:, None] * hits["p"][:, :dim].astype(np.float64)
+            target = (size[:, None] * cen).sum(0)
+            se_p = wp.std(0) / np.sqrt(n)
+            assert (np.abs(wp.mean(0) - target) <= 5 * se_p + 1e-9).all(), (wp.mean(0), target, se_p)
+        idx = hits["index"][hit].astype(np.int32)
+        assert oracle.prim_dist_f64(V, F, hits["p"][hit][:, :dim].copy(), idx).max() <= 1e-5 * np.abs(V).max()
