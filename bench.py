@@ -350,7 +350,22 @@ def run_torus(args, torch):
         res["rays"] = {"value": nq / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
                        "workload": f"{nq} rays, origins uniform in the AABB, directions uniform on S^2, tMax=inf"}
         log(f"[rays] {t_r * 1e3:.3f} ms/step, {nq / t_r:.4e} rays/s")
-        del rays, ray_out
+        del rays
+
+        # ---- stochastic sphere intersection at the query points, radius 0.1
+        balls = _spheres(torch, q, dev)
+        balls[:, 3] = 0.01
+        rnd = torch.rand((nq, 3), dtype=torch.float32, device=dev)
+
+        def sphere_step():
+            gpu.intersect_spheres_device(balls, rnd, ray_out, stream=stream)
+        t_s = timed(torch, stream, sphere_step, args.steps, args.warmup)
+        found = float((ray_out[:, 9].view(torch.int32) != -1).float().mean())
+        res["sphere_intersection"] = {"value": nq / t_s, "unit": "queries/s", "ms_per_step": t_s * 1e3,
+                                      "workload": f"{nq} spheres of radius 0.1 at the closest-point queries, "
+                                                  f"uniform random numbers; {found:.3f} find a primitive"}
+        log(f"[spheres] {t_s * 1e3:.3f} ms/step, {nq / t_s:.4e} queries/s, found {found:.3f}")
+        del balls, rnd, ray_out
 
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20

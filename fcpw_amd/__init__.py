@@ -1,8 +1,8 @@
 """fcpw_amd -- MI355X query backend for FCPW.
 
 Drop-in for the reference's GPU Python surface (python/fcpw_bindings.cpp:493-673):
-``GPUScene2D/3D`` with ``transfer_to_gpu``, ``refit``, ``intersect`` and
-``find_closest_points``, the ``GPU*`` record types and ``GPUInteractionList``
+``GPUScene2D/3D`` with ``transfer_to_gpu``, ``refit``, ``intersect`` (rays and
+spheres), ``find_closest_points`` and ``find_closest_silhouette_points``, the ``GPU*`` record types and ``GPUInteractionList``
 with its bulk ``get_*`` extraction, plus the ``Scene2D/3D`` subset that builds
 what the GPU consumes.  Queries run in hand-written HIP kernels for gfx950
 (``lib/libfcpw_hip.so``, C ABI in ``include/fcpw_hip.h``); there is no CPU
@@ -59,6 +59,7 @@ GPUFloat3List = _native.GPUFloat3List
 GPURayList = _native.GPURayList
 GPUBoundingSphereList = _native.GPUBoundingSphereList
 GPUInteractionList = _native.GPUInteractionList
+Uint32List = _native.Uint32List
 UINT_MAX = _native.UINT_MAX
 
 # The reference exports the directory of its packaged Slang shaders
@@ -102,7 +103,21 @@ class _GPUSceneMixin:
         return self._find_closest_points_arrays(*args, **kwargs)
 
     def find_closest_silhouette_points(self, *args, **kwargs):
-        raise RuntimeError("closest silhouette point queries are not built by the MI355X backend")
+        """find_closest_silhouette_points(query_points, squared_max_radii, flip_normal_orientation, interactions,
+                                          squared_min_radius=0.0, precision=1e-3)
+        find_closest_silhouette_points(bounding_spheres, flip_normal_orientation, interactions,
+                                       squared_min_radius=0.0, precision=1e-3)
+        (python/fcpw_bindings.cpp:620-627,664-671).  Needs a scene on which
+        compute_silhouettes() was called before build()."""
+        if "bounding_spheres" in kwargs or (args and _is_list(args[0], GPUBoundingSphereList)):
+            args = list(args)
+            if len(args) > 1 and not isinstance(args[1], Uint32List):
+                args[1] = Uint32List([int(v) for v in _np.asarray(args[1]).ravel()])
+            if "flip_normal_orientation" in kwargs and not isinstance(kwargs["flip_normal_orientation"], Uint32List):
+                kwargs["flip_normal_orientation"] = Uint32List(
+                    [int(v) for v in _np.asarray(kwargs["flip_normal_orientation"]).ravel()])
+            return self._find_closest_silhouette_points(*args, **kwargs)
+        return self._find_closest_silhouette_points_arrays(*args, **kwargs)
 
     # -- device-resident queries (torch tensors) ---------------------------------
     def find_closest_points_device(self, spheres, out, record_normals=False, stream=None):
@@ -116,6 +131,17 @@ class _GPUSceneMixin:
         """rays: (n, 10) float32 CUDA tensor of GPURay records; out: (n, 10) float32."""
         self._intersect_device(rays.data_ptr(), rays.shape[0], bool(check_for_occlusion), out.data_ptr(),
                                _stream_handle(rays, stream))
+
+    def find_closest_silhouette_points_device(self, spheres, flip_normal_orientation, out, squared_min_radius=0.0,
+                                              precision=1e-3, stream=None):
+        """spheres: (n, 4) GPUBoundingSphere records, flip_normal_orientation: (n,) int32/uint32,
+        out: (n, 10) GPUInteraction records (CUDA tensors)."""
+        n = spheres.shape[0]
+        fl = flip_normal_orientation
+        if fl.shape[0] != n or not fl.is_contiguous() or fl.element_size() != 4:
+            raise ValueError("flip_normal_orientation must be a contiguous (n,) 32-bit tensor")
+        self._closest_silhouette_points_device(spheres.data_ptr(), fl.data_ptr(), n, float(squared_min_radius),
+                                               float(precision), out.data_ptr(), _stream_handle(spheres, stream))
 
     def intersect_spheres_device(self, spheres, rand_nums, out, stream=None):
         """spheres: (n, 4) GPUBoundingSphere records, rand_nums: (n, 3) float32 in [0, 1),
@@ -153,5 +179,5 @@ class GPUScene3D(_GPUSceneMixin, _native.GPUScene3D):
 
 __all__ = ["PrimitiveType", "AggregateType", "Scene2D", "Scene3D", "GPUFloat2", "GPUFloat3", "GPURay",
            "GPUBoundingSphere", "GPUInteraction", "GPUFloat3List", "GPURayList", "GPUBoundingSphereList",
-           "GPUInteractionList", "GPUScene2D", "GPUScene3D", "slang_directory_path", "UINT_MAX",
+           "GPUInteractionList", "Uint32List", "GPUScene2D", "GPUScene3D", "slang_directory_path", "UINT_MAX",
            "INTERACTION_DTYPE", "RAY_DTYPE", "SPHERE_DTYPE"]

@@ -42,7 +42,7 @@ def build_ext():
     import pybind11
     py_inc = sysconfig.get_paths()["include"]
     tmp = EXT + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden", "-ffp-contract=off",
           f"-I{py_inc}", f"-I{pybind11.get_include()}",
           os.path.join(HERE, "csrc", "py", "bindings.cpp"),
           f"-L{os.path.dirname(HIP_LIB)}", "-lfcpw_hip", "-Wl,-rpath,$ORIGIN/lib",

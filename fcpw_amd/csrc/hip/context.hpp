@@ -26,6 +26,13 @@ struct fcpw_hip_ctx {
     unsigned int *counters = nullptr;
     int *flags = nullptr;
     unsigned long long *stats = nullptr; // internal nodes, primitive tests, queries (option "stats")
+    // silhouettes (fcpw_hip_upload_silhouettes): cones per child slot and root, leaf ranges, records
+    float4 *cones = nullptr, *sils = nullptr;
+    uint2 *sil_leaf = nullptr;
+    bool has_silhouettes = false;
+    int n_sil_refs = 0;
+    std::vector<int32_t> node_slot;     // flat node -> child slot (record * 2 + side), root = 2 * n_records
+    std::vector<int32_t> node_leaf_off; // flat node -> first leaf slot, -1 for internal nodes
     // staging for the host-pointer entry points
     void *stage_in = nullptr, *stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
@@ -64,12 +71,24 @@ static std::string g_create_error;
         }                                                                                                      \
     } while (0)
 
+static void free_silhouettes(fcpw_hip_ctx *c)
+{
+    void *bufs[] = {c->cones, c->sils, c->sil_leaf};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->cones = c->sils = nullptr;
+    c->sil_leaf = nullptr;
+    c->has_silhouettes = false;
+    c->n_sil_refs = 0;
+}
+
 static void free_scene(fcpw_hip_ctx *c)
 {
     void *bufs[] = {c->records, c->root, c->prims, c->prim_index, c->prim_vids, c->V,
                     c->rec_parent, c->leaf_refs, c->leaf_links, c->counters};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    free_silhouettes(c);
     c->records = c->root = c->prims = nullptr;
     c->prim_index = c->prim_vids = nullptr;
     c->V = nullptr;

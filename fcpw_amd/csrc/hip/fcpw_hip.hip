@@ -3,6 +3,7 @@
 // adjoint, options) and, from batch.hpp, the batched multi-scene launch.
 // Device code lives in device.hpp, the per-scene context in context.hpp.
 #include "device.hpp"
+#include "silhouette.hpp"
 #include "context.hpp"
 
 extern "C" {
@@ -73,6 +74,13 @@ static uint32_t make_link(bool leaf, int32_t offset, int32_t nrefs, const std::v
     return LEAF_BIT | ((uint32_t)(nrefs - 1) << CNT_SHIFT) | (uint32_t)offset;
 }
 
+static int check_ready(fcpw_hip_ctx *c, const char *fn)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    if (!c->uploaded) FAIL(c, FCPW_HIP_ERR_NOT_UPLOADED, "%s: no BVH has been uploaded", fn);
+    return FCPW_HIP_OK;
+}
+
 int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin, const float *pmax,
                         const int32_t *nrefs, const int32_t *offset, int n_prims, const int32_t *prim_vertices,
                         const int32_t *prim_index, int n_vertices, const float *positions)
@@ -131,12 +139,16 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
             else rec[(size_t)r * 3 + 2].y = bits_as_float(link);
         }
     };
+    std::vector<int32_t> node_slot((size_t)n_nodes, 2 * n_rec), node_leaf_off((size_t)n_nodes, -1);
+    for (int i = 0; i < n_nodes; i++)
+        if (nrefs[i] > 0) node_leaf_off[i] = offset[i];
     for (int i = 0; i < n_nodes; i++) {
         if (nrefs[i] > 0) continue;
         int r = rec_of[i];
         int kids[2] = {i + 1, i + offset[i]};
         for (int s = 0; s < 2; s++) {
             int k = kids[s];
+            node_slot[k] = r * 2 + s;
             depth[k] = depth[i] + 1;
             max_depth = std::max(max_depth, depth[k]);
             bool leaf = nrefs[k] > 0;
@@ -226,15 +238,67 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
     c->n_prims = n_prims;
     c->n_vertices = n_vertices;
     c->max_depth = max_depth;
+    c->node_slot = std::move(node_slot);
+    c->node_leaf_off = std::move(node_leaf_off);
     c->uploaded = true;
     c->generation++;
     return FCPW_HIP_OK;
 }
 
-static int check_ready(fcpw_hip_ctx *c, const char *fn)
+int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bounding_cone *cones,
+                                const int32_t *n_silhouettes, const int32_t *silhouette_offset, int n_refs,
+                                const void *silhouettes)
 {
-    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
-    if (!c->uploaded) FAIL(c, FCPW_HIP_ERR_NOT_UPLOADED, "%s: no BVH has been uploaded", fn);
+    int st = check_ready(c, "upload_silhouettes");
+    if (st) return st;
+    if (n_nodes != (int)c->node_slot.size() || !cones || !n_silhouettes || !silhouette_offset || n_refs < 0 ||
+        (n_refs > 0 && !silhouettes))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_silhouettes: expected %d nodes and non-NULL arrays",
+             (int)c->node_slot.size());
+    const int n_slots = 2 * c->n_records + 1;
+    std::vector<float4> cone_buf((size_t)n_slots * 2, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    std::vector<uint2> leaf_buf((size_t)c->n_prims, make_uint2(0u, 0u));
+    for (int i = 0; i < n_nodes; i++) {
+        const fcpw_hip_bounding_cone &k = cones[i];
+        const int slot = c->node_slot[i];
+        cone_buf[(size_t)slot * 2] = make_float4(k.axis.x, k.axis.y, k.axis.z, k.half_angle);
+        cone_buf[(size_t)slot * 2 + 1] = make_float4(k.radius, 0.0f, 0.0f, 0.0f);
+        const int32_t cnt = n_silhouettes[i], off = silhouette_offset[i];
+        if (cnt < 0 || off < 0 || (int64_t)off + cnt > n_refs)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_silhouettes: node %d silhouette range out of bounds", i);
+        if (c->node_leaf_off[i] < 0) {
+            if (cnt != 0) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_silhouettes: internal node %d has silhouettes", i);
+        } else {
+            leaf_buf[c->node_leaf_off[i]] = make_uint2((uint32_t)off, (uint32_t)cnt);
+        }
+    }
+    const int RW = c->dim == 3 ? 4 : 2;
+    std::vector<float4> rec_buf((size_t)std::max(n_refs, 1) * RW, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (int r = 0; r < n_refs; r++) {
+        if (c->dim == 2) {
+            const fcpw_hip_vertex &v = ((const fcpw_hip_vertex *)silhouettes)[r];
+            rec_buf[(size_t)r * 2] = make_float4(v.p.x, v.p.y, v.n0.x, v.n0.y);
+            rec_buf[(size_t)r * 2 + 1] =
+                make_float4(v.n1.x, v.n1.y, bits_as_float(v.index), bits_as_float(v.has_one_adjacent_face));
+        } else {
+            const fcpw_hip_edge &e = ((const fcpw_hip_edge *)silhouettes)[r];
+            rec_buf[(size_t)r * 4] = make_float4(e.pa.x, e.pa.y, e.pa.z, bits_as_float(e.index));
+            rec_buf[(size_t)r * 4 + 1] = make_float4(e.pb.x, e.pb.y, e.pb.z, bits_as_float(e.has_one_adjacent_face));
+            rec_buf[(size_t)r * 4 + 2] = make_float4(e.n0.x, e.n0.y, e.n0.z, 0.0f);
+            rec_buf[(size_t)r * 4 + 3] = make_float4(e.n1.x, e.n1.y, e.n1.z, 0.0f);
+        }
+    }
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipDeviceSynchronize());
+    free_silhouettes(c);
+    HIPCHECK(c, hipMalloc((void **)&c->cones, cone_buf.size() * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->sil_leaf, leaf_buf.size() * sizeof(uint2)));
+    HIPCHECK(c, hipMalloc((void **)&c->sils, rec_buf.size() * sizeof(float4)));
+    HIPCHECK(c, hipMemcpy(c->cones, cone_buf.data(), cone_buf.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->sil_leaf, leaf_buf.data(), leaf_buf.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->sils, rec_buf.data(), rec_buf.size() * sizeof(float4), hipMemcpyHostToDevice));
+    c->n_sil_refs = n_refs;
+    c->has_silhouettes = true;
     return FCPW_HIP_OK;
 }
 
@@ -292,6 +356,17 @@ int fcpw_hip_refit(fcpw_hip_ctx *c, void *stream)
         hipLaunchKernelGGL(k_refit<2>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->prims,
                            c->leaf_refs, c->leaf_links, c->rec_parent, c->counters, c->n_leaves);
     HIPCHECK(c, hipGetLastError());
+    if (c->has_silhouettes) {
+        if (c->dim == 3)
+            hipLaunchKernelGGL(k_refit_cones<3>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->cones,
+                               c->sil_leaf, c->sils, c->leaf_refs, c->leaf_links, c->rec_parent, c->counters,
+                               c->n_leaves, c->n_records);
+        else
+            hipLaunchKernelGGL(k_refit_cones<2>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->cones,
+                               c->sil_leaf, c->sils, c->leaf_refs, c->leaf_links, c->rec_parent, c->counters,
+                               c->n_leaves, c->n_records);
+        HIPCHECK(c, hipGetLastError());
+    }
     return FCPW_HIP_OK;
 }
 
@@ -590,11 +665,70 @@ int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *
     return FCPW_HIP_OK;
 }
 
-int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *, const uint32_t *, int64_t,
-                                       float, float, fcpw_hip_interaction *)
+static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32_t *flip, int64_t n, float r2min,
+                              float prec, void *out, hipStream_t s)
 {
-    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
-    FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "closest silhouette point queries are not built by this backend");
+    DevScene S = dev_scene(c);
+    SilScene T = {c->cones, c->sil_leaf, c->sils};
+    const unsigned blocks = (unsigned)((n + 127) / 128);
+    int st = timing_begin(c, s);
+    if (st) return st;
+    if (c->dim == 3)
+        hipLaunchKernelGGL((k_closest_silhouettes<3, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
+                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out);
+    else
+        hipLaunchKernelGGL((k_closest_silhouettes<2, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
+                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out);
+    HIPCHECK(c, hipGetLastError());
+    return timing_end(c, s);
+}
+
+static int check_silhouettes(fcpw_hip_ctx *c, const char *fn)
+{
+    int st = check_ready(c, fn);
+    if (st) return st;
+    if (!c->has_silhouettes)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "%s: the scene was transferred without silhouettes (computeSilhouettes)", fn);
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_closest_silhouette_points_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
+                                              const uint32_t *flip, int64_t n, float squared_min_radius,
+                                              float precision, fcpw_hip_interaction *out, void *stream)
+{
+    int st = check_silhouettes(c, "closest_silhouette_points_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !flip || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_silhouette_points_device: bad arguments");
+    if (((uintptr_t)spheres & 15) || ((uintptr_t)flip & 3) || ((uintptr_t)out & 7))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "closest_silhouette_points_device: spheres must be 16-byte, flags 4-byte and out 8-byte aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    return launch_silhouettes(c, spheres, flip, n, squared_min_radius, precision, out, (hipStream_t)stream);
+}
+
+int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, const uint32_t *flip,
+                                       int64_t n, float squared_min_radius, float precision,
+                                       fcpw_hip_interaction *out)
+{
+    int st = check_silhouettes(c, "closest_silhouette_points");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !flip || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_silhouette_points: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t sb = (size_t)n * sizeof(fcpw_hip_bounding_sphere), fb = (size_t)n * sizeof(uint32_t);
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, sb + fb);
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, sb, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy((char *)c->stage_in + sb, flip, fb, hipMemcpyHostToDevice));
+    st = launch_silhouettes(c, c->stage_in, (const uint32_t *)((char *)c->stage_in + sb), n, squared_min_radius,
+                            precision, c->stage_out, 0);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    return FCPW_HIP_OK;
 }
 
 int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *c, const float *points, const float *r2, int64_t n,

@@ -23,6 +23,7 @@
 
 #include "../../../include/fcpw_hip.h"
 #include "bvh_build.hpp"
+#include "snch_build.hpp"
 
 namespace fcpw_amd {
 
@@ -112,6 +113,8 @@ public:
         if (nObjects < 0) throw std::invalid_argument("setObjectCount: negative count");
         objects_.assign((size_t)nObjects, Object());
         bvh_.reset();
+        snch_.reset();
+        silhouettes_ = false;
     }
     void setObjectVertexCount(int nVertices, int objectIndex)
     {
@@ -151,6 +154,17 @@ public:
         o.indices.assign(indices, indices + (size_t)nPrims * DIM);
     }
 
+    // fcpw.inl:429-521: marks the scene for silhouette queries; the silhouette
+    // data is built with the BVH by the next build() (as the reference, whose
+    // build creates the SNCH variant when silhouettes were computed)
+    void computeSilhouettes(const IgnoreSilhouetteFn &ignoreSilhouette = {})
+    {
+        if (objects_.size() != 1)
+            throw std::runtime_error("the MI355X backend supports scenes with exactly one object (fcpw_gpu.h:17-20)");
+        silhouettes_ = true;
+        ignore_ = ignoreSilhouette;
+    }
+
     // Scene::build (fcpw.inl:1052-1126) for a single object with a binary BVH.
     // vectorize is accepted and ignored: the GPU path needs the binary BVH
     // (fcpw_gpu.h:17-20), and no vectorized CPU traversal exists here.
@@ -166,6 +180,10 @@ public:
         auto t0 = std::chrono::high_resolution_clock::now();
         bvh_ = std::make_unique<FlatBvh<DIM>>(build_bvh<DIM>(o.positions.data(), nVertices(), o.indices.data(),
                                                              nPrimitives(), heuristic_of(aggregateType)));
+        snch_.reset();
+        if (silhouettes_)
+            snch_ = std::make_unique<SnchData<DIM>>(
+                build_snch<DIM>(*bvh_, o.positions.data(), nVertices(), o.indices.data(), nPrimitives(), ignore_));
         if (printStats) {
             double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
             printf("BVH construction time: %g seconds\n", s);
@@ -193,6 +211,7 @@ public:
         require_bvh();
         auto t0 = std::chrono::high_resolution_clock::now();
         refit_bvh<DIM>(*bvh_, objects_[0].positions.data(), objects_[0].indices.data());
+        if (snch_) refit_snch_cones<DIM>(*snch_, *bvh_, objects_[0].positions.data());
         if (printStats) {
             double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
             printf("refit time: %g seconds\n", s);
@@ -207,6 +226,12 @@ public:
     int nVertices() const { return (int)(objects_.at(0).positions.size() / DIM); }
     int nPrimitives() const { return (int)(objects_.at(0).indices.size() / DIM); }
     uint64_t version() const { return version_; }
+    bool hasSilhouettes() const { return (bool)snch_; }
+    const SnchData<DIM> &snch() const
+    {
+        if (!snch_) throw std::runtime_error("the scene was built without silhouettes (call computeSilhouettes first)");
+        return *snch_;
+    }
 
 private:
     struct Object {
@@ -229,6 +254,9 @@ private:
     }
     std::vector<Object> objects_;
     std::unique_ptr<FlatBvh<DIM>> bvh_;
+    std::unique_ptr<SnchData<DIM>> snch_;
+    bool silhouettes_ = false;
+    IgnoreSilhouetteFn ignore_;
     AggregateType aggregate_ = AggregateType::Bvh_SurfaceArea;
     uint64_t version_ = 0;
 };
@@ -286,6 +314,7 @@ public:
                   fcpw_hip_upload_bvh(ctx_, DIM, (int)n, pmin.data(), pmax.data(), nrefs.data(), offset.data(), np,
                                       pv.data(), b.prim_order.data(), scene.nVertices(), scene.positions().data()),
                   "transferToGPU");
+        if (scene.hasSilhouettes()) uploadSilhouettes(scene);
         if (logs_) {
             int64_t rec = 0, depth = 0;
             fcpw_hip_get_info(ctx_, "n_records", &rec);
@@ -297,11 +326,16 @@ public:
 
     // fcpw_gpu.inl:64-80: with updateGeometry, upload the scene's current vertex
     // positions first; then refit every box on the device
+    // (with silhouettes, updateGeometry also re-extracts the silhouette records from
+    // the new positions, as allocateGeometryBuffers does, and the refit recomputes
+    // the cones from them)
     void refit(Scene<DIM> &scene, bool updateGeometry = true)
     {
         require();
-        if (updateGeometry)
+        if (updateGeometry) {
             hip_check(ctx_, fcpw_hip_update_positions(ctx_, scene.positions().data(), scene.nVertices()), "refit");
+            if (scene.hasSilhouettes()) uploadSilhouettes(scene);
+        }
         hip_check(ctx_, fcpw_hip_refit(ctx_, nullptr), "refit");
     }
 
@@ -389,12 +423,31 @@ public:
                   "findClosestPoints");
     }
 
-    // fcpw_gpu.inl:276-338: not built by this backend
+    // fcpw_gpu.inl:276-316 (row-major n x DIM points)
+    void findClosestSilhouettePoints(const float *queryPoints, const float *squaredMaxRadii,
+                                     const int32_t *flipNormalOrientation, int64_t n,
+                                     std::vector<GPUInteraction> &interactions, float squaredMinRadius = 0.0f,
+                                     float precision = 1e-3f)
+    {
+        std::vector<GPUBoundingSphere> spheres((size_t)n);
+        std::vector<uint32_t> flip((size_t)n);
+        parallel_rows(n, [&](int64_t s, int64_t e) {
+            for (int64_t i = s; i < e; i++) {
+                const float *q = queryPoints + i * DIM;
+                spheres[i] = GPUBoundingSphere(float3{q[0], q[1], DIM == 2 ? 0.0f : q[DIM - 1]}, squaredMaxRadii[i]);
+                flip[i] = (uint32_t)flipNormalOrientation[i];
+            }
+        });
+        findClosestSilhouettePoints(spheres, flip, interactions, squaredMinRadius, precision);
+    }
+    // fcpw_gpu.inl:318-338
     void findClosestSilhouettePoints(const std::vector<GPUBoundingSphere> &spheres,
                                      const std::vector<uint32_t> &flipNormalOrientation,
                                      std::vector<GPUInteraction> &interactions, float squaredMinRadius = 0.0f,
                                      float precision = 1e-3f)
     {
+        if (flipNormalOrientation.size() != spheres.size())
+            throw std::invalid_argument("findClosestSilhouettePoints: one flipNormalOrientation entry per sphere");
         require();
         interactions.assign(spheres.size(), GPUInteraction());
         hip_check(ctx_,
@@ -416,6 +469,22 @@ private:
     void require()
     {
         if (!ctx_) throw std::runtime_error("the BVH must be transferred to the GPU first (call transferToGPU)");
+    }
+    // the SNCH half of GPUBvhBuffers::allocate (bvh_interop_structures.h:193-300)
+    void uploadSilhouettes(Scene<DIM> &scene)
+    {
+        const SnchData<DIM> &S = scene.snch();
+        size_t n = S.sil_count.size();
+        std::vector<fcpw_hip_bounding_cone> cones(n);
+        for (size_t i = 0; i < n; i++) {
+            const float *k = &S.cone[i * 5];
+            cones[i] = fcpw_hip_bounding_cone{{k[0], k[1], k[2]}, k[3], k[4]};
+        }
+        std::vector<uint8_t> rec = snch_records<DIM>(S, scene.positions().data());
+        hip_check(ctx_,
+                  fcpw_hip_upload_silhouettes(ctx_, (int)n, cones.data(), S.sil_count.data(), S.sil_offset.data(),
+                                              (int)S.ref_obj.size(), rec.data()),
+                  "transferToGPU(silhouettes)");
     }
     std::string dir_;
     bool logs_;

@@ -3,6 +3,7 @@
 // subset that builds what the GPU consumes (python/fcpw_bindings.cpp:349-491).
 // Overloads that the reference separates by Eigen shapes are exposed under
 // distinct names here; fcpw_amd/__init__.py restores the reference signatures.
+#include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -20,6 +21,7 @@ PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::float3>);
 PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::GPURay>);
 PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::GPUBoundingSphere>);
 PYBIND11_MAKE_OPAQUE(std::vector<fcpw_amd::GPUInteraction>);
+PYBIND11_MAKE_OPAQUE(std::vector<uint32_t>);
 
 using F32 = py::array_t<float, py::array::c_style | py::array::forcecast>;
 using I32 = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
@@ -78,6 +80,30 @@ static void bind_scene(py::module_ &m, const char *name)
         .def("build", &S::build, "aggregate_type"_a, "vectorize"_a, "print_stats"_a = false,
              "reduce_memory_footprint"_a = false)
         .def("refit", &S::refit, "print_stats"_a = false)
+        .def(
+            "compute_silhouettes",
+            [](S &s, py::object ignore) {
+                IgnoreSilhouetteFn fn;
+                if (!ignore.is_none()) fn = ignore.cast<IgnoreSilhouetteFn>();
+                s.computeSilhouettes(fn);
+            },
+            "ignore_silhouette"_a = py::none())
+        .def_property_readonly("has_silhouettes", &S::hasSilhouettes)
+        .def(
+            "_silhouettes",
+            [](const S &s) {
+                const auto &d = s.snch();
+                size_t n = d.sil_count.size();
+                py::array_t<float> cone({(py::ssize_t)n, (py::ssize_t)5});
+                std::memcpy(cone.mutable_data(), d.cone.data(), d.cone.size() * sizeof(float));
+                py::array_t<int32_t> cnt((py::ssize_t)n), off((py::ssize_t)n);
+                std::memcpy(cnt.mutable_data(), d.sil_count.data(), n * sizeof(int32_t));
+                std::memcpy(off.mutable_data(), d.sil_offset.data(), n * sizeof(int32_t));
+                std::vector<uint8_t> rec = snch_records<DIM>(d, s.positions().data());
+                py::bytes recb(reinterpret_cast<const char *>(rec.data()), rec.size());
+                return py::make_tuple(cone, cnt, off, recb);
+            },
+            "cones (n_nodes, 5), n_silhouettes, silhouette_offset and the GPU records as bytes")
         .def_property_readonly("n_vertices", &S::nVertices)
         .def_property_readonly("n_primitives", &S::nPrimitives)
         .def(
@@ -215,9 +241,37 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
         .def(
             "_find_closest_silhouette_points",
             [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<uint32_t> &flip, InteractionList &out,
-               float r2min, float precision) { g.findClosestSilhouettePoints(s, flip, out, r2min, precision); },
+               float r2min, float precision) {
+                py::gil_scoped_release rel;
+                g.findClosestSilhouettePoints(s, flip, out, r2min, precision);
+            },
             "bounding_spheres"_a, "flip_normal_orientation"_a, "interactions"_a, "squared_min_radius"_a = 0.0f,
             "precision"_a = 1e-3f)
+        .def(
+            "_find_closest_silhouette_points_arrays",
+            [](G &g, const F32 &q, const F32 &r2, const I32 &flip, InteractionList &out, float r2min,
+               float precision) {
+                int64_t n;
+                const float *qp = rows(q, n, DIM, "query_points");
+                const float *rp = vec(r2, n, "squared_max_radii");
+                if (flip.ndim() != 1 || flip.shape(0) != n)
+                    throw std::invalid_argument("flip_normal_orientation must have shape (" + std::to_string(n) + ",)");
+                const int32_t *fp = flip.data();
+                py::gil_scoped_release rel;
+                g.findClosestSilhouettePoints(qp, rp, fp, n, out, r2min, precision);
+            },
+            "query_points"_a, "squared_max_radii"_a, "flip_normal_orientation"_a, "interactions"_a,
+            "squared_min_radius"_a = 0.0f, "precision"_a = 1e-3f)
+        .def(
+            "_closest_silhouette_points_device",
+            [](G &g, uintptr_t spheres, uintptr_t flip, int64_t n, float r2min, float precision, uintptr_t out,
+               uintptr_t stream) {
+                hip_check(g.context(),
+                          fcpw_hip_closest_silhouette_points_device(
+                              g.context(), (const fcpw_hip_bounding_sphere *)spheres, (const uint32_t *)flip, n, r2min,
+                              precision, (fcpw_hip_interaction *)out, (void *)stream),
+                          "closest_silhouette_points_device");
+            })
         // device-pointer entry points (torch tensors pass data_ptr() and the stream handle)
         .def(
             "_closest_points_device",
@@ -373,6 +427,9 @@ PYBIND11_MODULE(_fcpw_amd, m)
     py::bind_vector<std::vector<float3>>(m, "GPUFloat3List");
     py::bind_vector<std::vector<GPURay>>(m, "GPURayList");
     py::bind_vector<std::vector<GPUBoundingSphere>>(m, "GPUBoundingSphereList");
+    // UInt32List (python/fcpw_bindings.cpp:104-105); a Python list converts implicitly
+    py::bind_vector<std::vector<uint32_t>>(m, "Uint32List");
+    py::implicitly_convertible<py::list, std::vector<uint32_t>>();
     auto il = py::bind_vector<InteractionList>(m, "GPUInteractionList");
     // bulk extraction (fcpw_bindings.cpp:545-585)
     il.def("get_distances", [](const InteractionList &v) { return gather<float, 1>(v, offsetof(GPUInteraction, d)); },

@@ -59,6 +59,24 @@ typedef struct {
     uint32_t index;
 } fcpw_hip_interaction;
 
+/* GPUBoundingCone (bvh_interop_structures.h:41-52), 20 bytes; half_angle < 0
+   marks a node with no silhouette */
+typedef struct { fcpw_hip_float3 axis; float half_angle; float radius; } fcpw_hip_bounding_cone;
+
+/* GPUVertex (bvh_interop_structures.h:121-139), 44 bytes: a 2D silhouette
+   vertex with the unit normals of its two adjacent segments (z = 0) */
+typedef struct {
+    fcpw_hip_float3 p, n0, n1;
+    uint32_t index, has_one_adjacent_face;
+} fcpw_hip_vertex;
+
+/* GPUEdge (bvh_interop_structures.h:141-162), 56 bytes: a 3D silhouette edge
+   with the unit normals of its two adjacent triangles */
+typedef struct {
+    fcpw_hip_float3 pa, pb, n0, n1;
+    uint32_t index, has_one_adjacent_face;
+} fcpw_hip_edge;
+
 typedef struct fcpw_hip_ctx fcpw_hip_ctx;
 
 /* Stands in for GPUContext::initDevice (slang_rhi_utils.h:23-62) at
@@ -98,7 +116,9 @@ int fcpw_hip_update_positions_device(fcpw_hip_ctx *ctx, const float *positions, 
                                      void *stream);
 
 /* Replaces runBvhUpdate + the refit entry point (bvh_interop_structures.h:891-934,
-   bvh.cs.slang:135-149): one bottom-up launch recomputes every box. */
+   bvh.cs.slang:135-149): one bottom-up launch recomputes every box; with
+   silhouettes uploaded, a second one recomputes every cone from the current
+   silhouette records (Bvh.refit, bvh.slang:27-141). */
 int fcpw_hip_refit(fcpw_hip_ctx *ctx, void *stream);
 
 /*
@@ -138,12 +158,42 @@ int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding
                                       void *stream);
 
 /*
- * The closest-silhouette query of the reference (fcpw_gpu.inl:276-338,
- * bvh.cs.slang:107-133) is not built by this backend: FCPW_HIP_ERR_UNSUPPORTED.
+ * Replaces the silhouette half of GPUBvhBuffers::allocate<DIM> for a scene with
+ * silhouettes (the SNCH node type): the cone and silhouette range of every
+ * node (extractSnchNodes, bvh_interop_structures.h:193-218) and the silhouette
+ * records in reference order (extractSilhouetteVertices / extractSilhouetteEdges,
+ * :237-300).  Call after fcpw_hip_upload_bvh with the same n_nodes; node i's
+ * silhouettes are records [silhouette_offset[i], + n_silhouettes[i]) and only
+ * leaves have any.  silhouettes: n_refs fcpw_hip_vertex (dim 2) or
+ * fcpw_hip_edge (dim 3).  Calling it again replaces the records and cones (the
+ * refit with updateGeometry re-extracts the records from new positions).
+ */
+int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *ctx, int n_nodes, const fcpw_hip_bounding_cone *cones,
+                                const int32_t *n_silhouettes, const int32_t *silhouette_offset, int n_refs,
+                                const void *silhouettes);
+
+/*
+ * Replaces GPUScene::findClosestSilhouettePoints (fcpw_gpu.inl:276-338) and the
+ * closestSilhouettePoint entry point (bvh.cs.slang:107-133, bvh.slang:503-646):
+ * per sphere, the closest point on the visibility silhouette (2D: vertices where
+ * one adjacent segment faces the query and the other does not; 3D: such edges;
+ * boundary vertices/edges always count) within sqrt(r2), pruned by the node
+ * cones.  flip_normal_orientation[i] != 0 flips the orientation used for
+ * queries on the geometry (distance <= precision).  The search stops once a
+ * silhouette within sqrt(squared_min_radius) is found.  out: p, uv (edge
+ * parameter in uv.x), d, index (2D: the vertex id in leaf order, the
+ * reference's sorted numbering; 3D: the edge id); n = 0.  A sphere with no
+ * silhouette keeps the default record.  Needs fcpw_hip_upload_silhouettes
+ * (FCPW_HIP_ERR_UNSUPPORTED otherwise, as the reference kernel built without
+ * silhouette data finds nothing).  2D queries use z = 0.
  */
 int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
                                        const uint32_t *flip_normal_orientation, int64_t n,
                                        float squared_min_radius, float precision, fcpw_hip_interaction *out);
+int fcpw_hip_closest_silhouette_points_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
+                                              const uint32_t *flip_normal_orientation, int64_t n,
+                                              float squared_min_radius, float precision,
+                                              fcpw_hip_interaction *out, void *stream);
 
 /*
  * Compact closest-point query for differentiable use (no reference

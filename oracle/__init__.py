@@ -62,6 +62,15 @@ _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _hitp = np.ctypeslib.ndpointer(dtype=HIT_DTYPE, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+# GPU silhouette record, the reference GPUEdge layout (bvh_interop_structures.h:141-162);
+# 2D vertices keep p in pa and pb = 0
+SILHOUETTE_DTYPE = np.dtype([("pa", "<f4", (3,)), ("pb", "<f4", (3,)), ("n0", "<f4", (3,)), ("n1", "<f4", (3,)),
+                             ("index", "<u4"), ("has_one_adjacent_face", "<u4")])
+assert SILHOUETTE_DTYPE.itemsize == 56
+_silp = np.ctypeslib.ndpointer(dtype=SILHOUETTE_DTYPE, flags="C_CONTIGUOUS")
+IGNORE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_float, ctypes.c_int32, ctypes.c_void_p)
 
 
 def _opt(ptr_type):
@@ -102,6 +111,18 @@ def _declare(L):
     L.oracle_brute_ray_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, _f32p, c_int, c_int,
                                        _f64p, _i32p]
     L.oracle_prim_dist_f64.argtypes = [c_int, _f32p, _i32p, _f32p, _i32p, c_int, _f64p]
+    L.oracle_build_silhouettes.restype = c_int
+    L.oracle_build_silhouettes.argtypes = [c_void, _i32p, IGNORE_FN, c_void]
+    L.oracle_num_silhouette_refs.restype = c_int
+    L.oracle_num_silhouette_refs.argtypes = [c_void]
+    L.oracle_export_silhouettes.argtypes = [c_void, _opt(_f32p), oi32, oi32, _opt(_silp)]
+    L.oracle_update_silhouette_records.argtypes = [c_void]
+    L.oracle_refit_silhouette_cones_gpu.argtypes = [c_void]
+    L.oracle_closest_silhouettes_gpuorder.argtypes = [c_void, _f32p, _u32p, c_int, ctypes.c_float,
+                                                      ctypes.c_float, c_int, c_int, _hitp]
+    L.oracle_closest_silhouette_f64.restype = c_int
+    L.oracle_closest_silhouette_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, c_int,
+                                                ctypes.c_double, ctypes.c_double, c_int, _f64p, _f64p, _f64p]
 
 
 def _f32(a, cols=None):
@@ -216,6 +237,49 @@ class Bvh:
         lib().oracle_intersect_spheres(self._h, sph, rn, len(c), nthreads, out)
         return out
 
+    def build_silhouettes(self, ignore_silhouette=None):
+        """Scene::computeSilhouettes followed by the SNCH data of the build
+        (fcpw.inl:405-521,709-756; bvh.inl:236-420).  ignore_silhouette(angle,
+        primitive index) -> bool as in the reference."""
+        if ignore_silhouette is None:
+            cb = ctypes.cast(None, IGNORE_FN)
+        else:
+            cb = IGNORE_FN(lambda a, i, u: int(bool(ignore_silhouette(float(a), int(i)))))
+        self._ignore_cb = cb
+        return lib().oracle_build_silhouettes(self._h, self.indices, cb, None)
+
+    def silhouettes(self):
+        """(cones (n_nodes, 5), n_silhouettes (n_nodes,), silhouette_offset (n_nodes,),
+        records (n_refs,) SILHOUETTE_DTYPE)"""
+        n = self.n_nodes
+        cones = np.zeros((n, 5), np.float32)
+        cnt = np.zeros(n, np.int32)
+        off = np.zeros(n, np.int32)
+        rec = np.zeros(lib().oracle_num_silhouette_refs(self._h), SILHOUETTE_DTYPE)
+        lib().oracle_export_silhouettes(self._h, cones, cnt, off, rec)
+        return cones, cnt, off, rec
+
+    def update_silhouette_records(self):
+        lib().oracle_update_silhouette_records(self._h)
+
+    def refit_silhouette_cones_gpu(self):
+        """GPU refit of the cones (bvh.slang:27-141) from the current boxes and records."""
+        lib().oracle_refit_silhouette_cones_gpu(self._h)
+
+    def closest_silhouettes(self, q, r2=None, flip=None, squared_min_radius=0.0, precision=1e-3, brute=False,
+                            nthreads=0):
+        """GPU closest silhouette point (bvh.slang:503-646); brute tests every
+        reference in order instead of traversing the tree."""
+        q = self._q(q)
+        sph = np.zeros((len(q), 4), np.float32)
+        sph[:, :q.shape[1]] = q
+        sph[:, 3] = np.float32(np.finfo(np.float32).max) if r2 is None else _f32(r2).ravel()
+        fl = np.zeros(len(q), np.uint32) if flip is None else np.ascontiguousarray(flip, dtype=np.uint32)
+        out = np.zeros(len(q), HIT_DTYPE)
+        lib().oracle_closest_silhouettes_gpuorder(self._h, sph, fl, len(q), squared_min_radius, precision,
+                                                  int(brute), nthreads, out)
+        return out
+
     def brute_closest_f32(self, q, r2=None, nthreads=0):
         q = self._q(q)
         r2 = np.full(len(q), np.inf, np.float32) if r2 is None else _f32(r2).ravel()
@@ -270,3 +334,21 @@ def classify_index_mismatch(positions, indices, q, idx_a, idx_b, rel_tol=1e-6, a
     db = prim_dist_f64(positions, indices, q, idx_b)
     scale = np.maximum(np.maximum(da, db), 0.0)
     return np.abs(da - db) <= rel_tol * scale + abs_tol
+
+
+def closest_silhouette_f64(positions, indices, q, tol=2e-3, tol_d=2e-3, nthreads=0):
+    """FP64 closest silhouette point with an orientation-test classification
+    (independent of the FP32 normal-and-dot rules): returns (d_strict, d_loose,
+    p_strict (n, 3)); d_strict == d_loose where no candidate is ambiguous."""
+    positions = _f32(positions)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    dim = positions.shape[1]
+    q = _f32(q, dim)
+    ds = np.zeros(len(q), np.float64)
+    dl = np.zeros(len(q), np.float64)
+    ps = np.zeros((len(q), 3), np.float64)
+    rc = lib().oracle_closest_silhouette_f64(dim, positions, len(positions), indices, len(indices), q, len(q),
+                                             tol, tol_d, nthreads, ds, dl, ps)
+    if rc != 0:
+        raise ValueError("closest_silhouette_f64 needs manifold input")
+    return ds, dl, ps

@@ -26,6 +26,9 @@
 #define LOWF (-FLT_MAX)
 #define EPSF FLT_EPSILON
 #define BVH_MAX_DEPTH 64
+/* M_PI and M_PI_2 rounded to float, as the reference stores them */
+#define PI_F ((float)3.14159265358979323846)
+#define PI_2_F ((float)1.57079632679489661923)
 
 typedef struct {
     float pmin[3], pmax[3];
@@ -46,6 +49,8 @@ struct oracle_bvh {
     int32_t *index_map;   /* user vertex -> sorted vertex */
     node_t *nodes;
     int n_nodes, n_leaves, max_depth;
+    /* silhouettes, set by oracle_build_silhouettes (NULL until then) */
+    struct sil_data *sil;
 };
 
 /* ----------------------------------------------------------------------------
@@ -706,9 +711,12 @@ oracle_bvh *oracle_build(int dim, const float *positions, int n_vertices, const 
     return h;
 }
 
+static void sil_free(struct sil_data *s);
+
 void oracle_free(oracle_bvh *h)
 {
     if (!h) return;
+    sil_free(h->sil);
     free(h->positions); free(h->prim_vidx); free(h->prim_orig); free(h->index_map); free(h->nodes);
     free(h);
 }
@@ -1541,4 +1549,822 @@ void oracle_prim_dist_f64(int dim, const float *positions, const int32_t *indice
         load_prim_f64(&c, prims[qi], v);
         d[qi] = dim == 3 ? tri_dist_f64(v[0], v[1], v[2], x, pp) : seg_dist_f64(v[0], v[1], x, 2, pp, NULL);
     }
+}
+
+/* ============================================================================
+ * Silhouettes (the SNCH variant of the BVH):
+ *   - silhouette objects of Scene::computeSilhouettes (fcpw.inl:405-521) and
+ *     their renumbering by sortSoupPositions (fcpw.inl:709-756);
+ *   - per-leaf silhouette references and bounding cones of
+ *     Bvh::assignGeometricDataToNodes (bvh.inl:236-420);
+ *   - GPU records of extractSilhouetteVertices / extractSilhouetteEdges
+ *     (bvh_interop_structures.h:237-300);
+ *   - the GPU traversal Bvh.findClosestSilhouettePoint (bvh.slang:503-646) with
+ *     Vertex/Edge.findClosestSilhouettePoint (geometry.slang:490-708) and
+ *     BoundingCone.overlap (bounding-volumes.slang:122-236).
+ * Host-side quantities follow the Eigen expressions (c0 + (c1 + c2) reductions,
+ * normalized() leaves a zero vector unchanged); GPU-side ones use float3
+ * arithmetic summed left to right with z = 0 in 2D.
+ * ========================================================================= */
+struct sil_data {
+    int n_obj, n_refs;
+    int32_t (*ind)[4];  /* per object: 2D {prev, self, next, -}; 3D {opp. of face 1, a, b, opp. of face 0} */
+    int32_t *pindex;    /* per object: reported index */
+    int32_t *ref_obj;   /* per reference (leaf order): object id */
+    float *ref_n;       /* per reference: normal() used for the cone axis (dim floats) */
+    float *ref_fn;      /* per reference: the two unit face normals (2 x dim floats) */
+    float *cone;        /* per node: axis[3], halfAngle, radius */
+    int32_t *off, *cnt; /* per node: silhouetteReferenceOffset, nSilhouetteReferences */
+    oracle_silhouette *rec; /* per reference: GPU record */
+    int32_t *eidx;      /* 3D: edge id of (original triangle, local edge) */
+};
+
+static void sil_free(struct sil_data *s)
+{
+    if (!s) return;
+    free(s->ind); free(s->pindex); free(s->ref_obj); free(s->ref_n); free(s->ref_fn); free(s->cone);
+    free(s->off); free(s->cnt); free(s->rec); free(s->eidx);
+    free(s);
+}
+
+static const float *spos(const oracle_bvh *h, int v) { return h->positions + (size_t)v * h->dim; }
+
+/* Eigen normalized(): v / sqrt(squaredNorm) when squaredNorm > 0 */
+static void enormalize(float *v, int dim)
+{
+    float z = edot(v, v, dim);
+    if (z > 0.0f) {
+        float r = sqrtf(z);
+        for (int i = 0; i < dim; i++) v[i] = v[i] / r;
+    }
+}
+static void ecross(const float *a, const float *b, float *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static int sil_has_face(const oracle_bvh *h, const int32_t *ind, int f)
+{
+    if (h->dim == 2) return f == 0 ? ind[2] != -1 : ind[0] != -1;
+    return f == 0 ? ind[3] != -1 : ind[0] != -1;
+}
+/* SilhouetteVertex::normal(fIndex, normalize) (silhouette_vertices.inl:37-47),
+   SilhouetteEdge::normal(fIndex, normalize) (silhouette_edges.inl:47-68) */
+static void sil_face_normal(const oracle_bvh *h, const int32_t *ind, int f, int normalize, float *n)
+{
+    if (h->dim == 2) {
+        int i = f == 0 ? 1 : 0;
+        const float *pa = spos(h, ind[i]), *pb = spos(h, ind[i + 1]);
+        float s0 = pb[0] - pa[0], s1 = pb[1] - pa[1];
+        n[0] = s1; n[1] = -s0;
+        if (normalize) enormalize(n, 2);
+        return;
+    }
+    int i = f == 0 ? 3 : 0, j = f == 0 ? 1 : 2, k = f == 0 ? 2 : 1;
+    const float *pa = spos(h, ind[j]), *pb = spos(h, ind[k]), *pc = spos(h, ind[i]);
+    float v1[3], v2[3];
+    for (int c = 0; c < 3; c++) { v1[c] = pb[c] - pa[c]; v2[c] = pc[c] - pa[c]; }
+    ecross(v1, v2, n);
+    if (normalize) enormalize(n, 3);
+}
+/* normal() without precomputed soup normals: the normalized sum of the
+   unnormalized adjacent face normals */
+static void sil_normal(const oracle_bvh *h, const int32_t *ind, float *n)
+{
+    int dim = h->dim;
+    float t[3];
+    n[0] = n[1] = n[2] = 0.0f;
+    for (int f = 0; f < 2; f++) {
+        if (!sil_has_face(h, ind, f)) continue;
+        sil_face_normal(h, ind, f, 0, t);
+        for (int c = 0; c < dim; c++) n[c] = n[c] + t[c];
+    }
+    enormalize(n, dim);
+}
+static void sil_centroid(const oracle_bvh *h, const int32_t *ind, float *c)
+{
+    if (h->dim == 2) {
+        memcpy(c, spos(h, ind[1]), 2 * sizeof(float));
+        return;
+    }
+    const float *pa = spos(h, ind[1]), *pb = spos(h, ind[2]);
+    for (int i = 0; i < 3; i++) c[i] = (pa[i] + pb[i]) * 0.5f;
+}
+
+/* assignEdgeIndices (fcpw.inl:405-426): edges numbered by first appearance */
+typedef struct { int32_t a, b, pos; } edge_key;
+static int edge_key_cmp(const void *x, const void *y)
+{
+    const edge_key *p = (const edge_key *)x, *q = (const edge_key *)y;
+    if (p->a != q->a) return p->a < q->a ? -1 : 1;
+    if (p->b != q->b) return p->b < q->b ? -1 : 1;
+    return p->pos < q->pos ? -1 : (p->pos > q->pos);
+}
+static int int2_cmp(const void *x, const void *y)
+{
+    const int32_t *p = (const int32_t *)x, *q = (const int32_t *)y;
+    return p[0] < q[0] ? -1 : (p[0] > q[0]);
+}
+static int assign_edge_indices(const int32_t *idx, int n_tris, int32_t *eidx)
+{
+    int n = 3 * n_tris;
+    edge_key *k = (edge_key *)malloc((size_t)n * sizeof(edge_key));
+    for (int i = 0; i < n_tris; i++)
+        for (int j = 0; j < 3; j++) {
+            int32_t I = idx[3 * i + j], J = idx[3 * i + (j + 1) % 3];
+            if (I > J) { int32_t t = I; I = J; J = t; }
+            k[3 * i + j].a = I; k[3 * i + j].b = J; k[3 * i + j].pos = 3 * i + j;
+        }
+    qsort(k, n, sizeof(edge_key), edge_key_cmp);
+    /* (first position, group start) per distinct edge, then number groups by first position */
+    int32_t *grp = (int32_t *)malloc((size_t)n * 2 * sizeof(int32_t));
+    int ng = 0;
+    for (int i = 0; i < n; i++)
+        if (i == 0 || k[i].a != k[i - 1].a || k[i].b != k[i - 1].b) { grp[2 * ng] = k[i].pos; grp[2 * ng + 1] = i; ng++; }
+    qsort(grp, ng, 2 * sizeof(int32_t), int2_cmp);
+    for (int g = 0; g < ng; g++) {
+        int i = grp[2 * g + 1];
+        int32_t a = k[i].a, b = k[i].b;
+        for (; i < n && k[i].a == a && k[i].b == b; i++) eidx[k[i].pos] = g;
+    }
+    free(grp);
+    free(k);
+    return ng;
+}
+
+/* computeBoundingConesRecursive (bvh.inl:236-304) */
+static void cones_recursive(const oracle_bvh *h, struct sil_data *S, int start, int end)
+{
+    int dim = h->dim;
+    const node_t *node = &h->nodes[start];
+    float centroid[3], axis[3] = {0.0f, 0.0f, 0.0f}, radius = 0.0f;
+    for (int i = 0; i < dim; i++) centroid[i] = (node->box.pmin[i] + node->box.pmax[i]) * 0.5f;
+    int any = 0, two = 1;
+    for (int i = start; i < end; i++) {
+        for (int j = 0; j < S->cnt[i]; j++) {
+            int r = S->off[i] + j;
+            const int32_t *ind = S->ind[S->ref_obj[r]];
+            for (int c = 0; c < dim; c++) axis[c] = axis[c] + S->ref_n[(size_t)r * dim + c];
+            float sc[3], dv[3];
+            sil_centroid(h, ind, sc);
+            for (int c = 0; c < dim; c++) dv[c] = sc[c] - centroid[c];
+            radius = smax(radius, enorm(dv, dim));
+            two = two && sil_has_face(h, ind, 0) && sil_has_face(h, ind, 1);
+            any = 1;
+        }
+    }
+    float *cone = S->cone + (size_t)start * 5;
+    if (!any) {
+        cone[3] = -PI_F;
+    } else if (!two) {
+        cone[3] = PI_F;
+    } else {
+        float axis_norm = enorm(axis, dim);
+        if (axis_norm > EPSF) {
+            for (int c = 0; c < dim; c++) axis[c] = axis[c] / axis_norm;
+            float half = 0.0f;
+            for (int i = start; i < end; i++)
+                for (int j = 0; j < S->cnt[i]; j++) {
+                    int r = S->off[i] + j;
+                    for (int f = 0; f < 2; f++) {
+                        const float *n = S->ref_fn + ((size_t)r * 2 + f) * dim;
+                        float a = acosf(smax(-1.0f, smin(1.0f, edot(axis, n, dim))));
+                        half = smax(half, a);
+                    }
+                }
+            cone[0] = axis[0]; cone[1] = axis[1]; cone[2] = dim == 3 ? axis[2] : 0.0f;
+            cone[3] = half;
+            cone[4] = radius;
+        }
+    }
+    if (node->nrefs == 0) {
+        cones_recursive(h, S, start + 1, start + node->offset);
+        cones_recursive(h, S, start + node->offset, end);
+    }
+}
+
+/* GPU records (bvh_interop_structures.h:237-300) from the current positions */
+static void sil_make_records(const oracle_bvh *h, struct sil_data *S)
+{
+    int dim = h->dim;
+    for (int r = 0; r < S->n_refs; r++) {
+        int o = S->ref_obj[r];
+        const int32_t *ind = S->ind[o];
+        oracle_silhouette *g = &S->rec[r];
+        memset(g, 0, sizeof(*g));
+        int f0 = sil_has_face(h, ind, 0), f1 = sil_has_face(h, ind, 1);
+        memcpy(g->pa, spos(h, ind[1]), dim * sizeof(float));
+        if (dim == 3) memcpy(g->pb, spos(h, ind[2]), 3 * sizeof(float));
+        if (f0) sil_face_normal(h, ind, 0, 1, g->n0);
+        if (f1) sil_face_normal(h, ind, 1, 1, g->n1);
+        g->index = (uint32_t)S->pindex[o];
+        g->has_one_adjacent_face = (f0 && f1) ? 0u : 1u;
+    }
+}
+
+int oracle_build_silhouettes(oracle_bvh *h, const int32_t *user_idx, oracle_ignore_fn ignore, void *user)
+{
+    int dim = h->dim, np = h->n_prims;
+    sil_free(h->sil);
+    struct sil_data *S = (struct sil_data *)calloc(1, sizeof(struct sil_data));
+    h->sil = S;
+    if (dim == 2) {
+        /* Scene<2>::computeSilhouettes (fcpw.inl:436-473) on the user's vertex ids, then the
+           re-assignment of sortSoupPositions<2, SnchNode> (fcpw.inl:709-731) on sorted ids;
+           a field no sorted segment writes keeps its value from the first pass */
+        S->n_obj = h->n_vertices;
+        S->ind = (int32_t(*)[4])malloc((size_t)S->n_obj * sizeof(*S->ind));
+        S->pindex = (int32_t *)malloc((size_t)S->n_obj * sizeof(int32_t));
+        for (int v = 0; v < S->n_obj; v++) { S->ind[v][0] = S->ind[v][1] = S->ind[v][2] = S->ind[v][3] = -1; S->pindex[v] = -1; }
+        for (int i = 0; i < np; i++) {
+            int a = user_idx[2 * i], b = user_idx[2 * i + 1];
+            S->ind[a][1] = a; S->ind[a][2] = b; S->pindex[a] = a;
+            S->ind[b][0] = a; S->ind[b][1] = b; S->pindex[b] = b;
+        }
+        for (int s = 0; s < np; s++) {
+            int a = h->prim_vidx[2 * s], b = h->prim_vidx[2 * s + 1];
+            S->ind[a][1] = a; S->ind[a][2] = b; S->pindex[a] = a;
+            S->ind[b][0] = a; S->ind[b][1] = b; S->pindex[b] = b;
+        }
+    } else {
+        /* Scene<3>::computeSilhouettes (fcpw.inl:476-521) on the user's vertex ids, then the
+           remap of sortSoupPositions<3, SnchNode> (fcpw.inl:743-756) */
+        S->eidx = (int32_t *)malloc((size_t)np * 3 * sizeof(int32_t));
+        S->n_obj = assign_edge_indices(user_idx, np, S->eidx);
+        S->ind = (int32_t(*)[4])malloc((size_t)S->n_obj * sizeof(*S->ind));
+        S->pindex = (int32_t *)malloc((size_t)S->n_obj * sizeof(int32_t));
+        for (int e = 0; e < S->n_obj; e++) { S->ind[e][0] = S->ind[e][1] = S->ind[e][2] = S->ind[e][3] = -1; S->pindex[e] = -1; }
+        for (int i = 0; i < np; i++) {
+            const int32_t *t = user_idx + 3 * i;
+            for (int j = 0; j < 3; j++) {
+                int I = j == 0 ? 2 : j - 1, J = j, K = j == 2 ? 0 : j + 1;
+                int e = S->eidx[3 * i + j];
+                int orient = 1;
+                if (t[J] > t[K]) { int tmp = J; J = K; K = tmp; orient = -1; }
+                S->ind[e][orient == 1 ? 0 : 3] = t[I];
+                S->ind[e][1] = t[J];
+                S->ind[e][2] = t[K];
+                S->pindex[e] = e;
+            }
+        }
+        for (int e = 0; e < S->n_obj; e++)
+            for (int j = 0; j < 4; j++)
+                if (S->ind[e][j] != -1) S->ind[e][j] = h->index_map[S->ind[e][j]];
+    }
+
+    /* references per leaf, deduplicated within the leaf (bvh.inl:313-420) */
+    int cap = np * (dim == 3 ? 3 : 2);
+    S->ref_obj = (int32_t *)malloc((size_t)cap * sizeof(int32_t));
+    S->ref_n = (float *)malloc((size_t)cap * dim * sizeof(float));
+    S->ref_fn = (float *)malloc((size_t)cap * 2 * dim * sizeof(float));
+    S->off = (int32_t *)malloc((size_t)h->n_nodes * sizeof(int32_t));
+    S->cnt = (int32_t *)malloc((size_t)h->n_nodes * sizeof(int32_t));
+    int32_t seen[3 * 64];
+    for (int i = 0; i < h->n_nodes; i++) {
+        const node_t *node = &h->nodes[i];
+        int start = S->n_refs, n_seen = 0;
+        for (int j = 0; j < node->nrefs; j++) {
+            int slot = node->offset + j;
+            int prim = h->prim_orig[slot];
+            for (int k = 0; k < dim; k++) {
+                int o = dim == 2 ? h->prim_vidx[2 * slot + k] : S->eidx[3 * prim + k];
+                int dup = 0;
+                for (int q = 0; q < n_seen; q++) dup |= seen[q] == o;
+                if (dup) continue;
+                if (n_seen < 3 * 64) seen[n_seen++] = o;
+                const int32_t *ind = S->ind[o];
+                int f0 = sil_has_face(h, ind, 0), f1 = sil_has_face(h, ind, 1);
+                float n0[3] = {0.0f, 0.0f, 0.0f}, n1[3] = {0.0f, 0.0f, 0.0f}, n[3];
+                if (f0) sil_face_normal(h, ind, 0, 1, n0);
+                if (f1) sil_face_normal(h, ind, 1, 1, n1);
+                sil_normal(h, ind, n);
+                int ign = 0;
+                if (f0 && f1 && ignore) {
+                    float angle;
+                    if (dim == 2) {
+                        angle = n0[0] * n1[1] - n0[1] * n1[0];
+                    } else {
+                        const float *pa = spos(h, ind[1]), *pb = spos(h, ind[2]);
+                        float ed[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]}, cr[3];
+                        enormalize(ed, 3);
+                        ecross(n0, n1, cr);
+                        angle = atan2f(edot(ed, cr, 3), edot(n0, n1, 3));
+                    }
+                    ign = ignore(angle, prim, user) != 0;
+                }
+                if (ign) continue;
+                int r = S->n_refs++;
+                S->ref_obj[r] = o;
+                memcpy(S->ref_n + (size_t)r * dim, n, dim * sizeof(float));
+                memcpy(S->ref_fn + (size_t)r * 2 * dim, n0, dim * sizeof(float));
+                memcpy(S->ref_fn + (size_t)r * 2 * dim + dim, n1, dim * sizeof(float));
+            }
+        }
+        S->off[i] = start;
+        S->cnt[i] = S->n_refs - start;
+    }
+
+    /* cones: SnchNode default (axis 0, halfAngle pi, radius 0), then bvh.inl:236-304 */
+    S->cone = (float *)calloc((size_t)h->n_nodes * 5, sizeof(float));
+    for (int i = 0; i < h->n_nodes; i++) S->cone[(size_t)i * 5 + 3] = PI_F;
+    cones_recursive(h, S, 0, h->n_nodes);
+
+    S->rec = (oracle_silhouette *)calloc((size_t)(S->n_refs > 0 ? S->n_refs : 1), sizeof(oracle_silhouette));
+    sil_make_records(h, S);
+    return S->n_refs;
+}
+
+int oracle_num_silhouette_refs(const oracle_bvh *h) { return h->sil ? h->sil->n_refs : 0; }
+
+void oracle_export_silhouettes(const oracle_bvh *h, float *cones, int32_t *n_silhouettes, int32_t *silhouette_offset,
+                               oracle_silhouette *records)
+{
+    const struct sil_data *S = h->sil;
+    if (!S) return;
+    if (cones) memcpy(cones, S->cone, (size_t)h->n_nodes * 5 * sizeof(float));
+    if (n_silhouettes) memcpy(n_silhouettes, S->cnt, (size_t)h->n_nodes * sizeof(int32_t));
+    if (silhouette_offset) memcpy(silhouette_offset, S->off, (size_t)h->n_nodes * sizeof(int32_t));
+    if (records) memcpy(records, S->rec, (size_t)S->n_refs * sizeof(oracle_silhouette));
+}
+
+void oracle_update_silhouette_records(oracle_bvh *h)
+{
+    if (h->sil) sil_make_records(h, h->sil);
+}
+
+/* ---- GPU-side geometry, float3 arithmetic summed left to right ---- */
+static inline float g3dot(const float *a, const float *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+static inline float g3len(const float *a) { return sqrtf(g3dot(a, a)); }
+
+/* isSilhouetteVertex / isSilhouetteEdge (geometry.slang:490-522,599-633) */
+static int is_silhouette(const oracle_silhouette *s, int dim, const float *view, float d, int flip, float prec)
+{
+    float sign = flip ? 1.0f : -1.0f;
+    if (d <= prec) {
+        if (dim == 2) {
+            float det = s->n0[0] * s->n1[1] - s->n1[0] * s->n0[1];
+            return sign * det > prec;
+        }
+        float e[3] = {s->pb[0] - s->pa[0], s->pb[1] - s->pa[1], s->pb[2] - s->pa[2]}, cr[3];
+        float l = g3len(e);
+        for (int i = 0; i < 3; i++) e[i] = e[i] / l;
+        ecross(s->n0, s->n1, cr);
+        float angle = atan2f(g3dot(e, cr), g3dot(s->n0, s->n1));
+        return sign * angle > prec;
+    }
+    float u[3] = {view[0] / d, view[1] / d, view[2] / d};
+    float dot0 = g3dot(u, s->n0), dot1 = g3dot(u, s->n1);
+    if (fabsf(dot0) <= prec) return sign * dot1 > prec;
+    if (fabsf(dot1) <= prec) return sign * dot0 > prec;
+    return dot0 * dot1 < 0.0f;
+}
+
+/* Vertex/Edge.findClosestSilhouettePoint (geometry.slang:555-590,667-701) */
+static int silhouette_test(const oracle_silhouette *s, int dim, const float *x, float r2, int flip, float r2min,
+                           float prec, oracle_hit *c)
+{
+    if (r2min >= r2) return 0;
+    float p[3], t = 0.0f, d;
+    if (dim == 2) {
+        memcpy(p, s->pa, sizeof(p));
+    } else {
+        /* findClosestPointLineSegment (geometry.slang:79-103) on float3 */
+        float u[3], v[3];
+        for (int i = 0; i < 3; i++) { u[i] = s->pb[i] - s->pa[i]; v[i] = x[i] - s->pa[i]; }
+        float c1 = g3dot(u, v);
+        if (c1 <= 0.0f) {
+            memcpy(p, s->pa, sizeof(p));
+        } else {
+            float c2 = g3dot(u, u);
+            if (c2 <= c1) {
+                memcpy(p, s->pb, sizeof(p));
+                t = 1.0f;
+            } else {
+                t = c1 / c2;
+                for (int i = 0; i < 3; i++) p[i] = s->pa[i] + u[i] * t;
+            }
+        }
+    }
+    float view[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+    d = g3len(view);
+    if (d * d > r2) return 0;
+    int process = s->has_one_adjacent_face == 1;
+    if (!process) process = is_silhouette(s, dim, view, d, flip, prec);
+    if (process && d * d <= r2) {
+        hit_default(c);
+        memcpy(c->p, p, sizeof(p));
+        c->uv[0] = t;
+        c->d = d;
+        c->index = (int32_t)s->index;
+        return 1;
+    }
+    return 0;
+}
+
+/* computeOrthonormalBasis + projectToPlane (bounding-volumes.slang:122-142) */
+static float project_to_plane(const float *n, const float *e)
+{
+    float sign = n[2] >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sign + n[2]);
+    float b = n[0] * n[1] * a;
+    float b1[3] = {1.0f + sign * n[0] * n[0] * a, sign * b, -sign * n[0]};
+    float b2[3] = {b, sign + n[1] * n[1] * a, -n[1]};
+    float ab1[3] = {fabsf(b1[0]), fabsf(b1[1]), fabsf(b1[2])}, ab2[3] = {fabsf(b2[0]), fabsf(b2[1]), fabsf(b2[2])};
+    float r1 = g3dot(e, ab1), r2 = g3dot(e, ab2);
+    return sqrtf(r1 * r1 + r2 * r2);
+}
+static inline int in_range(float v, float lo, float hi) { return v >= lo && v <= hi; }
+
+/* BoundingCone.overlap (bounding-volumes.slang:170-236); box in 3 floats (z = 0 in 2D) */
+static int cone_overlap(const float *cone, const float *o, const float *lo, const float *hi, float dist_to_box)
+{
+    const float half_pi = PI_2_F;
+    float half = cone[3], radius = cone[4];
+    if (half >= half_pi || dist_to_box < EPSF) return 1;
+    float c[3], v[3];
+    for (int i = 0; i < 3; i++) { c[i] = 0.5f * (lo[i] + hi[i]); v[i] = c[i] - o[i]; }
+    float l = g3len(v);
+    for (int i = 0; i < 3; i++) v[i] = v[i] / l;
+    float d_axis = acosf(fmaxf(-1.0f, fminf(1.0f, g3dot(cone, v))));
+    if (in_range(half_pi, d_axis - half, d_axis + half)) return 1;
+    if (l > radius) {
+        float vh = asinf(radius / l);
+        float hs = half + vh;
+        return hs >= half_pi ? 1 : in_range(half_pi, d_axis - hs, d_axis + hs);
+    }
+    float e[3] = {hi[0] - c[0], hi[1] - c[1], hi[2] - c[2]}, av[3] = {fabsf(v[0]), fabsf(v[1]), fabsf(v[2])};
+    float d = g3dot(e, av);
+    float s = l - d;
+    if (s <= 0.0f) return 1;
+    d = project_to_plane(v, e);
+    float vh = atan2f(d, s);
+    float hs = half + vh;
+    return hs >= half_pi ? 1 : in_range(half_pi, d_axis - hs, d_axis + hs);
+}
+
+static float node_d2min(const oracle_bvh *h, int ni, const float *x)
+{
+    float d2min, d2max;
+    box_sqdist_gpu(&h->nodes[ni].box, x, h->dim, &d2min, &d2max);
+    return d2min;
+}
+
+typedef struct {
+    const oracle_bvh *h;
+    const float *spheres;
+    const uint32_t *flip;
+    float r2min, prec;
+    int brute;
+    oracle_hit *out;
+} csp_ctx;
+
+static void csp_range(void *vctx, int start, int end)
+{
+    const csp_ctx *C = (const csp_ctx *)vctx;
+    const oracle_bvh *h = C->h;
+    const struct sil_data *S = h->sil;
+    int dim = h->dim;
+    for (int qi = start; qi < end; qi++) {
+        const float *sp = C->spheres + (size_t)qi * 4;
+        float x[3] = {sp[0], sp[1], dim == 3 ? sp[2] : 0.0f};
+        float r2 = sp[3];
+        int flip = C->flip[qi] != 0;
+        oracle_hit best;
+        hit_default(&best);
+        if (!(C->r2min >= r2)) {
+            if (C->brute) {
+                /* every reference in order, the leaf loop of bvh.slang:540-566 */
+                for (int r = 0; r < S->n_refs; r++) {
+                    if ((int32_t)S->rec[r].index == best.index) continue;
+                    oracle_hit c;
+                    if (silhouette_test(&S->rec[r], dim, x, r2, flip, C->r2min, C->prec, &c)) {
+                        r2 = fminf(r2, c.d * c.d);
+                        best = c;
+                        if (C->r2min >= r2) break;
+                    }
+                }
+            } else if (node_d2min(h, 0, x) <= r2) {
+                int stack[BVH_MAX_DEPTH * 2 + 2];
+                int sp_ = 0;
+                stack[0] = 0;
+                while (sp_ >= 0) {
+                    int ni = stack[sp_--];
+                    const node_t *node = &h->nodes[ni];
+                    if (!(node_d2min(h, ni, x) <= r2)) continue;
+                    if (node->nrefs > 0) {
+                        for (int p = 0; p < S->cnt[ni]; p++) {
+                            const oracle_silhouette *s = &S->rec[S->off[ni] + p];
+                            if ((int32_t)s->index == best.index) continue;
+                            oracle_hit c;
+                            if (silhouette_test(s, dim, x, r2, flip, C->r2min, C->prec, &c)) {
+                                r2 = fminf(r2, c.d * c.d);
+                                best = c;
+                                if (C->r2min >= r2) break;
+                            }
+                        }
+                    } else {
+                        int kid[2] = {ni + 1, ni + node->offset};
+                        int ov[2];
+                        float dk[2] = {0.0f, 0.0f};
+                        for (int k = 0; k < 2; k++) {
+                            const float *cone = S->cone + (size_t)kid[k] * 5;
+                            ov[k] = cone[3] >= 0.0f;
+                            if (ov[k]) {
+                                dk[k] = node_d2min(h, kid[k], x);
+                                ov[k] = dk[k] <= r2;
+                                if (ov[k]) {
+                                    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+                                    memcpy(lo, h->nodes[kid[k]].box.pmin, dim * sizeof(float));
+                                    memcpy(hi, h->nodes[kid[k]].box.pmax, dim * sizeof(float));
+                                    ov[k] = cone_overlap(cone, x, lo, hi, dk[k]);
+                                }
+                            }
+                        }
+                        if (ov[0] && ov[1]) {
+                            int closer = kid[0], other = kid[1];
+                            if (dk[1] < dk[0]) { closer = kid[1]; other = kid[0]; }
+                            stack[++sp_] = other;
+                            stack[++sp_] = closer;
+                        } else if (ov[0]) {
+                            stack[++sp_] = kid[0];
+                        } else if (ov[1]) {
+                            stack[++sp_] = kid[1];
+                        }
+                    }
+                }
+            }
+        }
+        C->out[qi] = best;
+    }
+}
+
+void oracle_closest_silhouettes_gpuorder(const oracle_bvh *h, const float *spheres, const uint32_t *flip, int n,
+                                         float squared_min_radius, float precision, int brute, int nthreads,
+                                         oracle_hit *out)
+{
+    if (!h->sil) {
+        for (int i = 0; i < n; i++) hit_default(&out[i]);
+        return;
+    }
+    csp_ctx c = {h, spheres, flip, squared_min_radius, precision, brute, out};
+    parallel_for(n, nthreads, csp_range, &c);
+}
+
+/* ----------------------------------------------------------------------------
+ * FP64 closest silhouette point on raw geometry, classified by an independent
+ * orientation test: a vertex (2D) is on the silhouette seen from x when its two
+ * neighbours lie on the same side of the line through x and the vertex; an edge
+ * (3D) when the opposite vertices of its two triangles lie on the same side of
+ * the plane through x and the edge.  A vertex or edge with one neighbour/face is
+ * always on the silhouette.  Candidates whose normalised orientation is within
+ * tol of zero, or that lie within tol_d of x, are "ambiguous" (the FP32 path
+ * classifies those with its precision rules).  d_strict is the nearest certain
+ * silhouette point and d_loose the nearest certain-or-ambiguous one.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int dim, n_cand;
+    const float *pos;
+    int32_t *a, *b, *c, *d; /* per candidate; 2D: vertex a with neighbours c (prev), d (next) */
+    double tol, tol_d;
+    const float *q;
+    double *d_strict, *d_loose, *p_strict;
+} sf64_ctx;
+
+static void sf64_range(void *vctx, int start, int end)
+{
+    const sf64_ctx *C = (const sf64_ctx *)vctx;
+    int dim = C->dim;
+    for (int qi = start; qi < end; qi++) {
+        double x[3] = {C->q[(size_t)qi * dim], C->q[(size_t)qi * dim + 1], dim == 3 ? C->q[(size_t)qi * dim + 2] : 0.0};
+        double ds = INFINITY, dl = INFINITY, ps[3] = {0, 0, 0};
+        for (int k = 0; k < C->n_cand; k++) {
+            double A[3] = {0, 0, 0}, P[3], dist;
+            for (int i = 0; i < dim; i++) A[i] = C->pos[(size_t)C->a[k] * dim + i];
+            if (dim == 2) {
+                memcpy(P, A, sizeof(P));
+            } else {
+                double B[3], u[3], w[3];
+                for (int i = 0; i < 3; i++) { B[i] = C->pos[(size_t)C->b[k] * 3 + i]; u[i] = B[i] - A[i]; w[i] = x[i] - A[i]; }
+                double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                double t = uu > 0 ? (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / uu : 0.0;
+                t = t < 0 ? 0 : (t > 1 ? 1 : t);
+                for (int i = 0; i < 3; i++) P[i] = A[i] + t * u[i];
+            }
+            double r[3] = {x[0] - P[0], x[1] - P[1], x[2] - P[2]};
+            dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            int strict, ambiguous = 0;
+            if (C->c[k] < 0 || C->d[k] < 0) {
+                strict = 1;
+            } else if (dist <= C->tol_d) {
+                strict = 0;
+                ambiguous = 1;
+            } else {
+                double s[2];
+                int nb[2] = {C->c[k], C->d[k]};
+                for (int j = 0; j < 2; j++) {
+                    double N[3] = {0, 0, 0};
+                    for (int i = 0; i < dim; i++) N[i] = C->pos[(size_t)nb[j] * dim + i];
+                    if (dim == 2) {
+                        /* cross(A - x, N - A) / (|A - x| |N - A|) */
+                        double e0 = A[0] - x[0], e1 = A[1] - x[1], f0 = N[0] - A[0], f1 = N[1] - A[1];
+                        double den = sqrt(e0 * e0 + e1 * e1) * sqrt(f0 * f0 + f1 * f1);
+                        s[j] = den > 0 ? (e0 * f1 - e1 * f0) / den : 0.0;
+                    } else {
+                        /* det(x - A, B - A, N - A) / (|x - P| |(B - A) x (N - A)|) */
+                        double B[3], e[3], f[3], g[3], m[3];
+                        for (int i = 0; i < 3; i++) {
+                            B[i] = C->pos[(size_t)C->b[k] * 3 + i];
+                            e[i] = x[i] - A[i]; f[i] = B[i] - A[i]; g[i] = N[i] - A[i];
+                        }
+                        m[0] = f[1] * g[2] - f[2] * g[1];
+                        m[1] = f[2] * g[0] - f[0] * g[2];
+                        m[2] = f[0] * g[1] - f[1] * g[0];
+                        double den = dist * sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+                        s[j] = den > 0 ? (e[0] * m[0] + e[1] * m[1] + e[2] * m[2]) / den : 0.0;
+                    }
+                }
+                strict = s[0] * s[1] > 0;
+                ambiguous = fabs(s[0]) <= C->tol || fabs(s[1]) <= C->tol;
+                if (ambiguous) strict = 0;
+            }
+            if (strict && dist < ds) { ds = dist; memcpy(ps, P, sizeof(ps)); }
+            if ((strict || ambiguous) && dist < dl) dl = dist;
+        }
+        C->d_strict[qi] = ds;
+        C->d_loose[qi] = dl;
+        memcpy(C->p_strict + (size_t)qi * 3, ps, sizeof(ps));
+    }
+}
+
+int oracle_closest_silhouette_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,
+                                  int n_prims, const float *queries, int n, double tol, double tol_d, int nthreads,
+                                  double *d_strict, double *d_loose, double *p_strict)
+{
+    sf64_ctx C;
+    memset(&C, 0, sizeof(C));
+    C.dim = dim; C.pos = positions; C.tol = tol; C.tol_d = tol_d; C.q = queries;
+    C.d_strict = d_strict; C.d_loose = d_loose; C.p_strict = p_strict;
+    int nonmanifold = 0;
+    if (dim == 2) {
+        C.a = (int32_t *)malloc((size_t)n_vertices * sizeof(int32_t));
+        C.c = (int32_t *)malloc((size_t)n_vertices * sizeof(int32_t));
+        C.d = (int32_t *)malloc((size_t)n_vertices * sizeof(int32_t));
+        int32_t *used = (int32_t *)calloc((size_t)n_vertices, sizeof(int32_t));
+        for (int v = 0; v < n_vertices; v++) { C.a[v] = v; C.c[v] = -1; C.d[v] = -1; }
+        for (int i = 0; i < n_prims; i++) {
+            int a = indices[2 * i], b = indices[2 * i + 1];
+            if (C.d[a] >= 0 || C.c[b] >= 0) nonmanifold = 1;
+            C.d[a] = b; C.c[b] = a;
+            used[a] = used[b] = 1;
+        }
+        /* only vertices some segment uses are candidates */
+        int m = 0;
+        for (int v = 0; v < n_vertices; v++)
+            if (used[v]) { C.a[m] = C.a[v]; C.c[m] = C.c[v]; C.d[m] = C.d[v]; m++; }
+        C.n_cand = m;
+        free(used);
+    } else {
+        int32_t *eidx = (int32_t *)malloc((size_t)n_prims * 3 * sizeof(int32_t));
+        int E = assign_edge_indices(indices, n_prims, eidx);
+        C.n_cand = E;
+        C.a = (int32_t *)malloc((size_t)E * sizeof(int32_t));
+        C.b = (int32_t *)malloc((size_t)E * sizeof(int32_t));
+        C.c = (int32_t *)malloc((size_t)E * sizeof(int32_t));
+        C.d = (int32_t *)malloc((size_t)E * sizeof(int32_t));
+        for (int e = 0; e < E; e++) C.c[e] = C.d[e] = -1;
+        for (int i = 0; i < n_prims; i++)
+            for (int j = 0; j < 3; j++) {
+                int e = eidx[3 * i + j];
+                C.a[e] = indices[3 * i + j];
+                C.b[e] = indices[3 * i + (j + 1) % 3];
+                int opp = indices[3 * i + (j + 2) % 3];
+                if (C.c[e] < 0) C.c[e] = opp;
+                else if (C.d[e] < 0) C.d[e] = opp;
+                else nonmanifold = 1;
+            }
+        free(eidx);
+    }
+    if (!nonmanifold) parallel_for(n, nthreads, sf64_range, &C);
+    free(C.a); free(C.b); free(C.c); free(C.d);
+    return nonmanifold ? -1 : 0;
+}
+
+/* ----------------------------------------------------------------------------
+ * GPU refit of the cones (Bvh.refit, bvh.slang:27-141), after oracle_refit has
+ * recomputed the boxes and oracle_update_silhouette_records the records.
+ * Leaves rebuild their cone from their records (refitLeafNode); internal nodes
+ * merge their children's (refitInternalNode, mergeBoundingCones and rotate,
+ * bounding-volumes.slang:238-318).  rotate's w is length(cross(u, v)) in every
+ * component, as the reference declares it.
+ * ------------------------------------------------------------------------- */
+static void g3rotate(const float *u, const float *v, float theta, float *out)
+{
+    float c = cosf(theta), s = sinf(theta), cr[3];
+    ecross(u, v, cr);
+    float L = g3len(cr);
+    float w[3] = {L, L, L}, o[3];
+    for (int i = 0; i < 3; i++) o[i] = (1.0f - c) * w[i];
+    float R[3][3] = {{c + o[0] * w[0], o[1] * w[0] - s * w[2], o[2] * w[0] + s * w[1]},
+                     {o[0] * w[1] + s * w[2], c + o[1] * w[1], o[2] * w[1] - s * w[0]},
+                     {o[0] * w[2] - s * w[1], o[1] * w[2] + s * w[0], c + o[2] * w[2]}};
+    for (int i = 0; i < 3; i++) out[i] = (R[i][0] * u[0] + R[i][1] * u[1]) + R[i][2] * u[2];
+}
+
+static void g3merge_cones(const float *A, const float *B, const float *oA, const float *oB, const float *oN,
+                          float *C)
+{
+    C[0] = C[1] = C[2] = 0.0f;
+    C[3] = PI_F;
+    C[4] = 0.0f;
+    if (A[3] >= 0.0f && B[3] >= 0.0f) {
+        float axA[3] = {A[0], A[1], A[2]}, axB[3] = {B[0], B[1], B[2]};
+        float hA = A[3], hB = B[3];
+        float dA[3], dB[3];
+        for (int i = 0; i < 3; i++) { dA[i] = oN[i] - oA[i]; dB[i] = oN[i] - oB[i]; }
+        C[4] = sqrtf(fmaxf(A[4] * A[4] + g3dot(dA, dA), B[4] * B[4] + g3dot(dB, dB)));
+        if (hB > hA) {
+            for (int i = 0; i < 3; i++) { float t = axA[i]; axA[i] = axB[i]; axB[i] = t; }
+            float t = hA; hA = hB; hB = t;
+        }
+        float theta = acosf(fmaxf(-1.0f, fminf(1.0f, g3dot(axA, axB))));
+        if (fminf(theta + hB, PI_F) <= hA) {
+            memcpy(C, axA, sizeof(axA));
+            C[3] = hA;
+            return;
+        }
+        float o_theta = (hA + theta + hB) / 2.0f;
+        if (o_theta >= PI_F) {
+            memcpy(C, axA, sizeof(axA));
+            return;
+        }
+        g3rotate(axA, axB, o_theta - hA, C);
+        C[3] = o_theta;
+    } else if (A[3] >= 0.0f) {
+        memcpy(C, A, 5 * sizeof(float));
+    } else if (B[3] >= 0.0f) {
+        memcpy(C, B, 5 * sizeof(float));
+    } else {
+        C[3] = -PI_F;
+    }
+}
+
+static void node_centroid3(const oracle_bvh *h, int ni, float *c)
+{
+    c[2] = 0.0f;
+    for (int i = 0; i < h->dim; i++) c[i] = 0.5f * (h->nodes[ni].box.pmin[i] + h->nodes[ni].box.pmax[i]);
+}
+
+static void refit_cones_recursive(oracle_bvh *h, int ni)
+{
+    struct sil_data *S = h->sil;
+    const node_t *node = &h->nodes[ni];
+    float *C = S->cone + (size_t)ni * 5;
+    if (node->nrefs == 0) {
+        int l = ni + 1, r = ni + node->offset;
+        refit_cones_recursive(h, l);
+        refit_cones_recursive(h, r);
+        float cl[3], cr[3], cn[3];
+        node_centroid3(h, l, cl);
+        node_centroid3(h, r, cr);
+        node_centroid3(h, ni, cn);
+        g3merge_cones(S->cone + (size_t)l * 5, S->cone + (size_t)r * 5, cl, cr, cn, C);
+        return;
+    }
+    float axis[3] = {0.0f, 0.0f, 0.0f}, centroid[3], half = 0.0f, radius = 0.0f;
+    node_centroid3(h, ni, centroid);
+    int any = 0, two = 1;
+    for (int p = 0; p < S->cnt[ni]; p++) {
+        const oracle_silhouette *s = &S->rec[S->off[ni] + p];
+        for (int i = 0; i < 3; i++) axis[i] = axis[i] + s->n0[i];
+        for (int i = 0; i < 3; i++) axis[i] = axis[i] + s->n1[i];
+        float sc[3], dv[3];
+        for (int i = 0; i < 3; i++) sc[i] = h->dim == 2 ? s->pa[i] : 0.5f * (s->pa[i] + s->pb[i]);
+        for (int i = 0; i < 3; i++) dv[i] = sc[i] - centroid[i];
+        radius = fmaxf(radius, g3len(dv));
+        two = two && s->has_one_adjacent_face == 0;
+        any = 1;
+    }
+    if (!any) {
+        half = -PI_F;
+    } else if (!two) {
+        half = PI_F;
+    } else {
+        float an = g3len(axis);
+        if (an > EPSF) {
+            for (int i = 0; i < 3; i++) axis[i] = axis[i] / an;
+            for (int p = 0; p < S->cnt[ni]; p++) {
+                const oracle_silhouette *s = &S->rec[S->off[ni] + p];
+                half = fmaxf(half, acosf(fmaxf(-1.0f, fminf(1.0f, g3dot(axis, s->n0)))));
+                half = fmaxf(half, acosf(fmaxf(-1.0f, fminf(1.0f, g3dot(axis, s->n1)))));
+            }
+        }
+    }
+    C[0] = axis[0]; C[1] = axis[1]; C[2] = axis[2]; C[3] = half; C[4] = radius;
+}
+
+void oracle_refit_silhouette_cones_gpu(oracle_bvh *h)
+{
+    if (h->sil) refit_cones_recursive(h, 0);
 }

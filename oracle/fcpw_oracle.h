@@ -14,6 +14,11 @@
  *   - brute-force aggregate         include/fcpw/aggregates/baseline.inl:342-384
  *   - GPU traversal order           include/fcpw/gpu/bvh.slang:144-254,395-502
  *   - GPU sphere intersection       include/fcpw/gpu/bvh.slang:257-392
+ *   - silhouettes and SNCH cones    include/fcpw/fcpw.inl:405-521,709-756,
+ *                                   include/fcpw/aggregates/bvh.inl:236-420
+ *   - GPU closest silhouette point  include/fcpw/gpu/bvh.slang:503-646,
+ *                                   include/fcpw/gpu/geometry.slang:490-708,
+ *                                   include/fcpw/gpu/bounding-volumes.slang:122-236
  *   - primitive routines            include/fcpw/geometry/triangles.inl:12-32,222-260,374-487,
  *                                   include/fcpw/geometry/line_segments.inl:11-28,140-175,200-255,
  *                                   include/fcpw/gpu/geometry.slang:37-103,202-332
@@ -89,6 +94,42 @@ void oracle_intersect_spheres(const oracle_bvh *h, const float *spheres, const f
                               int nthreads, oracle_hit *out);
 void oracle_brute_closest_f32(const oracle_bvh *h, const float *queries, const float *r2, int n,
                               int nthreads, oracle_hit *out);
+
+/* ---- silhouettes (SNCH) ----
+   GPU silhouette record in the reference GPUEdge layout (bvh_interop_structures.h:141-162);
+   in 2D a GPUVertex (:121-139) is stored with p in pa, pb = 0, and z = 0 throughout. */
+typedef struct {
+    float pa[3], pb[3], n0[3], n1[3];
+    uint32_t index, has_one_adjacent_face;
+} oracle_silhouette;
+/* Scene::computeSilhouettes' ignoreSilhouette(angle, primitive index) callback */
+typedef int (*oracle_ignore_fn)(float angle, int32_t index, void *user);
+/* Scene::computeSilhouettes + the SNCH data of Bvh::assignGeometricDataToNodes
+   (fcpw.inl:405-521,709-756; bvh.inl:236-420).  user_idx: the primitives'
+   vertex ids as passed to oracle_build.  Returns the number of references. */
+int oracle_build_silhouettes(oracle_bvh *h, const int32_t *user_idx, oracle_ignore_fn ignore, void *user);
+int oracle_num_silhouette_refs(const oracle_bvh *h);
+/* per node: cones (axis[3], halfAngle, radius), nSilhouettes, silhouetteOffset;
+   per reference: the GPU record (any pointer may be NULL) */
+void oracle_export_silhouettes(const oracle_bvh *h, float *cones, int32_t *n_silhouettes, int32_t *silhouette_offset,
+                               oracle_silhouette *records);
+/* recompute the GPU records from the current positions (refit with updateGeometry) */
+void oracle_update_silhouette_records(oracle_bvh *h);
+/* GPU refit of the cones (bvh.slang:27-141) from the current boxes and records */
+void oracle_refit_silhouette_cones_gpu(oracle_bvh *h);
+/* GPU closest silhouette point (bvh.slang:503-646): spheres n x 4, flip n;
+   brute != 0 tests every reference in order instead of traversing */
+void oracle_closest_silhouettes_gpuorder(const oracle_bvh *h, const float *spheres, const uint32_t *flip, int n,
+                                         float squared_min_radius, float precision, int brute, int nthreads,
+                                         oracle_hit *out);
+
+/* FP64 closest silhouette point on raw geometry (user order) with an independent
+   orientation classification; returns -1 (and computes nothing) for
+   non-manifold input.  d_strict / d_loose bracket the exact answer (see the
+   definition beside the implementation); p_strict is n x 3. */
+int oracle_closest_silhouette_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,
+                                  int n_prims, const float *queries, int n, double tol, double tol_d, int nthreads,
+                                  double *d_strict, double *d_loose, double *p_strict);
 
 /* FP64 brute force on raw geometry (positions in user order) */
 void oracle_brute_closest_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,

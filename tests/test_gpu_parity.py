@@ -326,7 +326,8 @@ def test_errors_are_reported(gpu):
     scene, g = make_scene(V, F)
     with pytest.raises(ValueError):
         g.set_option("block", 100)
-    with pytest.raises(RuntimeError):
-        g.find_closest_silhouette_points(None)
+    with pytest.raises(RuntimeError, match="without silhouettes"):  # compute_silhouettes was not called
+        g.find_closest_silhouette_points(np.zeros((1, 3), np.float32), np.ones(1, np.float32), np.zeros(1, np.int32),
+                                         fcpw.GPUInteractionList())
     with pytest.raises(ValueError):
         g.find_closest_points(np.zeros((4, 2), np.float32), np.zeros(4, np.float32), fcpw.GPUInteractionList())

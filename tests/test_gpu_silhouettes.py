@@ -1,0 +1,169 @@
+"""Closest silhouette points on the GPU (GPUScene::findClosestSilhouettePoints,
+bvh.slang:503-646): against the oracle's restatement of the reference GPU
+traversal on the same silhouette data, through the array, list and device
+forms; after a GPU refit (cones recomputed on the device, bvh.slang:27-141) and
+after a host refit followed by a new transfer; and against the FP64
+orientation-test brute force.
+
+The cone test and the cone refit call acos/asin/atan2/cos/sin, which the device
+library and the host C library may round differently in the last place.  Such a
+difference can only move a pruning decision that sits exactly on its boundary,
+so results are compared bit for bit and at most MAX_ULP_FLIPS queries per case
+may differ; every query that differs must agree with the oracle's
+every-reference brute force in distance (an extra or a missing visit of a node
+whose silhouettes cannot win changes nothing else)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import meshes
+from tests.test_oracle_silhouettes import CASES, MAXF, product_scene, queries
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+fcpw = pytest.importorskip("fcpw_amd")
+
+MAX_ULP_FLIPS = 2
+
+
+def gpu_scene(s, dim):
+    g = fcpw.GPUScene3D("") if dim == 3 else fcpw.GPUScene2D("")
+    g.transfer_to_gpu(s)
+    return g
+
+
+def records(hits):
+    return np.frombuffer(hits._as_bytes(), dtype=fcpw.INTERACTION_DTYPE)
+
+
+def compare(rec, ref, brute, what):
+    same = (rec["index"].view(np.int32) == ref["index"]) & (rec["d"] == ref["d"])
+    for f in ("p", "uv", "n"):
+        same &= (rec[f] == ref[f]).all(axis=1)
+    bad = np.nonzero(~same)[0]
+    assert len(bad) <= MAX_ULP_FLIPS, (what, len(bad), bad[:8])
+    np.testing.assert_array_equal(rec["d"][bad], brute["d"][bad], err_msg=what)
+    assert (rec["n"] == 0).all()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_closest_silhouettes_match_the_reference_order(gpu, name):
+    V, F = CASES[name]
+    dim = V.shape[1]
+    s = product_scene(V, F)
+    g = gpu_scene(s, dim)
+    b = oracle.Bvh(V, F)
+    b.build_silhouettes()
+    q, r2, flip = queries(V, 20000, seed=11)
+    hits = fcpw.GPUInteractionList()
+    g.find_closest_silhouette_points(q, r2, flip.astype(np.int32), hits)
+    rec = records(hits).copy()
+    ref = b.closest_silhouettes(q, r2, flip)
+    brute = b.closest_silhouettes(q, r2, flip, brute=True)
+    compare(rec, ref, brute, f"{name} arrays")
+    found = ref["index"] >= 0
+    assert found.mean() > 0.3
+    # list form on a subset, and the device form on everything, with a minimum radius
+    spheres = fcpw.GPUBoundingSphereList()
+    for k in range(0, len(q), 50):
+        c = fcpw.GPUFloat3(float(q[k, 0]), float(q[k, 1]), float(q[k, 2]) if dim == 3 else 0.0)
+        spheres.append(fcpw.GPUBoundingSphere(c, float(r2[k])))
+    hits2 = fcpw.GPUInteractionList()
+    g.find_closest_silhouette_points(spheres, [int(v) for v in flip[::50]], hits2)
+    np.testing.assert_array_equal(records(hits2).view(np.uint32), rec[::50].copy().view(np.uint32))
+    r2min = np.float32((0.3 * float(np.abs(V).max())) ** 2)
+    sph = torch.zeros((len(q), 4), dtype=torch.float32, device="cuda")
+    sph[:, :dim] = torch.from_numpy(q).cuda()
+    sph[:, 3] = torch.from_numpy(r2).cuda()
+    fl = torch.from_numpy(flip.astype(np.int32)).cuda()
+    out = torch.empty((len(q), 10), dtype=torch.float32, device="cuda")
+    g.find_closest_silhouette_points_device(sph, fl, out, squared_min_radius=float(r2min), precision=2e-3)
+    torch.cuda.synchronize()
+    dev = out.cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel()
+    ref_m = b.closest_silhouettes(q, r2, flip, squared_min_radius=r2min, precision=2e-3)
+    brute_m = b.closest_silhouettes(q, r2, flip, squared_min_radius=r2min, precision=2e-3, brute=True)
+    same = (dev["index"].view(np.int32) == ref_m["index"]) & (dev["d"] == ref_m["d"])
+    assert (~same).sum() <= MAX_ULP_FLIPS, name
+    # with a minimum radius only the stop rule is common to traversal and brute force
+    stop = ~same
+    assert (dev["d"][stop].astype(np.float64) ** 2 <= r2min * (1 + 1e-6)).all() or \
+        np.array_equal(dev["d"][stop], brute_m["d"][stop])
+
+
+@pytest.mark.parametrize("name", ["icosphere", "torus", "star2d", "walker2d", "open2d"])
+def test_after_gpu_refit(gpu, name):
+    V, F = CASES[name]
+    dim = V.shape[1]
+    s = product_scene(V, F)
+    g = gpu_scene(s, dim)
+    b = oracle.Bvh(V, F)
+    b.build_silhouettes()
+    rng = np.random.default_rng(12)
+    Vk = V
+    q, r2, flip = queries(V, 12000, seed=13)
+    for step in range(3):
+        Vk = (Vk + rng.normal(0, 0.004 * float(np.abs(V).max()), V.shape)).astype(np.float32)
+        s.update_object_vertices(Vk, 0)
+        g.refit(s)  # device boxes and cones from re-extracted silhouette records
+        b.update_vertices(Vk)
+        b.refit()
+        b.update_silhouette_records()
+        b.refit_silhouette_cones_gpu()
+        hits = fcpw.GPUInteractionList()
+        g.find_closest_silhouette_points(q, r2, flip.astype(np.int32), hits)
+        rec = records(hits)
+        compare(rec, b.closest_silhouettes(q, r2, flip), b.closest_silhouettes(q, r2, flip, brute=True),
+                f"{name} refit step {step}")
+
+
+@pytest.mark.parametrize("name", ["torus", "walker2d"])
+def test_after_host_refit_and_transfer(gpu, name):
+    """Scene.refit recomputes the cones with the host formulas (bvh.inl:468-581);
+    a new transfer uploads them.  Any conservative cone gives the distances of
+    the every-reference brute force."""
+    V, F = CASES[name]
+    dim = V.shape[1]
+    s = product_scene(V, F)
+    rng = np.random.default_rng(14)
+    Vk = (V + rng.normal(0, 0.01 * float(np.abs(V).max()), V.shape)).astype(np.float32)
+    s.update_object_vertices(Vk, 0)
+    s.refit()
+    g = gpu_scene(s, dim)
+    b = oracle.Bvh(V, F)
+    b.build_silhouettes()
+    b.update_vertices(Vk)
+    b.refit()
+    b.update_silhouette_records()
+    q, r2, flip = queries(Vk, 12000, seed=15)
+    hits = fcpw.GPUInteractionList()
+    g.find_closest_silhouette_points(q, r2, flip.astype(np.int32), hits)
+    np.testing.assert_array_equal(records(hits)["d"], b.closest_silhouettes(q, r2, flip, brute=True)["d"])
+
+
+@pytest.mark.parametrize("name", ["icosphere", "bunny", "open3d", "star2d", "walker2d"])
+def test_fp64_orientation_bracket(gpu, name):
+    V, F = CASES[name]
+    dim = V.shape[1]
+    g = gpu_scene(product_scene(V, F), dim)
+    q = meshes.queries_around(V, 4000, seed=16)
+    hits = fcpw.GPUInteractionList()
+    g.find_closest_silhouette_points(q, np.full(len(q), MAXF, np.float32), np.zeros(len(q), np.int32), hits)
+    rec = records(hits)
+    ds, dl, _ = oracle.closest_silhouette_f64(V, F, q)
+    d32 = np.where(rec["index"] != fcpw.UINT_MAX, rec["d"].astype(np.float64), np.inf)
+    floor = 4 * np.finfo(np.float32).eps * float(np.abs(V).max())
+    lo = np.where(np.isinf(dl), np.inf, dl * (1 - 1e-5) - floor)
+    assert ((d32 >= lo) & (d32 <= ds * (1 + 1e-5) + floor)).all()
+
+
+def test_scene_without_silhouettes_is_rejected(gpu):
+    V, F = meshes.small_torus()
+    s = fcpw.Scene3D()
+    s.set_object_count(1)
+    s.set_object_vertices(V, 0)
+    s.set_object_triangles(F, 0)
+    s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    g = gpu_scene(s, 3)
+    with pytest.raises(RuntimeError, match="without silhouettes"):
+        g.find_closest_silhouette_points(V[:4], np.full(4, MAXF, np.float32), np.zeros(4, np.int32),
+                                         fcpw.GPUInteractionList())
