@@ -367,6 +367,34 @@ def run_torus(args, torch):
         log(f"[spheres] {t_s * 1e3:.3f} ms/step, {nq / t_s:.4e} queries/s, found {found:.3f}")
         del balls, rnd, ray_out
 
+        # ---- closest silhouette points on the same torus built with silhouettes,
+        # every fourth closest-point query (uniform and near-surface alike)
+        t0 = time.time()
+        ss = fcpw.Scene3D()
+        ss.set_object_count(1)
+        ss.set_object_vertices(V, 0)
+        ss.set_object_triangles(F, 0)
+        ss.compute_silhouettes()
+        ss.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+        gs = fcpw.GPUScene3D(fcpw.slang_directory_path)
+        gs.transfer_to_gpu(ss)
+        log(f"[silhouettes] silhouettes + cones built and transferred in {time.time() - t0:.1f}s")
+        qs = q[::4]
+        ns = len(qs)
+        balls = _spheres(torch, qs, dev)
+        flip = torch.zeros(ns, dtype=torch.int32, device=dev)
+        sil_out = torch.empty((ns, 10), dtype=torch.float32, device=dev)
+
+        def sil_step():
+            gs.find_closest_silhouette_points_device(balls, flip, sil_out, stream=stream)
+        t_sil = timed(torch, stream, sil_step, args.steps, args.warmup)
+        found = float((sil_out[:, 9].view(torch.int32) != -1).float().mean())
+        res["closest_silhouette"] = {"value": ns / t_sil, "unit": "queries/s", "ms_per_step": t_sil * 1e3,
+                                     "workload": f"{ns} queries (every 4th closest-point query), r2=inf, "
+                                                 f"flip=0; {found:.3f} find a silhouette"}
+        log(f"[silhouettes] {t_sil * 1e3:.3f} ms/step, {ns / t_sil:.4e} queries/s, found {found:.3f}")
+        del balls, flip, sil_out, gs, ss
+
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)

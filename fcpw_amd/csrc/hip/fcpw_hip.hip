@@ -670,15 +670,18 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
 {
     DevScene S = dev_scene(c);
     SilScene T = {c->cones, c->sil_leaf, c->sils};
+    const uint32_t *order = nullptr;
+    int st = morton_order(c, (const float *)spheres, 4, n, s, &order);
+    if (st) return st;
     const unsigned blocks = (unsigned)((n + 127) / 128);
-    int st = timing_begin(c, s);
+    st = timing_begin(c, s);
     if (st) return st;
     if (c->dim == 3)
         hipLaunchKernelGGL((k_closest_silhouettes<3, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
-                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out);
+                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, c->xcds);
     else
         hipLaunchKernelGGL((k_closest_silhouettes<2, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
-                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out);
+                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, c->xcds);
     HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
 }

@@ -164,12 +164,14 @@ __device__ __forceinline__ bool silhouette_test(const SilRecord &s, v3 x, float 
 template <int DIM, int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void k_closest_silhouettes(DevScene S, SilScene T, const float4 *spheres,
                                                                const uint32_t *flip, int64_t n, float r2min,
-                                                               float prec, float2 *out)
+                                                               float prec, float2 *out, const uint32_t *order,
+                                                               int xcds)
 {
     using G = Geo<DIM>;
     using vec = typename G::vec;
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = order ? (int64_t)order[k] : k;
     const float4 s4 = spheres[i];
     const v3 x = {s4.x, s4.y, DIM == 3 ? s4.z : 0.0f};
     const float q[3] = {s4.x, s4.y, s4.z};
@@ -192,8 +194,8 @@ __global__ __launch_bounds__(BLOCK) void k_closest_silhouettes(DevScene S, SilSc
             if (cur_d2 <= r2) {
                 if (cur & LEAF_BIT) {
                     const uint2 rng = T.sil_leaf[cur & OFF_MASK];
-                    for (uint32_t k = 0; k < rng.y; k++) {
-                        const SilRecord s = load_silhouette<DIM>(T.sils, rng.x + k);
+                    for (uint32_t j = 0; j < rng.y; j++) {
+                        const SilRecord s = load_silhouette<DIM>(T.sils, rng.x + j);
                         if (s.index == best) continue;
                         v3 p;
                         float t, d;

@@ -70,6 +70,9 @@ static_assert(sizeof(GPURay) == sizeof(fcpw_hip_ray), "GPURay layout");
 static_assert(sizeof(GPUBoundingSphere) == sizeof(fcpw_hip_bounding_sphere), "GPUBoundingSphere layout");
 static_assert(sizeof(GPUInteraction) == sizeof(fcpw_hip_interaction) && sizeof(GPUInteraction) == 40,
               "GPUInteraction layout");
+// GPUBoundingCone / GPUVertex / GPUEdge (bvh_interop_structures.h:41-52,121-162)
+static_assert(sizeof(fcpw_hip_bounding_cone) == 20 && sizeof(fcpw_hip_vertex) == 44 && sizeof(fcpw_hip_edge) == 56,
+              "silhouette record layouts");
 
 inline CostHeuristic heuristic_of(AggregateType t)
 {
