@@ -5,6 +5,235 @@
 #include "device.hpp"
 #include "silhouette.hpp"
 #include "context.hpp"
+#include "gpu_build.hpp"
+
+// in-place exclusive scan of n uint32 values (segments of SCAN_SEG, at most 16384 of them)
+static int exclusive_scan(fcpw_hip_ctx *c, uint32_t *v, int64_t n, uint32_t *sums, hipStream_t s)
+{
+    const int64_t segs = (n + SCAN_SEG - 1) / SCAN_SEG;
+    if (segs > 1024 * 16) FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "scan of %lld values exceeds the scan capacity", (long long)n);
+    hipLaunchKernelGGL(k_scan_segment_sums, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, v, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)segs);
+    hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, v, n, sums);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+// device allocations of one build, released together
+struct BuildScratch {
+    std::vector<void *> bufs;
+    ~BuildScratch()
+    {
+        for (void *b : bufs) (void)hipFree(b);
+    }
+    template <typename T>
+    hipError_t alloc(T **p, size_t count)
+    {
+        *p = nullptr;
+        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) bufs.push_back(*p);
+        return e;
+    }
+};
+
+static unsigned blocks_for(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+template <int DIM>
+static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, const int32_t *indices, int n_prims,
+                     int heuristic, float *out_pmin, float *out_pmax, int32_t *out_nrefs, int32_t *out_offset,
+                     int32_t *out_prim_order, int *out_n_nodes)
+{
+    using namespace gbuild;
+    const int n = n_prims;
+    const int cap_nodes = 2 * n - 1;
+    hipStream_t st = 0;
+    BuildScratch T;
+    float *V = nullptr;
+    int32_t *idx = nullptr;
+    HIPCHECK(c, hipMalloc((void **)&V, (size_t)n_vertices * DIM * sizeof(float)));
+    HIPCHECK(c, T.alloc(&idx, (size_t)n * DIM));
+    HIPCHECK(c, hipMemcpy(V, positions, (size_t)n_vertices * DIM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(idx, indices, (size_t)n * DIM * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->V = V; // owned by the context from here on
+
+    Build B = {};
+    B.n = n;
+    B.heuristic = heuristic;
+    float4 *plo, *phi, *pcen;
+    HIPCHECK(c, T.alloc(&plo, n));
+    HIPCHECK(c, T.alloc(&phi, n));
+    HIPCHECK(c, T.alloc(&pcen, n));
+    B.pbox_lo = plo; B.pbox_hi = phi; B.pcen = pcen;
+    HIPCHECK(c, T.alloc(&B.ref, n));
+    HIPCHECK(c, T.alloc(&B.ref2, n));
+    HIPCHECK(c, T.alloc(&B.elem_node, n));
+    HIPCHECK(c, T.alloc(&B.scan_r, (size_t)n + 1));
+    HIPCHECK(c, T.alloc(&B.scan_k, (size_t)n + 1));
+    for (int k = 0; k < 2; k++) {
+        HIPCHECK(c, T.alloc(&B.st_val[k], n));
+        HIPCHECK(c, T.alloc(&B.st_nxt[k], n));
+    }
+    int32_t **nd_arrays[] = {&B.nd_start, &B.nd_n, &B.nd_depth, &B.nd_left, &B.nd_dim, &B.nd_mid, &B.nd_rot,
+                             &B.nd_right, &B.nd_flags, &B.nd_size, &B.nd_leaves, &B.nd_idx, &B.nd_lb};
+    for (int32_t **a : nd_arrays) HIPCHECK(c, T.alloc(a, cap_nodes));
+    HIPCHECK(c, T.alloc(&B.nd_box, (size_t)cap_nodes * 6));
+    HIPCHECK(c, T.alloc(&B.nd_cbox, (size_t)cap_nodes * 6));
+    HIPCHECK(c, T.alloc(&B.nd_coord, cap_nodes));
+    HIPCHECK(c, T.alloc(&B.level_info, 4));
+    uint32_t *sums = nullptr;
+    HIPCHECK(c, T.alloc(&sums, 16384));
+    int bins_cap = 0;
+    size_t iscan_cap = 0;
+
+    const int t = 256;
+    hipLaunchKernelGGL(k_prim_data<DIM>, dim3(blocks_for(n, t)), dim3(t), 0, st, V, idx, n, plo, phi, pcen);
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, t)), dim3(t), 0, st, B.ref, n);
+    const int32_t root_init[3] = {0, n, 0};
+    HIPCHECK(c, hipMemcpy(B.nd_start, &root_init[0], sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(B.nd_n, &root_init[1], sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(B.nd_depth, &root_init[2], sizeof(int32_t), hipMemcpyHostToDevice));
+
+    std::vector<std::pair<int, int>> levels; // (first node, count)
+    int g0 = 0, m = 1, total = 1;
+    while (m > 0) {
+        if (m > bins_cap) {
+            int *nb = nullptr;
+            HIPCHECK(c, T.alloc(&nb, (size_t)m * BIN_NODE));
+            B.bins = nb;
+            bins_cap = m;
+        }
+        if ((size_t)m + 1 > iscan_cap) {
+            uint32_t *is = nullptr;
+            HIPCHECK(c, T.alloc(&is, (size_t)m + 1));
+            B.internal_scan = is;
+            iscan_cap = (size_t)m + 1;
+        }
+        hipLaunchKernelGGL(k_level_init, dim3(blocks_for(m, t)), dim3(t), 0, st, B, g0, m);
+        hipLaunchKernelGGL(k_elem_node, dim3(blocks_for(n, t)), dim3(t), 0, st, B, g0, m);
+        hipLaunchKernelGGL(k_bounds<DIM>, dim3(blocks_for(n, ACC_THREADS)), dim3(ACC_THREADS), 0, st, B);
+        hipLaunchKernelGGL(k_bins<DIM>, dim3(blocks_for(n, ACC_THREADS)), dim3(ACC_THREADS), 0, st, B, g0);
+        hipLaunchKernelGGL(k_split<DIM>, dim3(blocks_for(m, 64)), dim3(64), 0, st, B, g0, m);
+        hipLaunchKernelGGL(k_flag_right, dim3(blocks_for((int64_t)n + 1, t)), dim3(t), 0, st, B);
+        int e = exclusive_scan(c, B.scan_r, (int64_t)n + 1, sums, st);
+        if (e) return e;
+        hipLaunchKernelGGL(k_flag_rotate, dim3(blocks_for((int64_t)n + 1, t)), dim3(t), 0, st, B);
+        e = exclusive_scan(c, B.scan_k, (int64_t)n + 1, sums, st);
+        if (e) return e;
+        HIPCHECK(c, hipMemsetAsync(B.level_info, 0, 4 * sizeof(int), st));
+        hipLaunchKernelGGL(k_node_part, dim3(blocks_for((int64_t)m + 1, t)), dim3(t), 0, st, B, g0, m);
+        e = exclusive_scan(c, B.internal_scan, (int64_t)m + 1, sums, st);
+        if (e) return e;
+        uint32_t n_internal = 0;
+        int max_stream = 0;
+        HIPCHECK(c, hipMemcpy(&n_internal, B.internal_scan + m, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHECK(c, hipMemcpy(&max_stream, B.level_info, sizeof(int), hipMemcpyDeviceToHost));
+        hipLaunchKernelGGL(k_stream, dim3(blocks_for(n, t)), dim3(t), 0, st, B);
+        int in = 0;
+        if (max_stream > 0) {
+            int rounds = 1;
+            while ((1 << rounds) <= max_stream) rounds++;
+            for (int r = 0; r < rounds; r++) {
+                hipLaunchKernelGGL(k_jump, dim3(blocks_for(n, t)), dim3(t), 0, st, B, in);
+                in ^= 1;
+            }
+            hipLaunchKernelGGL(k_place_right, dim3(blocks_for(n, t)), dim3(t), 0, st, B, in);
+        }
+        const int next = g0 + m;
+        if ((int64_t)next + 2 * (int64_t)n_internal > cap_nodes)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "build_bvh: node capacity exceeded");
+        hipLaunchKernelGGL(k_children, dim3(blocks_for(m, t)), dim3(t), 0, st, B, g0, m, next);
+        HIPCHECK(c, hipGetLastError());
+        std::swap(B.ref, B.ref2);
+        levels.push_back({g0, m});
+        g0 = next;
+        m = 2 * (int)n_internal;
+        total += m;
+    }
+    for (int l = (int)levels.size() - 1; l >= 0; l--)
+        hipLaunchKernelGGL(k_sizes, dim3(blocks_for(levels[l].second, t)), dim3(t), 0, st, B, levels[l].first,
+                           levels[l].second);
+    HIPCHECK(c, hipMemsetAsync(B.nd_idx, 0, sizeof(int32_t), st));
+    HIPCHECK(c, hipMemsetAsync(B.nd_lb, 0, sizeof(int32_t), st));
+    for (size_t l = 0; l < levels.size(); l++)
+        hipLaunchKernelGGL(k_dfs, dim3(blocks_for(levels[l].second, t)), dim3(t), 0, st, B, levels[l].first,
+                           levels[l].second);
+    HIPCHECK(c, hipGetLastError());
+
+    const int n_leaf = (total + 1) / 2, n_rec = n_leaf - 1;
+    const int REC = DIM == 3 ? 4 : 3, PR = DIM == 3 ? 3 : 1;
+    HIPCHECK(c, hipMalloc((void **)&c->records, (size_t)std::max(n_rec, 1) * REC * sizeof(float4)));
+    HIPCHECK(c, hipMemset(c->records, 0, (size_t)std::max(n_rec, 1) * REC * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->root, 2 * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->prims, (size_t)n * PR * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->prim_index, (size_t)n * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->prim_vids, (size_t)n * DIM * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->rec_parent, (size_t)std::max(n_rec, 1) * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->leaf_refs, (size_t)n_leaf * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->leaf_links, (size_t)n_leaf * sizeof(uint32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->counters, (size_t)std::max(n_rec, 1) * sizeof(unsigned int)));
+    HIPCHECK(c, hipMemset(c->counters, 0, (size_t)std::max(n_rec, 1) * sizeof(unsigned int)));
+    Emit E = {};
+    E.records = c->records;
+    E.root = c->root;
+    E.rec_parent = c->rec_parent;
+    E.leaf_refs = c->leaf_refs;
+    E.leaf_links = c->leaf_links;
+    HIPCHECK(c, T.alloc(&E.pmin, (size_t)total * 3));
+    HIPCHECK(c, T.alloc(&E.pmax, (size_t)total * 3));
+    HIPCHECK(c, T.alloc(&E.nrefs, total));
+    HIPCHECK(c, T.alloc(&E.offset, total));
+    HIPCHECK(c, T.alloc(&E.status, 2));
+    HIPCHECK(c, hipMemset(E.status, 0, 2 * sizeof(int)));
+    hipLaunchKernelGGL(k_emit<DIM>, dim3(blocks_for(total, t)), dim3(t), 0, st, B, E, total);
+    hipLaunchKernelGGL(k_emit_prims<DIM>, dim3(blocks_for(n, t)), dim3(t), 0, st, B.ref, V, idx, n, c->prims,
+                       c->prim_index, c->prim_vids);
+    HIPCHECK(c, hipGetLastError());
+    int status[2] = {0, 0};
+    HIPCHECK(c, hipMemcpy(status, E.status, sizeof(status), hipMemcpyDeviceToHost));
+    if (status[0] & 1)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "build_bvh: a leaf at the depth cap holds more than %d primitives",
+             MAX_LEAF_PRIMS);
+    if (status[1] > STACK_TOTAL - 1)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "build_bvh: tree depth %d exceeds %d", status[1], STACK_TOTAL - 1);
+    float root_box[6];
+    {
+        float4 r[2];
+        HIPCHECK(c, hipMemcpy(r, c->root, sizeof(r), hipMemcpyDeviceToHost));
+        if (DIM == 3) {
+            const float v[6] = {r[0].x, r[0].y, r[0].z, r[1].x, r[1].y, r[1].z};
+            memcpy(root_box, v, sizeof(v));
+        } else {
+            const float v[6] = {r[0].x, r[0].y, 0.0f, r[0].z, r[0].w, 0.0f};
+            memcpy(root_box, v, sizeof(v));
+        }
+    }
+    if (out_pmin) HIPCHECK(c, hipMemcpy(out_pmin, E.pmin, (size_t)total * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    if (out_pmax) HIPCHECK(c, hipMemcpy(out_pmax, E.pmax, (size_t)total * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    if (out_nrefs) HIPCHECK(c, hipMemcpy(out_nrefs, E.nrefs, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (out_offset)
+        HIPCHECK(c, hipMemcpy(out_offset, E.offset, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (out_prim_order)
+        HIPCHECK(c, hipMemcpy(out_prim_order, c->prim_index, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (out_n_nodes) *out_n_nodes = total;
+    HIPCHECK(c, hipDeviceSynchronize());
+    for (int d = 0; d < 3; d++) {
+        float lo = d < DIM ? root_box[d] : 0.0f, hi = d < DIM ? root_box[3 + d] : 0.0f;
+        float ext = std::max(hi - lo, 1e-6f);
+        c->qlo[d] = 0.5f * (lo + hi) - 0.75f * ext;
+        c->qinv[d] = 1.0f / (1.5f * ext);
+    }
+    c->dim = DIM;
+    c->n_records = n_rec;
+    c->n_leaves = n_leaf;
+    c->n_prims = n;
+    c->n_vertices = n_vertices;
+    c->max_depth = status[1];
+    c->node_slot.clear();
+    c->node_leaf_off.clear();
+    c->uploaded = true;
+    c->generation++;
+    return FCPW_HIP_OK;
+}
 
 extern "C" {
 
@@ -243,6 +472,32 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
     c->uploaded = true;
     c->generation++;
     return FCPW_HIP_OK;
+}
+
+int fcpw_hip_build_bvh(fcpw_hip_ctx *c, int dim, const float *positions, int n_vertices, const int32_t *indices,
+                       int n_prims, int heuristic, float *node_pmin, float *node_pmax, int32_t *node_nrefs,
+                       int32_t *node_offset, int32_t *prim_order, int *n_nodes)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    if (dim != 2 && dim != 3) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "build_bvh: dim must be 2 or 3, got %d", dim);
+    if (n_prims <= 0 || n_vertices <= 0 || !positions || !indices)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "build_bvh: Cannot construct a BVH without primitives");
+    if (heuristic < 0 || heuristic > 4)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "build_bvh: unknown heuristic %d", heuristic);
+    if ((size_t)n_prims > (size_t)OFF_MASK || (int64_t)n_prims * 2 > (int64_t)INT32_MAX / 4)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "build_bvh: %d primitives exceed the supported range", n_prims);
+    for (int64_t k = 0; k < (int64_t)n_prims * dim; k++)
+        if (indices[k] < 0 || indices[k] >= n_vertices)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "build_bvh: vertex id out of range");
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipDeviceSynchronize());
+    free_scene(c);
+    int st = dim == 3 ? gpu_build<3>(c, positions, n_vertices, indices, n_prims, heuristic, node_pmin, node_pmax,
+                                     node_nrefs, node_offset, prim_order, n_nodes)
+                      : gpu_build<2>(c, positions, n_vertices, indices, n_prims, heuristic, node_pmin, node_pmax,
+                                     node_nrefs, node_offset, prim_order, n_nodes);
+    if (st) free_scene(c);
+    return st;
 }
 
 int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bounding_cone *cones,

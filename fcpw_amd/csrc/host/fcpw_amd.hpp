@@ -221,6 +221,15 @@ public:
         }
     }
 
+    // a tree built elsewhere (GPUScene::buildOnGPU); drops any silhouette data
+    void adoptBvh(FlatBvh<DIM> &&bvh, AggregateType aggregateType)
+    {
+        bvh_ = std::make_unique<FlatBvh<DIM>>(std::move(bvh));
+        snch_.reset();
+        aggregate_ = aggregateType;
+        ++version_;
+    }
+
     // accessors used by GPUScene
     bool built() const { return (bool)bvh_; }
     const FlatBvh<DIM> &bvh() const { require_bvh(); return *bvh_; }
@@ -230,6 +239,7 @@ public:
     int nPrimitives() const { return (int)(objects_.at(0).indices.size() / DIM); }
     uint64_t version() const { return version_; }
     bool hasSilhouettes() const { return (bool)snch_; }
+    bool silhouettesRequested() const { return silhouettes_; }
     const SnchData<DIM> &snch() const
     {
         if (!snch_) throw std::runtime_error("the scene was built without silhouettes (call computeSilhouettes first)");
@@ -324,6 +334,59 @@ public:
             fcpw_hip_get_info(ctx_, "max_depth", &depth);
             printf("transferToGPU: %zu nodes -> %lld records, depth %lld, %d primitives\n", n, (long long)rec,
                    (long long)depth, np);
+        }
+    }
+
+    // Scene::build + transferToGPU in one step on the device (fcpw_hip_build_bvh):
+    // the reference's binned SAH tree, built from the scene's current geometry.
+    // With exportTree the tree is also copied back into the scene, as if
+    // scene.build(aggregateType) had made it.  Silhouettes need the host build.
+    void buildOnGPU(Scene<DIM> &scene, AggregateType aggregateType = AggregateType::Bvh_SurfaceArea,
+                    bool exportTree = false)
+    {
+        if (scene.silhouettesRequested())
+            throw std::runtime_error("buildOnGPU: silhouettes are built on the host (use build + transferToGPU)");
+        if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
+        int h = (int)heuristic_of(aggregateType);
+        int np = scene.nPrimitives();
+        if (np <= 0) throw std::runtime_error("Cannot construct a BVH without primitives");
+        std::vector<float> pmin, pmax;
+        std::vector<int32_t> nrefs, offset, order;
+        if (exportTree) {
+            size_t cap = 2 * (size_t)np - 1;
+            pmin.resize(cap * 3);
+            pmax.resize(cap * 3);
+            nrefs.resize(cap);
+            offset.resize(cap);
+            order.resize((size_t)np);
+        }
+        int n_nodes = 0;
+        hip_check(ctx_,
+                  fcpw_hip_build_bvh(ctx_, DIM, scene.positions().data(), scene.nVertices(), scene.indices().data(),
+                                     np, h, exportTree ? pmin.data() : nullptr, exportTree ? pmax.data() : nullptr,
+                                     exportTree ? nrefs.data() : nullptr, exportTree ? offset.data() : nullptr,
+                                     exportTree ? order.data() : nullptr, &n_nodes),
+                  "buildOnGPU");
+        if (exportTree) {
+            FlatBvh<DIM> b;
+            b.nodes.resize((size_t)n_nodes);
+            int leaves = 0, depth = 0;
+            for (int i = 0; i < n_nodes; i++) {
+                for (int d = 0; d < DIM; d++) {
+                    b.nodes[i].box.lo[d] = pmin[(size_t)i * 3 + d];
+                    b.nodes[i].box.hi[d] = pmax[(size_t)i * 3 + d];
+                }
+                b.nodes[i].nrefs = nrefs[i];
+                b.nodes[i].offset = offset[i];
+                leaves += nrefs[i] > 0;
+            }
+            int64_t md = 0;
+            fcpw_hip_get_info(ctx_, "max_depth", &md);
+            depth = (int)md;
+            b.prim_order = std::move(order);
+            b.n_leaves = leaves;
+            b.max_depth = depth;
+            scene.adoptBvh(std::move(b), aggregateType);
         }
     }
 

@@ -174,6 +174,14 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
         .def("transfer_to_gpu", &G::transferToGPU, "scene"_a, "device_backend"_a = "default")
         .def("refit", &G::refit, "scene"_a, "update_geometry"_a = true)
         .def(
+            "build_on_gpu",
+            [](G &g, Scene<DIM> &s, AggregateType t, bool exportTree) {
+                py::gil_scoped_release rel;
+                g.buildOnGPU(s, t, exportTree);
+            },
+            "scene"_a, "aggregate_type"_a = AggregateType::Bvh_SurfaceArea, "export_tree"_a = false,
+            "Builds the reference's SAH BVH on the GPU and uploads it (no reference counterpart)")
+        .def(
             "_intersect_arrays",
             [](G &g, const F32 &o, const F32 &d, const F32 &t, InteractionList &out, bool occl) {
                 int64_t n, n2;

@@ -106,6 +106,27 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *ctx, int dim, int n_nodes, const float *no
                         int n_vertices, const float *positions);
 
 /*
+ * Builds the BVH on the device instead of the host: the reference's binned SAH
+ * build (Bvh::build, bvh.inl:3-234, with the leaf size 4 and 8 buckets of a
+ * non-vectorized Scene::build, fcpw.inl:796-867) followed by the upload of
+ * fcpw_hip_upload_bvh, in one call.  The tree is the reference's: the same
+ * nodes in the same DFS order, the same boxes, and each leaf's primitives in
+ * the same order.  positions: n_vertices x dim floats; indices: n_prims x dim
+ * vertex ids (caller's order); heuristic numbered as fcpw::CostHeuristic
+ * (0 LongestAxisCenter, 1 SurfaceArea, 2 OverlapSurfaceArea, 3 Volume,
+ * 4 OverlapVolume).  Optional outputs (NULL to skip) receive the flat tree as
+ * fcpw_hip_upload_bvh takes it: node_pmin / node_pmax (n x 3), node_nrefs,
+ * node_offset (n entries each, room for 2 * n_prims - 1 nodes), prim_order
+ * (n_prims: original primitive per leaf slot) and n_nodes.  Silhouette data
+ * is built on the host (fcpw_hip_upload_silhouettes needs an uploaded host
+ * tree).  Replaces any previous upload.  (No reference counterpart: the
+ * reference builds on the CPU and transfers, README.md:84-86.)
+ */
+int fcpw_hip_build_bvh(fcpw_hip_ctx *ctx, int dim, const float *positions, int n_vertices, const int32_t *indices,
+                       int n_prims, int heuristic, float *node_pmin, float *node_pmax, int32_t *node_nrefs,
+                       int32_t *node_offset, int32_t *prim_order, int *n_nodes);
+
+/*
  * Replaces the geometry re-upload of GPUScene::refit(scene, updateGeometry=true)
  * (fcpw_gpu.inl:64-80 -> bvh_interop_structures.h:582-603): new vertex
  * positions in the caller's vertex order; the primitive records are rebuilt on

@@ -78,3 +78,16 @@ def rays_from(positions, n, seed=82):
     o = synth.uniform_in_box(n, lo, hi, 1.0, rng)
     d = rng.sphere(n, positions.shape[1])
     return synth.f32(o), synth.f32(d)
+
+
+def degenerate_soup():
+    """An icosphere with degenerate and duplicate triangles appended: one
+    collapsed to a point, one with a sub-denormal extent, a collinear one, one
+    with a repeated vertex, and three duplicates."""
+    V, F = small_icosphere(2)
+    n = len(V)
+    extra_v = [V[0], V[0] + np.float32(1e-30), V[5], (V[5] + V[6]) / 2, V[6]]
+    V = np.vstack([V, np.asarray(extra_v, np.float32)]).astype(np.float32)
+    extra_f = [[0, 0, 0], [n, n + 1, 0], [n + 2, n + 3, n + 4], [5, 6, 5], F[0], F[0], F[1]]
+    F = np.vstack([F, np.asarray(extra_f, np.int32)]).astype(np.int32)
+    return V, F

@@ -36,23 +36,8 @@ def assert_same(rec, ref):
         assert ((a == b) | (np.isnan(a) & np.isnan(b))).all(), f
 
 
-def degenerate_soup(seed=3):
-    rng = np.random.default_rng(seed)
-    V, F = meshes.small_icosphere(2)
-    n = len(V)
-    extra_v = [V[0], V[0] + np.float32(1e-30), V[5], (V[5] + V[6]) / 2, V[6]]
-    V = np.vstack([V, np.asarray(extra_v, np.float32)]).astype(np.float32)
-    extra_f = [[0, 0, 0],                 # a point
-               [n, n + 1, 0],             # sub-denormal extent
-               [n + 2, n + 3, n + 4],     # collinear
-               [5, 6, 5],                 # a segment
-               F[0], F[0], F[1]]          # duplicates
-    F = np.vstack([F, np.asarray(extra_f, np.int32)]).astype(np.int32)
-    return V, F
-
-
 def test_degenerate_triangles_and_duplicates(gpu):
-    V, F = degenerate_soup()
+    V, F = meshes.degenerate_soup()
     scene, g = make_scene(V, F)
     q = meshes.queries_around(V, 20000, seed=4)
     q[:64] = V[:64]  # exactly on vertices: d = 0
