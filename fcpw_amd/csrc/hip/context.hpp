@@ -46,6 +46,10 @@ struct fcpw_hip_ctx {
     int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1, morton_bits = 24;
     int stats_on = 0, timing = 0;
     int n_cus = 256;
+    // device scratch of fcpw_hip_build_bvh, kept (grow-only) for the next build
+    static constexpr int BUILD_SLOTS = 48;
+    void *build_buf[BUILD_SLOTS] = {};
+    size_t build_bytes[BUILD_SLOTS] = {};
     // HIP event pairs around each traversal launch while "timing" is on
     static constexpr int MAX_TIMED = 1024;
     std::vector<hipEvent_t> ev_begin, ev_end;

@@ -19,20 +19,28 @@ static int exclusive_scan(fcpw_hip_ctx *c, uint32_t *v, int64_t n, uint32_t *sum
     return FCPW_HIP_OK;
 }
 
-// device allocations of one build, released together
+// device scratch of the build: numbered slots kept in the context and grown on
+// demand, so a rebuild of a scene of the same size allocates nothing
 struct BuildScratch {
-    std::vector<void *> bufs;
-    ~BuildScratch()
-    {
-        for (void *b : bufs) (void)hipFree(b);
-    }
+    fcpw_hip_ctx *c;
+    int next = 0;
     template <typename T>
     hipError_t alloc(T **p, size_t count)
     {
         *p = nullptr;
-        hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) bufs.push_back(*p);
-        return e;
+        if (next >= fcpw_hip_ctx::BUILD_SLOTS) return hipErrorOutOfMemory;
+        const int k = next++;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (c->build_bytes[k] < bytes) {
+            if (c->build_buf[k]) (void)hipFree(c->build_buf[k]);
+            c->build_buf[k] = nullptr;
+            c->build_bytes[k] = 0;
+            hipError_t e = hipMalloc(&c->build_buf[k], bytes);
+            if (e != hipSuccess) return e;
+            c->build_bytes[k] = bytes;
+        }
+        *p = (T *)c->build_buf[k];
+        return hipSuccess;
     }
 };
 
@@ -47,7 +55,7 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     const int n = n_prims;
     const int cap_nodes = 2 * n - 1;
     hipStream_t st = 0;
-    BuildScratch T;
+    BuildScratch T{c};
     float *V = nullptr;
     int32_t *idx = nullptr;
     HIPCHECK(c, hipMalloc((void **)&V, (size_t)n_vertices * DIM * sizeof(float)));
@@ -82,8 +90,11 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     HIPCHECK(c, T.alloc(&B.level_info, 4));
     uint32_t *sums = nullptr;
     HIPCHECK(c, T.alloc(&sums, 16384));
-    int bins_cap = 0;
-    size_t iscan_cap = 0;
+    // a level holds at most two children per internal node of the previous level,
+    // and an internal node has more than LEAF references
+    const int max_level = std::max(1, 2 * (n / (LEAF + 1)) + 2);
+    HIPCHECK(c, T.alloc(&B.bins, (size_t)max_level * BIN_NODE));
+    HIPCHECK(c, T.alloc(&B.internal_scan, (size_t)max_level + 1));
 
     const int t = 256;
     hipLaunchKernelGGL(k_prim_data<DIM>, dim3(blocks_for(n, t)), dim3(t), 0, st, V, idx, n, plo, phi, pcen);
@@ -96,20 +107,10 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     std::vector<std::pair<int, int>> levels; // (first node, count)
     int g0 = 0, m = 1, total = 1;
     while (m > 0) {
-        if (m > bins_cap) {
-            int *nb = nullptr;
-            HIPCHECK(c, T.alloc(&nb, (size_t)m * BIN_NODE));
-            B.bins = nb;
-            bins_cap = m;
-        }
-        if ((size_t)m + 1 > iscan_cap) {
-            uint32_t *is = nullptr;
-            HIPCHECK(c, T.alloc(&is, (size_t)m + 1));
-            B.internal_scan = is;
-            iscan_cap = (size_t)m + 1;
-        }
+        if (m > max_level) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "build_bvh: level size %d exceeds %d", m, max_level);
         hipLaunchKernelGGL(k_level_init, dim3(blocks_for(m, t)), dim3(t), 0, st, B, g0, m);
         hipLaunchKernelGGL(k_elem_node, dim3(blocks_for(n, t)), dim3(t), 0, st, B, g0, m);
+        hipLaunchKernelGGL(k_small<DIM>, dim3(blocks_for(m, 64)), dim3(64), 0, st, B, g0, m);
         hipLaunchKernelGGL(k_bounds<DIM>, dim3(blocks_for(n, ACC_THREADS)), dim3(ACC_THREADS), 0, st, B);
         hipLaunchKernelGGL(k_bins<DIM>, dim3(blocks_for(n, ACC_THREADS)), dim3(ACC_THREADS), 0, st, B, g0);
         hipLaunchKernelGGL(k_split<DIM>, dim3(blocks_for(m, 64)), dim3(64), 0, st, B, g0, m);
@@ -285,6 +286,8 @@ int fcpw_hip_destroy(fcpw_hip_ctx *c)
     if (c->stats) (void)hipFree(c->stats);
     void *ws[] = {c->stage_in, c->stage_out, c->ws_codes, c->ws_perm, c->ws_hist, c->ws_sums};
     for (void *b : ws)
+        if (b) (void)hipFree(b);
+    for (void *b : c->build_buf)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->ev_begin) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->ev_end) (void)hipEventDestroy(e);

@@ -150,53 +150,84 @@ __device__ __forceinline__ bool needs_split(const Build &B, int g)
     return B.nd_n[g] > LEAF && B.nd_depth[g] != DEPTH_CAP;
 }
 
-// A block whose first and last positions lie in one node (whose range is
-// contiguous) accumulates in LDS and flushes once; other blocks use global atomics.
+// nodes of at most SMALL references are bounded, binned and split by one thread
+// (k_small); larger ones by the element-parallel kernels below
+constexpr int SMALL = 64;
+__device__ __forceinline__ bool is_small(const Build &B, int g) { return B.nd_n[g] <= SMALL; }
+
+// Element-parallel accumulation for the large nodes.  The nodes a block touches
+// are consecutive in level order (node ranges are sorted and contiguous); when
+// they span fewer than SLOTS node ids the block accumulates in LDS, one slot per
+// node, and flushes each used slot with one set of global atomics.  Otherwise
+// every element goes to global memory directly.
 constexpr int ACC_THREADS = 256;
+constexpr int SLOTS = 8;
+
+struct BlockNodes {
+    int gmin, gmax;
+};
+
+__device__ __forceinline__ void block_node_range(const Build &B, int g, bool valid, BlockNodes &r, int *smin,
+                                                 int *smax)
+{
+    if (threadIdx.x == 0) {
+        *smin = INT_MAX;
+        *smax = -1;
+    }
+    __syncthreads();
+    if (valid) {
+        atomicMin(smin, g);
+        atomicMax(smax, g);
+    }
+    __syncthreads();
+    r.gmin = *smin;
+    r.gmax = *smax;
+}
 
 template <int DIM>
 __global__ __launch_bounds__(ACC_THREADS) void k_bounds(Build B)
 {
-    __shared__ int acc[12];
-    __shared__ int shared_node;
+    __shared__ int acc[SLOTS][12];
+    __shared__ int used[SLOTS];
+    __shared__ int smin, smax;
     const int pos = blockIdx.x * ACC_THREADS + threadIdx.x;
-    const int first = blockIdx.x * ACC_THREADS, last = min(B.n, first + ACC_THREADS) - 1;
-    if (threadIdx.x == 0) {
-        const int a = B.elem_node[first], b = B.elem_node[last];
-        shared_node = (a >= 0 && a == b) ? a : -1;
-        for (int k = 0; k < 3; k++) {
-            acc[k] = f2o(FLT_MAX); acc[3 + k] = f2o(-FLT_MAX);
-            acc[6 + k] = f2o(FLT_MAX); acc[9 + k] = f2o(-FLT_MAX);
-        }
+    for (int k = threadIdx.x; k < SLOTS * 12; k += ACC_THREADS) {
+        const int w = k % 12;
+        acc[k / 12][w] = (w / 3) % 2 == 0 ? f2o(FLT_MAX) : f2o(-FLT_MAX);
     }
-    __syncthreads();
-    const int sn = shared_node;
-    if (pos < B.n) {
-        const int g = B.elem_node[pos];
-        if (g >= 0) {
-            const int p = B.ref[pos];
-            const float4 l = B.pbox_lo[p], h = B.pbox_hi[p], c = B.pcen[p];
-            const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z}, cv[3] = {c.x, c.y, c.z};
-            for (int d = 0; d < DIM; d++) {
-                // the centroid box grows by the inflated point (expandToInclude(point))
-                const int a = f2o(lv[d]), b = f2o(hv[d]), ca = f2o(cv[d] - FLT_EPSILON), cb = f2o(cv[d] + FLT_EPSILON);
-                if (sn >= 0) {
-                    atomicMin(&acc[d], a); atomicMax(&acc[3 + d], b);
-                    atomicMin(&acc[6 + d], ca); atomicMax(&acc[9 + d], cb);
-                } else {
-                    atomicMin(&B.nd_box[g * 6 + d], a); atomicMax(&B.nd_box[g * 6 + 3 + d], b);
-                    atomicMin(&B.nd_cbox[g * 6 + d], ca); atomicMax(&B.nd_cbox[g * 6 + 3 + d], cb);
-                }
+    if (threadIdx.x < SLOTS) used[threadIdx.x] = 0;
+    const int g = pos < B.n ? B.elem_node[pos] : -1;
+    const bool valid = g >= 0 && !is_small(B, g);
+    BlockNodes r;
+    block_node_range(B, g, valid, r, &smin, &smax);
+    const bool lds = r.gmax >= 0 && r.gmax - r.gmin < SLOTS;
+    if (valid) {
+        const int p = B.ref[pos];
+        const float4 l = B.pbox_lo[p], h = B.pbox_hi[p], c = B.pcen[p];
+        const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z}, cv[3] = {c.x, c.y, c.z};
+        if (lds) used[g - r.gmin] = 1;
+        for (int d = 0; d < DIM; d++) {
+            // the centroid box grows by the inflated point (expandToInclude(point))
+            const int a = f2o(lv[d]), b = f2o(hv[d]), ca = f2o(cv[d] - FLT_EPSILON), cb = f2o(cv[d] + FLT_EPSILON);
+            if (lds) {
+                int *q = acc[g - r.gmin];
+                atomicMin(&q[d], a); atomicMax(&q[3 + d], b);
+                atomicMin(&q[6 + d], ca); atomicMax(&q[9 + d], cb);
+            } else {
+                atomicMin(&B.nd_box[g * 6 + d], a); atomicMax(&B.nd_box[g * 6 + 3 + d], b);
+                atomicMin(&B.nd_cbox[g * 6 + d], ca); atomicMax(&B.nd_cbox[g * 6 + 3 + d], cb);
             }
         }
     }
     __syncthreads();
-    if (sn >= 0 && threadIdx.x < 12) {
-        const int k = threadIdx.x, d = k % 3;
-        if (d < DIM) {
-            int *dst = k < 6 ? &B.nd_box[sn * 6 + k] : &B.nd_cbox[sn * 6 + (k - 6)];
-            if ((k / 3) % 2 == 0) atomicMin(dst, acc[k]);
-            else atomicMax(dst, acc[k]);
+    if (lds) {
+        for (int k = threadIdx.x; k < SLOTS * 12; k += ACC_THREADS) {
+            const int slot = k / 12, w = k % 12, d = w % 3;
+            if (!used[slot] || d >= DIM) continue;
+            const int node = r.gmin + slot;
+            int *dst = w < 6 ? &B.nd_box[node * 6 + w] : &B.nd_cbox[node * 6 + (w - 6)];
+            if ((w / 3) % 2 == 0) atomicMin(dst, acc[slot][w]);
+            else atomicMax(dst, acc[slot][w]);
         }
     }
 }
@@ -211,56 +242,54 @@ __device__ __forceinline__ int bucket_of(float c, float lo, float width)
 template <int DIM>
 __global__ __launch_bounds__(ACC_THREADS) void k_bins(Build B, int g0)
 {
-    __shared__ int acc[BIN_NODE];
-    __shared__ int shared_node;
+    __shared__ int acc[SLOTS * BIN_NODE];
+    __shared__ int used[SLOTS];
+    __shared__ int smin, smax;
     const int pos = blockIdx.x * ACC_THREADS + threadIdx.x;
-    const int first = blockIdx.x * ACC_THREADS, last = min(B.n, first + ACC_THREADS) - 1;
-    if (threadIdx.x == 0) {
-        const int a = B.elem_node[first], b = B.elem_node[last];
-        shared_node = (a >= 0 && a == b && needs_split(B, a)) ? a : -1;
-    }
-    for (int k = threadIdx.x; k < 3 * NB; k += ACC_THREADS) {
+    for (int k = threadIdx.x; k < SLOTS * 3 * NB; k += ACC_THREADS) {
         for (int d = 0; d < 3; d++) {
             acc[k * BIN_WORDS + d] = f2o(FLT_MAX);
             acc[k * BIN_WORDS + 3 + d] = f2o(-FLT_MAX);
         }
         acc[k * BIN_WORDS + 6] = 0;
     }
-    __syncthreads();
-    const int sn = shared_node;
-    if (pos < B.n && B.heuristic != LONGEST_AXIS_CENTER) {
-        const int g = B.elem_node[pos];
-        if (g >= 0 && needs_split(B, g)) {
-            const int p = B.ref[pos];
-            const float4 l = B.pbox_lo[p], h = B.pbox_hi[p], c = B.pcen[p];
-            const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z}, cv[3] = {c.x, c.y, c.z};
-            int *dst = sn >= 0 ? acc : B.bins + (size_t)(g - g0) * BIN_NODE;
-            for (int d = 0; d < DIM; d++) {
-                const float nlo = o2f(B.nd_box[g * 6 + d]), nhi = o2f(B.nd_box[g * 6 + 3 + d]);
-                const float extent = nhi - nlo;
-                if (extent < 1e-6f) continue;
-                const float width = extent / (float)NB;
-                int *bin = dst + (d * NB + bucket_of(cv[d], nlo, width)) * BIN_WORDS;
-                for (int e = 0; e < DIM; e++) {
-                    atomicMin(&bin[e], f2o(lv[e]));
-                    atomicMax(&bin[3 + e], f2o(hv[e]));
-                }
-                atomicAdd(&bin[6], 1);
+    if (threadIdx.x < SLOTS) used[threadIdx.x] = 0;
+    const int g = pos < B.n ? B.elem_node[pos] : -1;
+    const bool valid = B.heuristic != LONGEST_AXIS_CENTER && g >= 0 && needs_split(B, g) && !is_small(B, g);
+    BlockNodes r;
+    block_node_range(B, g, valid, r, &smin, &smax);
+    const bool lds = r.gmax >= 0 && r.gmax - r.gmin < SLOTS;
+    if (valid) {
+        const int p = B.ref[pos];
+        const float4 l = B.pbox_lo[p], h = B.pbox_hi[p], c = B.pcen[p];
+        const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z}, cv[3] = {c.x, c.y, c.z};
+        int *dst = lds ? acc + (g - r.gmin) * BIN_NODE : B.bins + (size_t)(g - g0) * BIN_NODE;
+        if (lds) used[g - r.gmin] = 1;
+        for (int d = 0; d < DIM; d++) {
+            const float nlo = o2f(B.nd_box[g * 6 + d]), nhi = o2f(B.nd_box[g * 6 + 3 + d]);
+            const float extent = nhi - nlo;
+            if (extent < 1e-6f) continue;
+            const float width = extent / (float)NB;
+            int *bin = dst + (d * NB + bucket_of(cv[d], nlo, width)) * BIN_WORDS;
+            for (int e = 0; e < DIM; e++) {
+                atomicMin(&bin[e], f2o(lv[e]));
+                atomicMax(&bin[3 + e], f2o(hv[e]));
             }
+            atomicAdd(&bin[6], 1);
         }
     }
     __syncthreads();
-    if (sn >= 0) {
-        int *dst = B.bins + (size_t)(sn - g0) * BIN_NODE;
-        for (int k = threadIdx.x; k < BIN_NODE; k += ACC_THREADS) {
-            const int w = k % BIN_WORDS;
-            if (w == 6) {
-                if (acc[k]) atomicAdd(&dst[k], acc[k]);
-            } else if (w < 3) {
-                atomicMin(&dst[k], acc[k]);
-            } else {
-                atomicMax(&dst[k], acc[k]);
+    if (lds) {
+        for (int k = threadIdx.x; k < SLOTS * 3 * NB; k += ACC_THREADS) {
+            const int slot = k / (3 * NB);
+            const int *a = acc + k * BIN_WORDS;
+            if (!used[slot] || a[6] == 0) continue;
+            int *dst = B.bins + (size_t)(r.gmin + slot - g0) * BIN_NODE + (k % (3 * NB)) * BIN_WORDS;
+            for (int e = 0; e < DIM; e++) {
+                atomicMin(&dst[e], a[e]);
+                atomicMax(&dst[3 + e], a[3 + e]);
             }
+            atomicAdd(&dst[6], a[6]);
         }
     }
 }
@@ -311,14 +340,69 @@ __device__ __forceinline__ float split_cost(int h, const float *llo, const float
     return valid ? c : -c;
 }
 
-// computeObjectSplit (bvh.inl:39-111) for every node of the level that splits
+// the bucket sweep of computeObjectSplit for one axis (bvh.inl:66-100): bucket
+// boxes blo/bhi (empty buckets hold the empty box) and counts bcnt
+template <int DIM>
+__device__ __forceinline__ void sweep_axis(int h, int d, float lo_d, float width, const float (*blo)[3],
+                                           const float (*bhi)[3], const int *bcnt, float &best, int &dim,
+                                           float &coord)
+{
+    float rlo[NB][3], rhi[NB][3];
+    int rcnt[NB];
+    float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int k = NB - 1; k > 0; k--) {
+        for (int e = 0; e < DIM; e++) {
+            alo[e] = smin_(alo[e], blo[k][e]);
+            ahi[e] = smax_(ahi[e], bhi[k][e]);
+            rlo[k][e] = alo[e];
+            rhi[k][e] = ahi[e];
+        }
+        rcnt[k] = bcnt[k] + (k != NB - 1 ? rcnt[k + 1] : 0);
+    }
+    float llo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    int nl = 0;
+    for (int k = 1; k < NB; k++) {
+        for (int e = 0; e < DIM; e++) {
+            llo[e] = smin_(llo[e], blo[k - 1][e]);
+            lhi[e] = smax_(lhi[e], bhi[k - 1][e]);
+        }
+        nl += bcnt[k - 1];
+        if (nl > 0 && rcnt[k] > 0) {
+            const float c = split_cost<DIM>(h, llo, lhi, rlo[k], rhi[k], nl, rcnt[k]);
+            if (c < best) {
+                best = c;
+                dim = d;
+                coord = lo_d + (float)k * width;
+            }
+        }
+    }
+}
+
+// LongestAxisCenter fallback on the centroid box (bvh.inl:104-108)
+template <int DIM>
+__device__ __forceinline__ void fallback_split(const float *clo, const float *chi, int &dim, float &coord)
+{
+    int md = 0;
+    float me = chi[0] - clo[0];
+    for (int d = 1; d < DIM; d++) {
+        const float e = chi[d] - clo[d];
+        if (e > me) {
+            me = e;
+            md = d;
+        }
+    }
+    dim = md;
+    coord = (clo[md] + chi[md]) * 0.5f;
+}
+
+// computeObjectSplit (bvh.inl:39-111) for every large node of the level that splits
 template <int DIM>
 __global__ void k_split(Build B, int g0, int m)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
     const int g = g0 + j;
-    if (!needs_split(B, g)) return;
+    if (!needs_split(B, g) || is_small(B, g)) return;
     float lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
         lo[d] = o2f(B.nd_box[g * 6 + d]);
@@ -332,8 +416,8 @@ __global__ void k_split(Build B, int g0, int m)
             const float extent = hi[d] - lo[d];
             if (extent < 1e-6f) continue;
             const float width = extent / (float)NB;
-            float blo[NB][3], bhi[NB][3], rlo[NB][3], rhi[NB][3];
-            int bcnt[NB], rcnt[NB];
+            float blo[NB][3], bhi[NB][3];
+            int bcnt[NB];
             for (int k = 0; k < NB; k++) {
                 const int *b = bins + (d * NB + k) * BIN_WORDS;
                 for (int e = 0; e < 3; e++) {
@@ -342,54 +426,82 @@ __global__ void k_split(Build B, int g0, int m)
                 }
                 bcnt[k] = b[6];
             }
-            float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-            for (int k = NB - 1; k > 0; k--) {
-                for (int e = 0; e < DIM; e++) {
-                    alo[e] = smin_(alo[e], blo[k][e]);
-                    ahi[e] = smax_(ahi[e], bhi[k][e]);
-                    rlo[k][e] = alo[e];
-                    rhi[k][e] = ahi[e];
-                }
-                rcnt[k] = bcnt[k] + (k != NB - 1 ? rcnt[k + 1] : 0);
-            }
-            float llo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-            int nl = 0;
-            for (int k = 1; k < NB; k++) {
-                for (int e = 0; e < DIM; e++) {
-                    llo[e] = smin_(llo[e], blo[k - 1][e]);
-                    lhi[e] = smax_(lhi[e], bhi[k - 1][e]);
-                }
-                nl += bcnt[k - 1];
-                if (nl > 0 && rcnt[k] > 0) {
-                    const float c = split_cost<DIM>(B.heuristic, llo, lhi, rlo[k], rhi[k], nl, rcnt[k]);
-                    if (c < best) {
-                        best = c;
-                        dim = d;
-                        coord = lo[d] + (float)k * width;
-                    }
-                }
-            }
+            sweep_axis<DIM>(B.heuristic, d, lo[d], width, blo, bhi, bcnt, best, dim, coord);
         }
     }
     if (dim == -1) {
-        // LongestAxisCenter fallback on the centroid box (bvh.inl:104-108)
         float clo[3], chi[3];
         for (int d = 0; d < 3; d++) {
             clo[d] = o2f(B.nd_cbox[g * 6 + d]);
             chi[d] = o2f(B.nd_cbox[g * 6 + 3 + d]);
         }
-        int md = 0;
-        float me = chi[0] - clo[0];
-        for (int d = 1; d < DIM; d++) {
-            const float e = chi[d] - clo[d];
-            if (e > me) {
-                me = e;
-                md = d;
-            }
-        }
-        dim = md;
-        coord = (clo[md] + chi[md]) * 0.5f;
+        fallback_split<DIM>(clo, chi, dim, coord);
     }
+    B.nd_dim[g] = dim;
+    B.nd_coord[g] = coord;
+}
+
+// small nodes: node box, centroid box, bins and split by one thread, growing
+// the boxes element by element in range order as the reference does
+template <int DIM>
+__global__ void k_small(Build B, int g0, int m)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int g = g0 + j;
+    if (!is_small(B, g)) return;
+    const int s = B.nd_start[g], n = B.nd_n[g];
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = s; i < s + n; i++) {
+        const int p = B.ref[i];
+        const float4 l = B.pbox_lo[p], h = B.pbox_hi[p], c = B.pcen[p];
+        const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z}, cv[3] = {c.x, c.y, c.z};
+        for (int d = 0; d < DIM; d++) {
+            lo[d] = smin_(lo[d], lv[d]);
+            hi[d] = smax_(hi[d], hv[d]);
+            clo[d] = smin_(clo[d], cv[d] - FLT_EPSILON);
+            chi[d] = smax_(chi[d], cv[d] + FLT_EPSILON);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        B.nd_box[g * 6 + d] = f2o(lo[d]);
+        B.nd_box[g * 6 + 3 + d] = f2o(hi[d]);
+        B.nd_cbox[g * 6 + d] = f2o(clo[d]);
+        B.nd_cbox[g * 6 + 3 + d] = f2o(chi[d]);
+    }
+    if (!needs_split(B, g)) return;
+    float best = FLT_MAX, coord = 0.0f;
+    int dim = -1;
+    if (B.heuristic != LONGEST_AXIS_CENTER) {
+        for (int d = 0; d < DIM; d++) {
+            const float extent = hi[d] - lo[d];
+            if (extent < 1e-6f) continue;
+            const float width = extent / (float)NB;
+            float blo[NB][3], bhi[NB][3];
+            int bcnt[NB];
+            for (int k = 0; k < NB; k++) {
+                for (int e = 0; e < 3; e++) {
+                    blo[k][e] = FLT_MAX;
+                    bhi[k][e] = -FLT_MAX;
+                }
+                bcnt[k] = 0;
+            }
+            for (int i = s; i < s + n; i++) {
+                const int p = B.ref[i];
+                const float4 l = B.pbox_lo[p], h = B.pbox_hi[p], c = B.pcen[p];
+                const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z}, cv[3] = {c.x, c.y, c.z};
+                const int k = bucket_of(cv[d], lo[d], width);
+                for (int e = 0; e < DIM; e++) {
+                    blo[k][e] = smin_(blo[k][e], lv[e]);
+                    bhi[k][e] = smax_(bhi[k][e], hv[e]);
+                }
+                bcnt[k] += 1;
+            }
+            sweep_axis<DIM>(B.heuristic, d, lo[d], width, blo, bhi, bcnt, best, dim, coord);
+        }
+    }
+    if (dim == -1) fallback_split<DIM>(clo, chi, dim, coord);
     B.nd_dim[g] = dim;
     B.nd_coord[g] = coord;
 }
