@@ -14,6 +14,12 @@ The printed JSON line carries
                dL/dV (dL/dd = 1); queries/s and its ratio to the forward rate
   rays         16,777,216 rays (origins uniform in the AABB, directions uniform
                on S^2, tMax = inf) through the ray-intersection kernel
+  sphere_intersection  16,777,216 spheres of radius 0.1 at the query points
+               through the stochastic sphere-intersection kernel
+  closest_silhouette   every 4th query (4,194,304) through the closest
+               silhouette point kernel, on the torus built with silhouettes
+  bvh_build    the torus' SAH tree built on the device against the host
+               build + transfer (the same tree either way)
   roofline     algorithmic bytes per query x queries per launch / the traversal
                kernel's mean launch duration (HIP events on the launch stream),
                against MI355X HBM bandwidth.  Bytes per query
@@ -485,7 +491,7 @@ def run_batch(args, torch):
 
     dev = torch.device("cuda:0")
     t0 = time.time()
-    meshes = synth.workload5()
+    meshes = synth.batch_workload()
     gpus = []
     for m in meshes:
         gpus.append(_scene(fcpw, m["positions"], m["indices"])[1])
@@ -568,7 +574,7 @@ def run_star2d(args, torch):
     from fcpw_amd import synth
 
     dev = torch.device("cuda:0")
-    w = synth.workload3()
+    w = synth.star_refit_workload()
     V, F = w["positions"], w["indices"]
     scene, gpu = _scene(fcpw, V, F)
     n = len(w["queries"])
@@ -618,7 +624,7 @@ def run_diff(args, torch):
     import fcpw_amd as fcpw
 
     dev = torch.device("cuda:0")
-    w = synth.workload4()
+    w = synth.geodesic_workload()
     V, F = w["positions"], w["indices"]
     scene, gpu = _scene(fcpw, V, F)
     n = len(w["queries"])

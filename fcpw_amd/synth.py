@@ -221,7 +221,7 @@ def f32(a):
 # ----------------------------------------------------------------------------
 # workloads
 # ----------------------------------------------------------------------------
-def workload1(n_queries=1 << 20):
+def icosphere_20k_workload(n_queries=1 << 20):
     v, f = icosphere(5)
     v = radial_noise(v, 0.01, Rng(seed(1, 1)))
     lo, hi = aabb(v)
@@ -237,7 +237,7 @@ def torus_mesh(U=1024, V=512, jitter=1e-3, stream_seed=None):
     return v, f
 
 
-def workload2(n_queries=1 << 24, n_rays=1 << 24, U=1024, V=512):
+def torus_1m_workload(n_queries=1 << 24, n_rays=1 << 24, U=1024, V=512):
     v, f = torus_mesh(U, V)
     lo, hi = aabb(v)
     rq = Rng(seed(2, 2))
@@ -251,7 +251,7 @@ def workload2(n_queries=1 << 24, n_rays=1 << 24, U=1024, V=512):
                 ray_o=f32(o), ray_d=f32(d), ray_tmax=np.full(n_rays, np.inf, np.float32))
 
 
-def workload3(n_segments=1 << 18, n_queries=1 << 23):
+def star_refit_workload(n_segments=1 << 18, n_queries=1 << 23):
     v, idx = star_polygon(n_segments, Rng(seed(3, 1)))
     rq = Rng(seed(3, 2))
     q = rq.uniform(n_queries * 2, -1.5, 1.5).reshape(-1, 2)
@@ -276,7 +276,7 @@ def perturb_star(v, step, fraction=0.1, amount=0.01):
     return f32(out)
 
 
-def workload4(n_queries=1 << 22, freq=100):
+def geodesic_workload(n_queries=1 << 22, freq=100):
     v, f = geodesic_sphere(freq)
     v = radial_noise(v, 0.01, Rng(seed(4, 1)))
     q = near_surface(n_queries, v, f, 0.1, Rng(seed(4, 2)))
@@ -284,7 +284,7 @@ def workload4(n_queries=1 << 22, freq=100):
                 r2=np.full(n_queries, np.inf, np.float32))
 
 
-def workload5(n_meshes=256, tri_range=(1e3, 5e5), query_range=(1e3, 1e6), max_meshes=None):
+def batch_workload(n_meshes=256, tri_range=(1e3, 5e5), query_range=(1e3, 1e6), max_meshes=None):
     rng = Rng(seed(5, 1))
     T = np.exp(rng.uniform(n_meshes, math.log(tri_range[0]), math.log(tri_range[1])))
     Q = np.exp(rng.uniform(n_meshes, math.log(query_range[0]), math.log(query_range[1]))).astype(np.int64)

@@ -1,8 +1,8 @@
-"""The benchmark configurations at their full mesh sizes, on subsamples of
+"""The benchmark workloads at their full mesh sizes, on subsamples of
 their queries: bit-exact parity with the oracle's reference-order traversal,
-and the error against an FP64 brute force stated per configuration.
+and the error against an FP64 brute force stated per workload.
 
-Error statistic (reported on stdout as one JSON line per configuration):
+Error statistic (reported on stdout as one JSON line per workload):
   rel = |d32 - d64| / max(d64, floor),   floor = 4 eps32 max|coordinate| / 1e-5
 An FP32 distance carries an absolute error of a few ulp of the coordinates
 (x - p cancels), so 1e-5 relative is attainable only where d >= floor; below
@@ -58,7 +58,7 @@ def fp64_report(name, V, F, q, rec):
     mism = idx != idx64
     ties = oracle.classify_index_mismatch(V, F, q[mism], idx[mism], idx64[mism]) if mism.any() else np.zeros(0, bool)
     above = d64 >= floor
-    report = dict(config=name, queries=len(q), max_rel=float(rel.max()), p999_rel=float(np.quantile(rel, 0.999)),
+    report = dict(workload=name, queries=len(q), max_rel=float(rel.max()), p999_rel=float(np.quantile(rel, 0.999)),
                   floor=floor, share_above_floor=float(above.mean()),
                   max_rel_above_floor=float(rel[above].max()) if above.any() else None,
                   max_abs_below_floor=float(np.abs(d32 - d64)[~above].max()) if (~above).any() else None,
@@ -69,8 +69,8 @@ def fp64_report(name, V, F, q, rec):
     return report
 
 
-def test_config1_icosphere_20k(gpu):
-    w = synth.workload1(65536)
+def test_icosphere_20k_uniform_queries(gpu):
+    w = synth.icosphere_20k_workload(65536)
     V, F, q = w["positions"], w["indices"], w["queries"]
     assert len(F) == 20480
     s, g = transfer(V, F)
@@ -78,10 +78,10 @@ def test_config1_icosphere_20k(gpu):
     ref = oracle.Bvh(V, F).closest_points_gpuorder(q)
     np.testing.assert_array_equal(rec["d"], ref["d"])
     np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
-    fp64_report("1: icosphere 20,480 tris, uniform in 1.5x box", V, F, q[:8192], rec[:8192])
+    fp64_report("icosphere 20,480 tris, uniform in 1.5x box", V, F, q[:8192], rec[:8192])
 
 
-def test_config2_torus_1m(gpu):
+def test_torus_1m_mixed_queries(gpu):
     v, f = synth.torus_mesh(1024, 512)
     V, F = synth.f32(v), f.astype(np.int32)
     lo, hi = V.min(0).astype(np.float64), V.max(0).astype(np.float64)
@@ -95,11 +95,11 @@ def test_config2_torus_1m(gpu):
     np.testing.assert_array_equal(rec["d"], ref["d"])
     np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
     sub = np.r_[0:1024, 16384:16384 + 1024]
-    fp64_report("2: torus 1,048,576 tris, half uniform / half near-surface", V, F, q[sub], rec[sub])
+    fp64_report("torus 1,048,576 tris, half uniform / half near-surface", V, F, q[sub], rec[sub])
 
 
-def test_config3_star_refit_steps(gpu):
-    w = synth.workload3(n_queries=1 << 16)
+def test_star_262k_segments_refit_steps(gpu):
+    w = synth.star_refit_workload(n_queries=1 << 16)
     V, F, q = w["positions"], w["indices"], w["queries"]
     s, g = transfer(V, F)
     ob = oracle.Bvh(V, F)
@@ -121,11 +121,11 @@ def test_config3_star_refit_steps(gpu):
             g.intersect(o[:16384], d[:16384], np.full(16384, np.inf, np.float32), hits)
             rr = np.frombuffer(hits._as_bytes(), dtype=fcpw.INTERACTION_DTYPE)
             np.testing.assert_array_equal(rr["d"], ob.intersect_gpuorder(o[:16384], d[:16384])["d"])
-    fp64_report("3: star polygon 262,144 segments after 10 refit steps", Vk, F, q[:8192], rec[:8192])
+    fp64_report("star polygon 262,144 segments after 10 refit steps", Vk, F, q[:8192], rec[:8192])
 
 
-def test_config4_geodesic_200k(gpu):
-    w = synth.workload4(n_queries=1 << 15)
+def test_geodesic_sphere_200k_near_surface(gpu):
+    w = synth.geodesic_workload(n_queries=1 << 15)
     V, F, q = w["positions"], w["indices"], w["queries"]
     assert len(F) == 200000
     s, g = transfer(V, F)
@@ -133,4 +133,4 @@ def test_config4_geodesic_200k(gpu):
     ref = oracle.Bvh(V, F).closest_points_gpuorder(q)
     np.testing.assert_array_equal(rec["d"], ref["d"])
     np.testing.assert_array_equal(rec["index"].view(np.int32), ref["index"])
-    fp64_report("4: geodesic sphere 200,000 tris, within 0.1 of the surface", V, F, q[:4096], rec[:4096])
+    fp64_report("geodesic sphere 200,000 tris, within 0.1 of the surface", V, F, q[:4096], rec[:4096])
