@@ -216,6 +216,23 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     if (out_prim_order)
         HIPCHECK(c, hipMemcpy(out_prim_order, c->prim_index, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (out_n_nodes) *out_n_nodes = total;
+    // with the flat tree on the host, map its nodes to record slots as fcpw_hip_upload_bvh
+    // does (internal nodes numbered in DFS order), so fcpw_hip_upload_silhouettes can follow
+    std::vector<int32_t> node_slot, node_leaf_off;
+    if (out_nrefs && out_offset) {
+        node_slot.assign((size_t)total, 2 * n_rec);
+        node_leaf_off.assign((size_t)total, -1);
+        int rank = 0;
+        for (int i = 0; i < total; i++) {
+            if (out_nrefs[i] > 0) {
+                node_leaf_off[i] = out_offset[i];
+                continue;
+            }
+            node_slot[i + 1] = rank * 2;
+            node_slot[i + out_offset[i]] = rank * 2 + 1;
+            rank++;
+        }
+    }
     HIPCHECK(c, hipDeviceSynchronize());
     for (int d = 0; d < 3; d++) {
         float lo = d < DIM ? root_box[d] : 0.0f, hi = d < DIM ? root_box[3 + d] : 0.0f;
@@ -229,8 +246,8 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     c->n_prims = n;
     c->n_vertices = n_vertices;
     c->max_depth = status[1];
-    c->node_slot.clear();
-    c->node_leaf_off.clear();
+    c->node_slot = std::move(node_slot);
+    c->node_leaf_off = std::move(node_leaf_off);
     c->uploaded = true;
     c->generation++;
     return FCPW_HIP_OK;
@@ -509,6 +526,9 @@ int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bou
 {
     int st = check_ready(c, "upload_silhouettes");
     if (st) return st;
+    if (c->node_slot.empty())
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "upload_silhouettes: the node layout is unknown (a device build without the exported tree)");
     if (n_nodes != (int)c->node_slot.size() || !cones || !n_silhouettes || !silhouette_offset || n_refs < 0 ||
         (n_refs > 0 && !silhouettes))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_silhouettes: expected %d nodes and non-NULL arrays",

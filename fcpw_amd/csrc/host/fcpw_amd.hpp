@@ -226,6 +226,11 @@ public:
     {
         bvh_ = std::make_unique<FlatBvh<DIM>>(std::move(bvh));
         snch_.reset();
+        if (silhouettes_) {
+            const Object &o = objects_.at(0);
+            snch_ = std::make_unique<SnchData<DIM>>(
+                build_snch<DIM>(*bvh_, o.positions.data(), nVertices(), o.indices.data(), nPrimitives(), ignore_));
+        }
         aggregate_ = aggregateType;
         ++version_;
     }
@@ -340,12 +345,13 @@ public:
     // Scene::build + transferToGPU in one step on the device (fcpw_hip_build_bvh):
     // the reference's binned SAH tree, built from the scene's current geometry.
     // With exportTree the tree is also copied back into the scene, as if
-    // scene.build(aggregateType) had made it.  Silhouettes need the host build.
+    // scene.build(aggregateType) had made it.  A scene marked for silhouettes
+    // always exports: its silhouette references and cones are then built on the
+    // host from that tree and uploaded.
     void buildOnGPU(Scene<DIM> &scene, AggregateType aggregateType = AggregateType::Bvh_SurfaceArea,
                     bool exportTree = false)
     {
-        if (scene.silhouettesRequested())
-            throw std::runtime_error("buildOnGPU: silhouettes are built on the host (use build + transferToGPU)");
+        if (scene.silhouettesRequested()) exportTree = true;
         if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
         int h = (int)heuristic_of(aggregateType);
         int np = scene.nPrimitives();
@@ -387,6 +393,7 @@ public:
             b.n_leaves = leaves;
             b.max_depth = depth;
             scene.adoptBvh(std::move(b), aggregateType);
+            if (scene.hasSilhouettes()) uploadSilhouettes(scene);
         }
     }
 

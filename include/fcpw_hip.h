@@ -118,8 +118,9 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *ctx, int dim, int n_nodes, const float *no
  * fcpw_hip_upload_bvh takes it: node_pmin / node_pmax (n x 3), node_nrefs,
  * node_offset (n entries each, room for 2 * n_prims - 1 nodes), prim_order
  * (n_prims: original primitive per leaf slot) and n_nodes.  Silhouette data
- * is built on the host (fcpw_hip_upload_silhouettes needs an uploaded host
- * tree).  Replaces any previous upload.  (No reference counterpart: the
+ * is built on the host from that exported tree: fcpw_hip_upload_silhouettes
+ * needs node_nrefs and node_offset to have been requested.  Replaces any
+ * previous upload.  (No reference counterpart: the
  * reference builds on the CPU and transfers, README.md:84-86.)
  */
 int fcpw_hip_build_bvh(fcpw_hip_ctx *ctx, int dim, const float *positions, int n_vertices, const int32_t *indices,

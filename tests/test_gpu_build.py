@@ -105,9 +105,33 @@ def test_gpu_build_at_scale_equals_host_build(gpu):
     print(f"\nbuild of {len(F)} triangles: host {t_host * 1e3:.1f} ms, device (with export) {t_dev * 1e3:.1f} ms")
 
 
-def test_gpu_build_rejects_silhouette_scenes(gpu):
-    V, F = meshes.small_torus()
+@pytest.mark.parametrize("name", ["torus", "walker2d"])
+def test_gpu_build_with_silhouettes(gpu, name):
+    """a scene marked for silhouettes: the device tree is exported, the silhouette
+    data built on the host from it, and the silhouette query matches the oracle"""
+    V, F = CASES[name]()
+    dim = V.shape[1]
     s = scene_of(V, F)
     s.compute_silhouettes()
-    with pytest.raises(RuntimeError, match="silhouettes"):
-        fcpw.GPUScene3D("").build_on_gpu(s)
+    g = fcpw.GPUScene3D("") if dim == 3 else fcpw.GPUScene2D("")
+    g.build_on_gpu(s)
+    ob = oracle.Bvh(V, F)
+    ob.build_silhouettes()
+    q = meshes.queries_around(V, 5000, seed=24)
+    hits = fcpw.GPUInteractionList()
+    r2 = np.full(len(q), np.finfo(np.float32).max, np.float32)
+    g.find_closest_silhouette_points(q, r2, np.zeros(len(q), np.int32), hits)
+    np.testing.assert_array_equal(records(hits)["d"], ob.closest_silhouettes(q, r2, brute=True)["d"])
+
+
+def test_gpu_build_of_the_star_polygon_equals_host_build(gpu):
+    from fcpw_amd import synth
+    w = synth.star_refit_workload(n_queries=1 << 10)
+    V, F = w["positions"], w["indices"]
+    host = scene_of(V, F)
+    host.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    dev = scene_of(V, F)
+    fcpw.GPUScene2D("").build_on_gpu(dev, fcpw.AggregateType.Bvh_SurfaceArea, export_tree=True)
+    a, b = host._bvh_nodes(), dev._bvh_nodes()
+    for x, y in zip(a[:5], b[:5]):
+        np.testing.assert_array_equal(x, y)
