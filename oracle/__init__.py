@@ -70,6 +70,10 @@ SILHOUETTE_DTYPE = np.dtype([("pa", "<f4", (3,)), ("pb", "<f4", (3,)), ("n0", "<
                              ("index", "<u4"), ("has_one_adjacent_face", "<u4")])
 assert SILHOUETTE_DTYPE.itemsize == 56
 _silp = np.ctypeslib.ndpointer(dtype=SILHOUETTE_DTYPE, flags="C_CONTIGUOUS")
+# one instance of a scene (oracle_instance): object, transformed flag, 3x4 t and its inverse
+INSTANCE_DTYPE = np.dtype([("object", "<i4"), ("transformed", "<i4"), ("t", "<f4", (12,)), ("tinv", "<f4", (12,))])
+assert INSTANCE_DTYPE.itemsize == 104
+_instp = np.ctypeslib.ndpointer(dtype=INSTANCE_DTYPE, flags="C_CONTIGUOUS")
 IGNORE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_float, ctypes.c_int32, ctypes.c_void_p)
 
 
@@ -120,6 +124,14 @@ def _declare(L):
     L.oracle_refit_silhouette_cones_gpu.argtypes = [c_void]
     L.oracle_closest_silhouettes_gpuorder.argtypes = [c_void, _f32p, _u32p, c_int, ctypes.c_float,
                                                       ctypes.c_float, c_int, c_int, _hitp]
+    L.oracle_invert_transform.argtypes = [c_int, _f32p, _f32p]
+    L.oracle_build_instances.restype = c_void
+    L.oracle_build_instances.argtypes = [c_int, c_void, c_int, _instp, c_int, c_int, c_int, c_int]
+    L.oracle_refit_instances.argtypes = [c_void, c_void, _instp]
+    L.oracle_instances_closest_points.argtypes = [c_void, c_void, _instp, _f32p, _f32p, c_int, c_int, c_int, _hitp,
+                                                  oi32]
+    L.oracle_instances_intersect.argtypes = [c_void, c_void, _instp, _f32p, _f32p, _f32p, c_int, c_int, c_int,
+                                             _hitp, oi32]
     L.oracle_closest_silhouette_f64.restype = c_int
     L.oracle_closest_silhouette_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, c_int,
                                                 ctypes.c_double, ctypes.c_double, c_int, _f64p, _f64p, _f64p]
@@ -352,3 +364,108 @@ def closest_silhouette_f64(positions, indices, q, tol=2e-3, tol_d=2e-3, nthreads
     if rc != 0:
         raise ValueError("closest_silhouette_f64 needs manifold input")
     return ds, dl, ps
+
+
+def transform_3x4(dim, m):
+    """A (dim+1) x (dim+1) affine matrix as the 3x4 rows the instance records
+    hold (2D: rows 0-1 with columns 0, 1 and 3)."""
+    m = np.asarray(m, dtype=np.float32)
+    assert m.shape == (dim + 1, dim + 1)
+    t = np.zeros((3, 4), np.float32)
+    t[:dim, :dim] = m[:dim, :dim]
+    t[:dim, 3] = m[:dim, dim]
+    if dim == 2:
+        t[2, 2] = 1.0
+    return t.ravel()
+
+
+def invert_transform(dim, t):
+    """Transform::inverse of a 3x4 affine record (cofactor inverse)."""
+    t = np.ascontiguousarray(t, dtype=np.float32).ravel()
+    out = np.zeros(12, np.float32)
+    lib().oracle_invert_transform(dim, t, out)
+    return out
+
+
+def make_instances(dim, n_objects, transforms):
+    """Scene::build's instance list (fcpw.inl:1066-1085).  transforms[i] is a
+    list of (dim+1) x (dim+1) matrices for object i (empty: the object itself)."""
+    rows = []
+    for o in range(n_objects):
+        ts = transforms.get(o, []) if isinstance(transforms, dict) else transforms[o]
+        if len(ts) == 0:
+            r = np.zeros(1, INSTANCE_DTYPE)
+            r["object"] = o
+            rows.append(r)
+        for m in ts:
+            r = np.zeros(1, INSTANCE_DTYPE)
+            r["object"] = o
+            r["transformed"] = 1
+            r["t"][0] = transform_3x4(dim, m)
+            r["tinv"][0] = invert_transform(dim, r["t"][0])
+            rows.append(r)
+    return np.concatenate(rows)
+
+
+class Instances:
+    """Reference Scene<DIM> with several objects and instance transforms: one
+    Bvh per object and the BVH over the instances (GPU order at both levels)."""
+
+    def __init__(self, objects, instances, heuristic=SURFACE_AREA, leaf_size=4, n_buckets=8):
+        self.objects = list(objects)
+        self.dim = self.objects[0].dim
+        self.instances = np.ascontiguousarray(instances, dtype=INSTANCE_DTYPE)
+        self._ptrs = (ctypes.c_void_p * len(self.objects))(*[o._h for o in self.objects])
+        self._h = lib().oracle_build_instances(self.dim, ctypes.cast(self._ptrs, ctypes.c_void_p),
+                                               len(self.objects), self.instances, len(self.instances),
+                                               heuristic, leaf_size, n_buckets)
+        if not self._h:
+            raise ValueError("oracle_build_instances rejected the input")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_free(self._h)
+            self._h = None
+
+    @property
+    def n_nodes(self):
+        return lib().oracle_num_nodes(self._h)
+
+    def nodes(self):
+        n = self.n_nodes
+        pmin = np.zeros((n, 3), np.float32)
+        pmax = np.zeros((n, 3), np.float32)
+        nrefs = np.zeros(n, np.int32)
+        offset = np.zeros(n, np.int32)
+        lib().oracle_export_nodes(self._h, pmin, pmax, nrefs, offset)
+        return pmin, pmax, nrefs, offset
+
+    def prim_order(self):
+        out = np.zeros(len(self.instances), np.int32)
+        lib().oracle_export_prim_order(self._h, out)
+        return out
+
+    def refit(self):
+        """refits every object, then the instance tree (bvh.inl:585-599)"""
+        for o in self.objects:
+            o.refit()
+        lib().oracle_refit_instances(self._h, ctypes.cast(self._ptrs, ctypes.c_void_p), self.instances)
+
+    def closest_points(self, q, r2=None, record_normal=False, nthreads=0):
+        """returns (hits, object index per query)"""
+        q = _f32(q, self.dim)
+        r2 = np.full(len(q), np.inf, np.float32) if r2 is None else _f32(r2).ravel()
+        out = np.zeros(len(q), HIT_DTYPE)
+        obj = np.zeros(len(q), np.int32)
+        lib().oracle_instances_closest_points(self._h, ctypes.cast(self._ptrs, ctypes.c_void_p), self.instances,
+                                              q, r2, len(q), int(record_normal), nthreads, out, obj)
+        return out, obj
+
+    def intersect(self, o, d, tmax=None, occlusion=False, nthreads=0):
+        o, d = _f32(o, self.dim), _f32(d, self.dim)
+        tmax = np.full(len(o), np.finfo(np.float32).max, np.float32) if tmax is None else _f32(tmax).ravel()
+        out = np.zeros(len(o), HIT_DTYPE)
+        obj = np.zeros(len(o), np.int32)
+        lib().oracle_instances_intersect(self._h, ctypes.cast(self._ptrs, ctypes.c_void_p), self.instances,
+                                         o, d, tmax, len(o), int(occlusion), nthreads, out, obj)
+        return out, obj

@@ -910,69 +910,86 @@ void oracle_closest_points_cpu(const oracle_bvh *h, const float *queries, const 
 /* ----------------------------------------------------------------------------
  * GPU-order closest point (bvh.slang:395-502), with traversal counters
  * ------------------------------------------------------------------------- */
+/* One query: x (3 floats, z = 0 in 2D) against h in the GPU order.  r2 is
+   updated in place, as the reference's inout BoundingSphere; returns the leaf
+   slot of the result (best filled) or -1 (best untouched). */
+static int cp_gpu_one(const oracle_bvh *h, const float *x, float *r2io, oracle_hit *best, int *n_int_out,
+                      int *n_prim_out, int *n_pop_out)
+{
+    int dim = h->dim;
+    int st[BVH_MAX_DEPTH];
+    float r2 = *r2io;
+    int best_slot = -1, n_int = 0, n_prim = 0, n_pop = 0;
+    float d0, d1, d2, d3;
+    box_sqdist_gpu(&h->nodes[0].box, x, dim, &d0, &d1);
+    if (d0 <= r2) {
+        r2 = fminf(r2, d1);
+        int sp = 0;
+        st[0] = 0;
+        while (sp >= 0) {
+            int ni = st[sp--];
+            const node_t *n = &h->nodes[ni];
+            n_pop++;
+            box_sqdist_gpu(&n->box, x, dim, &d0, &d1);
+            if (!(d0 <= r2)) continue;
+            if (n->nrefs > 0) {
+                for (int p = 0; p < n->nrefs; p++) {
+                    int slot = n->offset + p;
+                    float pt[3], uv[2];
+                    n_prim++;
+                    float dd = prim_closest(h, slot, x, pt, uv, gdot);
+                    if (dd * dd <= r2) {
+                        r2 = fminf(r2, dd * dd);
+                        hit_default(best);
+                        best->d = dd;
+                        memcpy(best->p, pt, sizeof(pt));
+                        memcpy(best->uv, uv, sizeof(uv));
+                        best->index = h->prim_orig[slot];
+                        best_slot = slot;
+                    }
+                }
+            } else {
+                n_int++;
+                int l = ni + 1, r = ni + n->offset;
+                box_sqdist_gpu(&h->nodes[l].box, x, dim, &d0, &d1);
+                int ol = d0 <= r2;
+                r2 = fminf(r2, d1);
+                box_sqdist_gpu(&h->nodes[r].box, x, dim, &d2, &d3);
+                int orr = d2 <= r2;
+                r2 = fminf(r2, d3);
+                if (ol && orr) {
+                    int closer = l, other = r;
+                    if (d2 < d0) { closer = r; other = l; }
+                    st[++sp] = other;
+                    st[++sp] = closer;
+                } else if (ol) {
+                    st[++sp] = l;
+                } else if (orr) {
+                    st[++sp] = r;
+                }
+            }
+        }
+    }
+    *r2io = r2;
+    if (n_int_out) *n_int_out = n_int;
+    if (n_prim_out) *n_prim_out = n_prim;
+    if (n_pop_out) *n_pop_out = n_pop;
+    return best_slot;
+}
+
 static void cp_gpu_range(void *vctx, int start, int end)
 {
     cp_ctx *c = (cp_ctx *)vctx;
     const oracle_bvh *h = c->h;
     int dim = h->dim;
-    int st[BVH_MAX_DEPTH];
     for (int qi = start; qi < end; qi++) {
         float x[3] = {0, 0, 0};
         for (int d = 0; d < dim; d++) x[d] = c->q[(size_t)qi * dim + d];
         float r2 = c->r2[qi];
         oracle_hit best;
         hit_default(&best);
-        int best_slot = -1, n_int = 0, n_prim = 0, n_pop = 0;
-        float d0, d1, d2, d3;
-        box_sqdist_gpu(&h->nodes[0].box, x, dim, &d0, &d1);
-        if (d0 <= r2) {
-            r2 = fminf(r2, d1);
-            int sp = 0;
-            st[0] = 0;
-            while (sp >= 0) {
-                int ni = st[sp--];
-                const node_t *n = &h->nodes[ni];
-                n_pop++;
-                box_sqdist_gpu(&n->box, x, dim, &d0, &d1);
-                if (!(d0 <= r2)) continue;
-                if (n->nrefs > 0) {
-                    for (int p = 0; p < n->nrefs; p++) {
-                        int slot = n->offset + p;
-                        float pt[3], uv[2];
-                        n_prim++;
-                        float dd = prim_closest(h, slot, x, pt, uv, gdot);
-                        if (dd * dd <= r2) {
-                            r2 = fminf(r2, dd * dd);
-                            hit_default(&best);
-                            best.d = dd;
-                            memcpy(best.p, pt, sizeof(pt));
-                            memcpy(best.uv, uv, sizeof(uv));
-                            best.index = h->prim_orig[slot];
-                            best_slot = slot;
-                        }
-                    }
-                } else {
-                    n_int++;
-                    int l = ni + 1, r = ni + n->offset;
-                    box_sqdist_gpu(&h->nodes[l].box, x, dim, &d0, &d1);
-                    int ol = d0 <= r2;
-                    r2 = fminf(r2, d1);
-                    box_sqdist_gpu(&h->nodes[r].box, x, dim, &d2, &d3);
-                    int orr = d2 <= r2;
-                    r2 = fminf(r2, d3);
-                    if (ol && orr) {
-                        int closer = l, other = r;
-                        if (d2 < d0) { closer = r; other = l; }
-                        st[++sp] = other;
-                        st[++sp] = closer;
-                    } else if (ol) {
-                        st[++sp] = l;
-                    } else if (orr) {
-                        st[++sp] = r;
-                    }
-                }
-            }
-        }
+        int n_int, n_prim, n_pop;
+        int best_slot = cp_gpu_one(h, x, &r2, &best, &n_int, &n_prim, &n_pop);
         if (best_slot >= 0 && c->record_normal) prim_normal_gpu(h, best_slot, best.n);
         c->out[qi] = best;
         if (c->n_internal) c->n_internal[qi] = n_int;
@@ -1123,12 +1140,83 @@ void oracle_intersect_cpu(const oracle_bvh *h, const float *origins, const float
     parallel_for(n, nthreads, ray_cpu_range, &c);
 }
 
+/* One ray (o, d, dinv: 3 floats, z = 0 in 2D) against h in the GPU order; tmax
+   is updated in place (the reference's inout Ray).  Returns the leaf slot of
+   the hit (best filled; with occlusion only its index) or -1. */
+static int ray_gpu_one(const oracle_bvh *h, const float *o, const float *d, const float *dinv, float *tmax_io,
+                       int occlusion, oracle_hit *best, int *n_int_out, int *n_prim_out)
+{
+    int dim = h->dim;
+    stack_entry st[BVH_MAX_DEPTH];
+    float tmax = *tmax_io;
+    int best_slot = -1, n_int = 0, n_prim = 0;
+    float b0, b1, b2, b3;
+    if (box_ray_gpu(&h->nodes[0].box, o, d, dinv, tmax, dim, &b0, &b1)) {
+        int sp = 0, done = 0;
+        st[0].node = 0;
+        st[0].dist = b0;
+        while (sp >= 0 && !done) {
+            int ni = st[sp].node;
+            float dist = st[sp].dist;
+            sp--;
+            if (dist > tmax) continue;
+            const node_t *n = &h->nodes[ni];
+            if (n->nrefs > 0) {
+                for (int p = 0; p < n->nrefs; p++) {
+                    int slot = n->offset + p, hit;
+                    oracle_hit cand;
+                    hit_default(&cand);
+                    n_prim++;
+                    if (dim == 3)
+                        hit = ray_triangle_gpu(vtx(h, slot, 0), vtx(h, slot, 1), vtx(h, slot, 2), o, d, tmax,
+                                               occlusion, &cand);
+                    else
+                        hit = ray_segment(vtx(h, slot, 0), vtx(h, slot, 1), o, d, tmax, occlusion, 0, &cand, NULL);
+                    if (hit) {
+                        best_slot = slot;
+                        if (occlusion) {
+                            /* bvh.slang:183-187: only the index of the occluder is written */
+                            best->index = h->prim_orig[slot];
+                            done = 1;
+                            break;
+                        }
+                        cand.index = h->prim_orig[slot];
+                        tmax = fminf(tmax, cand.d);
+                        *best = cand;
+                    }
+                }
+            } else {
+                n_int++;
+                int l = ni + 1, r = ni + n->offset;
+                int hl = box_ray_gpu(&h->nodes[l].box, o, d, dinv, tmax, dim, &b0, &b1);
+                int hr = box_ray_gpu(&h->nodes[r].box, o, d, dinv, tmax, dim, &b2, &b3);
+                if (hl && hr) {
+                    int closer = l, other = r;
+                    if (b2 < b0) {
+                        float t = b0; b0 = b2; b2 = t;
+                        closer = r; other = l;
+                    }
+                    st[++sp].node = other; st[sp].dist = b2;
+                    st[++sp].node = closer; st[sp].dist = b0;
+                } else if (hl) {
+                    st[++sp].node = l; st[sp].dist = b0;
+                } else if (hr) {
+                    st[++sp].node = r; st[sp].dist = b2;
+                }
+            }
+        }
+    }
+    *tmax_io = tmax;
+    if (n_int_out) *n_int_out = n_int;
+    if (n_prim_out) *n_prim_out = n_prim;
+    return best_slot;
+}
+
 static void ray_gpu_range(void *vctx, int start, int end)
 {
     ray_ctx *c = (ray_ctx *)vctx;
     const oracle_bvh *h = c->h;
     int dim = h->dim;
-    stack_entry st[BVH_MAX_DEPTH];
     for (int qi = start; qi < end; qi++) {
         float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, dinv[3] = {0, 0, 0};
         for (int k = 0; k < dim; k++) {
@@ -1139,63 +1227,8 @@ static void ray_gpu_range(void *vctx, int start, int end)
         float tmax = c->tmax[qi];
         oracle_hit best;
         hit_default(&best);
-        int n_int = 0, n_prim = 0;
-        float b0, b1, b2, b3;
-        if (box_ray_gpu(&h->nodes[0].box, o, d, dinv, tmax, dim, &b0, &b1)) {
-            int sp = 0, done = 0;
-            st[0].node = 0;
-            st[0].dist = b0;
-            while (sp >= 0 && !done) {
-                int ni = st[sp].node;
-                float dist = st[sp].dist;
-                sp--;
-                if (dist > tmax) continue;
-                const node_t *n = &h->nodes[ni];
-                if (n->nrefs > 0) {
-                    for (int p = 0; p < n->nrefs; p++) {
-                        int slot = n->offset + p, hit;
-                        oracle_hit cand;
-                        hit_default(&cand);
-                        n_prim++;
-                        if (dim == 3)
-                            hit = ray_triangle_gpu(vtx(h, slot, 0), vtx(h, slot, 1), vtx(h, slot, 2), o, d, tmax,
-                                                   c->occlusion, &cand);
-                        else
-                            hit = ray_segment(vtx(h, slot, 0), vtx(h, slot, 1), o, d, tmax, c->occlusion, 0, &cand,
-                                              NULL);
-                        if (hit) {
-                            if (c->occlusion) {
-                                /* bvh.slang:183-187: only the index of the occluder is written */
-                                best.index = h->prim_orig[slot];
-                                done = 1;
-                                break;
-                            }
-                            cand.index = h->prim_orig[slot];
-                            tmax = fminf(tmax, cand.d);
-                            best = cand;
-                        }
-                    }
-                } else {
-                    n_int++;
-                    int l = ni + 1, r = ni + n->offset;
-                    int hl = box_ray_gpu(&h->nodes[l].box, o, d, dinv, tmax, dim, &b0, &b1);
-                    int hr = box_ray_gpu(&h->nodes[r].box, o, d, dinv, tmax, dim, &b2, &b3);
-                    if (hl && hr) {
-                        int closer = l, other = r;
-                        if (b2 < b0) {
-                            float t = b0; b0 = b2; b2 = t;
-                            closer = r; other = l;
-                        }
-                        st[++sp].node = other; st[sp].dist = b2;
-                        st[++sp].node = closer; st[sp].dist = b0;
-                    } else if (hl) {
-                        st[++sp].node = l; st[sp].dist = b0;
-                    } else if (hr) {
-                        st[++sp].node = r; st[sp].dist = b2;
-                    }
-                }
-            }
-        }
+        int n_int, n_prim;
+        ray_gpu_one(h, o, d, dinv, &tmax, c->occlusion, &best, &n_int, &n_prim);
         c->out[qi] = best;
         if (c->n_internal) c->n_internal[qi] = n_int;
         if (c->n_leaf_prims) c->n_leaf_prims[qi] = n_prim;
@@ -2367,4 +2400,471 @@ static void refit_cones_recursive(oracle_bvh *h, int ni)
 void oracle_refit_silhouette_cones_gpu(oracle_bvh *h)
 {
     if (h->sil) refit_cones_recursive(h, 0);
+}
+
+/* ----------------------------------------------------------------------------
+ * Scenes of several objects and transformed instances.
+ *
+ * Scene::build (fcpw.inl:1052-1126) wraps each object in an aggregate; an
+ * object with instance transforms yields one TransformedAggregate per
+ * transform (primitive.h:289-440), in object order then transform order.  With
+ * more than one instance, a BVH over the instances is built by the same
+ * builder (makeAggregate over the aggregate pointers, fcpw.inl:1103-1112),
+ * each instance bounded by its object's root box (bvh.inl:614-618) and
+ * centred on the mean of its primitive centroids (bvh.inl:620-631); a
+ * transformed instance maps the 2^DIM root-box corners with epsilon-inflated
+ * expansion (bounding_volumes.h:183-202, primitive.h:295-298) and its
+ * centroid by t (primitive.h:300-303).
+ *
+ * Queries follow the GPU order at both levels (bvh.slang:144-254,395-502).
+ * A transformed instance is queried as TransformedAggregate is on the GPU
+ * (aggregate.slang:64-117, transform.slang:1-56): the sphere / ray is mapped
+ * by tInv, the object's tree traversed, r^2 / tMax mapped back by t, and on a
+ * hit the point is mapped by t, the normal by transpose(tInv) and normalized,
+ * and d = |p - x|.  An object without instance transforms is queried directly
+ * (bvh.inl:1317-1320,680-683).  Found results shrink r^2 / tMax as primitives
+ * do (bvh.inl:1322-1327,697-704).
+ *
+ * Transforms are 3x4 row-major affine matrices; a 2D transform uses rows 0-1
+ * and columns 0, 1 (linear) and 3 (translation).  transformPoint is
+ * mul(t, float4(p, 1)) with the row sums formed left to right.
+ * ------------------------------------------------------------------------- */
+static void xf_point(const float *t, const float *p, int dim, float *o)
+{
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < dim; i++) {
+        const float *row = t + 4 * i;
+        float s = row[0] * p[0] + row[1] * p[1];
+        if (dim == 3) s = s + row[2] * p[2];
+        r[i] = s + row[3];
+    }
+    o[0] = r[0]; o[1] = r[1]; o[2] = r[2];
+}
+
+/* transformSphere's radius (transform.slang:31-42): c is the sphere centre,
+   tc = transformPoint(t, c) */
+static float xf_r2(const float *t, const float *c, const float *tc, float r2, int dim)
+{
+    if (!(r2 < MAXF)) return MAXF;
+    float e[3] = {c[0] + sqrtf(r2), c[1] + 0.0f, c[2] + 0.0f}, m[3], dd[3];
+    xf_point(t, e, dim, m);
+    for (int i = 0; i < 3; i++) dd[i] = m[i] - tc[i];
+    return gdot(dd, dd, dim);
+}
+
+/* normal mapping of transformInteraction (transform.slang:45-56):
+   normalize(transpose(linear(tInv)) * n), written as v / length(v) */
+static void xf_normal(const float *tinv, const float *n, int dim, float *o)
+{
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < dim; i++) {
+        float s = tinv[0 * 4 + i] * n[0] + tinv[1 * 4 + i] * n[1];
+        if (dim == 3) s = s + tinv[2 * 4 + i] * n[2];
+        r[i] = s;
+    }
+    float l = sqrtf(gdot(r, r, dim));
+    for (int i = 0; i < dim; i++) r[i] = r[i] / l;
+    o[0] = r[0]; o[1] = r[1]; o[2] = r[2];
+}
+
+/* Transform::inverse for an affine transform: the linear part inverted by
+   cofactors (adjugate times 1/det), the translation -(Linv * v) */
+void oracle_invert_transform(int dim, const float *t, float *tinv)
+{
+    for (int i = 0; i < 12; i++) tinv[i] = 0.0f;
+    if (dim == 3) {
+        float cof[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                cof[i][j] = t[i1 * 4 + j1] * t[i2 * 4 + j2] - t[i1 * 4 + j2] * t[i2 * 4 + j1];
+            }
+        float det = (t[0] * cof[0][0] + t[1] * cof[0][1]) + t[2] * cof[0][2];
+        float inv = 1.0f / det;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) tinv[i * 4 + j] = cof[j][i] * inv;
+        for (int i = 0; i < 3; i++) {
+            float s = (tinv[i * 4 + 0] * t[3] + tinv[i * 4 + 1] * t[7]) + tinv[i * 4 + 2] * t[11];
+            tinv[i * 4 + 3] = -s;
+        }
+    } else {
+        float det = t[0] * t[5] - t[1] * t[4];
+        float inv = 1.0f / det;
+        tinv[0] = t[5] * inv;
+        tinv[1] = -t[1] * inv;
+        tinv[4] = -t[4] * inv;
+        tinv[5] = t[0] * inv;
+        for (int i = 0; i < 2; i++) tinv[i * 4 + 3] = -(tinv[i * 4 + 0] * t[3] + tinv[i * 4 + 1] * t[7]);
+        tinv[10] = 1.0f;
+    }
+}
+
+static box_t instance_box(oracle_bvh *const *objects, const oracle_instance *I, int dim)
+{
+    box_t r = objects[I->object]->nodes[0].box;
+    if (!I->transformed) return r;
+    box_t b;
+    box_init(&b);
+    for (int i = 0; i < (1 << dim); i++) {
+        float p[3] = {0.0f, 0.0f, 0.0f}, q[3];
+        int temp = i;
+        for (int j = 0; j < dim; j++) {
+            p[j] = (temp % 2) == 0 ? r.pmin[j] : r.pmax[j];
+            temp /= 2;
+        }
+        xf_point(I->t, p, dim, q);
+        box_expand_point(&b, q, dim);
+    }
+    return b;
+}
+
+static void instance_centroid(oracle_bvh *const *objects, const oracle_instance *I, int dim, float *c)
+{
+    const oracle_bvh *h = objects[I->object];
+    float s[3] = {0.0f, 0.0f, 0.0f};
+    for (int slot = 0; slot < h->n_prims; slot++) {
+        const float *v[3];
+        for (int k = 0; k < h->k; k++) v[k] = vtx(h, slot, k);
+        float pc[3];
+        prim_centroid(v, h->k, dim, pc);
+        for (int d = 0; d < dim; d++) s[d] = s[d] + pc[d];
+    }
+    float m[3] = {0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < dim; d++) m[d] = s[d] / (float)h->n_prims;
+    if (I->transformed) xf_point(I->t, m, dim, m);
+    for (int d = 0; d < dim; d++) c[d] = m[d];
+}
+
+oracle_bvh *oracle_build_instances(int dim, oracle_bvh *const *objects, int n_objects, const oracle_instance *inst,
+                                   int n_inst, int heuristic, int leaf_size, int n_buckets)
+{
+    if ((dim != 2 && dim != 3) || n_inst <= 0) return NULL;
+    for (int i = 0; i < n_inst; i++)
+        if (inst[i].object < 0 || inst[i].object >= n_objects || objects[inst[i].object]->dim != dim) return NULL;
+    oracle_bvh *h = (oracle_bvh *)calloc(1, sizeof(oracle_bvh));
+    h->dim = dim;
+    h->k = 0;
+    h->n_prims = n_inst;
+    h->prim_orig = (int32_t *)malloc((size_t)n_inst * sizeof(int32_t));
+    build_ctx b;
+    memset(&b, 0, sizeof(b));
+    b.h = h;
+    b.heuristic = heuristic;
+    b.leaf_size = leaf_size;
+    b.n_buckets = n_buckets;
+    b.ref_boxes = (box_t *)malloc((size_t)n_inst * sizeof(box_t));
+    b.ref_cent = (float *)malloc((size_t)n_inst * dim * sizeof(float));
+    b.bkt_box = (box_t *)malloc((size_t)n_buckets * sizeof(box_t));
+    b.rbkt_box = (box_t *)malloc((size_t)n_buckets * sizeof(box_t));
+    b.bkt_cnt = (int *)malloc((size_t)n_buckets * sizeof(int));
+    b.rbkt_cnt = (int *)malloc((size_t)n_buckets * sizeof(int));
+    for (int i = 0; i < n_inst; i++) {
+        h->prim_orig[i] = i;
+        b.ref_boxes[i] = instance_box(objects, &inst[i], dim);
+        instance_centroid(objects, &inst[i], dim, b.ref_cent + (size_t)i * dim);
+    }
+    build_recursive(&b, -1, 0, 0, n_inst, 0);
+    free(b.ref_boxes); free(b.ref_cent); free(b.bkt_box); free(b.rbkt_box); free(b.bkt_cnt); free(b.rbkt_cnt);
+    return h;
+}
+
+/* Bvh::refit over aggregates (bvh.inl:548-599): the objects are refit first
+   (by the caller), then the instance boxes and the tree above them */
+static void refit_instances_recursive(oracle_bvh *h, oracle_bvh *const *objects, const oracle_instance *inst, int ni)
+{
+    node_t *n = &h->nodes[ni];
+    if (n->nrefs == 0) {
+        refit_instances_recursive(h, objects, inst, ni + 1);
+        refit_instances_recursive(h, objects, inst, ni + n->offset);
+        n->box = h->nodes[ni + 1].box;
+        box_expand_box(&n->box, &h->nodes[ni + n->offset].box, h->dim);
+    } else {
+        box_init(&n->box);
+        for (int p = 0; p < n->nrefs; p++) {
+            box_t ib = instance_box(objects, &inst[h->prim_orig[n->offset + p]], h->dim);
+            box_expand_box(&n->box, &ib, h->dim);
+        }
+    }
+}
+
+void oracle_refit_instances(oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst)
+{
+    if (tlas->n_nodes > 0) refit_instances_recursive(tlas, objects, inst, 0);
+}
+
+typedef struct {
+    const oracle_bvh *tlas;
+    oracle_bvh *const *objects;
+    const oracle_instance *inst;
+    const float *q, *r2, *o, *d, *tmax;
+    int record_normal, occlusion;
+    oracle_hit *out;
+    int32_t *object_out;
+} inst_ctx;
+
+/* TransformedAggregate::findClosestPoint (aggregate.slang:95-117); r2 inout */
+static int inst_cp(const inst_ctx *c, int ii, const float *x, float *r2, oracle_hit *hit, int *slot_out)
+{
+    const oracle_instance *I = &c->inst[ii];
+    const oracle_bvh *h = c->objects[I->object];
+    int dim = h->dim;
+    float xl[3] = {x[0], x[1], x[2]}, r2l = *r2;
+    if (I->transformed) {
+        xf_point(I->tinv, x, dim, xl);
+        r2l = xf_r2(I->tinv, x, xl, *r2, dim);
+    }
+    oracle_hit cand;
+    hit_default(&cand);
+    int slot = cp_gpu_one(h, xl, &r2l, &cand, NULL, NULL, NULL);
+    if (I->transformed) {
+        float tc[3];
+        xf_point(I->t, xl, dim, tc);
+        *r2 = xf_r2(I->t, xl, tc, r2l, dim);
+    } else {
+        *r2 = r2l;
+    }
+    if (slot < 0) return 0;
+    if (I->transformed) {
+        float p[3], e[3];
+        xf_point(I->t, cand.p, dim, p);
+        memcpy(cand.p, p, sizeof(p));
+        for (int i = 0; i < 3; i++) e[i] = p[i] - x[i];
+        cand.d = sqrtf(gdot(e, e, dim));
+    }
+    *hit = cand;
+    *slot_out = slot;
+    return 1;
+}
+
+static void inst_cp_range(void *vctx, int start, int end)
+{
+    inst_ctx *c = (inst_ctx *)vctx;
+    const oracle_bvh *T = c->tlas;
+    int dim = T->dim;
+    int st[BVH_MAX_DEPTH];
+    for (int qi = start; qi < end; qi++) {
+        float x[3] = {0, 0, 0};
+        for (int d = 0; d < dim; d++) x[d] = c->q[(size_t)qi * dim + d];
+        float r2 = c->r2[qi];
+        oracle_hit best, cand;
+        hit_default(&best);
+        int best_inst = -1, best_slot = -1, slot;
+        if (T->n_prims == 1) {
+            /* a single instance is the scene's aggregate itself (fcpw.inl:1088-1092) */
+            if (inst_cp(c, T->prim_orig[0], x, &r2, &cand, &slot)) {
+                best = cand;
+                best_inst = T->prim_orig[0];
+                best_slot = slot;
+            }
+        } else {
+            float d0, d1, d2, d3;
+            box_sqdist_gpu(&T->nodes[0].box, x, dim, &d0, &d1);
+            if (d0 <= r2) {
+                r2 = fminf(r2, d1);
+                int sp = 0;
+                st[0] = 0;
+                while (sp >= 0) {
+                    int ni = st[sp--];
+                    const node_t *n = &T->nodes[ni];
+                    box_sqdist_gpu(&n->box, x, dim, &d0, &d1);
+                    if (!(d0 <= r2)) continue;
+                    if (n->nrefs > 0) {
+                        for (int p = 0; p < n->nrefs; p++) {
+                            int ii = T->prim_orig[n->offset + p];
+                            if (inst_cp(c, ii, x, &r2, &cand, &slot)) {
+                                r2 = fminf(r2, cand.d * cand.d);
+                                best = cand;
+                                best_inst = ii;
+                                best_slot = slot;
+                            }
+                        }
+                    } else {
+                        int l = ni + 1, r = ni + n->offset;
+                        box_sqdist_gpu(&T->nodes[l].box, x, dim, &d0, &d1);
+                        int ol = d0 <= r2;
+                        r2 = fminf(r2, d1);
+                        box_sqdist_gpu(&T->nodes[r].box, x, dim, &d2, &d3);
+                        int orr = d2 <= r2;
+                        r2 = fminf(r2, d3);
+                        if (ol && orr) {
+                            int closer = l, other = r;
+                            if (d2 < d0) { closer = r; other = l; }
+                            st[++sp] = other;
+                            st[++sp] = closer;
+                        } else if (ol) {
+                            st[++sp] = l;
+                        } else if (orr) {
+                            st[++sp] = r;
+                        }
+                    }
+                }
+            }
+        }
+        if (best_inst >= 0 && c->record_normal) {
+            const oracle_instance *I = &c->inst[best_inst];
+            float n[3];
+            prim_normal_gpu(c->objects[I->object], best_slot, n);
+            if (I->transformed) xf_normal(I->tinv, n, dim, n);
+            memcpy(best.n, n, sizeof(n));
+        }
+        c->out[qi] = best;
+        if (c->object_out) c->object_out[qi] = best_inst >= 0 ? c->inst[best_inst].object : -1;
+    }
+}
+
+void oracle_instances_closest_points(const oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst,
+                                     const float *queries, const float *r2, int n, int record_normal, int nthreads,
+                                     oracle_hit *out, int32_t *object_out)
+{
+    inst_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.tlas = tlas;
+    c.objects = objects;
+    c.inst = inst;
+    c.q = queries;
+    c.r2 = r2;
+    c.record_normal = record_normal;
+    c.out = out;
+    c.object_out = object_out;
+    parallel_for(n, nthreads, inst_cp_range, &c);
+}
+
+/* TransformedAggregate::intersect (aggregate.slang:59-79); tmax inout */
+static int inst_ray(const inst_ctx *c, int ii, const float *o, const float *d, const float *dinv, float *tmax,
+                    oracle_hit *hit)
+{
+    const oracle_instance *I = &c->inst[ii];
+    const oracle_bvh *h = c->objects[I->object];
+    int dim = h->dim;
+    float ol[3] = {o[0], o[1], o[2]}, dl[3] = {d[0], d[1], d[2]}, il[3] = {dinv[0], dinv[1], dinv[2]};
+    float tl = *tmax;
+    if (I->transformed) {
+        /* transformRay(tInv, r) (transform.slang:22-29) and the Ray constructor's 1/d */
+        float s = *tmax < MAXF ? *tmax : 1.0f, e[3], m[3];
+        xf_point(I->tinv, o, dim, ol);
+        for (int i = 0; i < 3; i++) e[i] = o[i] + d[i] * s;
+        xf_point(I->tinv, e, dim, m);
+        for (int i = 0; i < 3; i++) dl[i] = m[i] - ol[i];
+        float nrm = sqrtf(gdot(dl, dl, dim));
+        for (int i = 0; i < 3; i++) dl[i] = dl[i] / nrm;
+        tl = *tmax < MAXF ? nrm : MAXF;
+        for (int i = 0; i < dim; i++) il[i] = 1.0f / dl[i];
+    }
+    oracle_hit cand;
+    hit_default(&cand);
+    int slot = ray_gpu_one(h, ol, dl, il, &tl, c->occlusion, &cand, NULL, NULL);
+    if (I->transformed) {
+        if (tl < MAXF) {
+            float a[3], e[3], b[3], dd[3];
+            xf_point(I->t, ol, dim, a);
+            for (int i = 0; i < 3; i++) e[i] = ol[i] + dl[i] * tl;
+            xf_point(I->t, e, dim, b);
+            for (int i = 0; i < 3; i++) dd[i] = b[i] - a[i];
+            *tmax = sqrtf(gdot(dd, dd, dim));
+        } else {
+            *tmax = MAXF;
+        }
+    } else {
+        *tmax = tl;
+    }
+    if (slot < 0) return 0;
+    if (I->transformed && !c->occlusion) {
+        float p[3], e[3], n[3];
+        xf_point(I->t, cand.p, dim, p);
+        memcpy(cand.p, p, sizeof(p));
+        xf_normal(I->tinv, cand.n, dim, n);
+        memcpy(cand.n, n, sizeof(n));
+        for (int i = 0; i < 3; i++) e[i] = p[i] - o[i];
+        cand.d = sqrtf(gdot(e, e, dim));
+    }
+    *hit = cand;
+    return 1;
+}
+
+static void inst_ray_range(void *vctx, int start, int end)
+{
+    inst_ctx *c = (inst_ctx *)vctx;
+    const oracle_bvh *T = c->tlas;
+    int dim = T->dim;
+    stack_entry st[BVH_MAX_DEPTH];
+    for (int qi = start; qi < end; qi++) {
+        float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, dinv[3] = {0, 0, 0};
+        for (int k = 0; k < dim; k++) {
+            o[k] = c->o[(size_t)qi * dim + k];
+            d[k] = c->d[(size_t)qi * dim + k];
+            dinv[k] = 1.0f / d[k];
+        }
+        float tmax = c->tmax[qi];
+        oracle_hit best, cand;
+        hit_default(&best);
+        int best_inst = -1;
+        if (T->n_prims == 1) {
+            if (inst_ray(c, T->prim_orig[0], o, d, dinv, &tmax, &cand)) {
+                best = cand;
+                best_inst = T->prim_orig[0];
+            }
+        } else {
+            float b0, b1, b2, b3;
+            if (box_ray_gpu(&T->nodes[0].box, o, d, dinv, tmax, dim, &b0, &b1)) {
+                int sp = 0, done = 0;
+                st[0].node = 0;
+                st[0].dist = b0;
+                while (sp >= 0 && !done) {
+                    int ni = st[sp].node;
+                    float dist = st[sp].dist;
+                    sp--;
+                    if (dist > tmax) continue;
+                    const node_t *n = &T->nodes[ni];
+                    if (n->nrefs > 0) {
+                        for (int p = 0; p < n->nrefs; p++) {
+                            int ii = T->prim_orig[n->offset + p];
+                            if (inst_ray(c, ii, o, d, dinv, &tmax, &cand)) {
+                                best = cand;
+                                best_inst = ii;
+                                if (c->occlusion) { done = 1; break; }
+                                tmax = fminf(tmax, cand.d);
+                            }
+                        }
+                    } else {
+                        int l = ni + 1, r = ni + n->offset;
+                        int hl = box_ray_gpu(&T->nodes[l].box, o, d, dinv, tmax, dim, &b0, &b1);
+                        int hr = box_ray_gpu(&T->nodes[r].box, o, d, dinv, tmax, dim, &b2, &b3);
+                        if (hl && hr) {
+                            int closer = l, other = r;
+                            if (b2 < b0) {
+                                float t = b0; b0 = b2; b2 = t;
+                                closer = r; other = l;
+                            }
+                            st[++sp].node = other; st[sp].dist = b2;
+                            st[++sp].node = closer; st[sp].dist = b0;
+                        } else if (hl) {
+                            st[++sp].node = l; st[sp].dist = b0;
+                        } else if (hr) {
+                            st[++sp].node = r; st[sp].dist = b2;
+                        }
+                    }
+                }
+            }
+        }
+        c->out[qi] = best;
+        if (c->object_out) c->object_out[qi] = best_inst >= 0 ? c->inst[best_inst].object : -1;
+    }
+}
+
+void oracle_instances_intersect(const oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst,
+                                const float *origins, const float *dirs, const float *tmax, int n, int occlusion,
+                                int nthreads, oracle_hit *out, int32_t *object_out)
+{
+    inst_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.tlas = tlas;
+    c.objects = objects;
+    c.inst = inst;
+    c.o = origins;
+    c.d = dirs;
+    c.tmax = tmax;
+    c.occlusion = occlusion;
+    c.out = out;
+    c.object_out = object_out;
+    parallel_for(n, nthreads, inst_ray_range, &c);
 }

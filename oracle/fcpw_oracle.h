@@ -19,6 +19,10 @@
  *   - GPU closest silhouette point  include/fcpw/gpu/bvh.slang:503-646,
  *                                   include/fcpw/gpu/geometry.slang:490-708,
  *                                   include/fcpw/gpu/bounding-volumes.slang:122-236
+ *   - several objects, instances    include/fcpw/fcpw.inl:1052-1126,
+ *                                   include/fcpw/core/primitive.h:289-440,
+ *                                   include/fcpw/gpu/aggregate.slang:45-145,
+ *                                   include/fcpw/gpu/transform.slang:1-56
  *   - primitive routines            include/fcpw/geometry/triangles.inl:12-32,222-260,374-487,
  *                                   include/fcpw/geometry/line_segments.inl:11-28,140-175,200-255,
  *                                   include/fcpw/gpu/geometry.slang:37-103,202-332
@@ -130,6 +134,33 @@ void oracle_closest_silhouettes_gpuorder(const oracle_bvh *h, const float *spher
 int oracle_closest_silhouette_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,
                                   int n_prims, const float *queries, int n, double tol, double tol_d, int nthreads,
                                   double *d_strict, double *d_loose, double *p_strict);
+
+/* ---- scenes of several objects and transformed instances ----
+   Scene::build's instance list (fcpw.inl:1066-1085): per object, in order, one
+   untransformed instance when it has no instance transforms, else one
+   transformed instance per transform.  t / tinv: 3x4 row-major affine matrices
+   (2D: rows 0-1, columns 0, 1 and 3). */
+typedef struct {
+    int32_t object;
+    int32_t transformed;
+    float t[12];
+    float tinv[12];
+} oracle_instance;
+/* Transform::inverse of an affine transform (cofactor inverse of the linear part) */
+void oracle_invert_transform(int dim, const float *t, float *tinv);
+/* the BVH over the instances (makeAggregate over aggregates, fcpw.inl:1103-1112);
+   its "primitives" are instance ids: prim_order gives the instance of each leaf slot */
+oracle_bvh *oracle_build_instances(int dim, oracle_bvh *const *objects, int n_objects, const oracle_instance *inst,
+                                   int n_inst, int heuristic, int leaf_size, int n_buckets);
+/* Bvh::refit over aggregates, after the objects themselves were refit */
+void oracle_refit_instances(oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst);
+/* GPU order at both levels; object_out (optional) = Interaction::objectIndex, -1 on a miss */
+void oracle_instances_closest_points(const oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst,
+                                     const float *queries, const float *r2, int n, int record_normal, int nthreads,
+                                     oracle_hit *out, int32_t *object_out);
+void oracle_instances_intersect(const oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst,
+                                const float *origins, const float *dirs, const float *tmax, int n, int occlusion,
+                                int nthreads, oracle_hit *out, int32_t *object_out);
 
 /* FP64 brute force on raw geometry (positions in user order) */
 void oracle_brute_closest_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,

@@ -1,0 +1,214 @@
+"""Oracle for scenes of several objects and transformed instances
+(oracle.Instances: fcpw.inl:1052-1126, primitive.h:289-440,
+aggregate.slang:45-145, transform.slang:1-56), pinned by
+  * the FP64 brute force over the world-space soup every instance spans: d
+    within 1e-5 relative (floor below), the same (object, primitive) except
+    where FP64 calls the two candidates equidistant;
+  * exact identities: one instance with the identity transform returns the
+    untransformed object's answer bit for bit, and a power-of-two scale with no
+    rotation returns exactly that scale times the object's answer at the mapped
+    query;
+  * the tree over the instances: every instance in one leaf, boxes nested bit
+    for bit, each instance box holding its transformed vertices;
+  * the transform inverse against numpy's FP64 inverse.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import meshes, scenes
+
+# distances: 1e-5 relative plus 4 ulp of the largest world coordinate (the
+# FP32 transform of a point rounds each coordinate once more, ~8 at most here)
+REL, FLOOR = 1e-5, 1e-4
+EPS = float(np.finfo(np.float32).eps)
+
+
+def d_tolerance(W, d64):
+    return REL * d64 + 4 * EPS * float(np.abs(W).max())
+
+
+def build(name, heuristic=oracle.SURFACE_AREA):
+    objs, tr = scenes.scene(name)
+    dim = objs[0][0].shape[1]
+    bv = [oracle.Bvh(V, F) for V, F in objs]
+    inst = oracle.make_instances(dim, len(objs), tr)
+    return objs, tr, bv, inst, oracle.Instances(bv, inst, heuristic=heuristic)
+
+
+def _tie(W, WF, q, a, b):
+    da = oracle.prim_dist_f64(W, WF, q, a)
+    db = oracle.prim_dist_f64(W, WF, q, b)
+    return np.abs(da - db) <= 1e-5 * np.maximum(da, db) + 2e-6
+
+
+def _soup_index(obj, prim, s_obj, s_prim, n_instances_of):
+    """candidate soup primitives for each (object, primitive) answer"""
+    return [np.nonzero((s_obj == o) & (s_prim == p))[0] for o, p in zip(obj, prim)]
+
+
+@pytest.mark.parametrize("name", scenes.NAMES)
+def test_closest_points_match_fp64_world_brute_force(name):
+    objs, tr, bv, inst, I = build(name)
+    W, WF, s_obj, s_prim = scenes.world_soup(objs, tr)
+    q = scenes.queries(objs, tr, 3000, seed=31)
+    hits, obj = I.closest_points(q)
+    d64, p64, i64 = oracle.brute_closest_f64(W, WF, q)
+    assert (obj >= 0).all()
+    err = np.abs(hits["d"].astype(np.float64) - d64)
+    assert (err <= d_tolerance(W, d64)).all(), f"max abs error {err.max():.3g}"
+    # the same (object, primitive) as the FP64 answer, except at FP64 ties
+    mism = (obj != s_obj[i64]) | (hits["index"] != s_prim[i64])
+    if mism.any():
+        # find a soup primitive of the reported (object, primitive): the instance
+        # whose copy is nearest in FP64
+        cand = []
+        for k in np.nonzero(mism)[0]:
+            js = np.nonzero((s_obj == obj[k]) & (s_prim == hits["index"][k]))[0]
+            dj = oracle.prim_dist_f64(W, WF, np.repeat(q[k:k + 1], len(js), 0), js)
+            cand.append(js[np.argmin(dj)])
+        cand = np.asarray(cand, np.int32)
+        assert _tie(W, WF, q[mism], cand, i64[mism]).all()
+    dim = q.shape[1]
+    np.testing.assert_allclose(hits["p"][~mism, :dim], p64[~mism, :dim], atol=2e-5 * max(1.0, float(np.abs(W).max())))
+
+
+@pytest.mark.parametrize("name", ["mixed3d", "many3d", "mixed2d", "many2d"])
+def test_rays_match_fp64_world_brute_force(name):
+    objs, tr, bv, inst, I = build(name)
+    W, WF, s_obj, s_prim = scenes.world_soup(objs, tr)
+    o, d = scenes.rays(objs, tr, 3000, seed=32)
+    hits, obj = I.intersect(o, d)
+    t64, i64 = oracle.brute_ray_f64(W, WF, o, d)
+    hit64 = i64 >= 0
+    hit = obj >= 0
+    # hit/miss agree except for grazing rays
+    assert (hit != hit64).mean() < 0.002
+    both = hit & hit64
+    # unit directions: the parametric t of untransformed objects and the
+    # Euclidean distance of transformed instances coincide; the bound is the
+    # single-object one (|dt| / max(t, 1) < 1e-4)
+    err = np.abs(hits["d"][both] - t64[both]) / np.maximum(t64[both], 1.0)
+    assert err.max() < 1e-4
+    same = (obj[both] == s_obj[i64[both]]) & (hits["index"][both] == s_prim[i64[both]])
+    assert same.mean() > 0.995
+
+
+def test_identity_instance_is_the_object_bit_for_bit():
+    V, F = meshes.small_torus()
+    b = oracle.Bvh(V, F)
+    inst = oracle.make_instances(3, 1, [[np.eye(4)]])
+    I = oracle.Instances([b], inst)
+    q = meshes.queries_around(V, 4000, seed=33)
+    hits, obj = I.closest_points(q, record_normal=True)
+    ref = b.closest_points_gpuorder(q, record_normal=True)
+    for f in ("d", "p", "uv", "index"):
+        np.testing.assert_array_equal(hits[f], ref[f])
+    # the mapped normal is normalized once more (transform.slang:49): an ulp apart
+    np.testing.assert_allclose(hits["n"], ref["n"], rtol=0, atol=2.5e-7)
+    assert (obj == 0).all()
+    o, d = meshes.rays_from(V, 4000)
+    rh, robj = I.intersect(o, d)
+    rref = b.intersect_gpuorder(o, d)
+    # a mapped ray is re-normalized (transform.slang:22-29), so its direction
+    # and hit point move by an ulp; transformed instances report |p - o|, which
+    # with unit directions is t to a few ulp
+    hitm = rref["index"] >= 0
+    assert (rh["index"] != rref["index"]).mean() < 0.002
+    same = hitm & (rh["index"] == rref["index"])
+    np.testing.assert_allclose(rh["p"][same], rref["p"][same], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(rh["d"][same], rref["d"][same], rtol=2e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_power_of_two_scale_is_exact(dim):
+    V, F = meshes.small_icosphere(2) if dim == 3 else meshes.small_star(512)
+    b = oracle.Bvh(V, F)
+    M = np.eye(dim + 1)
+    M[:dim, :dim] *= 2.0
+    I = oracle.Instances([b], oracle.make_instances(dim, 1, [[M]]))
+    q = meshes.queries_around(2.0 * V, 3000, seed=34)
+    hits, _ = I.closest_points(q)
+    ref = b.closest_points_gpuorder(q * np.float32(0.5))
+    np.testing.assert_array_equal(hits["index"], ref["index"])
+    np.testing.assert_array_equal(hits["d"], np.float32(2.0) * ref["d"])
+    np.testing.assert_array_equal(hits["p"], np.float32(2.0) * ref["p"])
+
+
+def test_untransformed_objects_match_one_soup():
+    """several objects without transforms answer as one soup of them all (FP32
+    GPU order on the merged soup), distances bit for bit"""
+    objs, tr = scenes.scene("objects3d")
+    bv = [oracle.Bvh(V, F) for V, F in objs]
+    I = oracle.Instances(bv, oracle.make_instances(3, len(objs), tr))
+    W, WF, s_obj, s_prim = scenes.world_soup(objs, tr)
+    merged = oracle.Bvh(W, WF)
+    q = scenes.queries(objs, tr, 4000, seed=35)
+    hits, obj = I.closest_points(q)
+    ref = merged.closest_points_gpuorder(q)
+    np.testing.assert_array_equal(hits["d"], ref["d"])
+    same = (obj == s_obj[ref["index"]]) & (hits["index"] == s_prim[ref["index"]])
+    if not same.all():
+        assert _tie(W, WF, q[~same], ref["index"][~same],
+                    np.asarray([np.nonzero((s_obj == o) & (s_prim == p))[0][0]
+                                for o, p in zip(obj[~same], hits["index"][~same])], np.int32)).all()
+
+
+@pytest.mark.parametrize("name", ["mixed3d", "many3d", "mixed2d", "many2d"])
+def test_instance_tree_structure(name):
+    objs, tr, bv, inst, I = build(name)
+    dim = objs[0][0].shape[1]
+    pmin, pmax, nrefs, offset = I.nodes()
+    order = I.prim_order()
+    assert sorted(order.tolist()) == list(range(len(inst)))
+    covered = np.zeros(len(inst), np.int32)
+    for i in range(I.n_nodes):
+        if nrefs[i] > 0:
+            covered[offset[i]:offset[i] + nrefs[i]] += 1
+            assert nrefs[i] <= 4
+            for j in order[offset[i]:offset[i] + nrefs[i]]:
+                V, _ = objs[inst["object"][j]]
+                if inst["transformed"][j]:
+                    t = inst["t"][j].reshape(3, 4).astype(np.float64)
+                    Wv = V.astype(np.float64) @ t[:dim, :dim].T + t[:dim, 3]
+                else:
+                    Wv = V.astype(np.float64)
+                assert (Wv >= pmin[i, :dim] - 1e-6).all() and (Wv <= pmax[i, :dim] + 1e-6).all()
+        else:
+            l, r = i + 1, i + offset[i]
+            np.testing.assert_array_equal(pmin[i], np.minimum(pmin[l], pmin[r]))
+            np.testing.assert_array_equal(pmax[i], np.maximum(pmax[l], pmax[r]))
+    assert (covered == 1).all()
+
+
+def test_refit_after_no_change_is_identity_and_after_motion_contains():
+    objs, tr, bv, inst, I = build("many3d")
+    before = I.nodes()
+    I.refit()
+    for a, b in zip(before, I.nodes()):
+        np.testing.assert_array_equal(a, b)
+    rng = np.random.default_rng(36)
+    V0, F0 = objs[0]
+    V2 = (V0 + rng.normal(0, 0.05, V0.shape)).astype(np.float32)
+    bv[0].update_vertices(V2)
+    I.refit()
+    objs2 = [(V2, F0), objs[1]]
+    W, WF, _, _ = scenes.world_soup(objs2, tr)
+    q = scenes.queries(objs2, tr, 2000, seed=37)
+    hits, _ = I.closest_points(q)
+    d64, _, _ = oracle.brute_closest_f64(W, WF, q)
+    assert (np.abs(hits["d"].astype(np.float64) - d64) <= d_tolerance(W, d64)).all()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_invert_transform(dim):
+    rng = np.random.default_rng(38)
+    for _ in range(20):
+        R = scenes.rotation(dim, __import__("fcpw_amd").synth.Rng(int(rng.integers(1, 1 << 30))))
+        M = scenes.affine(dim, R, rng.uniform(0.2, 3.0), rng.uniform(-5, 5, dim))
+        t = oracle.transform_3x4(dim, M)
+        ti = oracle.invert_transform(dim, t).reshape(3, 4).astype(np.float64)
+        Mi = np.linalg.inv(t.reshape(3, 4).astype(np.float64)[[*range(dim)]][:, [*range(dim), 3]].tolist()
+                           + [[0.0] * dim + [1.0]])
+        np.testing.assert_allclose(ti[:dim, :dim], Mi[:dim, :dim], rtol=2e-6, atol=2e-6)
+        np.testing.assert_allclose(ti[:dim, 3], Mi[:dim, dim], rtol=2e-6, atol=2e-5)
