@@ -97,7 +97,9 @@ class _GPUSceneMixin:
 
     def find_closest_points(self, *args, **kwargs):
         """find_closest_points(query_points, squared_max_radii, interactions, record_normals=False)
-        find_closest_points(bounding_spheres, interactions, record_normals=False)"""
+        find_closest_points(bounding_spheres, interactions, record_normals=False)
+        object_indices= (optional, int32 array with one entry per query) receives each
+        result's object index (Interaction::objectIndex) or -1 on a miss."""
         if "bounding_spheres" in kwargs or (args and _is_list(args[0], GPUBoundingSphereList)):
             return self._find_closest_points_spheres(*args, **kwargs)
         return self._find_closest_points_arrays(*args, **kwargs)
@@ -120,17 +122,20 @@ class _GPUSceneMixin:
         return self._find_closest_silhouette_points_arrays(*args, **kwargs)
 
     # -- device-resident queries (torch tensors) ---------------------------------
-    def find_closest_points_device(self, spheres, out, record_normals=False, stream=None):
+    def find_closest_points_device(self, spheres, out, record_normals=False, stream=None, object_index=None):
         """spheres: (n, 4) float32 CUDA tensor of GPUBoundingSphere records;
-        out: (n, 10) float32 CUDA tensor receiving GPUInteraction records."""
+        out: (n, 10) float32 CUDA tensor receiving GPUInteraction records;
+        object_index: optional (n,) int32 CUDA tensor receiving object indices."""
         n = spheres.shape[0]
         self._closest_points_device(spheres.data_ptr(), n, bool(record_normals), out.data_ptr(),
-                                    _stream_handle(spheres, stream))
+                                    _stream_handle(spheres, stream), _object_ptr(object_index, n))
 
-    def intersect_device(self, rays, out, check_for_occlusion=False, stream=None):
-        """rays: (n, 10) float32 CUDA tensor of GPURay records; out: (n, 10) float32."""
-        self._intersect_device(rays.data_ptr(), rays.shape[0], bool(check_for_occlusion), out.data_ptr(),
-                               _stream_handle(rays, stream))
+    def intersect_device(self, rays, out, check_for_occlusion=False, stream=None, object_index=None):
+        """rays: (n, 10) float32 CUDA tensor of GPURay records; out: (n, 10) float32;
+        object_index: optional (n,) int32 CUDA tensor receiving object indices."""
+        n = rays.shape[0]
+        self._intersect_device(rays.data_ptr(), n, bool(check_for_occlusion), out.data_ptr(),
+                               _stream_handle(rays, stream), _object_ptr(object_index, n))
 
     def find_closest_silhouette_points_device(self, spheres, flip_normal_orientation, out, squared_min_radius=0.0,
                                               precision=1e-3, stream=None):
@@ -160,6 +165,14 @@ class _GPUSceneMixin:
 
     def info(self, key):
         return self._info(key)
+
+
+def _object_ptr(t, n):
+    if t is None:
+        return 0
+    if t.shape[0] != n or not t.is_contiguous() or t.element_size() != 4:
+        raise ValueError("object_index must be a contiguous (n,) 32-bit tensor")
+    return t.data_ptr()
 
 
 def _stream_handle(t, stream):

@@ -303,8 +303,8 @@ int fcpw_hip_batch_create(fcpw_hip_ctx *const *scenes, int n_scenes, fcpw_hip_ba
         return FCPW_HIP_ERR_INVALID_ARGUMENT;
     }
     for (int s = 0; s < n_scenes; s++) {
-        if (!scenes[s] || !scenes[s]->uploaded) {
-            g_create_error = "batch_create: scene " + std::to_string(s) + " has no uploaded BVH";
+        if (!scenes[s] || !scenes[s]->uploaded || scenes[s]->instanced) {
+            g_create_error = "batch_create: scene " + std::to_string(s) + " has no uploaded single-object BVH";
             return FCPW_HIP_ERR_NOT_UPLOADED;
         }
         if (scenes[s]->dim != scenes[0]->dim || scenes[s]->device != scenes[0]->device) {

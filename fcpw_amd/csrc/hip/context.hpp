@@ -31,6 +31,15 @@ struct fcpw_hip_ctx {
     uint2 *sil_leaf = nullptr;
     bool has_silhouettes = false;
     int n_sil_refs = 0;
+    // scene of several objects (fcpw_hip_upload_instances): records / root /
+    // prim_index then hold the tree over the instances (prim_index: leaf slot ->
+    // instance); the objects are contexts of their own
+    bool instanced = false;
+    std::vector<fcpw_hip_ctx *> inst_objects;
+    std::vector<uint64_t> inst_generations;
+    DevInstance *inst = nullptr;
+    DevScene *obj_table = nullptr;
+    int n_instances = 0;
     std::vector<int32_t> node_slot;     // flat node -> child slot (record * 2 + side), root = 2 * n_records
     std::vector<int32_t> node_leaf_off; // flat node -> first leaf slot, -1 for internal nodes
     // staging for the host-pointer entry points
@@ -89,10 +98,16 @@ static void free_silhouettes(fcpw_hip_ctx *c)
 static void free_scene(fcpw_hip_ctx *c)
 {
     void *bufs[] = {c->records, c->root, c->prims, c->prim_index, c->prim_vids, c->V,
-                    c->rec_parent, c->leaf_refs, c->leaf_links, c->counters};
+                    c->rec_parent, c->leaf_refs, c->leaf_links, c->counters, c->inst, c->obj_table};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_silhouettes(c);
+    c->inst = nullptr;
+    c->obj_table = nullptr;
+    c->instanced = false;
+    c->inst_objects.clear();
+    c->inst_generations.clear();
+    c->n_instances = 0;
     c->records = c->root = c->prims = nullptr;
     c->prim_index = c->prim_vids = nullptr;
     c->V = nullptr;

@@ -58,6 +58,14 @@ struct DevScene {
                                // loop iterations, leaf passes, 64-item evaluation groups
 };
 
+// one instance of a scene of several objects (fcpw_hip_instance on the device):
+// rows of t and of its inverse, the object drawn and whether it is transformed
+struct DevInstance {
+    float4 t[3];
+    float4 ti[3];
+    int32_t object, transformed, pad0, pad1;
+};
+
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
 
 // ---------------------------------------------------------------------------
@@ -256,8 +264,9 @@ struct CpResult {
     uint32_t n_iter, n_pass, n_group; // wave loop iterations, leaf passes, 64-item evaluation groups (wave-uniform)
 };
 
+// r2 is updated in place, as the reference's inout BoundingSphere
 template <int DIM, int BLOCK, int K>
-__device__ __forceinline__ void closest_point_query(const DevScene &S, typename Geo<DIM>::vec x, float r2,
+__device__ __forceinline__ void closest_point_query(const DevScene &S, typename Geo<DIM>::vec x, float &r2,
                                                     CpResult &res)
 {
     using G = Geo<DIM>;
@@ -397,9 +406,10 @@ struct RayGeo<2> {
     }
 };
 
+// tmax is updated in place, as the reference's inout Ray
 template <int DIM, int BLOCK, int K>
 __device__ __forceinline__ void ray_query(const DevScene &S, typename Geo<DIM>::vec o, typename Geo<DIM>::vec d,
-                                          typename Geo<DIM>::vec dinv, float tmax, bool occlusion,
+                                          typename Geo<DIM>::vec dinv, float &tmax, bool occlusion,
                                           RayResult &res)
 {
     using G = Geo<DIM>;
@@ -1108,36 +1118,11 @@ __global__ void k_rebuild_prims(float4 *prims, const int32_t *vids, const float 
 // involve min, max and the +-FLT_EPSILON inflation, so the result is bit-equal
 // to the reference refit (bvh.slang:27-141, bvh.inl:548-581) in any order.
 // The second arrival resets the counter, leaving all counters 0 for the next call.
+// refit_upward is the climb from one leaf box (ref = its slot, -1 for a root leaf).
 template <int DIM>
-__global__ void k_refit(float4 *records, float4 *root, const float4 *prims, const int32_t *leaf_refs,
-                        const uint32_t *leaf_links, const int32_t *rec_parent, unsigned int *counters, int n_leaves)
+__device__ __forceinline__ void refit_upward(float4 *records, float4 *root, const int32_t *rec_parent,
+                                             unsigned int *counters, int32_t ref, float lo[3], float hi[3])
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_leaves) return;
-    uint32_t link = leaf_links[j];
-    uint32_t off = link & OFF_MASK, cnt = ((link >> CNT_SHIFT) & CNT_MASK) + 1u;
-    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (uint32_t k = 0; k < cnt; k++) {
-        if (DIM == 3) {
-            const float4 *t = prims + (size_t)(off + k) * 3;
-            float4 a = t[0], b = t[1], c = t[2];
-            // Triangle.getBoundingBox (geometry.slang:342-346)
-            lo[0] = fminf(lo[0], fminf(fminf(a.x, b.x), c.x) - FLT_EPSILON);
-            lo[1] = fminf(lo[1], fminf(fminf(a.y, b.y), c.y) - FLT_EPSILON);
-            lo[2] = fminf(lo[2], fminf(fminf(a.z, b.z), c.z) - FLT_EPSILON);
-            hi[0] = fmaxf(hi[0], fmaxf(fmaxf(a.x, b.x), c.x) + FLT_EPSILON);
-            hi[1] = fmaxf(hi[1], fmaxf(fmaxf(a.y, b.y), c.y) + FLT_EPSILON);
-            hi[2] = fmaxf(hi[2], fmaxf(fmaxf(a.z, b.z), c.z) + FLT_EPSILON);
-        } else {
-            float4 s = prims[off + k];
-            // LineSegment.getBoundingBox (geometry.slang:112-116)
-            lo[0] = fminf(lo[0], fminf(s.x, s.z) - FLT_EPSILON);
-            lo[1] = fminf(lo[1], fminf(s.y, s.w) - FLT_EPSILON);
-            hi[0] = fmaxf(hi[0], fmaxf(s.x, s.z) + FLT_EPSILON);
-            hi[1] = fmaxf(hi[1], fmaxf(s.y, s.w) + FLT_EPSILON);
-        }
-    }
-    int32_t ref = leaf_refs[j];
     while (true) {
         if (ref < 0) { // the box belongs to the root
             if (DIM == 3) {
@@ -1176,6 +1161,38 @@ __global__ void k_refit(float4 *records, float4 *root, const float4 *prims, cons
         }
         ref = rec_parent[rec];
     }
+}
+
+template <int DIM>
+__global__ void k_refit(float4 *records, float4 *root, const float4 *prims, const int32_t *leaf_refs,
+                        const uint32_t *leaf_links, const int32_t *rec_parent, unsigned int *counters, int n_leaves)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_leaves) return;
+    uint32_t link = leaf_links[j];
+    uint32_t off = link & OFF_MASK, cnt = ((link >> CNT_SHIFT) & CNT_MASK) + 1u;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t k = 0; k < cnt; k++) {
+        if (DIM == 3) {
+            const float4 *t = prims + (size_t)(off + k) * 3;
+            float4 a = t[0], b = t[1], c = t[2];
+            // Triangle.getBoundingBox (geometry.slang:342-346)
+            lo[0] = fminf(lo[0], fminf(fminf(a.x, b.x), c.x) - FLT_EPSILON);
+            lo[1] = fminf(lo[1], fminf(fminf(a.y, b.y), c.y) - FLT_EPSILON);
+            lo[2] = fminf(lo[2], fminf(fminf(a.z, b.z), c.z) - FLT_EPSILON);
+            hi[0] = fmaxf(hi[0], fmaxf(fmaxf(a.x, b.x), c.x) + FLT_EPSILON);
+            hi[1] = fmaxf(hi[1], fmaxf(fmaxf(a.y, b.y), c.y) + FLT_EPSILON);
+            hi[2] = fmaxf(hi[2], fmaxf(fmaxf(a.z, b.z), c.z) + FLT_EPSILON);
+        } else {
+            float4 s = prims[off + k];
+            // LineSegment.getBoundingBox (geometry.slang:112-116)
+            lo[0] = fminf(lo[0], fminf(s.x, s.z) - FLT_EPSILON);
+            lo[1] = fminf(lo[1], fminf(s.y, s.w) - FLT_EPSILON);
+            hi[0] = fmaxf(hi[0], fmaxf(s.x, s.z) + FLT_EPSILON);
+            hi[1] = fmaxf(hi[1], fmaxf(s.y, s.w) + FLT_EPSILON);
+        }
+    }
+    refit_upward<DIM>(records, root, rec_parent, counters, leaf_refs[j], lo, hi);
 }
 
 // Adjoint of d(x, mesh): recompute p and its barycentric weights from the saved

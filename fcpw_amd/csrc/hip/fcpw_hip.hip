@@ -1,11 +1,15 @@
 // fcpw_hip.hip -- MI355X (gfx950) query backend for FCPW behind the C ABI in
 // include/fcpw_hip.h: the single-scene entry points (upload, refit, queries,
-// adjoint, options) and, from batch.hpp, the batched multi-scene launch.
+// adjoint, options), from batch.hpp the batched multi-scene launch and from
+// instances.hpp scenes of several objects and transformed instances.
 // Device code lives in device.hpp, the per-scene context in context.hpp.
 #include "device.hpp"
 #include "silhouette.hpp"
 #include "context.hpp"
 #include "gpu_build.hpp"
+
+// scenes of several objects (instances.hpp)
+static int instances_refit(fcpw_hip_ctx *c, hipStream_t s);
 
 // in-place exclusive scan of n uint32 values (segments of SCAN_SEG, at most 16384 of them)
 static int exclusive_scan(fcpw_hip_ctx *c, uint32_t *v, int64_t n, uint32_t *sums, hipStream_t s)
@@ -330,52 +334,64 @@ static int check_ready(fcpw_hip_ctx *c, const char *fn)
     return FCPW_HIP_OK;
 }
 
-int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin, const float *pmax,
-                        const int32_t *nrefs, const int32_t *offset, int n_prims, const int32_t *prim_vertices,
-                        const int32_t *prim_index, int n_vertices, const float *positions)
+// entry points served for one object's tree only
+static int check_single(fcpw_hip_ctx *c, const char *fn)
 {
-    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
-    if (dim != 2 && dim != 3) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: dim must be 2 or 3, got %d", dim);
-    if (n_nodes <= 0 || n_prims <= 0 || n_vertices <= 0 || !pmin || !pmax || !nrefs || !offset || !prim_vertices ||
-        !prim_index || !positions)
-        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: empty BVH or NULL array");
-    if ((size_t)n_prims > (size_t)OFF_MASK)
-        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: %d primitives exceed the 2^27 leaf-slot range", n_prims);
+    int st = check_ready(c, fn);
+    if (st) return st;
+    if (c->instanced)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "%s: not supported on a scene of several objects or instances", fn);
+    return FCPW_HIP_OK;
+}
 
+} // extern "C"
+
+// The child-pair records of a flat DFS tree (the layout fcpw_hip_upload_bvh
+// documents) with its refit schedule; validation errors name fn.
+struct TreeRecords {
+    std::vector<float4> rec;
+    float4 root[2];
+    std::vector<int32_t> parent, leaf_refs;
+    std::vector<uint32_t> leaf_links;
+    std::vector<int32_t> node_slot, node_leaf_off;
+    int n_rec = 0, n_leaf = 0, max_depth = 0;
+};
+
+static int tree_records(fcpw_hip_ctx *c, const char *fn, int dim, int n_nodes, const float *pmin, const float *pmax,
+                        const int32_t *nrefs, const int32_t *offset, int n_prims, TreeRecords &T)
+{
     // validate the reference layout and number internal nodes in DFS order
     std::vector<int32_t> rec_of(n_nodes, -1);
     int n_rec = 0, n_leaf = 0;
     for (int i = 0; i < n_nodes; i++) {
         if (nrefs[i] > 0) {
             if (nrefs[i] > MAX_LEAF_PRIMS)
-                FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: leaf %d holds %d primitives (at most %d)", i, nrefs[i],
+                FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "%s: leaf %d holds %d primitives (at most %d)", fn, i, nrefs[i],
                      MAX_LEAF_PRIMS);
             if (offset[i] < 0 || (int64_t)offset[i] + nrefs[i] > n_prims)
-                FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: leaf %d primitive range out of bounds", i);
+                FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: leaf %d primitive range out of bounds", fn, i);
             n_leaf++;
         } else if (nrefs[i] == 0) {
             if (offset[i] <= 1 || (int64_t)i + offset[i] >= n_nodes)
-                FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: internal node %d has a bad child offset %d", i,
+                FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: internal node %d has a bad child offset %d", fn, i,
                      offset[i]);
             rec_of[i] = n_rec++;
         } else {
-            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: node %d has a negative primitive count", i);
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: node %d has a negative primitive count", fn, i);
         }
     }
-    if (n_leaf != n_rec + 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: not a binary tree");
-    for (int64_t k = 0; k < (int64_t)n_prims * dim; k++)
-        if (prim_vertices[k] < 0 || prim_vertices[k] >= n_vertices)
-            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: vertex id out of range");
+    if (n_leaf != n_rec + 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: not a binary tree", fn);
 
     const int REC = dim == 3 ? 4 : 3;
-    std::vector<float4> rec((size_t)std::max(n_rec, 1) * REC, make_float4(0, 0, 0, 0));
-    std::vector<int32_t> parent((size_t)std::max(n_rec, 1), -1);
-    std::vector<int32_t> leaf_refs;
-    std::vector<uint32_t> leaf_links;
-    leaf_refs.reserve(n_leaf);
-    leaf_links.reserve(n_leaf);
+    T.rec.assign((size_t)std::max(n_rec, 1) * REC, make_float4(0, 0, 0, 0));
+    T.parent.assign((size_t)std::max(n_rec, 1), -1);
+    T.leaf_refs.clear();
+    T.leaf_links.clear();
+    T.leaf_refs.reserve(n_leaf);
+    T.leaf_links.reserve(n_leaf);
     std::vector<int32_t> depth(n_nodes, 0);
     int max_depth = 0;
+    auto &rec = T.rec;
 
     auto put_slot = [&](int r, int slot, int node, uint32_t link) {
         const float *lo = pmin + (size_t)node * 3, *hi = pmax + (size_t)node * 3;
@@ -388,49 +404,109 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
             else rec[(size_t)r * 3 + 2].y = bits_as_float(link);
         }
     };
-    std::vector<int32_t> node_slot((size_t)n_nodes, 2 * n_rec), node_leaf_off((size_t)n_nodes, -1);
+    T.node_slot.assign((size_t)n_nodes, 2 * n_rec);
+    T.node_leaf_off.assign((size_t)n_nodes, -1);
     for (int i = 0; i < n_nodes; i++)
-        if (nrefs[i] > 0) node_leaf_off[i] = offset[i];
+        if (nrefs[i] > 0) T.node_leaf_off[i] = offset[i];
     for (int i = 0; i < n_nodes; i++) {
         if (nrefs[i] > 0) continue;
         int r = rec_of[i];
         int kids[2] = {i + 1, i + offset[i]};
-        for (int s = 0; s < 2; s++) {
-            int k = kids[s];
-            node_slot[k] = r * 2 + s;
+        for (int sd = 0; sd < 2; sd++) {
+            int k = kids[sd];
+            T.node_slot[k] = r * 2 + sd;
             depth[k] = depth[i] + 1;
             max_depth = std::max(max_depth, depth[k]);
             bool leaf = nrefs[k] > 0;
             uint32_t link = make_link(leaf, offset[k], nrefs[k], rec_of, k);
-            put_slot(r, s, k, link);
+            put_slot(r, sd, k, link);
             if (leaf) {
-                leaf_refs.push_back(r * 2 + s);
-                leaf_links.push_back(link);
+                T.leaf_refs.push_back(r * 2 + sd);
+                T.leaf_links.push_back(link);
             } else {
-                parent[rec_of[k]] = r * 2 + s;
+                T.parent[rec_of[k]] = r * 2 + sd;
             }
         }
     }
     if (max_depth > STACK_TOTAL - 1)
-        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: tree depth %d exceeds %d", max_depth, STACK_TOTAL - 1);
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "%s: tree depth %d exceeds %d", fn, max_depth, STACK_TOTAL - 1);
 
-    float4 root[2];
     {
         bool leaf = nrefs[0] > 0;
         uint32_t link = make_link(leaf, offset[0], nrefs[0], rec_of, 0);
         if (leaf) {
-            leaf_refs.push_back(-1);
-            leaf_links.push_back(link);
+            T.leaf_refs.push_back(-1);
+            T.leaf_links.push_back(link);
         }
         if (dim == 3) {
-            root[0] = make_float4(pmin[0], pmin[1], pmin[2], bits_as_float(link));
-            root[1] = make_float4(pmax[0], pmax[1], pmax[2], 0.0f);
+            T.root[0] = make_float4(pmin[0], pmin[1], pmin[2], bits_as_float(link));
+            T.root[1] = make_float4(pmax[0], pmax[1], pmax[2], 0.0f);
         } else {
-            root[0] = make_float4(pmin[0], pmin[1], pmax[0], pmax[1]);
-            root[1] = make_float4(bits_as_float(link), 0.0f, 0.0f, 0.0f);
+            T.root[0] = make_float4(pmin[0], pmin[1], pmax[0], pmax[1]);
+            T.root[1] = make_float4(bits_as_float(link), 0.0f, 0.0f, 0.0f);
         }
     }
-    if ((int)leaf_refs.size() != n_leaf) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: inconsistent leaves");
+    if ((int)T.leaf_refs.size() != n_leaf) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: inconsistent leaves", fn);
+    T.n_rec = n_rec;
+    T.n_leaf = n_leaf;
+    T.max_depth = max_depth;
+    return FCPW_HIP_OK;
+}
+
+// allocates and fills the tree buffers of c (records, root, leaf-slot index,
+// refit schedule); the scene's previous buffers must have been freed
+static int upload_tree(fcpw_hip_ctx *c, const TreeRecords &T, const int32_t *slot_index, int n_slots)
+{
+    size_t rec_bytes = T.rec.size() * sizeof(float4);
+    HIPCHECK(c, hipMalloc((void **)&c->records, rec_bytes));
+    HIPCHECK(c, hipMalloc((void **)&c->root, 2 * sizeof(float4)));
+    HIPCHECK(c, hipMalloc((void **)&c->prim_index, (size_t)n_slots * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->rec_parent, T.parent.size() * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->leaf_refs, T.leaf_refs.size() * sizeof(int32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->leaf_links, T.leaf_links.size() * sizeof(uint32_t)));
+    HIPCHECK(c, hipMalloc((void **)&c->counters, T.parent.size() * sizeof(unsigned int)));
+    HIPCHECK(c, hipMemcpy(c->records, T.rec.data(), rec_bytes, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->root, T.root, sizeof(T.root), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->prim_index, slot_index, (size_t)n_slots * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->rec_parent, T.parent.data(), T.parent.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->leaf_refs, T.leaf_refs.data(), T.leaf_refs.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->leaf_links, T.leaf_links.data(), T.leaf_links.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(c->counters, 0, T.parent.size() * sizeof(unsigned int)));
+    return FCPW_HIP_OK;
+}
+
+// Morton grid of the query ordering: 1.5x the scene box
+static void set_query_grid(fcpw_hip_ctx *c, const float *pmin, const float *pmax, int dim)
+{
+    for (int d = 0; d < 3; d++) {
+        float lo = d < dim ? pmin[d] : 0.0f, hi = d < dim ? pmax[d] : 0.0f;
+        float ext = std::max(hi - lo, 1e-6f);
+        c->qlo[d] = 0.5f * (lo + hi) - 0.75f * ext;
+        c->qinv[d] = 1.0f / (1.5f * ext);
+    }
+}
+
+extern "C" {
+
+int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin, const float *pmax,
+                        const int32_t *nrefs, const int32_t *offset, int n_prims, const int32_t *prim_vertices,
+                        const int32_t *prim_index, int n_vertices, const float *positions)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    if (dim != 2 && dim != 3) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: dim must be 2 or 3, got %d", dim);
+    if (n_nodes <= 0 || n_prims <= 0 || n_vertices <= 0 || !pmin || !pmax || !nrefs || !offset || !prim_vertices ||
+        !prim_index || !positions)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: empty BVH or NULL array");
+    if ((size_t)n_prims > (size_t)OFF_MASK)
+        FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "upload_bvh: %d primitives exceed the 2^27 leaf-slot range", n_prims);
+    TreeRecords T;
+    int st = tree_records(c, "upload_bvh", dim, n_nodes, pmin, pmax, nrefs, offset, n_prims, T);
+    if (st) return st;
+    for (int64_t k = 0; k < (int64_t)n_prims * dim; k++)
+        if (prim_vertices[k] < 0 || prim_vertices[k] >= n_vertices)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_bvh: vertex id out of range");
 
     // primitive records in leaf-slot order
     const int PR = dim == 3 ? 3 : 1;
@@ -450,45 +526,25 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
 
     HIPCHECK(c, hipSetDevice(c->device));
     free_scene(c);
-    size_t rec_bytes = rec.size() * sizeof(float4);
-    HIPCHECK(c, hipMalloc((void **)&c->records, rec_bytes));
-    HIPCHECK(c, hipMalloc((void **)&c->root, 2 * sizeof(float4)));
+    st = upload_tree(c, T, prim_index, n_prims);
+    if (st) return st;
     HIPCHECK(c, hipMalloc((void **)&c->prims, prims.size() * sizeof(float4)));
-    HIPCHECK(c, hipMalloc((void **)&c->prim_index, (size_t)n_prims * sizeof(int32_t)));
     HIPCHECK(c, hipMalloc((void **)&c->prim_vids, (size_t)n_prims * dim * sizeof(int32_t)));
     HIPCHECK(c, hipMalloc((void **)&c->V, (size_t)n_vertices * dim * sizeof(float)));
-    HIPCHECK(c, hipMalloc((void **)&c->rec_parent, parent.size() * sizeof(int32_t)));
-    HIPCHECK(c, hipMalloc((void **)&c->leaf_refs, leaf_refs.size() * sizeof(int32_t)));
-    HIPCHECK(c, hipMalloc((void **)&c->leaf_links, leaf_links.size() * sizeof(uint32_t)));
-    HIPCHECK(c, hipMalloc((void **)&c->counters, parent.size() * sizeof(unsigned int)));
-    HIPCHECK(c, hipMemcpy(c->records, rec.data(), rec_bytes, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->root, root, sizeof(root), hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(c->prims, prims.data(), prims.size() * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->prim_index, prim_index, (size_t)n_prims * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(c->prim_vids, prim_vertices, (size_t)n_prims * dim * sizeof(int32_t),
                           hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(c->V, positions, (size_t)n_vertices * dim * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->rec_parent, parent.data(), parent.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->leaf_refs, leaf_refs.data(), leaf_refs.size() * sizeof(int32_t),
-                          hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->leaf_links, leaf_links.data(), leaf_links.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemset(c->counters, 0, parent.size() * sizeof(unsigned int)));
     HIPCHECK(c, hipDeviceSynchronize());
-    for (int d = 0; d < 3; d++) {
-        float lo = d < dim ? pmin[d] : 0.0f, hi = d < dim ? pmax[d] : 0.0f;
-        float ext = std::max(hi - lo, 1e-6f);
-        c->qlo[d] = 0.5f * (lo + hi) - 0.75f * ext;
-        c->qinv[d] = 1.0f / (1.5f * ext);
-    }
+    set_query_grid(c, pmin, pmax, dim);
     c->dim = dim;
-    c->n_records = n_rec;
-    c->n_leaves = n_leaf;
+    c->n_records = T.n_rec;
+    c->n_leaves = T.n_leaf;
     c->n_prims = n_prims;
     c->n_vertices = n_vertices;
-    c->max_depth = max_depth;
-    c->node_slot = std::move(node_slot);
-    c->node_leaf_off = std::move(node_leaf_off);
+    c->max_depth = T.max_depth;
+    c->node_slot = std::move(T.node_slot);
+    c->node_leaf_off = std::move(T.node_leaf_off);
     c->uploaded = true;
     c->generation++;
     return FCPW_HIP_OK;
@@ -524,7 +580,7 @@ int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bou
                                 const int32_t *n_silhouettes, const int32_t *silhouette_offset, int n_refs,
                                 const void *silhouettes)
 {
-    int st = check_ready(c, "upload_silhouettes");
+    int st = check_single(c, "upload_silhouettes");
     if (st) return st;
     if (c->node_slot.empty())
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
@@ -595,7 +651,7 @@ static int launch_rebuild(fcpw_hip_ctx *c, hipStream_t s)
 
 int fcpw_hip_update_positions(fcpw_hip_ctx *c, const float *positions, int n_vertices)
 {
-    int st = check_ready(c, "update_positions");
+    int st = check_single(c, "update_positions");
     if (st) return st;
     if (!positions || n_vertices != c->n_vertices)
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "update_positions: expected %d vertices, got %d", c->n_vertices,
@@ -610,7 +666,7 @@ int fcpw_hip_update_positions(fcpw_hip_ctx *c, const float *positions, int n_ver
 
 int fcpw_hip_update_positions_device(fcpw_hip_ctx *c, const float *positions, int n_vertices, void *stream)
 {
-    int st = check_ready(c, "update_positions_device");
+    int st = check_single(c, "update_positions_device");
     if (st) return st;
     if (!positions || n_vertices != c->n_vertices)
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "update_positions_device: expected %d vertices, got %d",
@@ -626,6 +682,7 @@ int fcpw_hip_refit(fcpw_hip_ctx *c, void *stream)
     int st = check_ready(c, "refit");
     if (st) return st;
     hipStream_t s = (hipStream_t)stream;
+    if (c->instanced) return instances_refit(c, s);
     int threads = 256, blocks = (c->n_leaves + threads - 1) / threads;
     if (c->dim == 3)
         hipLaunchKernelGGL(k_refit<3>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->prims,
@@ -836,6 +893,7 @@ int fcpw_hip_closest_points_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphe
 {
     int st = check_ready(c, "closest_points_device");
     if (st) return st;
+    if (c->instanced) return fcpw_hip_closest_points_objects_device(c, spheres, n, record_normals, out, nullptr, stream);
     if (n < 0 || (n > 0 && (!spheres || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: bad arguments");
     if (((uintptr_t)spheres & 15) || ((uintptr_t)out & 7))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: spheres must be 16-byte and out 8-byte aligned");
@@ -848,6 +906,7 @@ int fcpw_hip_closest_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *sph
 {
     int st = check_ready(c, "closest_points");
     if (st) return st;
+    if (c->instanced) return fcpw_hip_closest_points_objects(c, spheres, n, record_normals, out, nullptr);
     if (n < 0 || (n > 0 && (!spheres || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points: bad arguments");
     if (n == 0) return FCPW_HIP_OK;
     HIPCHECK(c, hipSetDevice(c->device));
@@ -867,6 +926,7 @@ int fcpw_hip_intersect_device(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t
 {
     int st = check_ready(c, "intersect_device");
     if (st) return st;
+    if (c->instanced) return fcpw_hip_intersect_objects_device(c, rays, n, occlusion, out, nullptr, stream);
     if (n < 0 || (n > 0 && (!rays || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_device: bad arguments");
     if (((uintptr_t)rays & 7) || ((uintptr_t)out & 7))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_device: rays and out must be 8-byte aligned");
@@ -878,6 +938,7 @@ int fcpw_hip_intersect(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int
 {
     int st = check_ready(c, "intersect");
     if (st) return st;
+    if (c->instanced) return fcpw_hip_intersect_objects(c, rays, n, occlusion, out, nullptr);
     if (n < 0 || (n > 0 && (!rays || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect: bad arguments");
     if (n == 0) return FCPW_HIP_OK;
     HIPCHECK(c, hipSetDevice(c->device));
@@ -910,7 +971,7 @@ int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_s
                                       const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
                                       void *stream)
 {
-    int st = check_ready(c, "intersect_spheres_device");
+    int st = check_single(c, "intersect_spheres_device");
     if (st) return st;
     if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres_device: bad arguments");
@@ -924,7 +985,7 @@ int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_s
 int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
                                const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out)
 {
-    int st = check_ready(c, "intersect_spheres");
+    int st = check_single(c, "intersect_spheres");
     if (st) return st;
     if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres: bad arguments");
@@ -966,7 +1027,7 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
 
 static int check_silhouettes(fcpw_hip_ctx *c, const char *fn)
 {
-    int st = check_ready(c, fn);
+    int st = check_single(c, fn);
     if (st) return st;
     if (!c->has_silhouettes)
         FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "%s: the scene was transferred without silhouettes (computeSilhouettes)", fn);
@@ -1015,7 +1076,7 @@ int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_
 int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *c, const float *points, const float *r2, int64_t n,
                                            float *d, float *p, int32_t *prim, int32_t *slot, void *stream)
 {
-    int st = check_ready(c, "closest_points_compact_device");
+    int st = check_single(c, "closest_points_compact_device");
     if (st) return st;
     if (n < 0 || (n > 0 && !points)) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_compact_device: bad arguments");
     if (n == 0) return FCPW_HIP_OK;
@@ -1033,7 +1094,7 @@ int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *c, const float *points,
 int fcpw_hip_distance_backward_device(fcpw_hip_ctx *c, const float *points, const int32_t *slot, const float *grad_d,
                                       int64_t n, float *grad_points, float *grad_vertices, void *stream)
 {
-    int st = check_ready(c, "distance_backward_device");
+    int st = check_single(c, "distance_backward_device");
     if (st) return st;
     if (n < 0 || (n > 0 && (!points || !slot || !grad_d)))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "distance_backward_device: bad arguments");
@@ -1158,3 +1219,4 @@ int fcpw_hip_get_info(const fcpw_hip_ctx *c, const char *key, int64_t *value)
 
 } // extern "C"
 #include "batch.hpp"
+#include "instances.hpp"

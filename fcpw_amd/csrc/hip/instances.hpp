@@ -1,0 +1,672 @@
+// instances.hpp -- scenes of several objects and transformed instances.
+// Included once, by fcpw_hip.hip, after batch.hpp.
+//
+// The reference GPU path serves one object (fcpw_gpu.h:17-20); its CPU Scene
+// puts a BVH over the objects' aggregates and wraps instanced objects in
+// TransformedAggregate (fcpw.inl:1052-1126, primitive.h:289-440), whose GPU
+// twin is aggregate.slang:45-145 with transform.slang:1-56.  Here:
+//   objects  contexts of their own, each with its uploaded tree
+//   inst     one DevInstance per instance: rows of t and tinv, object, flag
+//   table    one DevScene per object (refreshed when an object is re-uploaded)
+//   tree     the instance tree in the scene context's records / root, leaf
+//            slot j holding instance prim_index[j]
+// A query walks the instance tree in the reference GPU order and, at each
+// instance of a reached leaf, runs the object's traversal on the query mapped
+// by tinv, maps r^2 (or tMax) back by t and, on a hit, maps the point and
+// normal and recomputes d = |p - x| (TransformedAggregate.findClosestPoint /
+// intersect).  An object without instance transforms is traversed unmapped.
+// Point mapping is mul(t, float4(p, 1)) with each row summed left to right.
+#pragma once
+
+namespace fcpw_hip {
+
+__device__ __forceinline__ v3 xf(const float4 *r, v3 p)
+{
+    return {((r[0].x * p.x + r[0].y * p.y) + r[0].z * p.z) + r[0].w,
+            ((r[1].x * p.x + r[1].y * p.y) + r[1].z * p.z) + r[1].w,
+            ((r[2].x * p.x + r[2].y * p.y) + r[2].z * p.z) + r[2].w};
+}
+__device__ __forceinline__ v2 xf(const float4 *r, v2 p)
+{
+    return {(r[0].x * p.x + r[0].y * p.y) + r[0].w, (r[1].x * p.x + r[1].y * p.y) + r[1].w};
+}
+__device__ __forceinline__ v3 shift_x(v3 c, float s) { return {c.x + s, c.y + 0.0f, c.z + 0.0f}; }
+__device__ __forceinline__ v2 shift_x(v2 c, float s) { return {c.x + s, c.y + 0.0f}; }
+__device__ __forceinline__ v3 axpy(v3 o, v3 d, float s) { return {o.x + d.x * s, o.y + d.y * s, o.z + d.z * s}; }
+__device__ __forceinline__ v2 axpy(v2 o, v2 d, float s) { return {o.x + d.x * s, o.y + d.y * s}; }
+__device__ __forceinline__ v3 divs(v3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ v2 divs(v2 a, float s) { return {a.x / s, a.y / s}; }
+__device__ __forceinline__ v3 recip(v3 a) { return {1.0f / a.x, 1.0f / a.y, 1.0f / a.z}; }
+__device__ __forceinline__ v2 recip(v2 a) { return {1.0f / a.x, 1.0f / a.y}; }
+
+// transformSphere's radius (transform.slang:31-42): sphere (c, r2), tc = xf(c)
+template <typename V>
+__device__ __forceinline__ float xf_r2(const float4 *r, V c, V tc, float r2)
+{
+    if (!(r2 < FLT_MAX)) return FLT_MAX;
+    V dd = sub(xf(r, shift_x(c, sqrtf(r2))), tc);
+    return dot(dd, dd);
+}
+
+// normalize(transpose(linear(tinv)) * n), as v / length(v) (transform.slang:48-49)
+template <int DIM>
+__device__ __forceinline__ void xf_normal(const float4 *ti, float n[3])
+{
+    if (DIM == 3) {
+        v3 m = {(ti[0].x * n[0] + ti[1].x * n[1]) + ti[2].x * n[2], (ti[0].y * n[0] + ti[1].y * n[1]) + ti[2].y * n[2],
+                (ti[0].z * n[0] + ti[1].z * n[1]) + ti[2].z * n[2]};
+        float l = sqrtf(dot(m, m));
+        n[0] = m.x / l; n[1] = m.y / l; n[2] = m.z / l;
+    } else {
+        v2 m = {ti[0].x * n[0] + ti[1].x * n[1], ti[0].y * n[0] + ti[1].y * n[1]};
+        float l = sqrtf(dot(m, m));
+        n[0] = m.x / l; n[1] = m.y / l; n[2] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ void load_instance(const DevInstance *inst, int i, float4 t[3], float4 ti[3], int &object,
+                                              int &transformed)
+{
+    const DevInstance *I = inst + i;
+    for (int r = 0; r < 3; r++) {
+        t[r] = I->t[r];
+        ti[r] = I->ti[r];
+    }
+    object = I->object;
+    transformed = I->transformed;
+}
+
+// TransformedAggregate.findClosestPoint on one instance; r2 is updated in place
+template <int DIM, int BLOCK, int K>
+__device__ __forceinline__ bool instance_closest(const DevInstance *inst, const DevScene *objs, int ii,
+                                                 typename Geo<DIM>::vec x, float &r2, int &object, int32_t &slot,
+                                                 float &d, float p[3], float uv[2])
+{
+    using vec = typename Geo<DIM>::vec;
+    float4 t[3], ti[3];
+    int transformed;
+    load_instance(inst, ii, t, ti, object, transformed);
+    const DevScene S = global_scene(objs[object]);
+    vec xl = x;
+    float r2l = r2;
+    if (transformed) {
+        xl = xf(ti, x);
+        r2l = xf_r2(ti, x, xl, r2);
+    }
+    CpResult c;
+    closest_point_query<DIM, BLOCK, K>(S, xl, r2l, c);
+    r2 = transformed ? xf_r2(t, xl, xf(t, xl), r2l) : r2l;
+    if (c.slot < 0) return false;
+    slot = c.slot;
+    uv[0] = c.uv[0];
+    uv[1] = c.uv[1];
+    if (transformed) {
+        vec pw = xf(t, Geo<DIM>::load_point(c.p));
+        vec e = sub(pw, x);
+        d = sqrtf(dot(e, e));
+        store_p(pw, p);
+    } else {
+        d = c.d;
+        p[0] = c.p[0]; p[1] = c.p[1]; p[2] = c.p[2];
+    }
+    return true;
+}
+
+template <int DIM, int BLOCK, int K, int KT>
+__global__ __launch_bounds__(BLOCK) void k_instances_closest(DevScene T, const DevInstance *inst, const DevScene *objs,
+                                                             int n_inst, const float4 *spheres, int64_t n,
+                                                             int record_normals, float2 *out, int32_t *object_out,
+                                                             const uint32_t *order, int xcds)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    const int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = order ? (int64_t)order[k] : k;
+    const float4 s4 = spheres[i];
+    const float q[3] = {s4.x, s4.y, s4.z};
+    const vec x = G::load_point(q);
+    float r2 = s4.w;
+    int best_inst = -1, best_obj = -1;
+    int32_t best_slot = -1;
+    float best_d = FLT_MAX, bp[3] = {0.0f, 0.0f, 0.0f}, buv[2] = {0.0f, 0.0f};
+    int obj;
+    int32_t slot;
+    float d, p[3], uv[2];
+    if (n_inst == 1) {
+        // a single instance is the scene's aggregate itself (fcpw.inl:1088-1092)
+        const int ii = T.prim_index[0];
+        if (instance_closest<DIM, BLOCK, K>(inst, objs, ii, x, r2, obj, slot, d, p, uv)) {
+            best_inst = ii; best_obj = obj; best_slot = slot; best_d = d;
+            bp[0] = p[0]; bp[1] = p[1]; bp[2] = p[2];
+            buv[0] = uv[0]; buv[1] = uv[1];
+        }
+    } else {
+        vec lo, hi;
+        uint32_t cur;
+        G::root(T, lo, hi, cur);
+        float d2min, d2max;
+        box_sqdist(lo, hi, x, d2min, d2max);
+        if (d2min <= r2) {
+            r2 = fminf(r2, d2max);
+            float cur_d2 = d2min;
+            FCPW_STACK_DECL(BLOCK, KT);
+            while (true) {
+                if (cur_d2 <= r2) {
+                    if (cur & LEAF_BIT) {
+                        const uint32_t off = cur & OFF_MASK, cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                        for (uint32_t j = 0; j < cnt; j++) {
+                            const int ii = T.prim_index[off + j];
+                            if (instance_closest<DIM, BLOCK, K>(inst, objs, ii, x, r2, obj, slot, d, p, uv)) {
+                                r2 = fminf(r2, d * d);
+                                best_inst = ii; best_obj = obj; best_slot = slot; best_d = d;
+                                bp[0] = p[0]; bp[1] = p[1]; bp[2] = p[2];
+                                buv[0] = uv[0]; buv[1] = uv[1];
+                            }
+                        }
+                    } else {
+                        vec loA, hiA, loB, hiB;
+                        uint32_t lA, lB;
+                        G::record(T, cur, loA, hiA, lA, loB, hiB, lB);
+                        float a0, a1, b0, b1;
+                        box_sqdist(loA, hiA, x, a0, a1);
+                        bool oA = a0 <= r2;
+                        r2 = fminf(r2, a1);
+                        box_sqdist(loB, hiB, x, b0, b1);
+                        bool oB = b0 <= r2;
+                        r2 = fminf(r2, b1);
+                        if (oA && oB) {
+                            uint32_t nearL = lA, farL = lB;
+                            float nearD = a0, farD = b0;
+                            if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                            FCPW_PUSH(BLOCK, KT, farL, farD, { atomicOr(T.flags, 1); break; });
+                            cur = nearL;
+                            cur_d2 = nearD;
+                            continue;
+                        } else if (oA) {
+                            cur = lA; cur_d2 = a0;
+                            continue;
+                        } else if (oB) {
+                            cur = lB; cur_d2 = b0;
+                            continue;
+                        }
+                    }
+                }
+                if (sp == 0) break;
+                FCPW_POP(BLOCK, KT, cur, cur_d2);
+            }
+        }
+    }
+    float nrm[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t index = FCPW_HIP_UINT_MAX;
+    if (best_inst >= 0) {
+        const DevScene S = global_scene(objs[best_obj]);
+        index = (uint32_t)S.prim_index[best_slot];
+        if (record_normals) {
+            G::normal(S, (uint32_t)best_slot, nrm);
+            if (inst[best_inst].transformed) xf_normal<DIM>(inst[best_inst].ti, nrm);
+        }
+    }
+    write_interaction(out + i * 5, bp, nrm, buv, best_d, index);
+    if (object_out) object_out[i] = best_obj;
+}
+
+// TransformedAggregate.intersect on one instance; tmax is updated in place
+template <int DIM, int BLOCK, int K>
+__device__ __forceinline__ bool instance_ray(const DevInstance *inst, const DevScene *objs, int ii,
+                                             typename Geo<DIM>::vec o, typename Geo<DIM>::vec dir,
+                                             typename Geo<DIM>::vec dinv, float &tmax, bool occlusion, int &object,
+                                             RayResult &r)
+{
+    using vec = typename Geo<DIM>::vec;
+    float4 t[3], ti[3];
+    int transformed;
+    load_instance(inst, ii, t, ti, object, transformed);
+    const DevScene S = global_scene(objs[object]);
+    vec ol = o, dl = dir, il = dinv;
+    float tl = tmax;
+    if (transformed) {
+        // transformRay(tinv, r) (transform.slang:22-29) and the Ray constructor's 1/d
+        ol = xf(ti, o);
+        vec m = sub(xf(ti, axpy(o, dir, tmax < FLT_MAX ? tmax : 1.0f)), ol);
+        float nrm = sqrtf(dot(m, m));
+        dl = divs(m, nrm);
+        il = recip(dl);
+        tl = tmax < FLT_MAX ? nrm : FLT_MAX;
+    }
+    ray_query<DIM, BLOCK, K>(S, ol, dl, il, tl, occlusion, r);
+    if (transformed) {
+        if (tl < FLT_MAX) {
+            vec dd = sub(xf(t, axpy(ol, dl, tl)), xf(t, ol));
+            tmax = sqrtf(dot(dd, dd));
+        } else {
+            tmax = FLT_MAX;
+        }
+    } else {
+        tmax = tl;
+    }
+    if (r.slot < 0) return false;
+    if (transformed && !occlusion) {
+        vec pw = xf(t, Geo<DIM>::load_point(r.p));
+        vec e = sub(pw, o);
+        r.t = sqrtf(dot(e, e));
+        store_p(pw, r.p);
+        xf_normal<DIM>(ti, r.n);
+    }
+    return true;
+}
+
+template <int DIM, int BLOCK, int K, int KT>
+__global__ __launch_bounds__(BLOCK) void k_instances_intersect(DevScene T, const DevInstance *inst,
+                                                               const DevScene *objs, int n_inst, const float2 *rays,
+                                                               int64_t n, int occlusion, float2 *out,
+                                                               int32_t *object_out, const uint32_t *order, int xcds)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    const int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = order ? (int64_t)order[k] : k;
+    const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
+    const float2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
+    const float o3[3] = {r0.x, r0.y, r1.x}, d3[3] = {r1.y, r2.x, r2.y}, di3[3] = {r3.x, r3.y, r4.x};
+    float tmax = r4.y;
+    const vec o = G::load_point(o3), dir = G::load_point(d3), di = G::load_point(di3);
+    RayResult best, r;
+    best.slot = -1;
+    best.t = FLT_MAX;
+    for (int c = 0; c < 3; c++) { best.p[c] = 0.0f; best.n[c] = 0.0f; }
+    best.uv[0] = best.uv[1] = 0.0f;
+    int best_obj = -1, obj;
+    const bool occl = occlusion != 0;
+    if (n_inst == 1) {
+        if (instance_ray<DIM, BLOCK, K>(inst, objs, T.prim_index[0], o, dir, di, tmax, occl, obj, r)) {
+            best = r;
+            best_obj = obj;
+        }
+    } else {
+        vec lo, hi;
+        uint32_t cur;
+        G::root(T, lo, hi, cur);
+        float cur_t;
+        if (box_ray(lo, hi, o, dir, di, tmax, cur_t)) {
+            FCPW_STACK_DECL(BLOCK, KT);
+            bool done = false;
+            while (!done) {
+                if (!(cur_t > tmax)) {
+                    if (cur & LEAF_BIT) {
+                        const uint32_t off = cur & OFF_MASK, cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                        for (uint32_t j = 0; j < cnt; j++) {
+                            if (instance_ray<DIM, BLOCK, K>(inst, objs, T.prim_index[off + j], o, dir, di, tmax,
+                                                            occl, obj, r)) {
+                                best = r;
+                                best_obj = obj;
+                                if (occl) { done = true; break; }
+                                tmax = fminf(tmax, r.t);
+                            }
+                        }
+                        if (done) break;
+                    } else {
+                        vec loA, hiA, loB, hiB;
+                        uint32_t lA, lB;
+                        G::record(T, cur, loA, hiA, lA, loB, hiB, lB);
+                        float ta, tb;
+                        bool hA = box_ray(loA, hiA, o, dir, di, tmax, ta);
+                        bool hB = box_ray(loB, hiB, o, dir, di, tmax, tb);
+                        if (hA && hB) {
+                            uint32_t nearL = lA, farL = lB;
+                            float nearT = ta, farT = tb;
+                            if (tb < ta) { nearL = lB; farL = lA; nearT = tb; farT = ta; }
+                            FCPW_PUSH(BLOCK, KT, farL, farT, { atomicOr(T.flags, 1); break; });
+                            cur = nearL;
+                            cur_t = nearT;
+                            continue;
+                        } else if (hA) {
+                            cur = lA; cur_t = ta;
+                            continue;
+                        } else if (hB) {
+                            cur = lB; cur_t = tb;
+                            continue;
+                        }
+                    }
+                }
+                if (sp == 0) break;
+                FCPW_POP(BLOCK, KT, cur, cur_t);
+            }
+        }
+    }
+    uint32_t index = FCPW_HIP_UINT_MAX;
+    if (best.slot >= 0) index = (uint32_t)global_scene(objs[best_obj]).prim_index[best.slot];
+    write_interaction(out + i * 5, best.p, best.n, best.uv, best.t, index);
+    if (object_out) object_out[i] = best_obj;
+}
+
+// Bvh::refit over the instances (bvh.inl:548-599) once the objects are refit:
+// each leaf bounds its instances (root box of the object, or its 2^DIM
+// corners mapped by t and expanded with +-FLT_EPSILON), then the boxes are
+// merged bottom-up exactly as k_refit does
+template <int DIM>
+__global__ void k_instances_refit(float4 *records, float4 *root, const DevInstance *inst, const DevScene *objs,
+                                  const int32_t *leaf_inst, const int32_t *leaf_refs, const uint32_t *leaf_links,
+                                  const int32_t *rec_parent, unsigned int *counters, int n_leaves)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_leaves) return;
+    uint32_t link = leaf_links[j];
+    uint32_t off = link & OFF_MASK, cnt = ((link >> CNT_SHIFT) & CNT_MASK) + 1u;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t k = 0; k < cnt; k++) {
+        const DevInstance *I = inst + leaf_inst[off + k];
+        const float4 *rb = objs[I->object].root;
+        float blo[3], bhi[3];
+        if (DIM == 3) {
+            float4 a = rb[0], b = rb[1];
+            blo[0] = a.x; blo[1] = a.y; blo[2] = a.z;
+            bhi[0] = b.x; bhi[1] = b.y; bhi[2] = b.z;
+        } else {
+            float4 a = rb[0];
+            blo[0] = a.x; blo[1] = a.y; blo[2] = 0.0f;
+            bhi[0] = a.z; bhi[1] = a.w; bhi[2] = 0.0f;
+        }
+        if (I->transformed) {
+            float4 t[3] = {I->t[0], I->t[1], I->t[2]};
+            float tlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, thi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            for (int c = 0; c < (1 << DIM); c++) {
+                float q[3];
+                if (DIM == 3) {
+                    v3 pc = {(c & 1) ? bhi[0] : blo[0], (c & 2) ? bhi[1] : blo[1], (c & 4) ? bhi[2] : blo[2]};
+                    v3 m = xf(t, pc);
+                    q[0] = m.x; q[1] = m.y; q[2] = m.z;
+                } else {
+                    v2 pc = {(c & 1) ? bhi[0] : blo[0], (c & 2) ? bhi[1] : blo[1]};
+                    v2 m = xf(t, pc);
+                    q[0] = m.x; q[1] = m.y; q[2] = 0.0f;
+                }
+                for (int a = 0; a < DIM; a++) {
+                    tlo[a] = fminf(tlo[a], q[a] - FLT_EPSILON);
+                    thi[a] = fmaxf(thi[a], q[a] + FLT_EPSILON);
+                }
+            }
+            for (int a = 0; a < DIM; a++) { blo[a] = tlo[a]; bhi[a] = thi[a]; }
+        }
+        for (int a = 0; a < DIM; a++) {
+            lo[a] = fminf(lo[a], blo[a]);
+            hi[a] = fmaxf(hi[a], bhi[a]);
+        }
+    }
+    refit_upward<DIM>(records, root, rec_parent, counters, leaf_refs[j], lo, hi);
+}
+
+// object index of the hits of a one-object scene: 0, or -1 on a miss
+__global__ void k_single_object_index(const float2 *out, int64_t n, int32_t *object_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    object_out[i] = __float_as_uint(out[i * 5 + 4].y) == FCPW_HIP_UINT_MAX ? -1 : 0;
+}
+
+} // namespace fcpw_hip
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+constexpr int INST_BLOCK = 128, INST_K = 8, INST_KT = 4;
+
+// the object table, rebuilt when a member object was re-uploaded
+static int instances_ready(fcpw_hip_ctx *c, const char *fn)
+{
+    bool dirty = false;
+    for (size_t o = 0; o < c->inst_objects.size(); o++) {
+        fcpw_hip_ctx *m = c->inst_objects[o];
+        if (!m->uploaded || m->instanced || m->dim != c->dim)
+            FAIL(c, FCPW_HIP_ERR_NOT_UPLOADED, "%s: object %zu has no uploaded single-object BVH", fn, o);
+        if (m->generation != c->inst_generations[o]) dirty = true;
+    }
+    if (!dirty) return FCPW_HIP_OK;
+    std::vector<DevScene> table;
+    for (size_t o = 0; o < c->inst_objects.size(); o++) {
+        DevScene S = dev_scene(c->inst_objects[o]);
+        S.flags = c->flags; // overflow of any level is reported by the scene
+        table.push_back(S);
+        c->inst_generations[o] = c->inst_objects[o]->generation;
+    }
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipMemcpy(c->obj_table, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice));
+    return FCPW_HIP_OK;
+}
+
+static int instances_refit(fcpw_hip_ctx *c, hipStream_t s)
+{
+    int st = instances_ready(c, "refit");
+    if (st) return st;
+    std::vector<fcpw_hip_ctx *> done;
+    for (fcpw_hip_ctx *m : c->inst_objects) {
+        if (std::find(done.begin(), done.end(), m) != done.end()) continue;
+        done.push_back(m);
+        st = fcpw_hip_refit(m, s);
+        if (st) FAIL(c, st, "refit (object): %s", m->err.c_str());
+    }
+    int threads = 256, blocks = (c->n_leaves + threads - 1) / threads;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_instances_refit<3>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->inst,
+                           c->obj_table, c->prim_index, c->leaf_refs, c->leaf_links, c->rec_parent, c->counters,
+                           c->n_leaves);
+    else
+        hipLaunchKernelGGL(k_instances_refit<2>, dim3(blocks), dim3(threads), 0, s, c->records, c->root, c->inst,
+                           c->obj_table, c->prim_index, c->leaf_refs, c->leaf_links, c->rec_parent, c->counters,
+                           c->n_leaves);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+static int launch_instances_closest(fcpw_hip_ctx *c, const void *spheres, int64_t n, int record_normals, void *out,
+                                    int32_t *object_out, hipStream_t s)
+{
+    int st = instances_ready(c, "closest_points");
+    if (st) return st;
+    const uint32_t *order;
+    st = morton_order(c, (const float *)spheres, 4, n, s, &order);
+    if (st) return st;
+    if ((st = timing_begin(c, s))) return st;
+    DevScene T = dev_scene(c);
+    dim3 g = grid_for(c, n, INST_BLOCK);
+    if (c->dim == 3)
+        hipLaunchKernelGGL((k_instances_closest<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,
+                           c->obj_table, c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out,
+                           object_out, order, c->xcds);
+    else
+        hipLaunchKernelGGL((k_instances_closest<2, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,
+                           c->obj_table, c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out,
+                           object_out, order, c->xcds);
+    HIPCHECK(c, hipGetLastError());
+    return timing_end(c, s);
+}
+
+static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int occlusion, void *out,
+                                      int32_t *object_out, hipStream_t s)
+{
+    int st = instances_ready(c, "intersect");
+    if (st) return st;
+    const uint32_t *order;
+    st = morton_order(c, (const float *)rays, 10, n, s, &order); // ordered by origin
+    if (st) return st;
+    if ((st = timing_begin(c, s))) return st;
+    DevScene T = dev_scene(c);
+    dim3 g = grid_for(c, n, INST_BLOCK);
+    if (c->dim == 3)
+        hipLaunchKernelGGL((k_instances_intersect<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,
+                           c->inst, c->obj_table, c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out,
+                           object_out, order, c->xcds);
+    else
+        hipLaunchKernelGGL((k_instances_intersect<2, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,
+                           c->inst, c->obj_table, c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out,
+                           object_out, order, c->xcds);
+    HIPCHECK(c, hipGetLastError());
+    return timing_end(c, s);
+}
+
+static int fill_single_object_index(fcpw_hip_ctx *c, const void *out, int64_t n, int32_t *object_out, hipStream_t s)
+{
+    if (!object_out) return FCPW_HIP_OK;
+    hipLaunchKernelGGL(k_single_object_index, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float2 *)out,
+                       n, object_out);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
+extern "C" {
+
+int fcpw_hip_upload_instances(fcpw_hip_ctx *c, fcpw_hip_ctx *const *objects, int n_objects,
+                              const fcpw_hip_instance *instances, int n_instances, int n_nodes, const float *pmin,
+                              const float *pmax, const int32_t *nrefs, const int32_t *offset,
+                              const int32_t *leaf_instances)
+{
+    if (!c) return FCPW_HIP_ERR_INVALID_ARGUMENT;
+    if (n_objects <= 0 || n_instances <= 0 || n_nodes <= 0 || !objects || !instances || !pmin || !pmax || !nrefs ||
+        !offset || !leaf_instances)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_instances: empty scene or NULL array");
+    const int dim = objects[0] ? objects[0]->dim : 0;
+    for (int o = 0; o < n_objects; o++) {
+        fcpw_hip_ctx *m = objects[o];
+        if (!m || m == c || !m->uploaded || m->instanced)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_instances: object %d has no uploaded single-object BVH", o);
+        if (m->dim != dim || m->device != c->device)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_instances: objects must share dimension and device");
+    }
+    std::vector<int> seen(n_instances, 0);
+    for (int i = 0; i < n_instances; i++) {
+        if (instances[i].object < 0 || instances[i].object >= n_objects)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_instances: instance %d draws object %d of %d", i,
+                 instances[i].object, n_objects);
+        int li = leaf_instances[i];
+        if (li < 0 || li >= n_instances || seen[li]++)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_instances: leaf_instances is not a permutation");
+    }
+    TreeRecords T;
+    int st = tree_records(c, "upload_instances", dim, n_nodes, pmin, pmax, nrefs, offset, n_instances, T);
+    if (st) return st;
+    std::vector<DevInstance> inst((size_t)n_instances);
+    for (int i = 0; i < n_instances; i++) {
+        const fcpw_hip_instance &I = instances[i];
+        DevInstance &D = inst[i];
+        for (int r = 0; r < 3; r++) {
+            D.t[r] = make_float4(I.t[r * 4], I.t[r * 4 + 1], I.t[r * 4 + 2], I.t[r * 4 + 3]);
+            D.ti[r] = make_float4(I.tinv[r * 4], I.tinv[r * 4 + 1], I.tinv[r * 4 + 2], I.tinv[r * 4 + 3]);
+        }
+        D.object = I.object;
+        D.transformed = I.transformed ? 1 : 0;
+        D.pad0 = D.pad1 = 0;
+    }
+    HIPCHECK(c, hipSetDevice(c->device));
+    free_scene(c);
+    st = upload_tree(c, T, leaf_instances, n_instances);
+    if (st) return st;
+    HIPCHECK(c, hipMalloc((void **)&c->inst, inst.size() * sizeof(DevInstance)));
+    HIPCHECK(c, hipMemcpy(c->inst, inst.data(), inst.size() * sizeof(DevInstance), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMalloc((void **)&c->obj_table, (size_t)n_objects * sizeof(DevScene)));
+    c->inst_objects.assign(objects, objects + n_objects);
+    c->inst_generations.assign((size_t)n_objects, ~(uint64_t)0);
+    set_query_grid(c, pmin, pmax, dim);
+    c->dim = dim;
+    c->n_records = T.n_rec;
+    c->n_leaves = T.n_leaf;
+    c->n_prims = n_instances;
+    c->n_vertices = 0;
+    c->max_depth = T.max_depth;
+    c->node_slot = std::move(T.node_slot);
+    c->node_leaf_off = std::move(T.node_leaf_off);
+    c->n_instances = n_instances;
+    c->instanced = true;
+    c->uploaded = true;
+    c->generation++;
+    st = instances_ready(c, "upload_instances");
+    if (st) return st;
+    HIPCHECK(c, hipDeviceSynchronize());
+    return FCPW_HIP_OK;
+}
+
+int fcpw_hip_closest_points_objects_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                           int record_normals, fcpw_hip_interaction *out, int32_t *object_index,
+                                           void *stream)
+{
+    int st = check_ready(c, "closest_points_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: bad arguments");
+    if (((uintptr_t)spheres & 15) || ((uintptr_t)out & 7) || ((uintptr_t)object_index & 3))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "closest_points_device: spheres must be 16-byte, out 8-byte and object_index 4-byte aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->instanced) return launch_instances_closest(c, spheres, n, record_normals, out, object_index, s);
+    st = launch_closest_records(c, spheres, n, record_normals, out, s);
+    if (st) return st;
+    return fill_single_object_index(c, out, n, object_index, s);
+}
+
+int fcpw_hip_closest_points_objects(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                    int record_normals, fcpw_hip_interaction *out, int32_t *object_index)
+{
+    int st = check_ready(c, "closest_points");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t rec = (size_t)n * sizeof(fcpw_hip_interaction);
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, (size_t)n * sizeof(fcpw_hip_bounding_sphere));
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, rec + (size_t)n * sizeof(int32_t));
+    if (st) return st;
+    int32_t *obj_dev = object_index ? (int32_t *)((char *)c->stage_out + rec) : nullptr;
+    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), hipMemcpyHostToDevice));
+    st = fcpw_hip_closest_points_objects_device(c, (const fcpw_hip_bounding_sphere *)c->stage_in, n, record_normals,
+                                                (fcpw_hip_interaction *)c->stage_out, obj_dev, nullptr);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, rec, hipMemcpyDeviceToHost));
+    if (object_index)
+        HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return check_overflow(c);
+}
+
+int fcpw_hip_intersect_objects_device(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int occlusion,
+                                      fcpw_hip_interaction *out, int32_t *object_index, void *stream)
+{
+    int st = check_ready(c, "intersect_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!rays || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_device: bad arguments");
+    if (((uintptr_t)rays & 7) || ((uintptr_t)out & 7) || ((uintptr_t)object_index & 3))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "intersect_device: rays and out must be 8-byte and object_index 4-byte aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->instanced) return launch_instances_intersect(c, rays, n, occlusion, out, object_index, s);
+    st = launch_intersect(c, rays, n, occlusion, out, s);
+    if (st) return st;
+    return fill_single_object_index(c, out, n, object_index, s);
+}
+
+int fcpw_hip_intersect_objects(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int occlusion,
+                               fcpw_hip_interaction *out, int32_t *object_index)
+{
+    int st = check_ready(c, "intersect");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!rays || !out))) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t rec = (size_t)n * sizeof(fcpw_hip_interaction);
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, (size_t)n * sizeof(fcpw_hip_ray));
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, rec + (size_t)n * sizeof(int32_t));
+    if (st) return st;
+    int32_t *obj_dev = object_index ? (int32_t *)((char *)c->stage_out + rec) : nullptr;
+    HIPCHECK(c, hipMemcpy(c->stage_in, rays, (size_t)n * sizeof(fcpw_hip_ray), hipMemcpyHostToDevice));
+    st = fcpw_hip_intersect_objects_device(c, (const fcpw_hip_ray *)c->stage_in, n, occlusion,
+                                           (fcpw_hip_interaction *)c->stage_out, obj_dev, nullptr);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, rec, hipMemcpyDeviceToHost));
+    if (object_index)
+        HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return check_overflow(c);
+}
+
+} // extern "C"

@@ -270,6 +270,27 @@ private:
 
 } // namespace detail
 
+// the tree over references with precomputed boxes and centroids (primitives,
+// or the instances of a scene of several objects); refs is permuted into leaf
+// order, and prim_order lists each leaf slot's ref.prim
+template <int DIM>
+FlatBvh<DIM> build_bvh_refs(std::vector<detail::Ref<DIM>> &refs, CostHeuristic heuristic, int leaf_size = 4,
+                            int n_buckets = 8, int parallel_cutoff = 65536)
+{
+    using namespace detail;
+    const int n = (int)refs.size();
+    if (n <= 0) throw std::invalid_argument("Cannot construct a BVH without primitives");
+    FlatBvh<DIM> out;
+    out.nodes.reserve((size_t)n * 2);
+    Builder<DIM> b(refs, heuristic, leaf_size, n_buckets, parallel_cutoff);
+    auto res = b.build(out.nodes, 0, n, 0);
+    out.max_depth = res.first;
+    out.n_leaves = res.second;
+    out.prim_order.resize((size_t)n);
+    for (int i = 0; i < n; i++) out.prim_order[i] = refs[i].prim;
+    return out;
+}
+
 // positions: n_vertices x DIM; indices: n_prims x DIM (triangles in 3D,
 // segments in 2D)
 template <int DIM>
@@ -293,15 +314,7 @@ FlatBvh<DIM> build_bvh(const float *positions, int n_vertices, const int32_t *in
             r.c[d] = DIM == 3 ? ((v[0][d] + v[1][d]) + v[DIM - 1][d]) / 3.0f : (v[0][d] + v[1][d]) * 0.5f;
         r.prim = i;
     }
-    FlatBvh<DIM> out;
-    out.nodes.reserve((size_t)n_prims * 2);
-    Builder<DIM> b(refs, heuristic, leaf_size, n_buckets, parallel_cutoff);
-    auto res = b.build(out.nodes, 0, n_prims, 0);
-    out.max_depth = res.first;
-    out.n_leaves = res.second;
-    out.prim_order.resize((size_t)n_prims);
-    for (int i = 0; i < n_prims; i++) out.prim_order[i] = refs[i].prim;
-    return out;
+    return build_bvh_refs<DIM>(refs, heuristic, leaf_size, n_buckets, parallel_cutoff);
 }
 
 // Bvh::refit (bvh.inl:548-599): leaf boxes from the primitives' inflated

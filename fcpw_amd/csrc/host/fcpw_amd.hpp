@@ -23,6 +23,7 @@
 
 #include "../../../include/fcpw_hip.h"
 #include "bvh_build.hpp"
+#include "instances.hpp"
 #include "snch_build.hpp"
 
 namespace fcpw_amd {
@@ -117,6 +118,9 @@ public:
         objects_.assign((size_t)nObjects, Object());
         bvh_.reset();
         snch_.reset();
+        object_bvh_.clear();
+        instances_.clear();
+        instance_tree_.reset();
         silhouettes_ = false;
     }
     void setObjectVertexCount(int nVertices, int objectIndex)
@@ -157,26 +161,46 @@ public:
         o.indices.assign(indices, indices + (size_t)nPrims * DIM);
     }
 
+    // fcpw.inl:392-397: appends instance transforms to an object; each is a
+    // (DIM+1) x (DIM+1) row-major affine matrix with uniform scaling (fcpw.h:68)
+    void setObjectInstanceTransforms(const float *transforms, int nTransforms, int objectIndex)
+    {
+        Object &o = obj(objectIndex);
+        for (int k = 0; k < nTransforms; k++) {
+            float t[12];
+            transform_from_matrix<DIM>(transforms + (size_t)k * (DIM + 1) * (DIM + 1), t);
+            o.transforms.insert(o.transforms.end(), t, t + 12);
+        }
+    }
+
     // fcpw.inl:429-521: marks the scene for silhouette queries; the silhouette
     // data is built with the BVH by the next build() (as the reference, whose
     // build creates the SNCH variant when silhouettes were computed)
     void computeSilhouettes(const IgnoreSilhouetteFn &ignoreSilhouette = {})
     {
         if (objects_.size() != 1)
-            throw std::runtime_error("the MI355X backend supports scenes with exactly one object (fcpw_gpu.h:17-20)");
+            throw std::runtime_error("silhouettes need a scene of exactly one object on the MI355X backend");
         silhouettes_ = true;
         ignore_ = ignoreSilhouette;
     }
 
-    // Scene::build (fcpw.inl:1052-1126) for a single object with a binary BVH.
+    // Scene::build (fcpw.inl:1052-1126) with binary BVHs.  One object without
+    // instance transforms gets one tree; otherwise every object gets its own and
+    // the instances (one per transform, or the object itself) a tree above them.
     // vectorize is accepted and ignored: the GPU path needs the binary BVH
     // (fcpw_gpu.h:17-20), and no vectorized CPU traversal exists here.
     void build(AggregateType aggregateType, bool vectorize, bool printStats = false, bool reduceMemoryFootprint = false)
     {
         (void)vectorize;
         (void)reduceMemoryFootprint;
-        if (objects_.size() != 1)
-            throw std::runtime_error("the MI355X backend supports scenes with exactly one object (fcpw_gpu.h:17-20)");
+        if (objects_.empty()) throw std::runtime_error("Cannot construct a BVH without primitives");
+        object_bvh_.clear();
+        instances_.clear();
+        instance_tree_.reset();
+        if (objects_.size() > 1 || !objects_[0].transforms.empty()) {
+            buildInstanced(aggregateType, printStats);
+            return;
+        }
         Object &o = objects_[0];
         if (!o.has_prims || o.indices.empty()) throw std::runtime_error("Cannot construct a BVH without primitives");
         aggregate_ = aggregateType;
@@ -208,12 +232,19 @@ public:
             throw std::invalid_argument("updateObjectVertices: vertex count differs from the scene's");
         o.positions.assign(positions, positions + (size_t)nVertices * DIM);
     }
-    // Scene::refit (fcpw.inl:1145-1158) on the host copy of the BVH
+    // Scene::refit (fcpw.inl:1145-1158) on the host copy of the BVH (with
+    // instances: every object's tree, then the tree over the instances)
     void refit(bool printStats = false)
     {
-        require_bvh();
         auto t0 = std::chrono::high_resolution_clock::now();
-        refit_bvh<DIM>(*bvh_, objects_[0].positions.data(), objects_[0].indices.data());
+        if (instance_tree_) {
+            for (size_t i = 0; i < objects_.size(); i++)
+                refit_bvh<DIM>(*object_bvh_[i], objects_[i].positions.data(), objects_[i].indices.data());
+            refit_instance_tree<DIM>(*instance_tree_, objectBvhs(), instances_);
+        } else {
+            require_bvh();
+            refit_bvh<DIM>(*bvh_, objects_[0].positions.data(), objects_[0].indices.data());
+        }
         if (snch_) refit_snch_cones<DIM>(*snch_, *bvh_, objects_[0].positions.data());
         if (printStats) {
             double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
@@ -236,7 +267,27 @@ public:
     }
 
     // accessors used by GPUScene
-    bool built() const { return (bool)bvh_; }
+    bool built() const { return (bool)bvh_ || (bool)instance_tree_; }
+    // several objects or instance transforms: per-object trees and the instance tree
+    bool instanced() const { return (bool)instance_tree_; }
+    // build() will make an instance tree (several objects or instance transforms)
+    bool needsInstances() const { return objects_.size() > 1 || (!objects_.empty() && !objects_[0].transforms.empty()); }
+    int nObjects() const { return (int)objects_.size(); }
+    const FlatBvh<DIM> &objectBvh(int i) const
+    {
+        if (!instance_tree_) return bvh();
+        return *object_bvh_.at((size_t)i);
+    }
+    const std::vector<float> &objectPositions(int i) const { return objects_.at((size_t)i).positions; }
+    const std::vector<int32_t> &objectIndices(int i) const { return objects_.at((size_t)i).indices; }
+    int objectNVertices(int i) const { return (int)(objects_.at((size_t)i).positions.size() / DIM); }
+    int objectNPrimitives(int i) const { return (int)(objects_.at((size_t)i).indices.size() / DIM); }
+    const std::vector<fcpw_hip_instance> &instances() const { return instances_; }
+    const FlatBvh<DIM> &instanceTree() const
+    {
+        if (!instance_tree_) throw std::runtime_error("the scene has no instance tree (one object, no transforms)");
+        return *instance_tree_;
+    }
     const FlatBvh<DIM> &bvh() const { require_bvh(); return *bvh_; }
     const std::vector<float> &positions() const { return objects_.at(0).positions; }
     const std::vector<int32_t> &indices() const { return objects_.at(0).indices; }
@@ -255,8 +306,59 @@ private:
     struct Object {
         std::vector<float> positions;
         std::vector<int32_t> indices;
+        std::vector<float> transforms; // 12 floats (3x4 row-major) per instance transform
         bool has_prims = false;
     };
+    std::vector<const FlatBvh<DIM> *> objectBvhs() const
+    {
+        std::vector<const FlatBvh<DIM> *> v;
+        for (const auto &b : object_bvh_) v.push_back(b.get());
+        return v;
+    }
+    // fcpw.inl:1060-1112 for several objects or instances
+    void buildInstanced(AggregateType aggregateType, bool printStats)
+    {
+        if (silhouettes_)
+            throw std::runtime_error("silhouettes need a scene of exactly one object on the MI355X backend");
+        auto t0 = std::chrono::high_resolution_clock::now();
+        CostHeuristic h = heuristic_of(aggregateType);
+        std::vector<ObjectGeometry> geo;
+        for (size_t i = 0; i < objects_.size(); i++) {
+            const Object &o = objects_[i];
+            if (!o.has_prims || o.indices.empty()) throw std::runtime_error("Cannot construct a BVH without primitives");
+            object_bvh_.push_back(std::make_unique<FlatBvh<DIM>>(
+                build_bvh<DIM>(o.positions.data(), (int)(o.positions.size() / DIM), o.indices.data(),
+                               (int)(o.indices.size() / DIM), h)));
+            geo.push_back(ObjectGeometry{o.positions.data(), o.indices.data()});
+        }
+        for (size_t i = 0; i < objects_.size(); i++) {
+            const Object &o = objects_[i];
+            size_t nt = o.transforms.size() / 12;
+            if (nt == 0) {
+                fcpw_hip_instance I{};
+                I.object = (int32_t)i;
+                instances_.push_back(I);
+            }
+            for (size_t k = 0; k < nt; k++) {
+                fcpw_hip_instance I{};
+                I.object = (int32_t)i;
+                I.transformed = 1;
+                for (int j = 0; j < 12; j++) I.t[j] = o.transforms[k * 12 + j];
+                invert_transform<DIM>(I.t, I.tinv);
+                instances_.push_back(I);
+            }
+        }
+        instance_tree_ = std::make_unique<FlatBvh<DIM>>(build_instance_tree<DIM>(objectBvhs(), geo, instances_, h));
+        bvh_.reset();
+        snch_.reset();
+        aggregate_ = aggregateType;
+        if (printStats) {
+            double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+            printf("BVH construction time: %g seconds (%zu objects, %zu instances)\n", s, objects_.size(),
+                   instances_.size());
+        }
+        ++version_;
+    }
     Object &obj(int i)
     {
         if (i < 0 || (size_t)i >= objects_.size()) throw std::out_of_range("object index out of range");
@@ -272,6 +374,9 @@ private:
     }
     std::vector<Object> objects_;
     std::unique_ptr<FlatBvh<DIM>> bvh_;
+    std::vector<std::unique_ptr<FlatBvh<DIM>>> object_bvh_;
+    std::vector<fcpw_hip_instance> instances_;
+    std::unique_ptr<FlatBvh<DIM>> instance_tree_;
     std::unique_ptr<SnchData<DIM>> snch_;
     bool silhouettes_ = false;
     IgnoreSilhouetteFn ignore_;
@@ -302,6 +407,7 @@ public:
     ~GPUScene()
     {
         if (ctx_) fcpw_hip_destroy(ctx_);
+        for (fcpw_hip_ctx *o : objs_) fcpw_hip_destroy(o);
     }
     GPUScene(const GPUScene &) = delete;
     GPUScene &operator=(const GPUScene &) = delete;
@@ -311,33 +417,20 @@ public:
     {
         (void)deviceBackend; // HIP is the only backend
         if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
-        const FlatBvh<DIM> &b = scene.bvh();
-        size_t n = b.nodes.size();
-        std::vector<float> pmin(n * 3, 0.0f), pmax(n * 3, 0.0f);
-        std::vector<int32_t> nrefs(n), offset(n);
-        for (size_t i = 0; i < n; i++) {
-            for (int d = 0; d < DIM; d++) {
-                pmin[i * 3 + d] = b.nodes[i].box.lo[d];
-                pmax[i * 3 + d] = b.nodes[i].box.hi[d];
-            }
-            nrefs[i] = b.nodes[i].nrefs;
-            offset[i] = b.nodes[i].offset;
+        if (scene.instanced()) {
+            transferInstanced(scene);
+            return;
         }
-        int np = scene.nPrimitives();
-        std::vector<int32_t> pv((size_t)np * DIM);
-        const auto &idx = scene.indices();
-        for (int s = 0; s < np; s++)
-            for (int k = 0; k < DIM; k++) pv[(size_t)s * DIM + k] = idx[(size_t)b.prim_order[s] * DIM + k];
-        hip_check(ctx_,
-                  fcpw_hip_upload_bvh(ctx_, DIM, (int)n, pmin.data(), pmax.data(), nrefs.data(), offset.data(), np,
-                                      pv.data(), b.prim_order.data(), scene.nVertices(), scene.positions().data()),
-                  "transferToGPU");
+        const FlatBvh<DIM> &b = scene.bvh();
+        uploadTree(ctx_, b, scene.positions(), scene.indices(), "transferToGPU");
+        instanced_ = false;
+        int n = (int)b.nodes.size(), np = scene.nPrimitives();
         if (scene.hasSilhouettes()) uploadSilhouettes(scene);
         if (logs_) {
             int64_t rec = 0, depth = 0;
             fcpw_hip_get_info(ctx_, "n_records", &rec);
             fcpw_hip_get_info(ctx_, "max_depth", &depth);
-            printf("transferToGPU: %zu nodes -> %lld records, depth %lld, %d primitives\n", n, (long long)rec,
+            printf("transferToGPU: %d nodes -> %lld records, depth %lld, %d primitives\n", n, (long long)rec,
                    (long long)depth, np);
         }
     }
@@ -351,6 +444,9 @@ public:
     void buildOnGPU(Scene<DIM> &scene, AggregateType aggregateType = AggregateType::Bvh_SurfaceArea,
                     bool exportTree = false)
     {
+        if (scene.needsInstances())
+            throw std::runtime_error("buildOnGPU builds one object's tree; build scenes of several objects or "
+                                     "instances with Scene::build");
         if (scene.silhouettesRequested()) exportTree = true;
         if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
         int h = (int)heuristic_of(aggregateType);
@@ -367,6 +463,7 @@ public:
             order.resize((size_t)np);
         }
         int n_nodes = 0;
+        instanced_ = false;
         hip_check(ctx_,
                   fcpw_hip_build_bvh(ctx_, DIM, scene.positions().data(), scene.nVertices(), scene.indices().data(),
                                      np, h, exportTree ? pmin.data() : nullptr, exportTree ? pmax.data() : nullptr,
@@ -405,6 +502,17 @@ public:
     void refit(Scene<DIM> &scene, bool updateGeometry = true)
     {
         require();
+        if (instanced_) {
+            if ((int)objs_.size() != scene.nObjects()) throw std::runtime_error("refit: the scene's objects changed");
+            if (updateGeometry)
+                for (int i = 0; i < scene.nObjects(); i++)
+                    hip_check(objs_[i],
+                              fcpw_hip_update_positions(objs_[i], scene.objectPositions(i).data(),
+                                                        scene.objectNVertices(i)),
+                              "refit");
+            hip_check(ctx_, fcpw_hip_refit(ctx_, nullptr), "refit");
+            return;
+        }
         if (updateGeometry) {
             hip_check(ctx_, fcpw_hip_update_positions(ctx_, scene.positions().data(), scene.nVertices()), "refit");
             if (scene.hasSilhouettes()) uploadSilhouettes(scene);
@@ -414,7 +522,8 @@ public:
 
     // fcpw_gpu.inl:94-135 (row-major n x DIM origins and directions)
     void intersect(const float *rayOrigins, const float *rayDirections, const float *rayDistanceBounds, int64_t n,
-                   std::vector<GPUInteraction> &interactions, bool checkForOcclusion = false)
+                   std::vector<GPUInteraction> &interactions, bool checkForOcclusion = false,
+                   std::vector<int32_t> *objectIndices = nullptr)
     {
         std::vector<GPURay> rays((size_t)n);
         parallel_rows(n, [&](int64_t s, int64_t e) {
@@ -425,17 +534,19 @@ public:
                 rays[i] = GPURay(oo, dd, rayDistanceBounds[i]);
             }
         });
-        intersect(rays, interactions, checkForOcclusion);
+        intersect(rays, interactions, checkForOcclusion, objectIndices);
     }
     // fcpw_gpu.inl:137-156
     void intersect(const std::vector<GPURay> &rays, std::vector<GPUInteraction> &interactions,
-                   bool checkForOcclusion = false)
+                   bool checkForOcclusion = false, std::vector<int32_t> *objectIndices = nullptr)
     {
         require();
         interactions.assign(rays.size(), GPUInteraction());
+        if (objectIndices) objectIndices->assign(rays.size(), -1);
         hip_check(ctx_,
-                  fcpw_hip_intersect(ctx_, (const fcpw_hip_ray *)rays.data(), (int64_t)rays.size(),
-                                     checkForOcclusion ? 1 : 0, (fcpw_hip_interaction *)interactions.data()),
+                  fcpw_hip_intersect_objects(ctx_, (const fcpw_hip_ray *)rays.data(), (int64_t)rays.size(),
+                                             checkForOcclusion ? 1 : 0, (fcpw_hip_interaction *)interactions.data(),
+                                             objectIndices ? objectIndices->data() : nullptr),
                   "intersect");
     }
     // fcpw_gpu.inl:158-196 (row-major n x DIM centers, n x 3 random numbers in [0, 1)); with raw
@@ -472,7 +583,8 @@ public:
 
     // fcpw_gpu.inl:219-253 (row-major n x DIM points)
     void findClosestPoints(const float *queryPoints, const float *squaredMaxRadii, int64_t n,
-                           std::vector<GPUInteraction> &interactions, bool recordNormals = false)
+                           std::vector<GPUInteraction> &interactions, bool recordNormals = false,
+                           std::vector<int32_t> *objectIndices = nullptr)
     {
         std::vector<GPUBoundingSphere> spheres((size_t)n);
         parallel_rows(n, [&](int64_t s, int64_t e) {
@@ -481,18 +593,21 @@ public:
                 spheres[i] = GPUBoundingSphere(float3{q[0], q[1], DIM == 2 ? 0.0f : q[DIM - 1]}, squaredMaxRadii[i]);
             }
         });
-        findClosestPoints(spheres, interactions, recordNormals);
+        findClosestPoints(spheres, interactions, recordNormals, objectIndices);
     }
     // fcpw_gpu.inl:255-274
+    // objectIndices (optional): Interaction::objectIndex per query, -1 on a miss
     void findClosestPoints(const std::vector<GPUBoundingSphere> &spheres, std::vector<GPUInteraction> &interactions,
-                           bool recordNormals = false)
+                           bool recordNormals = false, std::vector<int32_t> *objectIndices = nullptr)
     {
         require();
         interactions.assign(spheres.size(), GPUInteraction());
+        if (objectIndices) objectIndices->assign(spheres.size(), -1);
         hip_check(ctx_,
-                  fcpw_hip_closest_points(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
-                                          (int64_t)spheres.size(), recordNormals ? 1 : 0,
-                                          (fcpw_hip_interaction *)interactions.data()),
+                  fcpw_hip_closest_points_objects(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
+                                                  (int64_t)spheres.size(), recordNormals ? 1 : 0,
+                                                  (fcpw_hip_interaction *)interactions.data(),
+                                                  objectIndices ? objectIndices->data() : nullptr),
                   "findClosestPoints");
     }
 
@@ -538,10 +653,74 @@ public:
         return ctx_;
     }
 
+    bool instanced() const { return instanced_; }
+
 private:
     void require()
     {
         if (!ctx_) throw std::runtime_error("the BVH must be transferred to the GPU first (call transferToGPU)");
+    }
+    // one object's flat tree and primitives (extractBvhNodes / extractTriangles,
+    // bvh_interop_structures.h:172-275)
+    static void uploadTree(fcpw_hip_ctx *ctx, const FlatBvh<DIM> &b, const std::vector<float> &positions,
+                           const std::vector<int32_t> &idx, const char *what)
+    {
+        size_t n = b.nodes.size();
+        std::vector<float> pmin(n * 3, 0.0f), pmax(n * 3, 0.0f);
+        std::vector<int32_t> nrefs(n), offset(n);
+        flatten(b, pmin, pmax, nrefs, offset);
+        int np = (int)b.prim_order.size();
+        std::vector<int32_t> pv((size_t)np * DIM);
+        for (int s = 0; s < np; s++)
+            for (int k = 0; k < DIM; k++) pv[(size_t)s * DIM + k] = idx[(size_t)b.prim_order[s] * DIM + k];
+        hip_check(ctx,
+                  fcpw_hip_upload_bvh(ctx, DIM, (int)n, pmin.data(), pmax.data(), nrefs.data(), offset.data(), np,
+                                      pv.data(), b.prim_order.data(), (int)(positions.size() / DIM),
+                                      positions.data()),
+                  what);
+    }
+    static void flatten(const FlatBvh<DIM> &b, std::vector<float> &pmin, std::vector<float> &pmax,
+                        std::vector<int32_t> &nrefs, std::vector<int32_t> &offset)
+    {
+        for (size_t i = 0; i < b.nodes.size(); i++) {
+            for (int d = 0; d < DIM; d++) {
+                pmin[i * 3 + d] = b.nodes[i].box.lo[d];
+                pmax[i * 3 + d] = b.nodes[i].box.hi[d];
+            }
+            nrefs[i] = b.nodes[i].nrefs;
+            offset[i] = b.nodes[i].offset;
+        }
+    }
+    // every object's tree into a context of its own, then the instance layer
+    void transferInstanced(Scene<DIM> &scene)
+    {
+        int n = scene.nObjects();
+        while ((int)objs_.size() > n) {
+            fcpw_hip_destroy(objs_.back());
+            objs_.pop_back();
+        }
+        while ((int)objs_.size() < n) {
+            fcpw_hip_ctx *o = nullptr;
+            hip_check(nullptr, fcpw_hip_create(device_, &o), "fcpw_hip_create");
+            objs_.push_back(o);
+        }
+        for (int i = 0; i < n; i++)
+            uploadTree(objs_[i], scene.objectBvh(i), scene.objectPositions(i), scene.objectIndices(i),
+                       "transferToGPU(object)");
+        const FlatBvh<DIM> &t = scene.instanceTree();
+        size_t nn = t.nodes.size();
+        std::vector<float> pmin(nn * 3, 0.0f), pmax(nn * 3, 0.0f);
+        std::vector<int32_t> nrefs(nn), offset(nn);
+        flatten(t, pmin, pmax, nrefs, offset);
+        const auto &inst = scene.instances();
+        hip_check(ctx_,
+                  fcpw_hip_upload_instances(ctx_, objs_.data(), n, inst.data(), (int)inst.size(), (int)nn,
+                                            pmin.data(), pmax.data(), nrefs.data(), offset.data(),
+                                            t.prim_order.data()),
+                  "transferToGPU(instances)");
+        instanced_ = true;
+        if (logs_)
+            printf("transferToGPU: %d objects, %zu instances, instance tree of %zu nodes\n", n, inst.size(), nn);
     }
     // the SNCH half of GPUBvhBuffers::allocate (bvh_interop_structures.h:193-300)
     void uploadSilhouettes(Scene<DIM> &scene)
@@ -563,6 +742,8 @@ private:
     bool logs_;
     int device_ = 0;
     fcpw_hip_ctx *ctx_ = nullptr;
+    std::vector<fcpw_hip_ctx *> objs_; // per-object contexts of an instanced scene
+    bool instanced_ = false;
 };
 
 // Several GPUScenes queried in one persistent launch (no reference counterpart:

@@ -39,6 +39,37 @@ static const float *vec(const F32 &a, int64_t n, const char *name)
     return a.data();
 }
 
+// copies per-query object indices into a caller's int32 array (None: nothing asked)
+static void give_objects(py::object dst, const std::vector<int32_t> &src)
+{
+    if (dst.is_none()) return;
+    py::array_t<int32_t> a = py::cast<py::array_t<int32_t>>(dst);
+    if (a.ndim() != 1 || (size_t)a.shape(0) != src.size() || !(a.flags() & py::array::c_style) || !a.writeable())
+        throw std::invalid_argument("object_indices must be a writable contiguous int32 array with one entry per query");
+    std::memcpy(a.mutable_data(), src.data(), src.size() * sizeof(int32_t));
+}
+
+template <int DIM>
+static py::tuple flat_nodes(const FlatBvh<DIM> &b)
+{
+    size_t n = b.nodes.size();
+    py::array_t<float> lo({(py::ssize_t)n, (py::ssize_t)3}), hi({(py::ssize_t)n, (py::ssize_t)3});
+    py::array_t<int32_t> nrefs((py::ssize_t)n), off((py::ssize_t)n);
+    auto L = lo.mutable_unchecked<2>(), H = hi.mutable_unchecked<2>();
+    auto R = nrefs.mutable_unchecked<1>(), O = off.mutable_unchecked<1>();
+    for (size_t i = 0; i < n; i++) {
+        for (int d = 0; d < 3; d++) {
+            L(i, d) = d < DIM ? b.nodes[i].box.lo[d] : 0.0f;
+            H(i, d) = d < DIM ? b.nodes[i].box.hi[d] : 0.0f;
+        }
+        R(i) = b.nodes[i].nrefs;
+        O(i) = b.nodes[i].offset;
+    }
+    py::array_t<int32_t> order((py::ssize_t)b.prim_order.size());
+    std::memcpy(order.mutable_data(), b.prim_order.data(), b.prim_order.size() * sizeof(int32_t));
+    return py::make_tuple(lo, hi, nrefs, off, order, b.max_depth, b.n_leaves);
+}
+
 template <int DIM>
 static void bind_scene(py::module_ &m, const char *name)
 {
@@ -106,6 +137,35 @@ static void bind_scene(py::module_ &m, const char *name)
             "cones (n_nodes, 5), n_silhouettes, silhouette_offset and the GPU records as bytes")
         .def_property_readonly("n_vertices", &S::nVertices)
         .def_property_readonly("n_primitives", &S::nPrimitives)
+        .def(
+            "set_object_instance_transforms",
+            [](S &s, py::object transforms, int o) {
+                F32 m = py::cast<F32>(py::module_::import("numpy").attr("asarray")(transforms, "float32"));
+                const int k = DIM + 1;
+                if (!((m.ndim() == 3 && m.shape(1) == k && m.shape(2) == k) || (m.ndim() == 2 && m.shape(0) == k &&
+                                                                                  m.shape(1) == k)))
+                    throw std::invalid_argument("transforms must be (n, " + std::to_string(k) + ", " +
+                                                std::to_string(k) + ") affine matrices");
+                int n = m.ndim() == 3 ? (int)m.shape(0) : 1;
+                s.setObjectInstanceTransforms(m.data(), n, o);
+            },
+            "transforms"_a, "object_index"_a,
+            "appends instance transforms ((DIM+1) x (DIM+1) affine matrices, uniform scaling) to an object")
+        .def_property_readonly("n_objects", &S::nObjects)
+        .def_property_readonly("is_instanced", &S::instanced)
+        .def(
+            "_instances",
+            [](const S &s) {
+                const auto &v = s.instances();
+                return py::bytes(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(fcpw_hip_instance));
+            },
+            "the instance records (fcpw_hip_instance) as bytes")
+        .def(
+            "_instance_tree_nodes", [](const S &s) { return flat_nodes<DIM>(s.instanceTree()); },
+            "instance tree as (pmin, pmax, nrefs, offset, leaf instances, max_depth, n_leaves)")
+        .def(
+            "_object_bvh_nodes", [](const S &s, int i) { return flat_nodes<DIM>(s.objectBvh(i)); }, "object_index"_a,
+            "one object's tree as (pmin, pmax, nrefs, offset, prim_order, max_depth, n_leaves)")
         .def(
             "_bvh_nodes",
             [](const S &s) {
@@ -183,24 +243,32 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
             "Builds the reference's SAH BVH on the GPU and uploads it (no reference counterpart)")
         .def(
             "_intersect_arrays",
-            [](G &g, const F32 &o, const F32 &d, const F32 &t, InteractionList &out, bool occl) {
+            [](G &g, const F32 &o, const F32 &d, const F32 &t, InteractionList &out, bool occl, py::object objs) {
                 int64_t n, n2;
                 const float *op = rows(o, n, DIM, "ray_origins");
                 const float *dp = rows(d, n2, DIM, "ray_directions");
                 if (n2 != n) throw std::invalid_argument("ray_directions and ray_origins differ in length");
                 const float *tp = vec(t, n, "ray_distance_bounds");
-                py::gil_scoped_release rel;
-                g.intersect(op, dp, tp, n, out, occl);
+                std::vector<int32_t> ob;
+                {
+                    py::gil_scoped_release rel;
+                    g.intersect(op, dp, tp, n, out, occl, objs.is_none() ? nullptr : &ob);
+                }
+                give_objects(objs, ob);
             },
             "ray_origins"_a, "ray_directions"_a, "ray_distance_bounds"_a, "interactions"_a,
-            "check_for_occlusion"_a = false)
+            "check_for_occlusion"_a = false, "object_indices"_a = py::none())
         .def(
             "_intersect_rays",
-            [](G &g, const std::vector<GPURay> &rays, InteractionList &out, bool occl) {
-                py::gil_scoped_release rel;
-                g.intersect(rays, out, occl);
+            [](G &g, const std::vector<GPURay> &rays, InteractionList &out, bool occl, py::object objs) {
+                std::vector<int32_t> ob;
+                {
+                    py::gil_scoped_release rel;
+                    g.intersect(rays, out, occl, objs.is_none() ? nullptr : &ob);
+                }
+                give_objects(objs, ob);
             },
-            "rays"_a, "interactions"_a, "check_for_occlusion"_a = false)
+            "rays"_a, "interactions"_a, "check_for_occlusion"_a = false, "object_indices"_a = py::none())
         .def(
             "_intersect_spheres",
             [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<float3> &r, InteractionList &out) {
@@ -231,21 +299,30 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
             })
         .def(
             "_find_closest_points_arrays",
-            [](G &g, const F32 &q, const F32 &r2, InteractionList &out, bool normals) {
+            [](G &g, const F32 &q, const F32 &r2, InteractionList &out, bool normals, py::object objs) {
                 int64_t n;
                 const float *qp = rows(q, n, DIM, "query_points");
                 const float *rp = vec(r2, n, "squared_max_radii");
-                py::gil_scoped_release rel;
-                g.findClosestPoints(qp, rp, n, out, normals);
+                std::vector<int32_t> ob;
+                {
+                    py::gil_scoped_release rel;
+                    g.findClosestPoints(qp, rp, n, out, normals, objs.is_none() ? nullptr : &ob);
+                }
+                give_objects(objs, ob);
             },
-            "query_points"_a, "squared_max_radii"_a, "interactions"_a, "record_normals"_a = false)
+            "query_points"_a, "squared_max_radii"_a, "interactions"_a, "record_normals"_a = false,
+            "object_indices"_a = py::none())
         .def(
             "_find_closest_points_spheres",
-            [](G &g, const std::vector<GPUBoundingSphere> &s, InteractionList &out, bool normals) {
-                py::gil_scoped_release rel;
-                g.findClosestPoints(s, out, normals);
+            [](G &g, const std::vector<GPUBoundingSphere> &s, InteractionList &out, bool normals, py::object objs) {
+                std::vector<int32_t> ob;
+                {
+                    py::gil_scoped_release rel;
+                    g.findClosestPoints(s, out, normals, objs.is_none() ? nullptr : &ob);
+                }
+                give_objects(objs, ob);
             },
-            "bounding_spheres"_a, "interactions"_a, "record_normals"_a = false)
+            "bounding_spheres"_a, "interactions"_a, "record_normals"_a = false, "object_indices"_a = py::none())
         .def(
             "_find_closest_silhouette_points",
             [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<uint32_t> &flip, InteractionList &out,
@@ -283,20 +360,24 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
         // device-pointer entry points (torch tensors pass data_ptr() and the stream handle)
         .def(
             "_closest_points_device",
-            [](G &g, uintptr_t spheres, int64_t n, bool normals, uintptr_t out, uintptr_t stream) {
+            [](G &g, uintptr_t spheres, int64_t n, bool normals, uintptr_t out, uintptr_t stream, uintptr_t obj) {
                 hip_check(g.context(),
-                          fcpw_hip_closest_points_device(g.context(), (const fcpw_hip_bounding_sphere *)spheres, n,
-                                                         normals ? 1 : 0, (fcpw_hip_interaction *)out, (void *)stream),
+                          fcpw_hip_closest_points_objects_device(g.context(), (const fcpw_hip_bounding_sphere *)spheres,
+                                                                 n, normals ? 1 : 0, (fcpw_hip_interaction *)out,
+                                                                 (int32_t *)obj, (void *)stream),
                           "closest_points_device");
-            })
+            },
+            "spheres"_a, "n"_a, "normals"_a, "out"_a, "stream"_a, "object_index"_a = 0)
         .def(
             "_intersect_device",
-            [](G &g, uintptr_t rays, int64_t n, bool occl, uintptr_t out, uintptr_t stream) {
+            [](G &g, uintptr_t rays, int64_t n, bool occl, uintptr_t out, uintptr_t stream, uintptr_t obj) {
                 hip_check(g.context(),
-                          fcpw_hip_intersect_device(g.context(), (const fcpw_hip_ray *)rays, n, occl ? 1 : 0,
-                                                    (fcpw_hip_interaction *)out, (void *)stream),
+                          fcpw_hip_intersect_objects_device(g.context(), (const fcpw_hip_ray *)rays, n, occl ? 1 : 0,
+                                                            (fcpw_hip_interaction *)out, (int32_t *)obj,
+                                                            (void *)stream),
                           "intersect_device");
-            })
+            },
+            "rays"_a, "n"_a, "occlusion"_a, "out"_a, "stream"_a, "object_index"_a = 0)
         .def(
             "_closest_points_compact_device",
             [](G &g, uintptr_t points, uintptr_t r2, int64_t n, uintptr_t d, uintptr_t p, uintptr_t prim,

@@ -242,6 +242,58 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, co
                                       void *stream);
 
 /*
+ * Scenes of several objects and transformed instances.  Replaces, for the GPU,
+ * the instance layer Scene::build puts above the objects' trees
+ * (fcpw.inl:1052-1126; TransformedAggregate, primitive.h:289-440, whose GPU
+ * twin is aggregate.slang:45-145 with transform.slang:1-56); the reference's
+ * own GPU path serves one object (fcpw_gpu.h:17-20).
+ *
+ * Each object is a context with its BVH uploaded (fcpw_hip_upload_bvh or
+ * fcpw_hip_build_bvh).  fcpw_hip_upload_instances turns ctx into the scene:
+ * instances[i] draws object instances[i].object, transformed by t when
+ * .transformed != 0 (an object without instance transforms is one
+ * untransformed instance), and the flat tree over the instances (same layout
+ * as fcpw_hip_upload_bvh; leaf slot j holds instance leaf_instances[j]) bounds
+ * them.  The object contexts must stay alive while ctx is used; re-uploading
+ * one is picked up by the next call.  A transform must be affine with uniform
+ * scaling (fcpw.h:68): 3x4 row-major, 2D in rows 0-1 with columns 0, 1
+ * (linear part) and 3 (translation); tinv is its inverse.
+ *
+ * On such a ctx, fcpw_hip_closest_points(_device) and fcpw_hip_intersect(_device)
+ * answer over all instances; index is the primitive index within its object,
+ * and the _objects variants also write the object index
+ * (Interaction::objectIndex, interaction.h:67) or -1 on a miss.  Per instance
+ * the query follows TransformedAggregate: sphere / ray mapped by tinv, the
+ * object's tree traversed, r^2 / tMax mapped back by t; a hit's point is
+ * mapped by t, its normal by transpose(tinv) (normalized) and d = |p - x|
+ * (rays: |p - o|, which equals t for unit directions).  With occlusion only
+ * index and object are written.  fcpw_hip_refit refits every object, then the
+ * instance boxes and the tree above them (bvh.inl:585-599).  Other query kinds
+ * return FCPW_HIP_ERR_UNSUPPORTED.
+ */
+typedef struct {
+    int32_t object;
+    int32_t transformed;
+    float t[12];
+    float tinv[12];
+} fcpw_hip_instance;
+
+int fcpw_hip_upload_instances(fcpw_hip_ctx *ctx, fcpw_hip_ctx *const *objects, int n_objects,
+                              const fcpw_hip_instance *instances, int n_instances, int n_nodes,
+                              const float *node_pmin, const float *node_pmax, const int32_t *node_nrefs,
+                              const int32_t *node_offset, const int32_t *leaf_instances);
+int fcpw_hip_closest_points_objects(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                    int record_normals, fcpw_hip_interaction *out, int32_t *object_index);
+int fcpw_hip_closest_points_objects_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                           int record_normals, fcpw_hip_interaction *out, int32_t *object_index,
+                                           void *stream);
+int fcpw_hip_intersect_objects(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int64_t n, int check_for_occlusion,
+                               fcpw_hip_interaction *out, int32_t *object_index);
+int fcpw_hip_intersect_objects_device(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int64_t n,
+                                      int check_for_occlusion, fcpw_hip_interaction *out, int32_t *object_index,
+                                      void *stream);
+
+/*
  * Batched queries over several uploaded contexts in one persistent launch (no
  * reference counterpart: the reference GPU path serves one object per
  * GPUScene, fcpw_gpu.h:17-20).  The scenes must share dim and device and stay

@@ -91,8 +91,14 @@ def test_scene_errors():
     s.set_object_triangles(F, 0)
     s.set_object_vertices(V, 1)
     s.set_object_triangles(F, 1)
+    # two objects build a tree per object and one over the instances; silhouettes
+    # need a scene of one object
+    s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    assert s.is_instanced and s.n_objects == 2
     with pytest.raises(RuntimeError, match="exactly one object"):
-        s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+        s.compute_silhouettes()
+    with pytest.raises(RuntimeError, match="several objects"):
+        fcpw.GPUScene3D("").build_on_gpu(s)
     s = fcpw.Scene3D()
     s.set_object_count(1)
     s.set_object_vertices(V, 0)
