@@ -20,6 +20,9 @@ The printed JSON line carries
                silhouette point kernel, on the torus built with silhouettes
   bvh_build    the torus' SAH tree built on the device against the host
                build + transfer (the same tree either way)
+  instances    a scene of 256 transformed instances of a 20,480-triangle
+               icosphere, 4,194,304 closest-point queries through the two-level
+               kernel, against the same triangles baked into one mesh
   roofline     algorithmic bytes per query x queries per launch / the traversal
                kernel's mean launch duration (HIP events on the launch stream),
                against MI355X HBM bandwidth.  Bytes per query
@@ -431,6 +434,8 @@ def run_torus(args, torch):
         log(f"[build] device {t_db * 1e3:.2f} ms vs host build {t_hb * 1e3:.1f} ms + transfer {t_tr * 1e3:.1f} ms")
         del gb, sb
 
+        res["instances"] = run_instances(args, torch, fcpw, synth, dev, stream)
+
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
@@ -447,6 +452,69 @@ def run_torus(args, torch):
         res["cpu_baseline"] = cpu_baseline(V, F, q, r2, args.cpu_sample)
         log(f"[cpu] {res['cpu_baseline']['value']:.3e} queries/s on {res['cpu_baseline']['cores']} threads")
     return res
+
+
+def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1 << 22):
+    """Scene of transformed instances: 256 copies of the 20,480-triangle noisy
+    icosphere (random rotation, uniform scale in [0.3, 1], centres uniform in
+    [-8, 8]^3), 4M closest-point queries uniform in the scene box, r2 = inf.
+    Timed against the same triangles baked into one 5.2M-triangle mesh."""
+    v, f = synth.icosphere(5)
+    V = synth.f32(synth.radial_noise(v, 0.01, synth.Rng(synth.seed(6, 1))))
+    F = f.astype(np.int32)
+    rng = synth.Rng(synth.seed(6, 2))
+    mats = []
+    for _ in range(n_inst):
+        ax = rng.sphere(1, 3)[0]
+        a = rng.uniform(1, 0.0, 2 * np.pi)[0]
+        K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+        R = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+        M = np.eye(4)
+        M[:3, :3] = rng.uniform(1, 0.3, 1.0)[0] * R
+        M[:3, 3] = rng.uniform(3, -8.0, 8.0)
+        mats.append(M)
+    mats = np.asarray(mats, np.float32)
+    t0 = time.time()
+    si = fcpw.Scene3D()
+    si.set_object_count(1)
+    si.set_object_vertices(V, 0)
+    si.set_object_triangles(F, 0)
+    si.set_object_instance_transforms(mats, 0)
+    si.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    gi = fcpw.GPUScene3D(fcpw.slang_directory_path)
+    gi.transfer_to_gpu(si)
+    t_inst_build = time.time() - t0
+    # the same instances baked into one mesh (FP64 transform, rounded once)
+    W = np.concatenate([V.astype(np.float64) @ M[:3, :3].T.astype(np.float64) + M[:3, 3] for M in mats])
+    WF = np.concatenate([F + k * len(V) for k in range(n_inst)]).astype(np.int32)
+    t0 = time.time()
+    _, gf = _scene(fcpw, synth.f32(W), WF)
+    t_flat_build = time.time() - t0
+    lo, hi = W.min(0), W.max(0)
+    q = synth.f32(synth.uniform_in_box(n_queries, lo, hi, 1.0, synth.Rng(synth.seed(6, 3))))
+    sph = _spheres(torch, q, dev)
+    out_i = torch.empty((n_queries, 10), dtype=torch.float32, device=dev)
+    out_f = torch.empty_like(out_i)
+    obj = torch.empty(n_queries, dtype=torch.int32, device=dev)
+
+    def inst_step():
+        gi.find_closest_points_device(sph, out_i, stream=stream, object_index=obj)
+
+    def flat_step():
+        gf.find_closest_points_device(sph, out_f, stream=stream)
+    t_i = timed(torch, stream, inst_step, args.steps, args.warmup)
+    t_f = timed(torch, stream, flat_step, args.steps, args.warmup)
+    di, df = out_i[:, 8].double(), out_f[:, 8].double()
+    rel = float(((di - df).abs() / df.clamp_min(1e-3)).max())
+    log(f"[instances] {n_inst} instances x {len(F)} tris: two-level {t_i * 1e3:.3f} ms "
+        f"({n_queries / t_i:.4e} q/s) vs baked mesh {t_f * 1e3:.3f} ms ({n_queries / t_f:.4e} q/s); "
+        f"max rel d difference {rel:.2e}; builds {t_inst_build:.2f}s / {t_flat_build:.2f}s")
+    return {"value": n_queries / t_i, "unit": "queries/s", "ms_per_step": t_i * 1e3,
+            "baked_mesh_value": n_queries / t_f, "baked_mesh_ms_per_step": t_f * 1e3,
+            "max_rel_d_difference": rel, "host_build_s": t_inst_build, "baked_host_build_s": t_flat_build,
+            "workload": f"{n_inst} transformed instances of a {len(F)}-triangle icosphere, {n_queries} "
+                        f"closest-point queries uniform in the scene box, r2=inf; baked = the same "
+                        f"{n_inst * len(F)} triangles as one mesh"}
 
 
 def _scene(fcpw, V, F):
