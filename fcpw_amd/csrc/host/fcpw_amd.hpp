@@ -370,6 +370,8 @@ private:
     }
     void require_bvh() const
     {
+        if (instance_tree_)
+            throw std::runtime_error("the scene has several objects or instances: its trees are per object");
         if (!bvh_) throw std::runtime_error("the scene has not been built");
     }
     std::vector<Object> objects_;

@@ -20,7 +20,7 @@ from tests import meshes, scenes
 
 # distances: 1e-5 relative plus 4 ulp of the largest world coordinate (the
 # FP32 transform of a point rounds each coordinate once more, ~8 at most here)
-REL, FLOOR = 1e-5, 1e-4
+REL = 1e-5
 EPS = float(np.finfo(np.float32).eps)
 
 
@@ -40,11 +40,6 @@ def _tie(W, WF, q, a, b):
     da = oracle.prim_dist_f64(W, WF, q, a)
     db = oracle.prim_dist_f64(W, WF, q, b)
     return np.abs(da - db) <= 1e-5 * np.maximum(da, db) + 2e-6
-
-
-def _soup_index(obj, prim, s_obj, s_prim, n_instances_of):
-    """candidate soup primitives for each (object, primitive) answer"""
-    return [np.nonzero((s_obj == o) & (s_prim == p))[0] for o, p in zip(obj, prim)]
 
 
 @pytest.mark.parametrize("name", scenes.NAMES)
