@@ -148,14 +148,14 @@ class _GPUSceneMixin:
         self._closest_silhouette_points_device(spheres.data_ptr(), fl.data_ptr(), n, float(squared_min_radius),
                                                float(precision), out.data_ptr(), _stream_handle(spheres, stream))
 
-    def intersect_spheres_device(self, spheres, rand_nums, out, stream=None):
+    def intersect_spheres_device(self, spheres, rand_nums, out, stream=None, object_index=None):
         """spheres: (n, 4) GPUBoundingSphere records, rand_nums: (n, 3) float32 in [0, 1),
-        out: (n, 10) GPUInteraction records (CUDA tensors)."""
+        out: (n, 10) GPUInteraction records, object_index: optional (n,) int32 (CUDA tensors)."""
         n = spheres.shape[0]
         if rand_nums.shape[0] != n or rand_nums.shape[-1] != 3 or not rand_nums.is_contiguous():
             raise ValueError("rand_nums must be a contiguous (n, 3) tensor")
         self._intersect_spheres_device(spheres.data_ptr(), rand_nums.data_ptr(), n, out.data_ptr(),
-                                       _stream_handle(spheres, stream))
+                                       _stream_handle(spheres, stream), _object_ptr(object_index, n))
 
     def set_option(self, key, value):
         self._set_option(key, int(value))

@@ -842,19 +842,18 @@ __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io,
 // lies inside), then a uniform point on the chosen primitive.  d = the pdf of
 // that point.  Random numbers: rnd[3i] drives the walk and the choice,
 // rnd[3i+1], rnd[3i+2] the point.
+// One sphere (x, r2) with random numbers u (walk and choice) and r1, r2s (the
+// point); returns the chosen leaf slot (-1: none) and fills p, nrm, uv and d
+// (the pdf; FLT_MAX when nothing is chosen).
 template <int DIM>
-__global__ __launch_bounds__(128) void k_intersect_spheres(DevScene S, const float4 *spheres, const float *rnd,
-                                                           int64_t n, float2 *out)
+__device__ __forceinline__ int32_t sphere_sample(const DevScene &S, typename Geo<DIM>::vec x, float r2, float u,
+                                                 float r1, float r2s, float p[3], float nrm[3], float uv[2], float &d)
 {
     using G = Geo<DIM>;
     using vec = typename G::vec;
-    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (i >= n) return;
-    const float4 s4 = spheres[i];
-    const float q[3] = {s4.x, s4.y, s4.z};
-    const float r2 = s4.w;
-    const vec x = G::load_point(q);
-    float p[3] = {0.0f, 0.0f, 0.0f}, nrm[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
+    p[0] = p[1] = p[2] = 0.0f;
+    nrm[0] = nrm[1] = nrm[2] = 0.0f;
+    uv[0] = uv[1] = 0.0f;
     float dsel = FLT_MAX;
     int32_t selected = -1;
     vec lo, hi;
@@ -863,7 +862,7 @@ __global__ __launch_bounds__(128) void k_intersect_spheres(DevScene S, const flo
     float d0, d1;
     box_sqdist(lo, hi, x, d0, d1);
     if (d0 <= r2) {
-        float maxd = d1, pdf = 1.0f, u = rnd[i * 3];
+        float maxd = d1, pdf = 1.0f;
         while (true) {
             if (cur & LEAF_BIT) {
                 const uint32_t off = cur & OFF_MASK, cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
@@ -915,14 +914,26 @@ __global__ __launch_bounds__(128) void k_intersect_spheres(DevScene S, const flo
             }
         }
     }
-    uint32_t index = FCPW_HIP_UINT_MAX;
-    if (selected >= 0) {
-        dsel *= G::sample(S, (uint32_t)selected, rnd[i * 3 + 1], rnd[i * 3 + 2], uv, p, nrm);
-        index = (uint32_t)S.prim_index[selected];
-    } else {
-        dsel = FLT_MAX;
-    }
-    write_interaction(out + i * 5, p, nrm, uv, dsel, index);
+    if (selected >= 0) dsel *= G::sample(S, (uint32_t)selected, r1, r2s, uv, p, nrm);
+    else dsel = FLT_MAX;
+    d = dsel;
+    return selected;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(128) void k_intersect_spheres(DevScene S, const float4 *spheres, const float *rnd,
+                                                           int64_t n, float2 *out)
+{
+    using G = Geo<DIM>;
+    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (i >= n) return;
+    const float4 s4 = spheres[i];
+    const float q[3] = {s4.x, s4.y, s4.z};
+    float p[3], nrm[3], uv[2], d;
+    const int32_t selected =
+        sphere_sample<DIM>(S, G::load_point(q), s4.w, rnd[i * 3], rnd[i * 3 + 1], rnd[i * 3 + 2], p, nrm, uv, d);
+    const uint32_t index = selected >= 0 ? (uint32_t)S.prim_index[selected] : FCPW_HIP_UINT_MAX;
+    write_interaction(out + i * 5, p, nrm, uv, d, index);
 }
 
 template <int DIM, int BLOCK, int K, bool STATS>

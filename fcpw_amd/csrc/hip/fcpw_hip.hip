@@ -971,8 +971,10 @@ int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_s
                                       const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
                                       void *stream)
 {
-    int st = check_single(c, "intersect_spheres_device");
+    int st = check_ready(c, "intersect_spheres_device");
     if (st) return st;
+    if (c->instanced)
+        return fcpw_hip_intersect_spheres_objects_device(c, spheres, rand_nums, n, out, nullptr, stream);
     if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres_device: bad arguments");
     if (((uintptr_t)spheres & 15) || ((uintptr_t)rand_nums & 3) || ((uintptr_t)out & 7))
@@ -985,8 +987,9 @@ int fcpw_hip_intersect_spheres_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_s
 int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
                                const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out)
 {
-    int st = check_single(c, "intersect_spheres");
+    int st = check_ready(c, "intersect_spheres");
     if (st) return st;
+    if (c->instanced) return fcpw_hip_intersect_spheres_objects(c, spheres, rand_nums, n, out, nullptr);
     if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres: bad arguments");
     if (n == 0) return FCPW_HIP_OK;

@@ -341,6 +341,123 @@ __global__ __launch_bounds__(BLOCK) void k_instances_intersect(DevScene T, const
     if (object_out) object_out[i] = best_obj;
 }
 
+// TransformedAggregate.intersect(sphere) (aggregate.slang:81-97) on one
+// instance: the sphere mapped by tinv, the object's walk and point sample with
+// (u, r1, r2s), the sampled point and normal mapped back; d (the pdf) is kept
+template <int DIM>
+__device__ __forceinline__ bool instance_sphere(const DevInstance *inst, const DevScene *objs, int ii,
+                                                typename Geo<DIM>::vec x, float r2, float u, float r1, float r2s,
+                                                int &object, int32_t &slot, float p[3], float n[3], float uv[2],
+                                                float &d)
+{
+    using vec = typename Geo<DIM>::vec;
+    float4 t[3], ti[3];
+    int transformed;
+    load_instance(inst, ii, t, ti, object, transformed);
+    const DevScene S = global_scene(objs[object]);
+    vec xl = x;
+    float r2l = r2;
+    if (transformed) {
+        xl = xf(ti, x);
+        r2l = xf_r2(ti, x, xl, r2);
+    }
+    slot = sphere_sample<DIM>(S, xl, r2l, u, r1, r2s, p, n, uv, d);
+    if (slot < 0) return false;
+    if (transformed) {
+        store_p(xf(t, Geo<DIM>::load_point(p)), p);
+        xf_normal<DIM>(ti, n);
+    }
+    return true;
+}
+
+// Sphere intersection over the instances: the random walk of
+// Bvh::processSubtreeForIntersection over aggregates (bvh.inl:1140-1250) with
+// the constant branch weight -- children overlapping the sphere chosen with
+// equal probability, then, in the leaf, each instance queried with the walk's
+// current u and the instances that find a primitive chosen uniformly
+// (reservoir weight 1); d = the instance's pdf x the walk's pdf / the number
+// of instances that found one.
+template <int DIM>
+__global__ __launch_bounds__(128) void k_instances_spheres(DevScene T, const DevInstance *inst, const DevScene *objs,
+                                                           int n_inst, const float4 *spheres, const float *rnd,
+                                                           int64_t n, float2 *out, int32_t *object_out)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (i >= n) return;
+    const float4 s4 = spheres[i];
+    const float q[3] = {s4.x, s4.y, s4.z};
+    const vec x = G::load_point(q);
+    const float r2 = s4.w, r1 = rnd[i * 3 + 1], r2s = rnd[i * 3 + 2];
+    float u = rnd[i * 3];
+    float bp[3] = {0.0f, 0.0f, 0.0f}, bn[3] = {0.0f, 0.0f, 0.0f}, buv[2] = {0.0f, 0.0f}, bd = FLT_MAX;
+    int best_obj = -1, obj;
+    int32_t best_slot = -1, slot;
+    float p[3], nn[3], uv[2], d;
+    if (n_inst == 1) {
+        if (instance_sphere<DIM>(inst, objs, T.prim_index[0], x, r2, u, r1, r2s, obj, slot, p, nn, uv, d)) {
+            best_obj = obj; best_slot = slot; bd = d;
+            for (int c = 0; c < 3; c++) { bp[c] = p[c]; bn[c] = nn[c]; }
+            buv[0] = uv[0]; buv[1] = uv[1];
+        }
+    } else {
+        vec lo, hi;
+        uint32_t cur;
+        G::root(T, lo, hi, cur);
+        float d0, d1;
+        box_sqdist(lo, hi, x, d0, d1);
+        if (d0 <= r2) {
+            float pdf = 1.0f;
+            while (true) {
+                if (cur & LEAF_BIT) {
+                    const uint32_t off = cur & OFF_MASK, cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                    float total = 0.0f;
+                    for (uint32_t k = 0; k < cnt; k++) {
+                        if (instance_sphere<DIM>(inst, objs, T.prim_index[off + k], x, r2, u, r1, r2s, obj, slot, p,
+                                                 nn, uv, d)) {
+                            total += 1.0f;
+                            const float prob = 1.0f / total;
+                            if (u < prob) {
+                                u = u / prob;
+                                best_obj = obj; best_slot = slot; bd = d * pdf;
+                                for (int c = 0; c < 3; c++) { bp[c] = p[c]; bn[c] = nn[c]; }
+                                buv[0] = uv[0]; buv[1] = uv[1];
+                            } else {
+                                u = (u - prob) / (1.0f - prob);
+                            }
+                        }
+                    }
+                    if (total > 0.0f) bd = bd / total;
+                    break;
+                }
+                vec loA, hiA, loB, hiB;
+                uint32_t lA, lB;
+                G::record(T, cur, loA, hiA, lA, loB, hiB, lB);
+                float a0, a1, b0, b1;
+                box_sqdist(loA, hiA, x, a0, a1);
+                box_sqdist(loB, hiB, x, b0, b1);
+                const float wl = a0 <= r2 ? 1.0f : 0.0f, wr = b0 <= r2 ? 1.0f : 0.0f;
+                const float tot = wl + wr;
+                if (!(tot > 0.0f)) break;
+                const float pl = wl / tot, pr = 1.0f - pl;
+                if (u < pl) {
+                    u = u / pl;
+                    cur = lA;
+                    pdf *= pl;
+                } else {
+                    u = (u - pl) / pr;
+                    cur = lB;
+                    pdf *= pr;
+                }
+            }
+        }
+    }
+    const uint32_t index = best_slot >= 0 ? (uint32_t)objs[best_obj].prim_index[best_slot] : FCPW_HIP_UINT_MAX;
+    write_interaction(out + i * 5, bp, bn, buv, bd, index);
+    if (object_out) object_out[i] = best_obj;
+}
+
 // Bvh::refit over the instances (bvh.inl:548-599) once the objects are refit:
 // each leaf bounds its instances (root box of the object, or its 2^DIM
 // corners mapped by t and expanded with +-FLT_EPSILON), then the boxes are
@@ -505,6 +622,23 @@ static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t
     return timing_end(c, s);
 }
 
+static int launch_instances_spheres(fcpw_hip_ctx *c, const void *spheres, const float *rnd, int64_t n, void *out,
+                                    int32_t *object_out, hipStream_t s)
+{
+    int st = instances_ready(c, "intersect_spheres");
+    if (st) return st;
+    DevScene T = dev_scene(c);
+    const unsigned blocks = (unsigned)((n + 127) / 128);
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_instances_spheres<3>, dim3(blocks), dim3(128), 0, s, T, c->inst, c->obj_table,
+                           c->n_instances, (const float4 *)spheres, rnd, n, (float2 *)out, object_out);
+    else
+        hipLaunchKernelGGL(k_instances_spheres<2>, dim3(blocks), dim3(128), 0, s, T, c->inst, c->obj_table,
+                           c->n_instances, (const float4 *)spheres, rnd, n, (float2 *)out, object_out);
+    HIPCHECK(c, hipGetLastError());
+    return FCPW_HIP_OK;
+}
+
 static int fill_single_object_index(fcpw_hip_ctx *c, const void *out, int64_t n, int32_t *object_out, hipStream_t s)
 {
     if (!object_out) return FCPW_HIP_OK;
@@ -667,6 +801,56 @@ int fcpw_hip_intersect_objects(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_
     if (object_index)
         HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
     return check_overflow(c);
+}
+
+int fcpw_hip_intersect_spheres_objects_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
+                                             const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
+                                             int32_t *object_index, void *stream)
+{
+    int st = check_ready(c, "intersect_spheres_device");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres_device: bad arguments");
+    if (((uintptr_t)spheres & 15) || ((uintptr_t)rand_nums & 3) || ((uintptr_t)out & 7) ||
+        ((uintptr_t)object_index & 3))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "intersect_spheres_device: spheres must be 16-byte, out 8-byte, rand_nums and object_index 4-byte "
+             "aligned");
+    if (n == 0) return FCPW_HIP_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->instanced) return launch_instances_spheres(c, spheres, (const float *)rand_nums, n, out, object_index, s);
+    st = fcpw_hip_intersect_spheres_device(c, spheres, rand_nums, n, out, stream);
+    if (st) return st;
+    return fill_single_object_index(c, out, n, object_index, s);
+}
+
+int fcpw_hip_intersect_spheres_objects(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres,
+                                      const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
+                                      int32_t *object_index)
+{
+    int st = check_ready(c, "intersect_spheres");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !rand_nums || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "intersect_spheres: bad arguments");
+    if (n == 0) return FCPW_HIP_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t sb = (size_t)n * sizeof(fcpw_hip_bounding_sphere), rb = (size_t)n * sizeof(fcpw_hip_float3);
+    const size_t rec = (size_t)n * sizeof(fcpw_hip_interaction);
+    st = grow(c, &c->stage_in, &c->stage_in_bytes, sb + rb);
+    if (st) return st;
+    st = grow(c, &c->stage_out, &c->stage_out_bytes, rec + (size_t)n * sizeof(int32_t));
+    if (st) return st;
+    int32_t *obj_dev = object_index ? (int32_t *)((char *)c->stage_out + rec) : nullptr;
+    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, sb, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy((char *)c->stage_in + sb, rand_nums, rb, hipMemcpyHostToDevice));
+    st = fcpw_hip_intersect_spheres_objects_device(c, (const fcpw_hip_bounding_sphere *)c->stage_in,
+                                                   (const fcpw_hip_float3 *)((char *)c->stage_in + sb), n,
+                                                   (fcpw_hip_interaction *)c->stage_out, obj_dev, nullptr);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(out, c->stage_out, rec, hipMemcpyDeviceToHost));
+    if (object_index)
+        HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return FCPW_HIP_OK;
 }
 
 } // extern "C"

@@ -554,7 +554,8 @@ public:
     // fcpw_gpu.inl:158-196 (row-major n x DIM centers, n x 3 random numbers in [0, 1)); with raw
     // pointers this overload would collide with the ray one, hence the distinct name
     void intersectSpheres(const float *sphereCenters, const float *sphereSquaredRadii, const float *randNums,
-                          int64_t n, std::vector<GPUInteraction> &interactions)
+                          int64_t n, std::vector<GPUInteraction> &interactions,
+                          std::vector<int32_t> *objectIndices = nullptr)
     {
         std::vector<GPUBoundingSphere> spheres((size_t)n);
         std::vector<float3> rnd((size_t)n);
@@ -566,20 +567,23 @@ public:
                 rnd[i] = float3{r[0], r[1], DIM == 2 ? 0.0f : r[2]};
             }
         });
-        intersect(spheres, rnd, interactions);
+        intersect(spheres, rnd, interactions, objectIndices);
     }
     // fcpw_gpu.inl:198-217
     void intersect(const std::vector<GPUBoundingSphere> &spheres, const std::vector<float3> &randNums,
-                   std::vector<GPUInteraction> &interactions)
+                   std::vector<GPUInteraction> &interactions, std::vector<int32_t> *objectIndices = nullptr)
     {
         if (randNums.size() != spheres.size())
             throw std::invalid_argument("intersect(spheres): one rand_nums entry per sphere");
         require();
         interactions.assign(spheres.size(), GPUInteraction());
+        if (objectIndices) objectIndices->assign(spheres.size(), -1);
         hip_check(ctx_,
-                  fcpw_hip_intersect_spheres(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
-                                             (const fcpw_hip_float3 *)randNums.data(), (int64_t)spheres.size(),
-                                             (fcpw_hip_interaction *)interactions.data()),
+                  fcpw_hip_intersect_spheres_objects(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
+                                                     (const fcpw_hip_float3 *)randNums.data(),
+                                                     (int64_t)spheres.size(),
+                                                     (fcpw_hip_interaction *)interactions.data(),
+                                                     objectIndices ? objectIndices->data() : nullptr),
                   "intersect(spheres)");
     }
 

@@ -271,32 +271,43 @@ static void bind_gpu_scene(py::module_ &m, const char *name)
             "rays"_a, "interactions"_a, "check_for_occlusion"_a = false, "object_indices"_a = py::none())
         .def(
             "_intersect_spheres",
-            [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<float3> &r, InteractionList &out) {
-                py::gil_scoped_release rel;
-                g.intersect(s, r, out);
+            [](G &g, const std::vector<GPUBoundingSphere> &s, const std::vector<float3> &r, InteractionList &out,
+               py::object objs) {
+                std::vector<int32_t> ob;
+                {
+                    py::gil_scoped_release rel;
+                    g.intersect(s, r, out, objs.is_none() ? nullptr : &ob);
+                }
+                give_objects(objs, ob);
             },
-            "bounding_spheres"_a, "rand_nums"_a, "interactions"_a)
+            "bounding_spheres"_a, "rand_nums"_a, "interactions"_a, "object_indices"_a = py::none())
         .def(
             "_intersect_spheres_arrays",
-            [](G &g, const F32 &c, const F32 &r2, const F32 &rn, InteractionList &out) {
+            [](G &g, const F32 &c, const F32 &r2, const F32 &rn, InteractionList &out, py::object objs) {
                 int64_t n, n2;
                 const float *cp = rows(c, n, DIM, "sphere_centers");
                 const float *rp = vec(r2, n, "sphere_squared_radii");
                 const float *np_ = rows(rn, n2, 3, "rand_nums");
                 if (n2 != n) throw std::invalid_argument("rand_nums and sphere_centers differ in length");
-                py::gil_scoped_release rel;
-                g.intersectSpheres(cp, rp, np_, n, out);
+                std::vector<int32_t> ob;
+                {
+                    py::gil_scoped_release rel;
+                    g.intersectSpheres(cp, rp, np_, n, out, objs.is_none() ? nullptr : &ob);
+                }
+                give_objects(objs, ob);
             },
-            "sphere_centers"_a, "sphere_squared_radii"_a, "rand_nums"_a, "interactions"_a)
+            "sphere_centers"_a, "sphere_squared_radii"_a, "rand_nums"_a, "interactions"_a,
+            "object_indices"_a = py::none())
         .def(
             "_intersect_spheres_device",
-            [](G &g, uintptr_t spheres, uintptr_t rnd, int64_t n, uintptr_t out, uintptr_t stream) {
+            [](G &g, uintptr_t spheres, uintptr_t rnd, int64_t n, uintptr_t out, uintptr_t stream, uintptr_t obj) {
                 hip_check(g.context(),
-                          fcpw_hip_intersect_spheres_device(g.context(), (const fcpw_hip_bounding_sphere *)spheres,
-                                                            (const fcpw_hip_float3 *)rnd, n,
-                                                            (fcpw_hip_interaction *)out, (void *)stream),
+                          fcpw_hip_intersect_spheres_objects_device(
+                              g.context(), (const fcpw_hip_bounding_sphere *)spheres, (const fcpw_hip_float3 *)rnd, n,
+                              (fcpw_hip_interaction *)out, (int32_t *)obj, (void *)stream),
                           "intersect_spheres_device");
-            })
+            },
+            "spheres"_a, "rand_nums"_a, "n"_a, "out"_a, "stream"_a, "object_index"_a = 0)
         .def(
             "_find_closest_points_arrays",
             [](G &g, const F32 &q, const F32 &r2, InteractionList &out, bool normals, py::object objs) {

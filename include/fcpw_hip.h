@@ -259,16 +259,22 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, co
  * scaling (fcpw.h:68): 3x4 row-major, 2D in rows 0-1 with columns 0, 1
  * (linear part) and 3 (translation); tinv is its inverse.
  *
- * On such a ctx, fcpw_hip_closest_points(_device) and fcpw_hip_intersect(_device)
- * answer over all instances; index is the primitive index within its object,
- * and the _objects variants also write the object index
- * (Interaction::objectIndex, interaction.h:67) or -1 on a miss.  Per instance
- * the query follows TransformedAggregate: sphere / ray mapped by tinv, the
- * object's tree traversed, r^2 / tMax mapped back by t; a hit's point is
- * mapped by t, its normal by transpose(tinv) (normalized) and d = |p - x|
- * (rays: |p - o|, which equals t for unit directions).  With occlusion only
- * index and object are written.  fcpw_hip_refit refits every object, then the
- * instance boxes and the tree above them (bvh.inl:585-599).  Other query kinds
+ * On such a ctx, fcpw_hip_closest_points(_device), fcpw_hip_intersect(_device)
+ * and fcpw_hip_intersect_spheres(_device) answer over all instances; index is
+ * the primitive index within its object, and the _objects variants also write
+ * the object index (Interaction::objectIndex, interaction.h:67) or -1 on a
+ * miss (0 for every hit of a one-object scene).  Per instance the query follows
+ * TransformedAggregate: sphere / ray mapped by tinv, the object's tree
+ * traversed, r^2 / tMax mapped back by t; a hit's point is mapped by t, its
+ * normal by transpose(tinv) (normalized) and d = |p - x| (rays: |p - o|, which
+ * equals t for unit directions).  With occlusion only index and object are
+ * written.  Sphere intersection walks the instance tree as
+ * Bvh::processSubtreeForIntersection does over aggregates (bvh.inl:1140-1250):
+ * in the reached leaf, each instance is sampled with the walk's current random
+ * number and one of those that find a primitive is chosen uniformly; d = the
+ * instance's pdf (in its own space, as TransformedAggregate keeps it) times the
+ * walk's pdf over that count.  fcpw_hip_refit refits every object, then the
+ * instance boxes and the tree above them (bvh.inl:585-599).  Silhouette queries
  * return FCPW_HIP_ERR_UNSUPPORTED.
  */
 typedef struct {
@@ -292,6 +298,12 @@ int fcpw_hip_intersect_objects(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int6
 int fcpw_hip_intersect_objects_device(fcpw_hip_ctx *ctx, const fcpw_hip_ray *rays, int64_t n,
                                       int check_for_occlusion, fcpw_hip_interaction *out, int32_t *object_index,
                                       void *stream);
+int fcpw_hip_intersect_spheres_objects(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
+                                       const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
+                                       int32_t *object_index);
+int fcpw_hip_intersect_spheres_objects_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres,
+                                              const fcpw_hip_float3 *rand_nums, int64_t n, fcpw_hip_interaction *out,
+                                              int32_t *object_index, void *stream);
 
 /*
  * Batched queries over several uploaded contexts in one persistent launch (no

@@ -132,6 +132,8 @@ def _declare(L):
                                                   oi32]
     L.oracle_instances_intersect.argtypes = [c_void, c_void, _instp, _f32p, _f32p, _f32p, c_int, c_int, c_int,
                                              _hitp, oi32]
+    L.oracle_instances_intersect_spheres.argtypes = [c_void, c_void, _instp, _f32p, _f32p, c_int, c_int, _hitp,
+                                                     oi32]
     L.oracle_closest_silhouette_f64.restype = c_int
     L.oracle_closest_silhouette_f64.argtypes = [c_int, _f32p, c_int, _i32p, c_int, _f32p, c_int,
                                                 ctypes.c_double, ctypes.c_double, c_int, _f64p, _f64p, _f64p]
@@ -468,4 +470,17 @@ class Instances:
         obj = np.zeros(len(o), np.int32)
         lib().oracle_instances_intersect(self._h, ctypes.cast(self._ptrs, ctypes.c_void_p), self.instances,
                                          o, d, tmax, len(o), int(occlusion), nthreads, out, obj)
+        return out, obj
+
+    def intersect_spheres(self, centers, r2, rand_nums, nthreads=0):
+        """sphere intersection over the instances: (hits, object index per query)"""
+        c = _f32(centers, self.dim)
+        sph = np.zeros((len(c), 4), np.float32)
+        sph[:, :self.dim] = c
+        sph[:, 3] = _f32(r2).ravel()
+        rn = np.ascontiguousarray(rand_nums, dtype=np.float32).reshape(len(c), 3)
+        out = np.zeros(len(c), HIT_DTYPE)
+        obj = np.zeros(len(c), np.int32)
+        lib().oracle_instances_intersect_spheres(self._h, ctypes.cast(self._ptrs, ctypes.c_void_p), self.instances,
+                                                 sph, rn, len(c), nthreads, out, obj)
         return out, obj

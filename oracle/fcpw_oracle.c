@@ -1309,6 +1309,89 @@ typedef struct {
     oracle_hit *out;
 } sph_ctx;
 
+/* One sphere (x, r2) with random numbers rn[0] (walk and choice), rn[1], rn[2]
+   (the point); returns 1 and fills out, or 0 with out the default record. */
+static int sph_gpu_one(const oracle_bvh *h, const float *x, float r2, const float *rn, oracle_hit *out)
+{
+    int dim = h->dim;
+    oracle_hit it;
+    hit_default(&it);
+    int selected = -1, did = 0;
+    float d0, d1, d2, d3;
+    box_sqdist_gpu(&h->nodes[0].box, x, dim, &d0, &d1);
+    if (d0 <= r2) {
+        float maxd = d1, pdf = 1.0f, u = rn[0];
+        int cur = 0;
+        for (;;) {
+            const node_t *nd = &h->nodes[cur];
+            if (nd->nrefs > 0) {
+                float total = 0.0f;
+                for (int k = 0; k < nd->nrefs; k++) {
+                    int slot = nd->offset + k, hitp = 0;
+                    oracle_hit cand;
+                    hit_default(&cand);
+                    if (maxd <= r2) {
+                        hitp = 1;
+                    } else {
+                        float pt[3], uv[2];
+                        float dd = prim_closest(h, slot, x, pt, uv, gdot);
+                        if (dd * dd <= r2) {
+                            hitp = 1;
+                            memcpy(cand.p, pt, sizeof(pt));
+                            memcpy(cand.uv, uv, sizeof(uv));
+                        }
+                    }
+                    if (hitp) {
+                        cand.d = prim_area_gpu(h, slot);
+                        cand.index = h->prim_orig[slot];
+                        did = 1;
+                        total += cand.d;
+                        float prob = cand.d / total;
+                        if (u < prob) {
+                            u = u / prob;
+                            it = cand;
+                            it.d *= pdf;
+                            selected = slot;
+                        } else {
+                            u = (u - prob) / (1.0f - prob);
+                        }
+                    }
+                }
+                if (total > 0.0f) it.d /= total;
+                break;
+            }
+            int l = cur + 1, r = cur + nd->offset;
+            box_sqdist_gpu(&h->nodes[l].box, x, dim, &d0, &d1);
+            float wl = (d0 <= r2) ? 1.0f : 0.0f;
+            box_sqdist_gpu(&h->nodes[r].box, x, dim, &d2, &d3);
+            float wr = (d2 <= r2) ? 1.0f : 0.0f;
+            float tot = wl + wr;
+            if (!(tot > 0.0f)) break;
+            float pl = wl / tot, pr = 1.0f - pl;
+            if (u < pl) {
+                u = u / pl;
+                cur = l;
+                pdf *= pl;
+                maxd = d1;
+            } else {
+                u = (u - pl) / pr;
+                cur = r;
+                pdf *= pr;
+                maxd = d3;
+            }
+        }
+    }
+    if (did && (it.index == -1 || selected < 0)) did = 0;
+    if (did) {
+        float spdf = prim_sample_gpu(h, selected, rn[1], rn[2], it.uv, it.p, it.n);
+        it.d *= spdf;
+    } else {
+        hit_default(&it);
+    }
+    *out = it;
+    return did;
+}
+
 static void sph_range(void *vctx, int start, int end)
 {
     sph_ctx *c = (sph_ctx *)vctx;
@@ -1317,83 +1400,7 @@ static void sph_range(void *vctx, int start, int end)
     for (int qi = start; qi < end; qi++) {
         float x[3] = {c->spheres[(size_t)qi * 4], c->spheres[(size_t)qi * 4 + 1], 0.0f};
         if (dim == 3) x[2] = c->spheres[(size_t)qi * 4 + 2];
-        float r2 = c->spheres[(size_t)qi * 4 + 3];
-        const float *rn = c->rnd + (size_t)qi * 3;
-        oracle_hit it;
-        hit_default(&it);
-        int selected = -1, did = 0;
-        float d0, d1, d2, d3;
-        box_sqdist_gpu(&h->nodes[0].box, x, dim, &d0, &d1);
-        if (d0 <= r2) {
-            float maxd = d1, pdf = 1.0f, u = rn[0];
-            int cur = 0;
-            for (;;) {
-                const node_t *nd = &h->nodes[cur];
-                if (nd->nrefs > 0) {
-                    float total = 0.0f;
-                    for (int k = 0; k < nd->nrefs; k++) {
-                        int slot = nd->offset + k, hitp = 0;
-                        oracle_hit cand;
-                        hit_default(&cand);
-                        if (maxd <= r2) {
-                            hitp = 1;
-                        } else {
-                            float pt[3], uv[2];
-                            float dd = prim_closest(h, slot, x, pt, uv, gdot);
-                            if (dd * dd <= r2) {
-                                hitp = 1;
-                                memcpy(cand.p, pt, sizeof(pt));
-                                memcpy(cand.uv, uv, sizeof(uv));
-                            }
-                        }
-                        if (hitp) {
-                            cand.d = prim_area_gpu(h, slot);
-                            cand.index = h->prim_orig[slot];
-                            did = 1;
-                            total += cand.d;
-                            float prob = cand.d / total;
-                            if (u < prob) {
-                                u = u / prob;
-                                it = cand;
-                                it.d *= pdf;
-                                selected = slot;
-                            } else {
-                                u = (u - prob) / (1.0f - prob);
-                            }
-                        }
-                    }
-                    if (total > 0.0f) it.d /= total;
-                    break;
-                }
-                int l = cur + 1, r = cur + nd->offset;
-                box_sqdist_gpu(&h->nodes[l].box, x, dim, &d0, &d1);
-                float wl = (d0 <= r2) ? 1.0f : 0.0f;
-                box_sqdist_gpu(&h->nodes[r].box, x, dim, &d2, &d3);
-                float wr = (d2 <= r2) ? 1.0f : 0.0f;
-                float tot = wl + wr;
-                if (!(tot > 0.0f)) break;
-                float pl = wl / tot, pr = 1.0f - pl;
-                if (u < pl) {
-                    u = u / pl;
-                    cur = l;
-                    pdf *= pl;
-                    maxd = d1;
-                } else {
-                    u = (u - pl) / pr;
-                    cur = r;
-                    pdf *= pr;
-                    maxd = d3;
-                }
-            }
-        }
-        if (did && (it.index == -1 || selected < 0)) did = 0;
-        if (did) {
-            float spdf = prim_sample_gpu(h, selected, rn[1], rn[2], it.uv, it.p, it.n);
-            it.d *= spdf;
-        } else {
-            hit_default(&it);
-        }
-        c->out[qi] = it;
+        sph_gpu_one(h, x, c->spheres[(size_t)qi * 4 + 3], c->rnd + (size_t)qi * 3, &c->out[qi]);
     }
 }
 
@@ -2867,4 +2874,122 @@ void oracle_instances_intersect(const oracle_bvh *tlas, oracle_bvh *const *objec
     c.out = out;
     c.object_out = object_out;
     parallel_for(n, nthreads, inst_ray_range, &c);
+}
+
+/* TransformedAggregate.intersect(sphere) (aggregate.slang:81-97): the sphere
+   mapped by tInv, the object's walk and point sample, point and normal mapped
+   back; d (the pdf) is kept */
+static int inst_sph(const inst_ctx *c, int ii, const float *x, float r2, const float *rn, oracle_hit *hit)
+{
+    const oracle_instance *I = &c->inst[ii];
+    const oracle_bvh *h = c->objects[I->object];
+    int dim = h->dim;
+    float xl[3] = {x[0], x[1], x[2]}, r2l = r2;
+    if (I->transformed) {
+        xf_point(I->tinv, x, dim, xl);
+        r2l = xf_r2(I->tinv, x, xl, r2, dim);
+    }
+    if (!sph_gpu_one(h, xl, r2l, rn, hit)) return 0;
+    if (I->transformed) {
+        float p[3], n[3];
+        xf_point(I->t, hit->p, dim, p);
+        memcpy(hit->p, p, sizeof(p));
+        xf_normal(I->tinv, hit->n, dim, n);
+        memcpy(hit->n, n, sizeof(n));
+    }
+    return 1;
+}
+
+/* Bvh::processSubtreeForIntersection over aggregates (bvh.inl:1140-1250) with
+   the constant branch weight, in GPU arithmetic: the walk picks, with equal
+   probability, a child overlapping the sphere; in the reached leaf every
+   instance is sampled with the walk's current u (and rn[1], rn[2]) and those
+   that find a primitive are chosen among uniformly (reservoir weight 1); the
+   chosen record's d is multiplied by the walk's pdf and divided by that count */
+static void inst_sph_range(void *vctx, int start, int end)
+{
+    inst_ctx *c = (inst_ctx *)vctx;
+    const oracle_bvh *T = c->tlas;
+    int dim = T->dim;
+    for (int qi = start; qi < end; qi++) {
+        float x[3] = {c->q[(size_t)qi * 4], c->q[(size_t)qi * 4 + 1], 0.0f};
+        if (dim == 3) x[2] = c->q[(size_t)qi * 4 + 2];
+        float r2 = c->q[(size_t)qi * 4 + 3];
+        const float *rn = c->r2 + (size_t)qi * 3;
+        oracle_hit best, cand;
+        hit_default(&best);
+        int best_inst = -1;
+        if (T->n_prims == 1) {
+            if (inst_sph(c, T->prim_orig[0], x, r2, rn, &cand)) {
+                best = cand;
+                best_inst = T->prim_orig[0];
+            }
+        } else {
+            float d0, d1, d2, d3;
+            box_sqdist_gpu(&T->nodes[0].box, x, dim, &d0, &d1);
+            if (d0 <= r2) {
+                float pdf = 1.0f, u = rn[0];
+                int cur = 0;
+                for (;;) {
+                    const node_t *nd = &T->nodes[cur];
+                    if (nd->nrefs > 0) {
+                        float total = 0.0f;
+                        for (int k = 0; k < nd->nrefs; k++) {
+                            int ii = T->prim_orig[nd->offset + k];
+                            float mr[3] = {u, rn[1], rn[2]};
+                            if (inst_sph(c, ii, x, r2, mr, &cand)) {
+                                total += 1.0f;
+                                float prob = 1.0f / total;
+                                if (u < prob) {
+                                    u = u / prob;
+                                    best = cand;
+                                    best.d *= pdf;
+                                    best_inst = ii;
+                                } else {
+                                    u = (u - prob) / (1.0f - prob);
+                                }
+                            }
+                        }
+                        if (total > 0.0f) best.d /= total;
+                        break;
+                    }
+                    int l = cur + 1, r = cur + nd->offset;
+                    box_sqdist_gpu(&T->nodes[l].box, x, dim, &d0, &d1);
+                    float wl = (d0 <= r2) ? 1.0f : 0.0f;
+                    box_sqdist_gpu(&T->nodes[r].box, x, dim, &d2, &d3);
+                    float wr = (d2 <= r2) ? 1.0f : 0.0f;
+                    float tot = wl + wr;
+                    if (!(tot > 0.0f)) break;
+                    float pl = wl / tot, pr = 1.0f - pl;
+                    if (u < pl) {
+                        u = u / pl;
+                        cur = l;
+                        pdf *= pl;
+                    } else {
+                        u = (u - pl) / pr;
+                        cur = r;
+                        pdf *= pr;
+                    }
+                }
+            }
+        }
+        c->out[qi] = best;
+        if (c->object_out) c->object_out[qi] = best_inst >= 0 ? c->inst[best_inst].object : -1;
+    }
+}
+
+void oracle_instances_intersect_spheres(const oracle_bvh *tlas, oracle_bvh *const *objects,
+                                        const oracle_instance *inst, const float *spheres, const float *rand_nums,
+                                        int n, int nthreads, oracle_hit *out, int32_t *object_out)
+{
+    inst_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.tlas = tlas;
+    c.objects = objects;
+    c.inst = inst;
+    c.q = spheres;
+    c.r2 = rand_nums;
+    c.out = out;
+    c.object_out = object_out;
+    parallel_for(n, nthreads, inst_sph_range, &c);
 }

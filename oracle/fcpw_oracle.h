@@ -161,6 +161,11 @@ void oracle_instances_closest_points(const oracle_bvh *tlas, oracle_bvh *const *
 void oracle_instances_intersect(const oracle_bvh *tlas, oracle_bvh *const *objects, const oracle_instance *inst,
                                 const float *origins, const float *dirs, const float *tmax, int n, int occlusion,
                                 int nthreads, oracle_hit *out, int32_t *object_out);
+/* spheres n x 4 (center, r^2), rand_nums n x 3: the walk over the instance tree
+   of bvh.inl:1140-1250, each instance sampled as aggregate.slang:81-97 */
+void oracle_instances_intersect_spheres(const oracle_bvh *tlas, oracle_bvh *const *objects,
+                                        const oracle_instance *inst, const float *spheres, const float *rand_nums,
+                                        int n, int nthreads, oracle_hit *out, int32_t *object_out);
 
 /* FP64 brute force on raw geometry (positions in user order) */
 void oracle_brute_closest_f64(int dim, const float *positions, int n_vertices, const int32_t *indices,

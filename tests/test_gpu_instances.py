@@ -1,9 +1,9 @@
 """Scenes of several objects and transformed instances on the GPU
-(fcpw_hip_upload_instances and the two-level kernels): closest points and
-rays equal the oracle's restatement (oracle.Instances, GPU order at both
-levels) bit for bit -- d, p, n, uv, primitive and object index -- on every
-scene of tests/scenes.py, with bounded and unbounded radii, occlusion, after a
-device refit, and through the device-pointer entry points."""
+(fcpw_hip_upload_instances and the two-level kernels): closest points, rays
+and sphere intersection equal the oracle's restatement (oracle.Instances, GPU
+order at both levels) bit for bit -- d, p, n, uv, primitive and object index --
+on every scene of tests/scenes.py, with bounded and unbounded radii, occlusion,
+after a device refit, and through the device-pointer entry points."""
 import numpy as np
 import pytest
 
@@ -139,13 +139,36 @@ def test_single_object_scene_reports_object_zero(gpu):
     np.testing.assert_array_equal(obj, np.where(hit, 0, -1))
 
 
-def test_unsupported_queries_raise(gpu):
+@pytest.mark.parametrize("name", scenes.NAMES)
+def test_sphere_intersection_equals_oracle(gpu, name):
+    objs, tr, s, g, bv, I = setup(name)
+    c = scenes.queries(objs, tr, 6000, seed=62)
+    rng = np.random.default_rng(63)
+    r2 = rng.uniform(0.0, 1.0, len(c)).astype(np.float32)
+    rn = rng.random((len(c), 3), dtype=np.float32)
+    hits = fcpw.GPUInteractionList()
+    obj = np.zeros(len(c), np.int32)
+    g.intersect(c, r2, rn, hits, object_indices=obj)
+    ref, ref_obj = I.intersect_spheres(c, r2, rn)
+    assert (ref_obj >= 0).mean() > 0.05 and (ref_obj < 0).any()
+    assert_hits_equal(records(hits), obj, ref, ref_obj)
+    # device form
+    dev = torch.device("cuda:0")
+    sph = torch.zeros((len(c), 4), dtype=torch.float32, device=dev)
+    sph[:, :c.shape[1]] = torch.from_numpy(c).to(dev)
+    sph[:, 3] = torch.from_numpy(r2).to(dev)
+    out = torch.empty((len(c), 10), dtype=torch.float32, device=dev)
+    objt = torch.empty(len(c), dtype=torch.int32, device=dev)
+    g.intersect_spheres_device(sph, torch.from_numpy(rn).to(dev), out, object_index=objt)
+    torch.cuda.synchronize()
+    assert_hits_equal(out.cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel(), objt.cpu().numpy(), ref, ref_obj)
+
+
+def test_silhouette_queries_raise(gpu):
     objs, tr, s, g, bv, I = setup("mixed3d")
     n = 4
     c = np.zeros((n, 3), np.float32)
     hits = fcpw.GPUInteractionList()
-    with pytest.raises(RuntimeError, match="several objects"):
-        g.intersect(c, np.full(n, 0.1, np.float32), np.zeros((n, 3), np.float32), hits)
     with pytest.raises(RuntimeError, match="several objects|without silhouettes"):
         g.find_closest_silhouette_points(c, np.full(n, 1.0, np.float32), np.zeros(n, np.int32), hits)
 

@@ -207,3 +207,74 @@ def test_invert_transform(dim):
                            + [[0.0] * dim + [1.0]])
         np.testing.assert_allclose(ti[:dim, :dim], Mi[:dim, :dim], rtol=2e-6, atol=2e-6)
         np.testing.assert_allclose(ti[:dim, 3], Mi[:dim, dim], rtol=2e-6, atol=2e-5)
+
+
+def _two_apart(transform_b=None):
+    """two icospheres 5 apart along x, untransformed (or B through a transform):
+    both instances sit in the root leaf, so the walk's pdf is 1"""
+    V, F = meshes.small_icosphere(2)
+    VB = (V + np.float32([5.0, 0.0, 0.0])).astype(np.float32)
+    bv = [oracle.Bvh(V, F), oracle.Bvh(VB, F)]
+    tr = [[], [] if transform_b is None else [transform_b]]
+    return V, VB, F, bv, oracle.Instances(bv, oracle.make_instances(3, 2, tr))
+
+
+def test_sphere_on_one_instance_is_that_objects_sample():
+    """a sphere reaching one object only: the instance answer is that object's own
+    sample bit for bit (count 1, walk pdf 1)"""
+    V, VB, F, bv, I = _two_apart()
+    rng = np.random.default_rng(71)
+    c = (V[rng.integers(0, len(V), 2000)] * rng.uniform(0.9, 1.1, (2000, 1))).astype(np.float32)
+    r2 = np.full(len(c), 0.04, np.float32)
+    rn = rng.random((len(c), 3), dtype=np.float32)
+    hits, obj = I.intersect_spheres(c, r2, rn)
+    ref = bv[0].intersect_spheres(c, r2, rn)
+    found = ref["index"] >= 0
+    assert found.mean() > 0.5
+    np.testing.assert_array_equal(obj, np.where(found, 0, -1))
+    for f in ("d", "p", "n", "uv", "index"):
+        np.testing.assert_array_equal(hits[f], ref[f], err_msg=f)
+
+
+def test_sphere_on_two_instances_is_the_reservoir_choice():
+    """a sphere reaching both objects: both are sampled with the same u; the
+    second replaces the first when u < 1/2, and d is the object's pdf / 2"""
+    V, VB, F, bv, I = _two_apart()
+    n = 2000
+    rng = np.random.default_rng(73)
+    c = np.tile(np.float32([[2.5, 0.0, 0.0]]), (n, 1)) + rng.uniform(-0.05, 0.05, (n, 3)).astype(np.float32)
+    r2 = np.full(n, 16.0, np.float32)      # radius 4: both spheres inside
+    rn = rng.random((n, 3), dtype=np.float32)
+    hits, obj = I.intersect_spheres(c, r2, rn)
+    a = bv[0].intersect_spheres(c, r2, rn)
+    b = bv[1].intersect_spheres(c, r2, rn)
+    assert (a["index"] >= 0).all() and (b["index"] >= 0).all()
+    second = rn[:, 0] < 0.5
+    np.testing.assert_array_equal(obj, np.where(second, 1, 0))
+    ref = np.where(second, b, a)
+    for f in ("p", "n", "uv", "index"):
+        np.testing.assert_array_equal(hits[f], ref[f], err_msg=f)
+    np.testing.assert_array_equal(hits["d"], ref["d"] / np.float32(2.0))
+
+
+def test_sphere_on_a_transformed_instance_lands_on_it():
+    """a rigid transform: the sample lies on the FP64-mapped primitive it names,
+    and its pdf is the object's pdf for the mapped sphere (d is kept)"""
+    R = scenes.rotation(3, __import__("fcpw_amd").synth.Rng(74))
+    M = scenes.affine(3, R, 1.0, (5.0, 1.0, -2.0))
+    V, VB, F, bv, I = _two_apart(M)
+    W = VB.astype(np.float64) @ M[:3, :3].T + M[:3, 3]
+    rng = np.random.default_rng(75)
+    c = (W[rng.integers(0, len(W), 2000)] + rng.normal(0, 0.05, (2000, 3))).astype(np.float32)
+    r2 = np.full(len(c), 0.09, np.float32)
+    rn = rng.random((len(c), 3), dtype=np.float32)
+    hits, obj = I.intersect_spheres(c, r2, rn)
+    found = hits["index"] >= 0
+    assert found.mean() > 0.5 and (obj[found] == 1).all()
+    tri = W[F[hits["index"][found]]]
+    p = hits["p"][found].astype(np.float64)
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    assert np.abs(((p - tri[:, 0]) * n).sum(1)).max() < 1e-5      # on the plane
+    np.testing.assert_allclose(np.abs((hits["n"][found] * n).sum(1)), 1.0, atol=1e-5)
+    assert (hits["d"][found] > 0).all()
