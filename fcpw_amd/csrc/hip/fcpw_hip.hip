@@ -1191,6 +1191,8 @@ int fcpw_hip_get_info(const fcpw_hip_ctx *c, const char *key, int64_t *value)
     else if (k == "max_depth") *value = c->max_depth;
     else if (k == "n_cus") *value = c->n_cus;
     else if (k == "uploaded") *value = c->uploaded ? 1 : 0;
+    else if (k == "n_instances") *value = c->instanced ? c->n_instances : 0;
+    else if (k == "n_objects") *value = c->instanced ? (int64_t)c->inst_objects.size() : (c->uploaded ? 1 : 0);
     else if (k.rfind("stat_", 0) == 0) {
         static const char *names[7] = {"stat_internal_nodes", "stat_prim_tests", "stat_queries", "stat_wave_iterations",
                                        "stat_leaf_passes", "stat_leaf_groups", "stat_waves"};

@@ -360,7 +360,12 @@ int fcpw_hip_set_option(fcpw_hip_ctx *ctx, const char *key, int value);
 int fcpw_hip_get_option(const fcpw_hip_ctx *ctx, const char *key, int *value);
 
 /* Uploaded BVH facts: n_records (internal nodes), n_leaves, n_prims, n_vertices,
-   dim, max_depth, n_cus.  Measurement: stat_internal_nodes, stat_prim_tests,
+   dim, max_depth, n_cus, n_objects, n_instances (0 unless instanced; on a scene of
+   several objects the tree facts are the instance tree's: n_prims = instances,
+   n_vertices = 0).  The instanced kernels run with fixed shapes (128 threads, 8
+   LDS stack entries per object traversal, 4 for the instance tree); of the
+   tuning knobs below only "sort", "morton_bits" and "xcds" apply to them.
+   Measurement: stat_internal_nodes, stat_prim_tests,
    stat_queries and, for closest-point launches, stat_waves, stat_wave_iterations,
    stat_leaf_passes, stat_leaf_groups (cooperative leaf passes and their 64-item
    evaluation groups); totals since "stats" was set to 1; synchronizes the device;
