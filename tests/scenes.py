@@ -57,6 +57,17 @@ def _scene(name):
     elif name == "many2d":
         objs = [meshes.small_star(256)]
         tr = [_random_transforms(2, 30, rng, box=5.0)]
+    elif name == "crowd3d":
+        # 4,000 instances of two small meshes (a deep instance tree) and 50
+        # untransformed objects
+        objs = [meshes.small_icosphere(0), meshes.small_torus(8, 4)]
+        tr = [_random_transforms(3, 2500, rng, box=40.0, scales=(0.2, 1.0)),
+              _random_transforms(3, 1500, rng, box=40.0, scales=(0.2, 1.0))]
+        for k in range(50):
+            V, F = meshes.small_icosphere(0)
+            objs.append(((V * np.float32(0.5) + np.float32([45.0 + 2.0 * (k % 10), 2.0 * (k // 10), 0.0]))
+                         .astype(np.float32), F))
+            tr.append([])
     else:
         raise KeyError(name)
     return objs, tr

@@ -186,3 +186,24 @@ def test_reupload_switches_between_instanced_and_single(gpu):
     q = scenes.queries(objs, tr, 3000, seed=61)
     g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits)
     np.testing.assert_array_equal(records(hits)["d"], I.closest_points(q)[0]["d"])
+
+
+def test_crowd_of_instances_equals_oracle(gpu):
+    """4,000 instances (a deep instance tree) next to 50 untransformed objects"""
+    objs, tr, s, g, bv, I = setup("crowd3d")
+    assert g.info("n_instances") == 4050 and g.info("n_objects") == 52
+    q = scenes.queries(objs, tr, 20000, seed=82)
+    hits = fcpw.GPUInteractionList()
+    obj = np.zeros(len(q), np.int32)
+    g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits, record_normals=True, object_indices=obj)
+    ref, ref_obj = I.closest_points(q, record_normal=True)
+    assert_hits_equal(records(hits), obj, ref, ref_obj)
+    o, d = scenes.rays(objs, tr, 20000, seed=83)
+    g.intersect(o, d, np.full(len(o), np.finfo(np.float32).max, np.float32), hits, object_indices=obj)
+    ref, ref_obj = I.intersect(o, d)
+    assert_hits_equal(records(hits), obj, ref, ref_obj)
+    r2 = np.full(len(q), 4.0, np.float32)
+    rn = np.random.default_rng(84).random((len(q), 3), dtype=np.float32)
+    g.intersect(q, r2, rn, hits, object_indices=obj)
+    ref, ref_obj = I.intersect_spheres(q, r2, rn)
+    assert_hits_equal(records(hits), obj, ref, ref_obj)

@@ -107,3 +107,12 @@ def test_transform_errors():
     s.set_object_instance_transforms(np.zeros((4, 4), np.float32), 0)
     with pytest.raises(ValueError, match="singular"):
         s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+
+
+def test_crowd_instance_tree_matches_oracle():
+    objs, tr = scenes.scene("crowd3d")
+    s = make_scene(objs, tr)
+    bv = [oracle.Bvh(V, F) for V, F in objs]
+    I = oracle.Instances(bv, oracle.make_instances(3, len(objs), tr))
+    assert_same_nodes(s._instance_tree_nodes(), I)
+    assert s._instance_tree_nodes()[5] >= 10   # a deep instance tree
