@@ -1018,26 +1018,31 @@ __device__ __forceinline__ uint32_t morton_cell(const float *q, float4 lo, float
 
 template <int DIM>
 __global__ void k_morton_hist(const float *pts, int stride, int64_t n, float4 lo, float4 inv, int axis_bits,
-                              uint32_t *codes, uint32_t *hist)
+                              uint32_t *codes, uint32_t *ranks, uint32_t *hist)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t code = morton_cell<DIM>(pts + i * stride, lo, inv, axis_bits);
     codes[i] = code;
-    atomicAdd(&hist[code], 1u);
+    ranks[i] = atomicAdd(&hist[code], 1u); // the query's place within its cell
 }
 
 // exclusive scan of n counters in place, three phases: per-segment sums, a
-// one-block scan of the sums, then each segment rescanned from its base
+// one-block scan of the sums, then each segment rescanned from its base.
+// Global memory is read and written lane-contiguously; the per-thread runs of
+// SCAN_PER consecutive counters are formed in LDS (padded one word per 32 so
+// the runs of a wave's lanes fall in distinct banks).
 constexpr int SCAN_THREADS = 256, SCAN_PER = 32, SCAN_SEG = SCAN_THREADS * SCAN_PER;
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_sums(const uint32_t *v, int64_t n, uint32_t *sums)
 {
     __shared__ uint32_t part[SCAN_THREADS];
-    int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * SCAN_PER;
+    const int64_t seg = (int64_t)blockIdx.x * SCAN_SEG;
     uint32_t s = 0;
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) s += v[base + k];
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int64_t i = seg + (int64_t)k * SCAN_THREADS + threadIdx.x;
+        if (i < n) s += v[i];
+    }
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = SCAN_THREADS / 2; off > 0; off >>= 1) {
@@ -1073,13 +1078,21 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint32_t *sums, int m)
         }
 }
 
+__device__ __forceinline__ int scan_pad(int j) { return j + (j >> 5); }
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_apply(uint32_t *v, int64_t n, const uint32_t *sums)
 {
     __shared__ uint32_t part[SCAN_THREADS];
-    int64_t base = (int64_t)blockIdx.x * SCAN_SEG + (int64_t)threadIdx.x * SCAN_PER;
+    __shared__ uint32_t seg_v[SCAN_SEG + SCAN_SEG / 32];
+    const int64_t seg = (int64_t)blockIdx.x * SCAN_SEG;
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int j = k * SCAN_THREADS + (int)threadIdx.x;
+        seg_v[scan_pad(j)] = seg + j < n ? v[seg + j] : 0u;
+    }
+    __syncthreads();
+    const int base = (int)threadIdx.x * SCAN_PER;
     uint32_t s = 0;
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) s += v[base + k];
+    for (int k = 0; k < SCAN_PER; k++) s += seg_v[scan_pad(base + k)];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < SCAN_THREADS; off <<= 1) {
@@ -1089,12 +1102,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_segment_apply(uint32_t *v
         __syncthreads();
     }
     uint32_t run = sums[blockIdx.x] + part[threadIdx.x] - s;
-    for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) {
-            uint32_t c = v[base + k];
-            v[base + k] = run;
-            run += c;
-        }
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int p = scan_pad(base + k);
+        const uint32_t c = seg_v[p];
+        seg_v[p] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int j = k * SCAN_THREADS + (int)threadIdx.x;
+        if (seg + j < n) v[seg + j] = seg_v[scan_pad(j)];
+    }
+}
+
+// each query to its cell's first position plus its place within the cell
+__global__ void k_morton_place(const uint32_t *codes, const uint32_t *ranks, int64_t n, const uint32_t *offsets,
+                               uint32_t *perm)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    perm[offsets[codes[i]] + ranks[i]] = (uint32_t)i;
 }
 
 __global__ void k_morton_scatter(const uint32_t *codes, int64_t n, uint32_t *offsets, uint32_t *perm)

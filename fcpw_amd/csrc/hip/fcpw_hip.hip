@@ -750,12 +750,13 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     const int axis_bits = c->morton_bits / c->dim;
     const int64_t cells = (int64_t)1 << (axis_bits * c->dim);
     const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
-    int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 4);
+    int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 8); // codes, then ranks
     if (!st) st = grow(c, &c->ws_perm, &c->ws_perm_bytes, (size_t)n * 4);
     if (!st) st = grow(c, &c->ws_hist, &c->ws_hist_bytes, (size_t)cells * 4);
     if (!st) st = grow(c, &c->ws_sums, &c->ws_sums_bytes, (size_t)segs * 4);
     if (st) return st;
     uint32_t *codes = (uint32_t *)c->ws_codes, *perm = (uint32_t *)c->ws_perm, *hist = (uint32_t *)c->ws_hist;
+    uint32_t *ranks = codes + n;
     uint32_t *sums = (uint32_t *)c->ws_sums;
     unsigned nb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, hist, 0u, cells);
@@ -763,14 +764,14 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     float4 inv = make_float4(c->qinv[0], c->qinv[1], c->qinv[2], 0.0f);
     if (c->dim == 3)
         hipLaunchKernelGGL(k_morton_hist<3>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, axis_bits, codes,
-                           hist);
+                           ranks, hist);
     else
         hipLaunchKernelGGL(k_morton_hist<2>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, axis_bits, codes,
-                           hist);
+                           ranks, hist);
     hipLaunchKernelGGL(k_scan_segment_sums, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)segs);
     hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
-    hipLaunchKernelGGL(k_morton_scatter, dim3(nb), dim3(256), 0, s, codes, n, hist, perm);
+    hipLaunchKernelGGL(k_morton_place, dim3(nb), dim3(256), 0, s, codes, ranks, n, hist, perm);
     HIPCHECK(c, hipGetLastError());
     *perm_out = perm;
     return FCPW_HIP_OK;
