@@ -65,7 +65,7 @@ __device__ __forceinline__ int batch_find(const BatchScene *T, int n, int64_t ke
 
 template <int DIM>
 __global__ __launch_bounds__(64) void k_batch_morton_hist(const BatchScene *T, int n_scenes, const float *pts,
-                                                          int stride, uint32_t *codes, uint32_t *hist)
+                                                          int stride, uint32_t *codes, uint32_t *ranks, uint32_t *hist)
 {
     const int t = batch_find(T, n_scenes, (int64_t)blockIdx.x, [](const BatchScene &b) { return b.c64_begin; });
     const BatchScene &B = T[t];
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(64) void k_batch_morton_hist(const BatchScene *T, i
     code = morton_cell<DIM>(q, B.qlo, B.qinv, DIM == 3 ? BATCH_CELL_BITS / 3 : BATCH_CELL_BITS / 2);
     code += (uint32_t)t << BATCH_CELL_BITS;
     codes[i] = code;
-    atomicAdd(&hist[code], 1u);
+    ranks[i] = atomicAdd(&hist[code], 1u); // the query's place within its cell
 }
 
 // Persistent closest-point launch: each wave pops work items until none are
@@ -232,18 +232,19 @@ static int batch_order(fcpw_hip_batch *b, const float *pts, int stride, hipStrea
     const int64_t cells = (int64_t)n << BATCH_CELL_BITS;
     const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
     uint32_t *codes = (uint32_t *)b->codes, *perm = (uint32_t *)b->perm, *hist = (uint32_t *)b->hist;
+    uint32_t *ranks = codes + b->n_total;
     uint32_t *sums = (uint32_t *)b->sums;
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, hist, 0u, cells);
     if (b->dim == 3)
         hipLaunchKernelGGL(k_batch_morton_hist<3>, dim3((unsigned)b->n_c64), dim3(64), 0, s, b->table, n, pts, stride,
-                           codes, hist);
+                           codes, ranks, hist);
     else
         hipLaunchKernelGGL(k_batch_morton_hist<2>, dim3((unsigned)b->n_c64), dim3(64), 0, s, b->table, n, pts, stride,
-                           codes, hist);
+                           codes, ranks, hist);
     hipLaunchKernelGGL(k_scan_segment_sums, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)segs);
     hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
-    hipLaunchKernelGGL(k_morton_scatter, dim3((unsigned)((b->n_total + 255) / 256)), dim3(256), 0, s, codes,
+    hipLaunchKernelGGL(k_morton_place, dim3((unsigned)((b->n_total + 255) / 256)), dim3(256), 0, s, codes, ranks,
                        b->n_total, hist, perm);
     HIPCHECK(b, hipGetLastError());
     *perm_out = perm;
@@ -357,7 +358,7 @@ int fcpw_hip_batch_set_queries(fcpw_hip_batch *b, const int64_t *counts)
     if (st) return st;
     const size_t cells = (size_t)b->scenes.size() << BATCH_CELL_BITS;
     const size_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
-    st = batch_grow(b, &b->codes, &b->codes_bytes, std::max<size_t>(4, (size_t)total * 4));
+    st = batch_grow(b, &b->codes, &b->codes_bytes, std::max<size_t>(8, (size_t)total * 8)); // codes, then ranks
     if (!st) st = batch_grow(b, &b->perm, &b->perm_bytes, std::max<size_t>(4, (size_t)total * 4));
     if (!st) st = batch_grow(b, &b->hist, &b->hist_bytes, cells * 4);
     if (!st) st = batch_grow(b, &b->sums, &b->sums_bytes, segs * 4);

@@ -1124,14 +1124,6 @@ __global__ void k_morton_place(const uint32_t *codes, const uint32_t *ranks, int
     perm[offsets[codes[i]] + ranks[i]] = (uint32_t)i;
 }
 
-__global__ void k_morton_scatter(const uint32_t *codes, int64_t n, uint32_t *offsets, uint32_t *perm)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t pos = atomicAdd(&offsets[codes[i]], 1u);
-    perm[pos] = (uint32_t)i;
-}
-
 // rebuild the primitive records from vertex positions (caller's vertex order)
 template <int DIM>
 __global__ void k_rebuild_prims(float4 *prims, const int32_t *vids, const float *V, int n_prims)
