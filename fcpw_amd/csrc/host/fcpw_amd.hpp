@@ -266,6 +266,18 @@ public:
         ++version_;
     }
 
+    // trees built elsewhere, one per object (GPUScene::buildOnGPU on a scene of
+    // several objects): the instance list and tree are then built on the host
+    void adoptInstanced(std::vector<FlatBvh<DIM>> &&objectTrees, AggregateType aggregateType)
+    {
+        if (silhouettes_)
+            throw std::runtime_error("silhouettes need a scene of exactly one object on the MI355X backend");
+        if (objectTrees.size() != objects_.size()) throw std::invalid_argument("adoptInstanced: one tree per object");
+        object_bvh_.clear();
+        for (auto &t : objectTrees) object_bvh_.push_back(std::make_unique<FlatBvh<DIM>>(std::move(t)));
+        finishInstanced(aggregateType);
+    }
+
     // accessors used by GPUScene
     bool built() const { return (bool)bvh_ || (bool)instance_tree_; }
     // several objects or instance transforms: per-object trees and the instance tree
@@ -322,15 +334,28 @@ private:
             throw std::runtime_error("silhouettes need a scene of exactly one object on the MI355X backend");
         auto t0 = std::chrono::high_resolution_clock::now();
         CostHeuristic h = heuristic_of(aggregateType);
-        std::vector<ObjectGeometry> geo;
         for (size_t i = 0; i < objects_.size(); i++) {
             const Object &o = objects_[i];
             if (!o.has_prims || o.indices.empty()) throw std::runtime_error("Cannot construct a BVH without primitives");
             object_bvh_.push_back(std::make_unique<FlatBvh<DIM>>(
                 build_bvh<DIM>(o.positions.data(), (int)(o.positions.size() / DIM), o.indices.data(),
                                (int)(o.indices.size() / DIM), h)));
-            geo.push_back(ObjectGeometry{o.positions.data(), o.indices.data()});
         }
+        finishInstanced(aggregateType);
+        if (printStats) {
+            double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+            printf("BVH construction time: %g seconds (%zu objects, %zu instances)\n", s, objects_.size(),
+                   instances_.size());
+        }
+    }
+    // the instance list and the tree over the instances, once every object has its tree
+    void finishInstanced(AggregateType aggregateType)
+    {
+        CostHeuristic h = heuristic_of(aggregateType);
+        std::vector<ObjectGeometry> geo;
+        for (size_t i = 0; i < objects_.size(); i++)
+            geo.push_back(ObjectGeometry{objects_[i].positions.data(), objects_[i].indices.data()});
+        instances_.clear();
         for (size_t i = 0; i < objects_.size(); i++) {
             const Object &o = objects_[i];
             size_t nt = o.transforms.size() / 12;
@@ -352,11 +377,6 @@ private:
         bvh_.reset();
         snch_.reset();
         aggregate_ = aggregateType;
-        if (printStats) {
-            double s = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
-            printf("BVH construction time: %g seconds (%zu objects, %zu instances)\n", s, objects_.size(),
-                   instances_.size());
-        }
         ++version_;
     }
     Object &obj(int i)
@@ -442,57 +462,25 @@ public:
     // With exportTree the tree is also copied back into the scene, as if
     // scene.build(aggregateType) had made it.  A scene marked for silhouettes
     // always exports: its silhouette references and cones are then built on the
-    // host from that tree and uploaded.
+    // host from that tree and uploaded.  A scene of several objects or instance
+    // transforms builds every object's tree on the device and always exports
+    // them: the tree over the instances is built from them on the host.
     void buildOnGPU(Scene<DIM> &scene, AggregateType aggregateType = AggregateType::Bvh_SurfaceArea,
                     bool exportTree = false)
     {
-        if (scene.needsInstances())
-            throw std::runtime_error("buildOnGPU builds one object's tree; build scenes of several objects or "
-                                     "instances with Scene::build");
+        if (scene.needsInstances()) {
+            buildInstancedOnGPU(scene, aggregateType);
+            return;
+        }
         if (scene.silhouettesRequested()) exportTree = true;
         if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
         int h = (int)heuristic_of(aggregateType);
-        int np = scene.nPrimitives();
-        if (np <= 0) throw std::runtime_error("Cannot construct a BVH without primitives");
-        std::vector<float> pmin, pmax;
-        std::vector<int32_t> nrefs, offset, order;
-        if (exportTree) {
-            size_t cap = 2 * (size_t)np - 1;
-            pmin.resize(cap * 3);
-            pmax.resize(cap * 3);
-            nrefs.resize(cap);
-            offset.resize(cap);
-            order.resize((size_t)np);
-        }
-        int n_nodes = 0;
         instanced_ = false;
-        hip_check(ctx_,
-                  fcpw_hip_build_bvh(ctx_, DIM, scene.positions().data(), scene.nVertices(), scene.indices().data(),
-                                     np, h, exportTree ? pmin.data() : nullptr, exportTree ? pmax.data() : nullptr,
-                                     exportTree ? nrefs.data() : nullptr, exportTree ? offset.data() : nullptr,
-                                     exportTree ? order.data() : nullptr, &n_nodes),
-                  "buildOnGPU");
         if (exportTree) {
-            FlatBvh<DIM> b;
-            b.nodes.resize((size_t)n_nodes);
-            int leaves = 0, depth = 0;
-            for (int i = 0; i < n_nodes; i++) {
-                for (int d = 0; d < DIM; d++) {
-                    b.nodes[i].box.lo[d] = pmin[(size_t)i * 3 + d];
-                    b.nodes[i].box.hi[d] = pmax[(size_t)i * 3 + d];
-                }
-                b.nodes[i].nrefs = nrefs[i];
-                b.nodes[i].offset = offset[i];
-                leaves += nrefs[i] > 0;
-            }
-            int64_t md = 0;
-            fcpw_hip_get_info(ctx_, "max_depth", &md);
-            depth = (int)md;
-            b.prim_order = std::move(order);
-            b.n_leaves = leaves;
-            b.max_depth = depth;
-            scene.adoptBvh(std::move(b), aggregateType);
+            scene.adoptBvh(buildTree(ctx_, scene.positions(), scene.indices(), h, true), aggregateType);
             if (scene.hasSilhouettes()) uploadSilhouettes(scene);
+        } else {
+            buildTree(ctx_, scene.positions(), scene.indices(), h, false);
         }
     }
 
@@ -662,6 +650,79 @@ public:
     bool instanced() const { return instanced_; }
 
 private:
+    // fcpw_hip_build_bvh into ctx; with exportTree the tree is returned as the
+    // host build would have made it
+    FlatBvh<DIM> buildTree(fcpw_hip_ctx *ctx, const std::vector<float> &positions, const std::vector<int32_t> &indices,
+                           int h, bool exportTree)
+    {
+        int np = (int)(indices.size() / DIM);
+        if (np <= 0) throw std::runtime_error("Cannot construct a BVH without primitives");
+        std::vector<float> pmin, pmax;
+        std::vector<int32_t> nrefs, offset, order;
+        if (exportTree) {
+            size_t cap = 2 * (size_t)np - 1;
+            pmin.resize(cap * 3);
+            pmax.resize(cap * 3);
+            nrefs.resize(cap);
+            offset.resize(cap);
+            order.resize((size_t)np);
+        }
+        int n_nodes = 0;
+        hip_check(ctx,
+                  fcpw_hip_build_bvh(ctx, DIM, positions.data(), (int)(positions.size() / DIM), indices.data(), np, h,
+                                     exportTree ? pmin.data() : nullptr, exportTree ? pmax.data() : nullptr,
+                                     exportTree ? nrefs.data() : nullptr, exportTree ? offset.data() : nullptr,
+                                     exportTree ? order.data() : nullptr, &n_nodes),
+                  "buildOnGPU");
+        FlatBvh<DIM> b;
+        if (!exportTree) return b;
+        b.nodes.resize((size_t)n_nodes);
+        int leaves = 0;
+        for (int i = 0; i < n_nodes; i++) {
+            for (int d = 0; d < DIM; d++) {
+                b.nodes[i].box.lo[d] = pmin[(size_t)i * 3 + d];
+                b.nodes[i].box.hi[d] = pmax[(size_t)i * 3 + d];
+            }
+            b.nodes[i].nrefs = nrefs[i];
+            b.nodes[i].offset = offset[i];
+            leaves += nrefs[i] > 0;
+        }
+        int64_t md = 0;
+        fcpw_hip_get_info(ctx, "max_depth", &md);
+        b.prim_order = std::move(order);
+        b.n_leaves = leaves;
+        b.max_depth = (int)md;
+        return b;
+    }
+    // every object's tree built on the device into its own context (and copied
+    // back: the instance tree needs each root box and primitive order), then the
+    // instance list and tree on the host and the instance layer uploaded
+    void buildInstancedOnGPU(Scene<DIM> &scene, AggregateType aggregateType)
+    {
+        if (scene.silhouettesRequested())
+            throw std::runtime_error("silhouettes need a scene of exactly one object on the MI355X backend");
+        if (!ctx_) hip_check(nullptr, fcpw_hip_create(device_, &ctx_), "fcpw_hip_create");
+        int n = scene.nObjects();
+        resizeObjects(n);
+        int h = (int)heuristic_of(aggregateType);
+        std::vector<FlatBvh<DIM>> trees;
+        for (int i = 0; i < n; i++)
+            trees.push_back(buildTree(objs_[i], scene.objectPositions(i), scene.objectIndices(i), h, true));
+        scene.adoptInstanced(std::move(trees), aggregateType);
+        uploadInstances(scene);
+    }
+    void resizeObjects(int n)
+    {
+        while ((int)objs_.size() > n) {
+            fcpw_hip_destroy(objs_.back());
+            objs_.pop_back();
+        }
+        while ((int)objs_.size() < n) {
+            fcpw_hip_ctx *o = nullptr;
+            hip_check(nullptr, fcpw_hip_create(device_, &o), "fcpw_hip_create");
+            objs_.push_back(o);
+        }
+    }
     void require()
     {
         if (!ctx_) throw std::runtime_error("the BVH must be transferred to the GPU first (call transferToGPU)");
@@ -701,18 +762,16 @@ private:
     void transferInstanced(Scene<DIM> &scene)
     {
         int n = scene.nObjects();
-        while ((int)objs_.size() > n) {
-            fcpw_hip_destroy(objs_.back());
-            objs_.pop_back();
-        }
-        while ((int)objs_.size() < n) {
-            fcpw_hip_ctx *o = nullptr;
-            hip_check(nullptr, fcpw_hip_create(device_, &o), "fcpw_hip_create");
-            objs_.push_back(o);
-        }
+        resizeObjects(n);
         for (int i = 0; i < n; i++)
             uploadTree(objs_[i], scene.objectBvh(i), scene.objectPositions(i), scene.objectIndices(i),
                        "transferToGPU(object)");
+        uploadInstances(scene);
+    }
+    // the instance layer over the object contexts
+    void uploadInstances(Scene<DIM> &scene)
+    {
+        int n = scene.nObjects();
         const FlatBvh<DIM> &t = scene.instanceTree();
         size_t nn = t.nodes.size();
         std::vector<float> pmin(nn * 3, 0.0f), pmax(nn * 3, 0.0f);

@@ -207,3 +207,30 @@ def test_crowd_of_instances_equals_oracle(gpu):
     g.intersect(q, r2, rn, hits, object_indices=obj)
     ref, ref_obj = I.intersect_spheres(q, r2, rn)
     assert_hits_equal(records(hits), obj, ref, ref_obj)
+
+
+@pytest.mark.parametrize("name", ["mixed3d", "mixed2d", "crowd3d"])
+def test_build_on_gpu_of_several_objects(gpu, name):
+    """every object's tree built on the device: the scene's trees equal the host
+    build's, and queries equal the oracle"""
+    objs, tr = scenes.scene(name)
+    dim = objs[0][0].shape[1]
+    host = make_scene(objs, tr)
+    dev = make_scene(objs, tr)
+    g = fcpw.GPUScene3D("") if dim == 3 else fcpw.GPUScene2D("")
+    g.build_on_gpu(dev)
+    for o in range(len(objs)):
+        a, b = host._object_bvh_nodes(o), dev._object_bvh_nodes(o)
+        for x, y in zip(a[:5], b[:5]):
+            np.testing.assert_array_equal(x, y)
+    a, b = host._instance_tree_nodes(), dev._instance_tree_nodes()
+    for x, y in zip(a[:5], b[:5]):
+        np.testing.assert_array_equal(x, y)
+    bv = [oracle.Bvh(V, F) for V, F in objs]
+    I = oracle.Instances(bv, oracle.make_instances(dim, len(objs), tr))
+    q = scenes.queries(objs, tr, 5000, seed=85)
+    hits = fcpw.GPUInteractionList()
+    obj = np.zeros(len(q), np.int32)
+    g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits, object_indices=obj)
+    ref, ref_obj = I.closest_points(q)
+    assert_hits_equal(records(hits), obj, ref, ref_obj, fields=("d", "p", "uv"))

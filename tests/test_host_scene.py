@@ -97,8 +97,6 @@ def test_scene_errors():
     assert s.is_instanced and s.n_objects == 2
     with pytest.raises(RuntimeError, match="exactly one object"):
         s.compute_silhouettes()
-    with pytest.raises(RuntimeError, match="several objects"):
-        fcpw.GPUScene3D("").build_on_gpu(s)
     s = fcpw.Scene3D()
     s.set_object_count(1)
     s.set_object_vertices(V, 0)
