@@ -357,9 +357,20 @@ def run_torus(args, torch):
             gpu.intersect_device(rays, ray_out, stream=stream)
         t_r = timed(torch, stream, ray_step, args.steps, args.warmup)
         res["rays"] = {"value": nq / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
-                       "workload": f"{nq} rays, origins uniform in the AABB, directions uniform on S^2, tMax=inf"}
-        log(f"[rays] {t_r * 1e3:.3f} ms/step, {nq / t_r:.4e} rays/s")
-        del rays
+                       "workload": f"{nq} rays, origins uniform in the AABB, directions uniform on S^2, tMax=inf",
+                       "ray_coop": gpu.get_option("ray_coop")}
+        log(f"[rays] {t_r * 1e3:.3f} ms/step, {nq / t_r:.4e} rays/s (ray_coop={gpu.get_option('ray_coop')})")
+        # the other ray kernel on the same rays: same bits, its own time
+        first = ray_out.clone()
+        other = 1 - gpu.get_option("ray_coop")
+        gpu.set_option("ray_coop", other)
+        t_o = timed(torch, stream, ray_step, args.steps, args.warmup)
+        assert torch.equal(first.view(torch.int32), ray_out.view(torch.int32)), "ray kernels differ"
+        gpu.set_option("ray_coop", 1 - other)
+        res["rays_other_kernel"] = {"value": nq / t_o, "unit": "rays/s", "ms_per_step": t_o * 1e3,
+                                    "ray_coop": other}
+        log(f"[rays] ray_coop={other}: {t_o * 1e3:.3f} ms/step, {nq / t_o:.4e} rays/s")
+        del rays, first
 
         # ---- stochastic sphere intersection at the query points, radius 0.1
         balls = _spheres(torch, q, dev)

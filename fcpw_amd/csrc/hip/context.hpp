@@ -53,6 +53,7 @@ struct fcpw_hip_ctx {
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
     int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1, morton_bits = 24;
+    int ray_coop = 1; // wave-cooperative leaves for ray queries
     int stats_on = 0, timing = 0;
     int n_cus = 256;
     // device scratch of fcpw_hip_build_bvh, kept (grow-only) for the next build

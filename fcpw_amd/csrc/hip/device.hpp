@@ -834,6 +834,235 @@ __global__ __launch_bounds__(64) void k_closest_points_coop(DevScene S, CpIO io,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-cooperative ray traversal: the leaf passes of the closest-point kernel
+// applied to rays.  A ray-primitive test computes t without reference to
+// tmax and accepts the hit when 0 <= t <= tmax (geometry.slang:37-77,
+// 123-160), so the wave evaluates t for every (lane, primitive) pair of the
+// held leaves with no upper limit, and each owner replays the acceptance
+// test, tmax = t, in leaf order; an occlusion query stops at its first
+// accepted hit.  The winner's point, normal and uv are recomputed at the end.
+// ---------------------------------------------------------------------------
+template <int DIM>
+struct RayCoopLane {
+    typename Geo<DIM>::vec o, d, dinv;
+    float tmax, cur_t;
+    uint32_t cur, loff, lcnt;
+    int32_t best;
+    bool active, live, holding;
+    uint32_t n_internal, n_prims;
+};
+
+// t of a ray-primitive hit with no upper limit on t; -1 when the primitive is missed
+template <int DIM>
+__device__ __forceinline__ float ray_t(const DevScene &S, uint32_t slot, typename Geo<DIM>::vec o,
+                                       typename Geo<DIM>::vec d)
+{
+    if constexpr (DIM == 3) {
+        const float4 *t = S.prims + (size_t)slot * 3;
+        const float4 a = t[0], b = t[1], c = t[2];
+        float tt, v, w;
+        const v3 pa = {a.x, a.y, a.z}, pb = {b.x, b.y, b.z}, pc = {c.x, c.y, c.z};
+        return ray_triangle(pa, pb, pc, o, d, __builtin_inff(), tt, v, w) ? tt : -1.0f;
+    } else {
+        const float4 s4 = S.prims[slot];
+        const v2 pa = {s4.x, s4.y}, pb = {s4.z, s4.w};
+        float tt, s;
+        return ray_segment(pa, pb, o, d, __builtin_inff(), tt, s) ? tt : -1.0f;
+    }
+}
+
+// One traversal step of an active ray lane that holds no leaf (the order of
+// ray_query): pop until a node's entry distance is within tmax, then hold it
+// (leaf) or expand it (internal: continue with the nearer child hit, push the
+// farther).
+#define FCPW_RAY_COOP_STEP(DIM, BLOCK, K, S, L, OVERFLOW)                                                    \
+    do {                                                                                                      \
+        using G_ = Geo<DIM>;                                                                                  \
+        using vec_ = typename G_::vec;                                                                        \
+        if ((L).active && !(L).holding) {                                                                     \
+            if (!((L).live && !((L).cur_t > (L).tmax))) {                                                    \
+                (L).live = false;                                                                             \
+                while (sp > 0) {                                                                              \
+                    FCPW_POP(BLOCK, K, (L).cur, (L).cur_t);                                                   \
+                    if (!((L).cur_t > (L).tmax)) { (L).live = true; break; }                                  \
+                }                                                                                             \
+            }                                                                                                 \
+            if (!(L).live) {                                                                                  \
+                (L).active = false;                                                                           \
+            } else if ((L).cur & LEAF_BIT) {                                                                  \
+                (L).holding = true;                                                                           \
+                (L).live = false;                                                                             \
+                (L).loff = (L).cur & OFF_MASK;                                                                \
+                (L).lcnt = (((L).cur >> CNT_SHIFT) & CNT_MASK) + 1u;                                          \
+            } else {                                                                                          \
+                vec_ loA, hiA, loB, hiB;                                                                      \
+                uint32_t lA, lB;                                                                              \
+                G_::record(S, (L).cur, loA, hiA, lA, loB, hiB, lB);                                           \
+                (L).n_internal++;                                                                             \
+                float ta, tb;                                                                                 \
+                const bool hA = box_ray(loA, hiA, (L).o, (L).d, (L).dinv, (L).tmax, ta);                      \
+                const bool hB = box_ray(loB, hiB, (L).o, (L).d, (L).dinv, (L).tmax, tb);                      \
+                (L).live = hA || hB;                                                                          \
+                if (hA && hB) {                                                                               \
+                    uint32_t nearL = lA, farL = lB;                                                           \
+                    float nearT = ta, farT = tb;                                                              \
+                    if (tb < ta) { nearL = lB; farL = lA; nearT = tb; farT = ta; }                           \
+                    FCPW_PUSH(BLOCK, K, farL, farT, { OVERFLOW = true; (L).active = false; (L).live = false; }); \
+                    (L).cur = nearL;                                                                          \
+                    (L).cur_t = nearT;                                                                        \
+                } else if (hA) {                                                                              \
+                    (L).cur = lA;                                                                             \
+                    (L).cur_t = ta;                                                                           \
+                } else if (hB) {                                                                              \
+                    (L).cur = lB;                                                                             \
+                    (L).cur_t = tb;                                                                           \
+                }                                                                                             \
+            }                                                                                                 \
+        }                                                                                                     \
+    } while (0)
+
+// The ray leaf pass, run by the whole wave: t of every (lane, primitive) pair
+// of the held leaves, then each owner's in-order acceptance (0 <= t <= tmax,
+// tmax = t; an occlusion query ends at its first hit).
+template <int DIM>
+__device__ __forceinline__ uint32_t ray_coop_leaf_pass(const DevScene &S, CoopShared &sh, RayCoopLane<DIM> &L,
+                                                       bool occlusion)
+{
+    uint32_t groups = 0;
+    using G = Geo<DIM>;
+    const int lane = threadIdx.x & 63;
+    float os[3], ds[3];
+    store_p(L.o, os);
+    store_p(L.d, ds);
+    for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
+        bool h0 = L.holding && L.lcnt > j0, h1 = L.holding && L.lcnt > j0 + 1;
+        bool h2 = L.holding && L.lcnt > j0 + 2, h3 = L.holding && L.lcnt > j0 + 3;
+        uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+        if (m0 == 0) break;
+        uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+        uint32_t total = c0 + c1 + c2 + __popcll(m3);
+        uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                 i3 = c0 + c1 + c2 + lanes_below(m3);
+        if (h0) sh.item[i0] = (uint16_t)((lane << 4) | j0);
+        if (h1) sh.item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
+        if (h2) sh.item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
+        if (h3) sh.item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
+        __syncthreads();
+        for (uint32_t pass = 0; pass < total; pass += 64) {
+            groups++;
+            const uint32_t k = pass + (uint32_t)lane;
+            // every lane takes part in the permutes; lanes past the end read item 0
+            const uint32_t it = sh.item[k < total ? k : 0u];
+            const int src = (int)(it >> 4);
+            float oo[3], dd[3];
+            oo[0] = __shfl(os[0], src, 64);
+            oo[1] = __shfl(os[1], src, 64);
+            oo[2] = DIM == 3 ? __shfl(os[2], src, 64) : 0.0f;
+            dd[0] = __shfl(ds[0], src, 64);
+            dd[1] = __shfl(ds[1], src, 64);
+            dd[2] = DIM == 3 ? __shfl(ds[2], src, 64) : 0.0f;
+            const uint32_t off = __shfl(L.loff, src, 64);
+            if (k < total) sh.d[k] = ray_t<DIM>(S, off + (it & 15u), G::load_point(oo), G::load_point(dd));
+        }
+        __syncthreads();
+        if (L.holding) {
+            const uint32_t idx[4] = {i0, i1, i2, i3};
+            for (uint32_t jj = 0; jj < 4; jj++) {
+                if (L.lcnt > j0 + jj) {
+                    L.n_prims++;
+                    const float t = sh.d[idx[jj]];
+                    if (t >= 0.0f && t <= L.tmax) {
+                        L.best = (int32_t)(L.loff + j0 + jj);
+                        if (occlusion) {
+                            // the query ends here: no later group of this leaf, no further traversal
+                            L.lcnt = 0;
+                            L.active = false;
+                            break;
+                        }
+                        L.tmax = fminf(L.tmax, t);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (m3 == 0) break;
+    }
+    L.holding = false;
+    return groups;
+}
+
+template <int DIM, int K, bool STATS>
+__global__ __launch_bounds__(64) void k_intersect_coop(DevScene S, const float2 *rays, int64_t n, int occlusion,
+                                                       float2 *out, const uint32_t *order, int xcds, int hold_min)
+{
+    using G = Geo<DIM>;
+    constexpr int BLOCK = 64;
+    __shared__ CoopShared sh;
+    const int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
+    const bool valid = k < n;
+    const int64_t i = valid ? (order ? (int64_t)order[k] : k) : 0;
+    float o3[3] = {0.0f, 0.0f, 0.0f}, d3[3] = {0.0f, 0.0f, 0.0f}, di3[3] = {0.0f, 0.0f, 0.0f};
+    float tmax = -1.0f;
+    if (valid) {
+        const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
+        const float2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
+        o3[0] = r0.x; o3[1] = r0.y; o3[2] = r1.x;
+        d3[0] = r1.y; d3[1] = r2.x; d3[2] = r2.y;
+        di3[0] = r3.x; di3[1] = r3.y; di3[2] = r4.x;
+        tmax = r4.y;
+    }
+    RayCoopLane<DIM> L;
+    L.o = G::load_point(o3);
+    L.d = G::load_point(d3);
+    L.dinv = G::load_point(di3);
+    L.tmax = tmax;
+    L.holding = false;
+    L.loff = L.lcnt = 0;
+    L.best = -1;
+    L.n_internal = L.n_prims = 0;
+    {
+        typename G::vec lo, hi;
+        G::root(S, lo, hi, L.cur);
+        L.active = valid && box_ray(lo, hi, L.o, L.d, L.dinv, L.tmax, L.cur_t);
+        L.live = L.active;
+    }
+    bool overflow = false;
+    const bool occ = occlusion != 0;
+    uint32_t n_iter = 0, n_pass = 0, n_group = 0;
+    FCPW_STACK_DECL(BLOCK, K);
+    __syncthreads();
+    while (true) {
+        FCPW_RAY_COOP_STEP(DIM, BLOCK, K, S, L, overflow);
+        const uint64_t act = __ballot(L.active);
+        if (act == 0) break;
+        n_iter++;
+        const uint64_t hold = __ballot(L.holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+        n_pass++;
+        n_group += ray_coop_leaf_pass<DIM>(S, sh, L, occ);
+    }
+    if (overflow) atomicOr(S.flags, 1);
+    if (valid) {
+        const uint32_t index = L.best >= 0 ? (uint32_t)S.prim_index[L.best] : FCPW_HIP_UINT_MAX;
+        float z[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f};
+        if (occ) {
+            // only the occluder's index is written (bvh.slang:183-187)
+            write_interaction(out + i * 5, z, z, uv, FLT_MAX, index);
+        } else if (L.best >= 0) {
+            RayResult r;
+            RayGeo<DIM>::hit(S, (uint32_t)L.best, L.o, L.d, __builtin_inff(), r, true);
+            write_interaction(out + i * 5, r.p, r.n, r.uv, r.t, index);
+        } else {
+            write_interaction(out + i * 5, z, z, uv, FLT_MAX, index);
+        }
+    }
+    if (STATS) {
+        wave_stats_add(S, L.n_internal, L.n_prims, valid ? 1u : 0u);
+        wave_loop_stats_add(S, n_iter, n_pass, n_group);
+    }
+}
+
 // Sphere intersection (bvh.slang:257-392) with the constant branch weight of
 // the sphereIntersection entry point (bvh.cs.slang:59-82): one random walk
 // from the root that picks, with equal probability, one of the children

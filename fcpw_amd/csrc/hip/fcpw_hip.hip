@@ -862,7 +862,26 @@ static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int oc
     int st = morton_order(c, (const float *)rays, 10, n, s, &order); // ordered by origin
     if (st) return st;
     if ((st = timing_begin(c, s))) return st;
-    if (c->stats_on && c->dim == 3)
+    if (c->ray_coop) {
+        const dim3 g = grid_for(c, n, 64);
+        const int k = c->stack >= 16 ? 16 : (c->stack >= 8 ? 8 : 4);
+#define FCPW_RCOOP(D, K, ST)                                                                                  \
+    hipLaunchKernelGGL((k_intersect_coop<D, K, ST>), g, dim3(64), 0, s, S, (const float2 *)rays, n, occlusion, \
+                       (float2 *)out, order, c->xcds, c->hold_min)
+        if (c->stats_on) {
+            if (c->dim == 3) FCPW_RCOOP(3, 16, true);
+            else FCPW_RCOOP(2, 16, true);
+        } else if (c->dim == 3) {
+            if (k == 4) FCPW_RCOOP(3, 4, false);
+            else if (k == 8) FCPW_RCOOP(3, 8, false);
+            else FCPW_RCOOP(3, 16, false);
+        } else {
+            if (k == 4) FCPW_RCOOP(2, 4, false);
+            else if (k == 8) FCPW_RCOOP(2, 8, false);
+            else FCPW_RCOOP(2, 16, false);
+        }
+#undef FCPW_RCOOP
+    } else if (c->stats_on && c->dim == 3)
         hipLaunchKernelGGL((k_intersect<3, 128, 16, true>), grid_for(c, n, 128), dim3(128), 0, s, S,
                            (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds);
     else if (c->stats_on)
@@ -1136,7 +1155,9 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "hold_min") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "hold_min must be in [1, 64]");
         c->hold_min = value;
-
+    } else if (k == "ray_coop") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "ray_coop must be 0 or 1");
+        c->ray_coop = value;
     } else if (k == "xcds") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcds must be in [1, 64]");
         c->xcds = value;
@@ -1173,7 +1194,7 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "xcds") *value = c->xcds;
     else if (k == "morton_bits") *value = c->morton_bits;
     else if (k == "hold_min") *value = c->hold_min;
-
+    else if (k == "ray_coop") *value = c->ray_coop;
     else if (k == "stats") *value = c->stats_on;
     else if (k == "timing") *value = c->timing;
     else return FCPW_HIP_ERR_INVALID_ARGUMENT;

@@ -225,6 +225,50 @@ def test_options_do_not_change_results(gpu, cache):
         g.set_option(key, val)
 
 
+@pytest.mark.parametrize("name", ["bunny", "star2d"])
+@pytest.mark.parametrize("occlusion", [False, True])
+def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
+    """the cooperative ray kernel (evaluate t with no upper limit, replay the
+    acceptance in leaf order) gives the bits of the one-ray-per-lane kernel
+    and of the oracle, for every knob it reads"""
+    V, F, scene, g, ob = get(cache, name)
+    o, d = meshes.rays_from(V, 20000, seed=31)
+    tmax = np.full(len(o), np.inf, np.float32)
+    tmax[::7] = 0.3
+    ref = ob.intersect_gpuorder(o, d, tmax, occlusion=occlusion)
+    configs = [dict(ray_coop=0, block=b, stack=k, sort=srt) for b in (64, 256) for k in (8, 16) for srt in (0, 1)]
+    configs += [dict(ray_coop=1, stack=k, hold_min=h, sort=srt, xcds=x) for k in (4, 8, 16) for h in (1, 8, 64)
+                for srt in (0, 1) for x in (1, 8)]
+    for cfg in configs:
+        for key, val in cfg.items():
+            g.set_option(key, val)
+        hits = fcpw.GPUInteractionList()
+        g.intersect(o, d, tmax, hits, check_for_occlusion=occlusion)
+        rec = as_records(hits)
+        try:
+            assert_bit_equal(rec, ref)
+        except AssertionError as e:
+            raise AssertionError(f"{cfg}: {e}") from None
+    for key, val in dict(ray_coop=1, stack=8, hold_min=1, sort=1, xcds=8, block=128).items():
+        g.set_option(key, val)
+
+
+@pytest.mark.parametrize("name", ["torus", "star2d"])
+def test_cooperative_ray_counters_equal_reference_order_counts(gpu, cache, name):
+    V, F, scene, g, ob = get(cache, name)
+    o, d = meshes.rays_from(V, 20000, seed=32)
+    g.set_option("ray_coop", 1)
+    g.set_option("stats", 1)
+    hits = fcpw.GPUInteractionList()
+    g.intersect(o, d, np.full(len(o), np.inf, np.float32), hits)
+    got = (g.info("stat_internal_nodes"), g.info("stat_prim_tests"), g.info("stat_queries"))
+    ref, ni, npr = ob.intersect_gpuorder(o, d, counters=True)
+    assert_bit_equal(as_records(hits), ref)
+    assert got == (int(ni.sum()), int(npr.sum()), len(o))
+    assert g.info("stat_leaf_passes") > 0
+    g.set_option("stats", 0)
+
+
 @pytest.mark.parametrize("n", [4097, 4096 + 64 * 8 - 1, 50001, 64 * 8 * 13 + 65])
 def test_every_query_of_a_ragged_batch_is_written(gpu, cache, n):
     """batch sizes that leave a partial group of workgroups after the XCD
@@ -253,9 +297,13 @@ def test_every_query_of_a_ragged_batch_is_written(gpu, cache, n):
     for key, val in dict(coop=1, stack=8, hold_min=1, sort=1, xcds=8, block=128, morton_bits=24).items():
         g.set_option(key, val)
     o, dr = meshes.rays_from(V, n, seed=12)
-    hits = fcpw.GPUInteractionList()
-    g.intersect(o, dr, np.full(n, np.inf, np.float32), hits)
-    assert_bit_equal(as_records(hits), ob.intersect_gpuorder(o, dr))
+    ref = ob.intersect_gpuorder(o, dr)
+    for rc in (0, 1):
+        g.set_option("ray_coop", rc)
+        hits = fcpw.GPUInteractionList()
+        g.intersect(o, dr, np.full(n, np.inf, np.float32), hits)
+        assert_bit_equal(as_records(hits), ref)
+    g.set_option("ray_coop", 1)
 
 
 @pytest.mark.parametrize("name", ["torus", "bunny", "star2d"])
