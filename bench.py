@@ -514,6 +514,13 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     def flat_step():
         gf.find_closest_points_device(sph, out_f, stream=stream)
     t_i = timed(torch, stream, inst_step, args.steps, args.warmup)
+    # the instances through the one-query-per-lane kernel: same records, its own time
+    first = out_i.clone()
+    gi.set_option("coop", 0)
+    t_i1 = timed(torch, stream, inst_step, args.steps, args.warmup)
+    gi.set_option("coop", 1)
+    assert torch.equal(first.view(torch.int32), out_i.view(torch.int32)), "instance kernels differ"
+    del first
     t_f = timed(torch, stream, flat_step, args.steps, args.warmup)
     # the baked mesh through the one-query-per-lane kernel, the instanced kernels' structure
     gf.set_option("coop", 0)
@@ -521,13 +528,14 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     gf.set_option("coop", 1)
     # the two scenes round the mapped geometry differently (FP32 per query vs FP64 once)
     dmax = float((out_i[:, 8].double() - out_f[:, 8].double()).abs().max())
+    log(f"[instances] one query per lane: {t_i1 * 1e3:.3f} ms")
     log(f"[instances] {n_inst} instances x {len(F)} tris: two-level {t_i * 1e3:.3f} ms "
         f"({n_queries / t_i:.4e} q/s) vs baked mesh {t_f * 1e3:.3f} ms ({n_queries / t_f:.4e} q/s; "
         f"one query per lane {t_f1 * 1e3:.3f} ms); "
         f"max |d difference| {dmax:.2e}; builds {t_inst_build:.2f}s / {t_flat_build:.2f}s")
     return {"value": n_queries / t_i, "unit": "queries/s", "ms_per_step": t_i * 1e3,
             "baked_mesh_value": n_queries / t_f, "baked_mesh_ms_per_step": t_f * 1e3,
-            "baked_mesh_per_lane_ms_per_step": t_f1 * 1e3,
+            "baked_mesh_per_lane_ms_per_step": t_f1 * 1e3, "per_lane_ms_per_step": t_i1 * 1e3,
             "max_abs_d_difference": dmax, "host_build_s": t_inst_build, "baked_host_build_s": t_flat_build,
             "workload": f"{n_inst} transformed instances of a {len(F)}-triangle icosphere, {n_queries} "
                         f"closest-point queries uniform in the scene box, r2=inf; baked = the same "

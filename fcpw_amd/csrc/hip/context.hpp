@@ -40,6 +40,7 @@ struct fcpw_hip_ctx {
     DevInstance *inst = nullptr;
     DevScene *obj_table = nullptr;
     int n_instances = 0;
+    int inst_obj_depth = 0; // deepest object tree
     std::vector<int32_t> node_slot;     // flat node -> child slot (record * 2 + side), root = 2 * n_records
     std::vector<int32_t> node_leaf_off; // flat node -> first leaf slot, -1 for internal nodes
     // staging for the host-pointer entry points

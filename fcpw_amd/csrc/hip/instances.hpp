@@ -219,6 +219,291 @@ __global__ __launch_bounds__(BLOCK, 6) void k_instances_closest(DevScene T, cons
     if (object_out) object_out[i] = best_obj;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-cooperative closest points on a scene of instances.  Each lane walks the
+// instance tree and, per instance of a reached leaf, the object's tree on the
+// mapped query, with one stack for both levels (an object's entries sit above
+// the index `base` where its walk began).  Object leaves are evaluated by the
+// whole wave as in k_closest_points_coop: every (lane, primitive) pair of the
+// held leaves, from the holder's object and mapped query, then each holder
+// replays d*d <= r2 in leaf order.  Instances are visited in the order of
+// k_instances_closest and the winner is recomputed from its instance and slot,
+// so the records are that kernel's bit for bit.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ float closest_prim(const float4 *prims, uint32_t slot, typename Geo<DIM>::vec x,
+                                              typename Geo<DIM>::vec &p, float &u0, float &u1)
+{
+    DevScene S = {};
+    S.prims = prims;
+    return Geo<DIM>::closest(S, slot, x, p, u0, u1);
+}
+
+template <int DIM, int K>
+__global__ __launch_bounds__(64) void k_instances_closest_coop(DevScene T, const DevInstance *inst,
+                                                               const DevScene *objs, int n_inst,
+                                                               const float4 *spheres, int64_t n, int record_normals,
+                                                               float2 *out, int32_t *object_out,
+                                                               const uint32_t *order, int xcds, int hold_min)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    constexpr int BLOCK = 64;
+    __shared__ CoopShared sh;
+    const int lane = threadIdx.x & 63;
+    const int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
+    const bool valid = k < n;
+    const int64_t i = valid ? (order ? (int64_t)order[k] : k) : 0;
+    float q[3] = {0.0f, 0.0f, 0.0f};
+    float r2 = -1.0f;
+    if (valid) {
+        const float4 s4 = spheres[i];
+        q[0] = s4.x; q[1] = s4.y; q[2] = s4.z;
+        r2 = s4.w;
+    }
+    const vec x = G::load_point(q);
+    int best_inst = -1;
+    int32_t best_slot = -1;
+    // traversal state of either level
+    uint32_t cur = 0;
+    float cur_d2 = 0.0f;
+    bool live = false, active = false, holding = false, start = false;
+    bool inner = false;                              // walking an object's tree
+    uint32_t leaf_off = 0, leaf_cnt = 0, leaf_j = 0; // the instance leaf being visited
+    int base = 0;
+    // the instance being visited
+    int ii = 0, transformed = 0;
+    vec xl = x;
+    float r2l = 0.0f;
+    DevScene O = {};
+    uint32_t loff = 0, lcnt = 0;
+    int32_t lbest = -1;
+    bool overflow = false;
+    FCPW_STACK_DECL(BLOCK, K);
+    if (valid) {
+        if (n_inst == 1) {
+            // a single instance is the scene's aggregate itself (fcpw.inl:1088-1092)
+            active = start = true;
+            leaf_off = 0;
+            leaf_cnt = 1;
+        } else {
+            vec lo, hi;
+            G::root(T, lo, hi, cur);
+            float d2min, d2max;
+            box_sqdist(lo, hi, x, d2min, d2max);
+            if (d2min <= r2) {
+                r2 = fminf(r2, d2max);
+                cur_d2 = d2min;
+                active = live = true;
+            }
+        }
+    }
+    __syncthreads();
+    while (true) {
+        if (active && !holding) {
+            if (start) {
+                // TransformedAggregate.findClosestPoint: map the query into the object's space
+                start = false;
+                ii = T.prim_index[leaf_off + leaf_j];
+                float4 ti[3];
+                int object;
+                load_instance(inst, ii, ti, object, transformed);
+                O = global_scene(objs[object]);
+                xl = x;
+                r2l = r2;
+                if (transformed) {
+                    xl = xf(ti, x);
+                    r2l = xf_r2(ti, x, xl, r2);
+                }
+                inner = true;
+                base = sp;
+                lbest = -1;
+                vec lo, hi;
+                G::root(O, lo, hi, cur);
+                float d2min, d2max;
+                box_sqdist(lo, hi, xl, d2min, d2max);
+                live = d2min <= r2l;
+                if (live) {
+                    r2l = fminf(r2l, d2max);
+                    cur_d2 = d2min;
+                }
+            } else {
+                const float lim = inner ? r2l : r2;
+                if (!(live && cur_d2 <= lim)) {
+                    live = false;
+                    while (sp > base) {
+                        FCPW_POP(BLOCK, K, cur, cur_d2);
+                        if (cur_d2 <= lim) { live = true; break; }
+                    }
+                }
+                if (!live) {
+                    if (inner) {
+                        // the object's walk is over: map its radius and result back
+                        float4 t[3];
+                        if (transformed) load_forward(inst, ii, t);
+                        r2 = transformed ? xf_r2(t, xl, xf(t, xl), r2l) : r2l;
+                        if (lbest >= 0) {
+                            vec pl;
+                            float u0, u1;
+                            float d = closest_prim<DIM>(O.prims, (uint32_t)lbest, xl, pl, u0, u1);
+                            if (transformed) {
+                                const vec e = sub(xf(t, pl), x);
+                                d = sqrtf(dot(e, e));
+                            }
+                            r2 = fminf(r2, d * d);
+                            best_inst = ii;
+                            best_slot = lbest;
+                        }
+                        inner = false;
+                        base = 0;
+                        if (++leaf_j < leaf_cnt) start = true;
+                        else if (n_inst == 1) active = false;
+                    } else {
+                        active = false;
+                    }
+                } else if (cur & LEAF_BIT) {
+                    if (inner) {
+                        holding = true;
+                        loff = cur & OFF_MASK;
+                        lcnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                    } else {
+                        leaf_off = cur & OFF_MASK;
+                        leaf_cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                        leaf_j = 0;
+                        start = true;
+                    }
+                    live = false;
+                } else {
+                    DevScene R = {};
+                    R.records = inner ? O.records : T.records;
+                    const vec xq = inner ? xl : x;
+                    float rr = inner ? r2l : r2;
+                    vec loA, hiA, loB, hiB;
+                    uint32_t lA, lB;
+                    G::record(R, cur, loA, hiA, lA, loB, hiB, lB);
+                    float a0, a1, b0, b1;
+                    box_sqdist(loA, hiA, xq, a0, a1);
+                    const bool oA = a0 <= rr;
+                    rr = fminf(rr, a1);
+                    box_sqdist(loB, hiB, xq, b0, b1);
+                    const bool oB = b0 <= rr;
+                    rr = fminf(rr, b1);
+                    if (inner) r2l = rr;
+                    else r2 = rr;
+                    live = oA || oB;
+                    if (oA && oB) {
+                        uint32_t nearL = lA, farL = lB;
+                        float nearD = a0, farD = b0;
+                        if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                        FCPW_PUSH(BLOCK, K, farL, farD, { overflow = true; active = false; live = false; });
+                        cur = nearL;
+                        cur_d2 = nearD;
+                    } else if (oA) {
+                        cur = lA;
+                        cur_d2 = a0;
+                    } else if (oB) {
+                        cur = lB;
+                        cur_d2 = b0;
+                    }
+                }
+            }
+        }
+        const uint64_t act = __ballot(active);
+        if (act == 0) break;
+        const uint64_t hold = __ballot(holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+        // the leaf pass over the held object leaves
+        float xs[3];
+        store_p(xl, xs);
+        const uint64_t prims_ptr = (uint64_t)(uintptr_t)O.prims;
+        const uint32_t plo = (uint32_t)prims_ptr, phi = (uint32_t)(prims_ptr >> 32);
+        for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
+            const bool h0 = holding && lcnt > j0, h1 = holding && lcnt > j0 + 1;
+            const bool h2 = holding && lcnt > j0 + 2, h3 = holding && lcnt > j0 + 3;
+            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            if (m0 == 0) break;
+            const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+            const uint32_t total = c0 + c1 + c2 + __popcll(m3);
+            const uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                           i3 = c0 + c1 + c2 + lanes_below(m3);
+            if (h0) sh.item[i0] = (uint16_t)((lane << 4) | j0);
+            if (h1) sh.item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
+            if (h2) sh.item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
+            if (h3) sh.item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
+            __syncthreads();
+            for (uint32_t pass = 0; pass < total; pass += 64) {
+                const uint32_t kk = pass + (uint32_t)lane;
+                // every lane takes part in the permutes; lanes past the end read item 0
+                const uint32_t it = sh.item[kk < total ? kk : 0u];
+                const int src = (int)(it >> 4);
+                float xo[3];
+                xo[0] = __shfl(xs[0], src, 64);
+                xo[1] = __shfl(xs[1], src, 64);
+                xo[2] = DIM == 3 ? __shfl(xs[2], src, 64) : 0.0f;
+                const uint32_t off = __shfl(loff, src, 64);
+                const uint32_t lo32 = __shfl(plo, src, 64), hi32 = __shfl(phi, src, 64);
+                if (kk < total) {
+                    const float4 *pr = (const float4 *)(uintptr_t)(((uint64_t)hi32 << 32) | lo32);
+                    vec pp;
+                    float u0, u1;
+                    sh.d[kk] = closest_prim<DIM>(pr, off + (it & 15u), G::load_point(xo), pp, u0, u1);
+                }
+            }
+            __syncthreads();
+            if (holding) {
+                const uint32_t idx[4] = {i0, i1, i2, i3};
+                for (uint32_t jj = 0; jj < 4; jj++) {
+                    if (lcnt > j0 + jj) {
+                        const float d = sh.d[idx[jj]];
+                        if (d * d <= r2l) {
+                            r2l = fminf(r2l, d * d);
+                            lbest = (int32_t)(loff + j0 + jj);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (m3 == 0) break;
+        }
+        holding = false;
+    }
+    if (overflow) atomicOr(T.flags, 1);
+    if (!valid) return;
+    float bp[3] = {0.0f, 0.0f, 0.0f}, buv[2] = {0.0f, 0.0f}, nrm[3] = {0.0f, 0.0f, 0.0f};
+    float best_d = FLT_MAX;
+    uint32_t index = FCPW_HIP_UINT_MAX;
+    int best_obj = -1;
+    if (best_inst >= 0) {
+        float4 ti[3];
+        int tr;
+        load_instance(inst, best_inst, ti, best_obj, tr);
+        const DevScene S = global_scene(objs[best_obj]);
+        const vec xb = tr ? xf(ti, x) : x;
+        vec pl;
+        float u0, u1;
+        best_d = G::closest(S, (uint32_t)best_slot, xb, pl, u0, u1);
+        buv[0] = u0;
+        buv[1] = u1;
+        if (tr) {
+            float4 t[3];
+            load_forward(inst, best_inst, t);
+            const vec pw = xf(t, pl);
+            const vec e = sub(pw, x);
+            best_d = sqrtf(dot(e, e));
+            store_p(pw, bp);
+        } else {
+            store_p(pl, bp);
+        }
+        index = (uint32_t)S.prim_index[best_slot];
+        if (record_normals) {
+            G::normal(S, (uint32_t)best_slot, nrm);
+            if (tr) xf_normal<DIM>(ti, nrm);
+        }
+    }
+    write_interaction(out + i * 5, bp, nrm, buv, best_d, index);
+    if (object_out) object_out[i] = best_obj;
+}
+
 // TransformedAggregate.intersect on one instance; tmax is updated in place
 template <int DIM, int BLOCK, int K>
 __device__ __forceinline__ bool instance_ray(const DevInstance *inst, const DevScene *objs, int ii,
@@ -552,14 +837,17 @@ static int instances_ready(fcpw_hip_ctx *c, const char *fn)
     }
     if (!dirty) return FCPW_HIP_OK;
     std::vector<DevScene> table;
+    int depth = 0;
     for (size_t o = 0; o < c->inst_objects.size(); o++) {
         DevScene S = dev_scene(c->inst_objects[o]);
         S.flags = c->flags; // overflow of any level is reported by the scene
+        depth = std::max(depth, c->inst_objects[o]->max_depth);
         table.push_back(S);
         c->inst_generations[o] = c->inst_objects[o]->generation;
     }
     HIPCHECK(c, hipSetDevice(c->device));
     HIPCHECK(c, hipMemcpy(c->obj_table, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice));
+    c->inst_obj_depth = depth;
     return FCPW_HIP_OK;
 }
 
@@ -597,6 +885,20 @@ static int launch_instances_closest(fcpw_hip_ctx *c, const void *spheres, int64_
     if (st) return st;
     if ((st = timing_begin(c, s))) return st;
     DevScene T = dev_scene(c);
+    if (c->coop && c->max_depth + c->inst_obj_depth < STACK_TOTAL) {
+        // one stack serves both levels, so the two trees' depths must fit it together
+        const dim3 g = grid_for(c, n, 64);
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_instances_closest_coop<3, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
+                               c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
+                               order, c->xcds, c->hold_min);
+        else
+            hipLaunchKernelGGL((k_instances_closest_coop<2, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
+                               c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
+                               order, c->xcds, c->hold_min);
+        HIPCHECK(c, hipGetLastError());
+        return timing_end(c, s);
+    }
     dim3 g = grid_for(c, n, INST_BLOCK);
     if (c->dim == 3)
         hipLaunchKernelGGL((k_instances_closest<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,

@@ -50,6 +50,32 @@ def test_closest_points_equal_oracle(gpu, name, normals):
     assert_hits_equal(records(hits), obj, ref, ref_obj)
 
 
+@pytest.mark.parametrize("name", scenes.NAMES + ["crowd3d"])
+def test_cooperative_and_per_lane_kernels_equal_oracle(gpu, name):
+    """the wave-cooperative instance kernel (one stack for the instance tree and
+    the object trees, object leaves evaluated by the whole wave) and the
+    one-query-per-lane kernel give the oracle's records, for every leaf-pass
+    threshold, with bounded and unbounded radii"""
+    objs, tr, s, g, bv, I = setup(name)
+    q = scenes.queries(objs, tr, 8000, seed=57)
+    r2 = np.full(len(q), np.inf, np.float32)
+    r2[1::3] = np.float32(0.25)
+    ref, ref_obj = I.closest_points(q, r2, record_normal=True)
+    for cfg in (dict(coop=1, hold_min=1), dict(coop=1, hold_min=8), dict(coop=1, hold_min=64),
+                dict(coop=1, hold_min=1, sort=0, xcds=1), dict(coop=0)):
+        for key, val in cfg.items():
+            g.set_option(key, val)
+        hits = fcpw.GPUInteractionList()
+        obj = np.full(len(q), -7, np.int32)
+        g.find_closest_points(q, r2, hits, record_normals=True, object_indices=obj)
+        try:
+            assert_hits_equal(records(hits), obj, ref, ref_obj)
+        except AssertionError as e:
+            raise AssertionError(f"{cfg}: {e}") from None
+    for key, val in dict(coop=1, hold_min=1, sort=1, xcds=8).items():
+        g.set_option(key, val)
+
+
 @pytest.mark.parametrize("name", ["mixed3d", "many3d", "mixed2d"])
 def test_bounded_radii_equal_oracle(gpu, name):
     objs, tr, s, g, bv, I = setup(name)
