@@ -327,7 +327,9 @@ __global__ __launch_bounds__(64) void k_instances_closest_coop(DevScene T, const
                     r2l = fminf(r2l, d2max);
                     cur_d2 = d2min;
                 }
-            } else {
+            }
+            // the first step into the object is taken in the iteration that mapped the query
+            {
                 const float lim = inner ? r2l : r2;
                 if (!(live && cur_d2 <= lim)) {
                     live = false;
