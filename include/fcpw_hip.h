@@ -339,7 +339,8 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
 /* Tuning knobs read by the kernels (values chosen by fcpw_amd.autotune).  None
    of them changes a result bit: every setting visits nodes in the same order.
      "coop"      1 = wave-cooperative leaf processing (default), 0 = one query per
-                 lane with the leaf loop inline (closest points)
+                 lane with the leaf loop inline (closest points, on single-object and
+                 instanced scenes)
      "ray_coop"  the same choice for ray queries (default 1); with 1 the ray
                  kernel also reads "hold_min" and "stack" (4, 8 or 16 LDS entries)
      "hold_min"  coop: lanes that must hold a leaf before the wave processes
