@@ -413,7 +413,15 @@ def run_torus(args, torch):
                                      "workload": f"{ns} queries (every 4th closest-point query), r2=inf, "
                                                  f"flip=0; {found:.3f} find a silhouette"}
         log(f"[silhouettes] {t_sil * 1e3:.3f} ms/step, {ns / t_sil:.4e} queries/s, found {found:.3f}")
-        del balls, flip, sil_out, gs, ss
+        # the one-query-per-lane kernel on the same queries: same records, its own time
+        first = sil_out.clone()
+        gs.set_option("coop", 0)
+        t_sil1 = timed(torch, stream, sil_step, args.steps, args.warmup)
+        assert torch.equal(first.view(torch.int32), sil_out.view(torch.int32)), "silhouette kernels differ"
+        gs.set_option("coop", 1)
+        res["closest_silhouette"]["per_lane_ms_per_step"] = t_sil1 * 1e3
+        log(f"[silhouettes] one query per lane: {t_sil1 * 1e3:.3f} ms/step")
+        del balls, flip, sil_out, gs, ss, first
 
         # ---- BVH construction: the reference's SAH tree built on the device
         # (geometry upload included) against the host build + transfer

@@ -31,6 +31,7 @@ struct fcpw_hip_ctx {
     uint2 *sil_leaf = nullptr;
     bool has_silhouettes = false;
     int n_sil_refs = 0;
+    int sil_leaf_max = 0; // most silhouette records in one leaf
     // scene of several objects (fcpw_hip_upload_instances): records / root /
     // prim_index then hold the tree over the instances (prim_index: leaf slot ->
     // instance); the objects are contexts of their own
@@ -95,6 +96,7 @@ static void free_silhouettes(fcpw_hip_ctx *c)
     c->sil_leaf = nullptr;
     c->has_silhouettes = false;
     c->n_sil_refs = 0;
+    c->sil_leaf_max = 0;
 }
 
 static void free_scene(fcpw_hip_ctx *c)

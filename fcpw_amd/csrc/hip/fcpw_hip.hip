@@ -592,6 +592,7 @@ int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bou
     const int n_slots = 2 * c->n_records + 1;
     std::vector<float4> cone_buf((size_t)n_slots * 2, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     std::vector<uint2> leaf_buf((size_t)c->n_prims, make_uint2(0u, 0u));
+    int leaf_max = 0;
     for (int i = 0; i < n_nodes; i++) {
         const fcpw_hip_bounding_cone &k = cones[i];
         const int slot = c->node_slot[i];
@@ -604,6 +605,7 @@ int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bou
             if (cnt != 0) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_silhouettes: internal node %d has silhouettes", i);
         } else {
             leaf_buf[c->node_leaf_off[i]] = make_uint2((uint32_t)off, (uint32_t)cnt);
+            leaf_max = std::max(leaf_max, (int)cnt);
         }
     }
     const int RW = c->dim == 3 ? 4 : 2;
@@ -632,6 +634,7 @@ int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bou
     HIPCHECK(c, hipMemcpy(c->sil_leaf, leaf_buf.data(), leaf_buf.size() * sizeof(uint2), hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(c->sils, rec_buf.data(), rec_buf.size() * sizeof(float4), hipMemcpyHostToDevice));
     c->n_sil_refs = n_refs;
+    c->sil_leaf_max = leaf_max;
     c->has_silhouettes = true;
     return FCPW_HIP_OK;
 }
@@ -1038,7 +1041,16 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
     const unsigned blocks = (unsigned)((n + 127) / 128);
     st = timing_begin(c, s);
     if (st) return st;
-    if (c->dim == 3)
+    if (c->coop && c->sil_leaf_max <= 1023) {
+        // record indices within a leaf travel in 10 bits of a leaf-pass item
+        const dim3 g = grid_for(c, n, 64);
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_closest_silhouettes_coop<3, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
+                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->hold_min);
+        else
+            hipLaunchKernelGGL((k_closest_silhouettes_coop<2, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
+                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->hold_min);
+    } else if (c->dim == 3)
         hipLaunchKernelGGL((k_closest_silhouettes<3, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
                            (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, c->xcds);
     else

@@ -125,11 +125,10 @@ __device__ __forceinline__ bool is_silhouette(const SilRecord &s, v3 view, float
 }
 
 // Vertex / Edge .findClosestSilhouettePoint (geometry.slang:555-590,667-701)
+// The closest point p (segment parameter t) of a silhouette record to x; returns |x - p|.
 template <int DIM>
-__device__ __forceinline__ bool silhouette_test(const SilRecord &s, v3 x, float r2, bool flip, float r2min,
-                                                float prec, v3 &p, float &t, float &d)
+__device__ __forceinline__ float silhouette_point(const SilRecord &s, v3 x, v3 &p, float &t)
 {
-    if (r2min >= r2) return false;
     t = 0.0f;
     if (DIM == 2) {
         p = s.pa;
@@ -151,11 +150,64 @@ __device__ __forceinline__ bool silhouette_test(const SilRecord &s, v3 x, float 
         }
     }
     const v3 view = sub(x, p);
-    d = sqrtf(dot(view, view));
+    return sqrtf(dot(view, view));
+}
+
+template <int DIM>
+__device__ __forceinline__ bool silhouette_test(const SilRecord &s, v3 x, float r2, bool flip, float r2min,
+                                                float prec, v3 &p, float &t, float &d)
+{
+    if (r2min >= r2) return false;
+    d = silhouette_point<DIM>(s, x, p, t);
     if (d * d > r2) return false;
     bool process = s.one == 1u;
-    if (!process) process = is_silhouette<DIM>(s, view, d, flip, prec);
+    if (!process) process = is_silhouette<DIM>(s, sub(x, p), d, flip, prec);
     return process && d * d <= r2;
+}
+
+// The distance of a silhouette record to x when the record is on the silhouette
+// seen from x, NaN otherwise: silhouette_test without its radius checks, which
+// a NaN fails as the test's own rejection does.
+template <int DIM>
+__device__ __forceinline__ float silhouette_distance(const SilRecord &s, v3 x, bool flip, float prec)
+{
+    v3 p;
+    float t;
+    const float d = silhouette_point<DIM>(s, x, p, t);
+    bool process = s.one == 1u;
+    if (!process) process = is_silhouette<DIM>(s, sub(x, p), d, flip, prec);
+    return process ? d : __builtin_nanf("");
+}
+
+// The children of record cur that the silhouette traversal visits (bvh.slang:580-627):
+// a child with a valid cone whose box overlaps the sphere and whose cone passes
+// BoundingCone.overlap; a0, b0 are the children's squared box distances.
+template <int DIM>
+__device__ __forceinline__ void silhouette_children(const DevScene &S, const SilScene &T, uint32_t cur, v3 x,
+                                                    typename Geo<DIM>::vec xb, float r2, bool &oA, bool &oB,
+                                                    float &a0, float &b0, uint32_t &lA, uint32_t &lB)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    vec loA, hiA, loB, hiB;
+    G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
+    const float4 *cp = T.cones + (size_t)cur * 4;
+    const float4 cA = cp[0], rA = cp[1], cB = cp[2], rB = cp[3];
+    float unused;
+    a0 = 0.0f;
+    b0 = 0.0f;
+    oA = cA.w >= 0.0f;
+    if (oA) {
+        box_sqdist(loA, hiA, xb, a0, unused);
+        oA = a0 <= r2;
+        if (oA) oA = cone_overlap(cA, rA.x, x, to3(loA), to3(hiA), a0);
+    }
+    oB = cB.w >= 0.0f;
+    if (oB) {
+        box_sqdist(loB, hiB, xb, b0, unused);
+        oB = b0 <= r2;
+        if (oB) oB = cone_overlap(cB, rB.x, x, to3(loB), to3(hiB), b0);
+    }
 }
 
 // Bvh.findClosestSilhouettePoint (bvh.slang:503-646) for one query per lane.
@@ -209,24 +261,10 @@ __global__ __launch_bounds__(BLOCK) void k_closest_silhouettes(DevScene S, SilSc
                         }
                     }
                 } else {
-                    vec loA, hiA, loB, hiB;
                     uint32_t lA, lB;
-                    G::record(S, cur, loA, hiA, lA, loB, hiB, lB);
-                    const float4 *cp = T.cones + (size_t)cur * 4;
-                    const float4 cA = cp[0], rA = cp[1], cB = cp[2], rB = cp[3];
-                    float a0 = 0.0f, b0 = 0.0f;
-                    bool oA = cA.w >= 0.0f, oB;
-                    if (oA) {
-                        box_sqdist(loA, hiA, xb, a0, unused);
-                        oA = a0 <= r2;
-                        if (oA) oA = cone_overlap(cA, rA.x, x, to3(loA), to3(hiA), a0);
-                    }
-                    oB = cB.w >= 0.0f;
-                    if (oB) {
-                        box_sqdist(loB, hiB, xb, b0, unused);
-                        oB = b0 <= r2;
-                        if (oB) oB = cone_overlap(cB, rB.x, x, to3(loB), to3(hiB), b0);
-                    }
+                    float a0, b0;
+                    bool oA, oB;
+                    silhouette_children<DIM>(S, T, cur, x, xb, r2, oA, oB, a0, b0, lA, lB);
                     if (oA && oB) {
                         uint32_t nearL = lA, farL = lB;
                         float nearD = a0, farD = b0;
@@ -253,6 +291,170 @@ __global__ __launch_bounds__(BLOCK) void k_closest_silhouettes(DevScene S, SilSc
     const float p3[3] = {best_p.x, best_p.y, DIM == 3 ? best_p.z : 0.0f};
     const float nz[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {best_t, 0.0f};
     write_interaction(out + i * 5, p3, nz, uv, best_d, best);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-cooperative closest silhouette points (one wave per workgroup).  Lanes
+// descend as in k_closest_silhouettes; a lane that reaches a leaf with
+// silhouette records holds it, and the wave evaluates the distance of every
+// (lane, record) pair of the held leaves together (NaN for a record that is
+// not on the silhouette seen from its lane's query).  Each holder then replays
+// the reference's leaf loop in order: skip the record of the current best,
+// nothing passes once r2min >= r2, accept when d*d <= r2.  Neither distance nor
+// classification depends on r2, so the replay is exact; the winner's point is
+// recomputed at the end.  Items are (lane << 10 | record within the leaf), so
+// leaves hold at most 1023 records (the launch falls back to the per-lane
+// kernel otherwise).
+// ---------------------------------------------------------------------------
+struct SilShared {
+    uint16_t item[256];
+    float d[256];
+    uint32_t index[256];
+};
+
+template <int DIM, int K>
+__global__ __launch_bounds__(64) void k_closest_silhouettes_coop(DevScene S, SilScene T, const float4 *spheres,
+                                                                 const uint32_t *flip, int64_t n, float r2min,
+                                                                 float prec, float2 *out, const uint32_t *order,
+                                                                 int xcds, int hold_min)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    constexpr int BLOCK = 64;
+    __shared__ SilShared sh;
+    const int lane = threadIdx.x & 63;
+    const int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
+    const bool valid = k < n;
+    const int64_t i = valid ? (order ? (int64_t)order[k] : k) : 0;
+    float4 s4 = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    bool fl = false;
+    if (valid) {
+        s4 = spheres[i];
+        fl = flip[i] != 0u;
+    }
+    const v3 x = {s4.x, s4.y, DIM == 3 ? s4.z : 0.0f};
+    const float q[3] = {s4.x, s4.y, s4.z};
+    const vec xb = G::load_point(q);
+    float r2 = s4.w;
+    uint32_t best = FCPW_HIP_UINT_MAX;
+    int32_t best_rec = -1;
+    uint32_t cur, loff = 0, lcnt = 0;
+    float cur_d2, unused;
+    {
+        vec lo, hi;
+        G::root(S, lo, hi, cur);
+        box_sqdist(lo, hi, xb, cur_d2, unused);
+    }
+    bool active = valid && !(r2min >= r2) && cur_d2 <= r2, live = active, holding = false, overflow = false;
+    FCPW_STACK_DECL(BLOCK, K);
+    __syncthreads();
+    while (true) {
+        if (active && !holding) {
+            if (!(live && cur_d2 <= r2)) {
+                live = false;
+                while (sp > 0) {
+                    FCPW_POP(BLOCK, K, cur, cur_d2);
+                    if (cur_d2 <= r2) { live = true; break; }
+                }
+            }
+            if (!live) {
+                active = false;
+            } else if (cur & LEAF_BIT) {
+                const uint2 rng = T.sil_leaf[cur & OFF_MASK];
+                loff = rng.x;
+                lcnt = rng.y;
+                holding = lcnt > 0;
+                live = false;
+            } else {
+                uint32_t lA, lB;
+                float a0, b0;
+                bool oA, oB;
+                silhouette_children<DIM>(S, T, cur, x, xb, r2, oA, oB, a0, b0, lA, lB);
+                live = oA || oB;
+                if (oA && oB) {
+                    uint32_t nearL = lA, farL = lB;
+                    float nearD = a0, farD = b0;
+                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }
+                    FCPW_PUSH(BLOCK, K, farL, farD, { overflow = true; active = false; live = false; });
+                    cur = nearL;
+                    cur_d2 = nearD;
+                } else if (oA) {
+                    cur = lA;
+                    cur_d2 = a0;
+                } else if (oB) {
+                    cur = lB;
+                    cur_d2 = b0;
+                }
+            }
+        }
+        const uint64_t act = __ballot(active);
+        if (act == 0) break;
+        const uint64_t hold = __ballot(holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+        const uint32_t flu = fl ? 1u : 0u;
+        for (uint32_t j0 = 0;; j0 += 4) {
+            const bool h0 = holding && lcnt > j0, h1 = holding && lcnt > j0 + 1;
+            const bool h2 = holding && lcnt > j0 + 2, h3 = holding && lcnt > j0 + 3;
+            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            if (m0 == 0) break;
+            const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+            const uint32_t total = c0 + c1 + c2 + __popcll(m3);
+            const uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                           i3 = c0 + c1 + c2 + lanes_below(m3);
+            if (h0) sh.item[i0] = (uint16_t)((lane << 10) | j0);
+            if (h1) sh.item[i1] = (uint16_t)((lane << 10) | (j0 + 1));
+            if (h2) sh.item[i2] = (uint16_t)((lane << 10) | (j0 + 2));
+            if (h3) sh.item[i3] = (uint16_t)((lane << 10) | (j0 + 3));
+            __syncthreads();
+            for (uint32_t pass = 0; pass < total; pass += 64) {
+                const uint32_t kk = pass + (uint32_t)lane;
+                // every lane takes part in the permutes; lanes past the end read item 0
+                const uint32_t it = sh.item[kk < total ? kk : 0u];
+                const int src = (int)(it >> 10);
+                const v3 xo = {__shfl(x.x, src, 64), __shfl(x.y, src, 64), DIM == 3 ? __shfl(x.z, src, 64) : 0.0f};
+                const bool fo = __shfl(flu, src, 64) != 0u;
+                const uint32_t off = __shfl(loff, src, 64);
+                if (kk < total) {
+                    const SilRecord rec = load_silhouette<DIM>(T.sils, off + (it & 1023u));
+                    sh.d[kk] = silhouette_distance<DIM>(rec, xo, fo, prec);
+                    sh.index[kk] = rec.index;
+                }
+            }
+            __syncthreads();
+            if (holding) {
+                const uint32_t idx[4] = {i0, i1, i2, i3};
+                for (uint32_t jj = 0; jj < 4; jj++) {
+                    if (lcnt > j0 + jj) {
+                        const uint32_t index = sh.index[idx[jj]];
+                        if (index == best || r2min >= r2) continue;
+                        const float d = sh.d[idx[jj]];
+                        if (d * d <= r2) {
+                            r2 = fminf(r2, d * d);
+                            best = index;
+                            best_rec = (int32_t)(loff + j0 + jj);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (m3 == 0) break;
+        }
+        holding = false;
+    }
+    if (overflow) atomicOr(S.flags, 1);
+    if (!valid) return;
+    float p3[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f}, d = FLT_MAX;
+    const float nz[3] = {0.0f, 0.0f, 0.0f};
+    if (best_rec >= 0) {
+        v3 p;
+        float t;
+        d = silhouette_point<DIM>(load_silhouette<DIM>(T.sils, (uint32_t)best_rec), x, p, t);
+        p3[0] = p.x;
+        p3[1] = p.y;
+        p3[2] = DIM == 3 ? p.z : 0.0f;
+        uv[0] = t;
+    }
+    write_interaction(out + i * 5, p3, nz, uv, d, best);
 }
 
 // ---------------------------------------------------------------------------

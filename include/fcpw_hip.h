@@ -340,7 +340,7 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
    of them changes a result bit: every setting visits nodes in the same order.
      "coop"      1 = wave-cooperative leaf processing (default), 0 = one query per
                  lane with the leaf loop inline (closest points, on single-object and
-                 instanced scenes)
+                 instanced scenes, and closest silhouette points)
      "ray_coop"  the same choice for ray queries (default 1); with 1 the ray
                  kernel also reads "hold_min" and "stack" (4, 8 or 16 LDS entries)
      "hold_min"  coop: lanes that must hold a leaf before the wave processes

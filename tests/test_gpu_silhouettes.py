@@ -167,3 +167,45 @@ def test_scene_without_silhouettes_is_rejected(gpu):
     with pytest.raises(RuntimeError, match="without silhouettes"):
         g.find_closest_silhouette_points(V[:4], np.full(4, MAXF, np.float32), np.zeros(4, np.int32),
                                          fcpw.GPUInteractionList())
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_cooperative_and_per_lane_kernels_give_the_same_bits(gpu, name):
+    """The wave-cooperative silhouette kernel (records of the held leaves
+    evaluated by the whole wave, the leaf loop replayed by each holder) and the
+    one-query-per-lane kernel return identical records, for every leaf-pass
+    threshold, with and without a minimum radius, and after a device refit."""
+    V, F = CASES[name]
+    dim = V.shape[1]
+    s = product_scene(V, F)
+    g = gpu_scene(s, dim)
+    q, r2, flip = queries(V, 20000, seed=47)
+    ext = float(np.abs(V).max())
+    q2 = np.random.default_rng(48).uniform(-3 * ext, 3 * ext, (6000, dim)).astype(np.float32)
+    q = np.vstack([q, q2])
+    r2 = np.concatenate([r2, np.full(len(q2), MAXF, np.float32)])
+    flip = np.concatenate([flip, np.zeros(len(q2), np.uint32)])
+    sph = torch.zeros((len(q), 4), dtype=torch.float32, device="cuda")
+    sph[:, :dim] = torch.from_numpy(q).cuda()
+    sph[:, 3] = torch.from_numpy(r2).cuda()
+    fl = torch.from_numpy(flip.astype(np.int32)).cuda()
+    r2min = float((0.2 * ext) ** 2)
+
+    def run(cfg, **kw):
+        for key, val in cfg.items():
+            g.set_option(key, val)
+        out = torch.full((len(q), 10), -7.0, dtype=torch.float32, device="cuda")
+        g.find_closest_silhouette_points_device(sph, fl, out, **kw)
+        torch.cuda.synchronize()
+        return out.cpu().numpy().view(np.uint32)
+
+    for kw in ({}, dict(squared_min_radius=r2min, precision=2e-3)):
+        base = run(dict(coop=0), **kw)
+        for cfg in (dict(coop=1, hold_min=1), dict(coop=1, hold_min=8), dict(coop=1, hold_min=64, sort=0)):
+            np.testing.assert_array_equal(run(cfg, **kw), base, err_msg=f"{cfg} {kw}")
+        g.set_option("sort", 1)
+    V2 = (V + np.random.default_rng(49).uniform(-0.01, 0.01, V.shape) * ext).astype(np.float32)
+    s.update_object_vertices(V2, 0)
+    g.refit(s)
+    np.testing.assert_array_equal(run(dict(coop=1, hold_min=1)), run(dict(coop=0)))
+    g.set_option("coop", 1)
