@@ -536,6 +536,20 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     gf.set_option("coop", 1)
     # the two scenes round the mapped geometry differently (FP32 per query vs FP64 once)
     dmax = float((out_i[:, 8].double() - out_f[:, 8].double()).abs().max())
+    # rays through the instances: origins uniform in [-9, 9]^3, directions uniform on S^2, tMax = inf
+    rr = synth.Rng(synth.seed(6, 4))
+    ro = synth.f32(synth.uniform_in_box(n_queries, np.array([-9.0] * 3), np.array([9.0] * 3), 1.0, rr))
+    rays = _rays(torch, ro, synth.f32(rr.sphere(n_queries, 3)), dev)
+    t_r = timed(torch, stream, lambda: gi.intersect_device(rays, out_i, stream=stream), args.steps, args.warmup)
+    first = out_i.clone()
+    gi.set_option("ray_coop", 0)
+    t_r1 = timed(torch, stream, lambda: gi.intersect_device(rays, out_i, stream=stream), args.steps, args.warmup)
+    gi.set_option("ray_coop", 1)
+    assert torch.equal(first.view(torch.int32), out_i.view(torch.int32)), "instance ray kernels differ"
+    ray_hits = float((first[:, 9].view(torch.int32) != -1).float().mean())
+    del first, rays
+    log(f"[instances] {n_queries} rays: {t_r * 1e3:.3f} ms ({n_queries / t_r:.4e} rays/s; one ray per lane "
+        f"{t_r1 * 1e3:.3f} ms), {ray_hits:.3f} hit")
     log(f"[instances] one query per lane: {t_i1 * 1e3:.3f} ms")
     log(f"[instances] {n_inst} instances x {len(F)} tris: two-level {t_i * 1e3:.3f} ms "
         f"({n_queries / t_i:.4e} q/s) vs baked mesh {t_f * 1e3:.3f} ms ({n_queries / t_f:.4e} q/s; "
@@ -544,6 +558,9 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     return {"value": n_queries / t_i, "unit": "queries/s", "ms_per_step": t_i * 1e3,
             "baked_mesh_value": n_queries / t_f, "baked_mesh_ms_per_step": t_f * 1e3,
             "baked_mesh_per_lane_ms_per_step": t_f1 * 1e3, "per_lane_ms_per_step": t_i1 * 1e3,
+            "rays": {"value": n_queries / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
+                     "per_lane_ms_per_step": t_r1 * 1e3, "hit_fraction": ray_hits,
+                     "workload": f"{n_queries} rays, origins uniform in [-9, 9]^3, directions uniform, tMax=inf"},
             "max_abs_d_difference": dmax, "host_build_s": t_inst_build, "baked_host_build_s": t_flat_build,
             "workload": f"{n_inst} transformed instances of a {len(F)}-triangle icosphere, {n_queries} "
                         f"closest-point queries uniform in the scene box, r2=inf; baked = the same "

@@ -638,6 +638,297 @@ __global__ __launch_bounds__(BLOCK) void k_instances_intersect(DevScene T, const
     if (object_out) object_out[i] = best_obj;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-cooperative rays on a scene of instances: the structure of
+// k_instances_closest_coop (one stack for both levels) with the ray leaf pass
+// of k_intersect_coop on the mapped rays.  The winner is recomputed from its
+// instance, its slot and the world tMax at the start of its instance's visit
+// (the mapped direction depends on it, transform.slang:22-29).
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ void map_ray(const float4 ti[3], typename Geo<DIM>::vec o, typename Geo<DIM>::vec dir,
+                                        float tmax, typename Geo<DIM>::vec &ol, typename Geo<DIM>::vec &dl,
+                                        typename Geo<DIM>::vec &il, float &tl, float &nrm)
+{
+    // transformRay(tinv, r) (transform.slang:22-29) and the Ray constructor's 1/d
+    ol = xf(ti, o);
+    const typename Geo<DIM>::vec m = sub(xf(ti, axpy(o, dir, tmax < FLT_MAX ? tmax : 1.0f)), ol);
+    nrm = sqrtf(dot(m, m));
+    dl = divs(m, nrm);
+    il = recip(dl);
+    tl = tmax < FLT_MAX ? nrm : FLT_MAX;
+}
+
+template <int DIM, int K>
+__global__ __launch_bounds__(64) void k_instances_intersect_coop(DevScene T, const DevInstance *inst,
+                                                                 const DevScene *objs, int n_inst, const float2 *rays,
+                                                                 int64_t n, int occlusion, float2 *out,
+                                                                 int32_t *object_out, const uint32_t *order, int xcds,
+                                                                 int hold_min)
+{
+    using G = Geo<DIM>;
+    using vec = typename G::vec;
+    constexpr int BLOCK = 64;
+    __shared__ CoopShared sh;
+    const int lane = threadIdx.x & 63;
+    const int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
+    const bool valid = k < n;
+    const int64_t i = valid ? (order ? (int64_t)order[k] : k) : 0;
+    float o3[3] = {0.0f, 0.0f, 0.0f}, d3[3] = {0.0f, 0.0f, 0.0f}, di3[3] = {0.0f, 0.0f, 0.0f};
+    float tmax = -1.0f;
+    if (valid) {
+        const float2 *rr = rays + i * 5; // GPURay: o(3) d(3) dinv(3) tmax
+        const float2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3], r4 = rr[4];
+        o3[0] = r0.x; o3[1] = r0.y; o3[2] = r1.x;
+        d3[0] = r1.y; d3[1] = r2.x; d3[2] = r2.y;
+        di3[0] = r3.x; di3[1] = r3.y; di3[2] = r4.x;
+        tmax = r4.y;
+    }
+    const vec o = G::load_point(o3), dir = G::load_point(d3), di = G::load_point(di3);
+    const bool occl = occlusion != 0;
+    int best_inst = -1;
+    int32_t best_slot = -1;
+    float best_tv = 0.0f; // world tMax when the winner's instance was entered
+    uint32_t cur = 0;
+    float cur_t = 0.0f;
+    bool live = false, active = false, holding = false, start = false, inner = false;
+    uint32_t leaf_off = 0, leaf_cnt = 0, leaf_j = 0;
+    int base = 0, ii = 0, transformed = 0;
+    vec ol = o, dl = dir, il = di;
+    float tl = 0.0f, tv = 0.0f;
+    DevScene O = {};
+    uint32_t loff = 0, lcnt = 0;
+    int32_t lbest = -1;
+    bool overflow = false;
+    FCPW_STACK_DECL(BLOCK, K);
+    if (valid) {
+        if (n_inst == 1) {
+            // a single instance is the scene's aggregate itself (fcpw.inl:1088-1092)
+            active = start = true;
+            leaf_cnt = 1;
+        } else {
+            vec lo, hi;
+            G::root(T, lo, hi, cur);
+            active = live = box_ray(lo, hi, o, dir, di, tmax, cur_t);
+        }
+    }
+    __syncthreads();
+    while (true) {
+        if (active && !holding) {
+            if (start) {
+                // TransformedAggregate.intersect: map the ray into the object's space
+                start = false;
+                ii = T.prim_index[leaf_off + leaf_j];
+                float4 ti[3];
+                int object;
+                load_instance(inst, ii, ti, object, transformed);
+                O = global_scene(objs[object]);
+                tv = tmax;
+                ol = o;
+                dl = dir;
+                il = di;
+                tl = tmax;
+                if (transformed) {
+                    float nrm;
+                    map_ray<DIM>(ti, o, dir, tmax, ol, dl, il, tl, nrm);
+                }
+                inner = true;
+                base = sp;
+                lbest = -1;
+                vec lo, hi;
+                G::root(O, lo, hi, cur);
+                live = box_ray(lo, hi, ol, dl, il, tl, cur_t);
+            }
+            {
+                const float lim = inner ? tl : tmax;
+                if (!(live && !(cur_t > lim))) {
+                    live = false;
+                    while (sp > base) {
+                        FCPW_POP(BLOCK, K, cur, cur_t);
+                        if (!(cur_t > lim)) { live = true; break; }
+                    }
+                }
+                if (!live) {
+                    if (inner) {
+                        // the object's walk is over: map its tMax and hit back
+                        float4 t[3];
+                        if (transformed) {
+                            load_forward(inst, ii, t);
+                            if (tl < FLT_MAX) {
+                                const vec dd = sub(xf(t, axpy(ol, dl, tl)), xf(t, ol));
+                                tmax = sqrtf(dot(dd, dd));
+                            } else {
+                                tmax = FLT_MAX;
+                            }
+                        } else {
+                            tmax = tl;
+                        }
+                        if (lbest >= 0) {
+                            RayResult r;
+                            RayGeo<DIM>::hit(O, (uint32_t)lbest, ol, dl, __builtin_inff(), r, true);
+                            float tw = r.t;
+                            if (transformed) {
+                                const vec e = sub(xf(t, G::load_point(r.p)), o);
+                                tw = sqrtf(dot(e, e));
+                            }
+                            tmax = fminf(tmax, tw);
+                            best_inst = ii;
+                            best_slot = lbest;
+                            best_tv = tv;
+                        }
+                        inner = false;
+                        base = 0;
+                        if (++leaf_j < leaf_cnt) start = true;
+                        else if (n_inst == 1) active = false;
+                    } else {
+                        active = false;
+                    }
+                } else if (cur & LEAF_BIT) {
+                    if (inner) {
+                        holding = true;
+                        loff = cur & OFF_MASK;
+                        lcnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                    } else {
+                        leaf_off = cur & OFF_MASK;
+                        leaf_cnt = ((cur >> CNT_SHIFT) & CNT_MASK) + 1u;
+                        leaf_j = 0;
+                        start = true;
+                    }
+                    live = false;
+                } else {
+                    DevScene R = {};
+                    R.records = inner ? O.records : T.records;
+                    const vec oq = inner ? ol : o, dq = inner ? dl : dir, iq = inner ? il : di;
+                    vec loA, hiA, loB, hiB;
+                    uint32_t lA, lB;
+                    G::record(R, cur, loA, hiA, lA, loB, hiB, lB);
+                    float ta, tb;
+                    const bool hA = box_ray(loA, hiA, oq, dq, iq, lim, ta);
+                    const bool hB = box_ray(loB, hiB, oq, dq, iq, lim, tb);
+                    live = hA || hB;
+                    if (hA && hB) {
+                        uint32_t nearL = lA, farL = lB;
+                        float nearT = ta, farT = tb;
+                        if (tb < ta) { nearL = lB; farL = lA; nearT = tb; farT = ta; }
+                        FCPW_PUSH(BLOCK, K, farL, farT, { overflow = true; active = false; live = false; });
+                        cur = nearL;
+                        cur_t = nearT;
+                    } else if (hA) {
+                        cur = lA;
+                        cur_t = ta;
+                    } else if (hB) {
+                        cur = lB;
+                        cur_t = tb;
+                    }
+                }
+            }
+        }
+        const uint64_t act = __ballot(active);
+        if (act == 0) break;
+        const uint64_t hold = __ballot(holding);
+        if (hold == 0 || ((int)__popcll(hold) < hold_min && hold != act)) continue;
+        // the leaf pass over the held object leaves
+        float os[3], ds[3];
+        store_p(ol, os);
+        store_p(dl, ds);
+        const uint64_t prims_ptr = (uint64_t)(uintptr_t)O.prims;
+        const uint32_t plo = (uint32_t)prims_ptr, phi = (uint32_t)(prims_ptr >> 32);
+        for (uint32_t j0 = 0; j0 < (uint32_t)MAX_LEAF_PRIMS; j0 += 4) {
+            const bool h0 = holding && lcnt > j0, h1 = holding && lcnt > j0 + 1;
+            const bool h2 = holding && lcnt > j0 + 2, h3 = holding && lcnt > j0 + 3;
+            const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2), m3 = __ballot(h3);
+            if (m0 == 0) break;
+            const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+            const uint32_t total = c0 + c1 + c2 + __popcll(m3);
+            const uint32_t i0 = lanes_below(m0), i1 = c0 + lanes_below(m1), i2 = c0 + c1 + lanes_below(m2),
+                           i3 = c0 + c1 + c2 + lanes_below(m3);
+            if (h0) sh.item[i0] = (uint16_t)((lane << 4) | j0);
+            if (h1) sh.item[i1] = (uint16_t)((lane << 4) | (j0 + 1));
+            if (h2) sh.item[i2] = (uint16_t)((lane << 4) | (j0 + 2));
+            if (h3) sh.item[i3] = (uint16_t)((lane << 4) | (j0 + 3));
+            __syncthreads();
+            for (uint32_t pass = 0; pass < total; pass += 64) {
+                const uint32_t kk = pass + (uint32_t)lane;
+                // every lane takes part in the permutes; lanes past the end read item 0
+                const uint32_t it = sh.item[kk < total ? kk : 0u];
+                const int src = (int)(it >> 4);
+                float oo[3], dd[3];
+                oo[0] = __shfl(os[0], src, 64);
+                oo[1] = __shfl(os[1], src, 64);
+                oo[2] = DIM == 3 ? __shfl(os[2], src, 64) : 0.0f;
+                dd[0] = __shfl(ds[0], src, 64);
+                dd[1] = __shfl(ds[1], src, 64);
+                dd[2] = DIM == 3 ? __shfl(ds[2], src, 64) : 0.0f;
+                const uint32_t off = __shfl(loff, src, 64);
+                const uint32_t lo32 = __shfl(plo, src, 64), hi32 = __shfl(phi, src, 64);
+                if (kk < total) {
+                    DevScene P = {};
+                    P.prims = (const float4 *)(uintptr_t)(((uint64_t)hi32 << 32) | lo32);
+                    sh.d[kk] = ray_t<DIM>(P, off + (it & 15u), G::load_point(oo), G::load_point(dd));
+                }
+            }
+            __syncthreads();
+            if (holding) {
+                const uint32_t idx[4] = {i0, i1, i2, i3};
+                for (uint32_t jj = 0; jj < 4; jj++) {
+                    if (lcnt > j0 + jj) {
+                        const float t = sh.d[idx[jj]];
+                        if (t >= 0.0f && t <= tl) {
+                            lbest = (int32_t)(loff + j0 + jj);
+                            if (occl) {
+                                // the first occluder ends the query
+                                best_inst = ii;
+                                best_slot = lbest;
+                                lcnt = 0;
+                                active = false;
+                                break;
+                            }
+                            tl = fminf(tl, t);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (m3 == 0) break;
+        }
+        holding = false;
+    }
+    if (overflow) atomicOr(T.flags, 1);
+    if (!valid) return;
+    float p[3] = {0.0f, 0.0f, 0.0f}, nv[3] = {0.0f, 0.0f, 0.0f}, uv[2] = {0.0f, 0.0f}, tw = FLT_MAX;
+    uint32_t index = FCPW_HIP_UINT_MAX;
+    int best_obj = -1;
+    if (best_inst >= 0) {
+        float4 ti[3];
+        int tr;
+        load_instance(inst, best_inst, ti, best_obj, tr);
+        const DevScene S = global_scene(objs[best_obj]);
+        index = (uint32_t)S.prim_index[best_slot];
+        if (!occl) {
+            vec ob = o, db = dir, ib = di;
+            float tb = best_tv, nrm;
+            if (tr) map_ray<DIM>(ti, o, dir, best_tv, ob, db, ib, tb, nrm);
+            RayResult r;
+            RayGeo<DIM>::hit(S, (uint32_t)best_slot, ob, db, __builtin_inff(), r, true);
+            tw = r.t;
+            p[0] = r.p[0]; p[1] = r.p[1]; p[2] = r.p[2];
+            nv[0] = r.n[0]; nv[1] = r.n[1]; nv[2] = r.n[2];
+            uv[0] = r.uv[0]; uv[1] = r.uv[1];
+            if (tr) {
+                float4 t[3];
+                load_forward(inst, best_inst, t);
+                const vec pw = xf(t, G::load_point(r.p));
+                const vec e = sub(pw, o);
+                tw = sqrtf(dot(e, e));
+                store_p(pw, p);
+                xf_normal<DIM>(ti, nv);
+            }
+        }
+    }
+    write_interaction(out + i * 5, p, nv, uv, tw, index);
+    if (object_out) object_out[i] = best_obj;
+}
+
 // TransformedAggregate.intersect(sphere) (aggregate.slang:81-97) on one
 // instance: the sphere mapped by tinv, the object's walk and point sample with
 // (u, r1, r2s), the sampled point and normal mapped back; d (the pdf) is kept
@@ -924,6 +1215,20 @@ static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t
     if (st) return st;
     if ((st = timing_begin(c, s))) return st;
     DevScene T = dev_scene(c);
+    if (c->ray_coop && c->max_depth + c->inst_obj_depth < STACK_TOTAL) {
+        // one stack serves both levels, so the two trees' depths must fit it together
+        const dim3 g = grid_for(c, n, 64);
+        if (c->dim == 3)
+            hipLaunchKernelGGL((k_instances_intersect_coop<3, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
+                               c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out, object_out, order,
+                               c->xcds, c->hold_min);
+        else
+            hipLaunchKernelGGL((k_instances_intersect_coop<2, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
+                               c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out, object_out, order,
+                               c->xcds, c->hold_min);
+        HIPCHECK(c, hipGetLastError());
+        return timing_end(c, s);
+    }
     dim3 g = grid_for(c, n, INST_BLOCK);
     if (c->dim == 3)
         hipLaunchKernelGGL((k_instances_intersect<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,

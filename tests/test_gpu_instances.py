@@ -76,6 +76,32 @@ def test_cooperative_and_per_lane_kernels_equal_oracle(gpu, name):
         g.set_option(key, val)
 
 
+@pytest.mark.parametrize("occlusion", [False, True])
+@pytest.mark.parametrize("name", scenes.NAMES + ["crowd3d"])
+def test_cooperative_and_per_lane_ray_kernels_equal_oracle(gpu, name, occlusion):
+    """rays through the wave-cooperative instance kernel (ray leaf pass on the
+    mapped rays, winner recomputed from the tMax its instance was entered with)
+    and through the one-ray-per-lane kernel, against the oracle"""
+    objs, tr, s, g, bv, I = setup(name)
+    o, d = scenes.rays(objs, tr, 8000, seed=58)
+    tmax = np.full(len(o), np.finfo(np.float32).max, np.float32)
+    tmax[::4] = np.float32(2.5)
+    ref, ref_obj = I.intersect(o, d, tmax, occlusion=occlusion)
+    for cfg in (dict(ray_coop=1, hold_min=1), dict(ray_coop=1, hold_min=8), dict(ray_coop=1, hold_min=64),
+                dict(ray_coop=1, hold_min=1, sort=0, xcds=1), dict(ray_coop=0)):
+        for key, val in cfg.items():
+            g.set_option(key, val)
+        hits = fcpw.GPUInteractionList()
+        obj = np.full(len(o), -7, np.int32)
+        g.intersect(o, d, tmax, hits, check_for_occlusion=occlusion, object_indices=obj)
+        try:
+            assert_hits_equal(records(hits), obj, ref, ref_obj)
+        except AssertionError as e:
+            raise AssertionError(f"{cfg}: {e}") from None
+    for key, val in dict(ray_coop=1, hold_min=1, sort=1, xcds=8).items():
+        g.set_option(key, val)
+
+
 @pytest.mark.parametrize("name", ["mixed3d", "many3d", "mixed2d"])
 def test_bounded_radii_equal_oracle(gpu, name):
     objs, tr, s, g, bv, I = setup(name)
