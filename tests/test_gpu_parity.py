@@ -374,6 +374,10 @@ def test_errors_are_reported(gpu):
     scene, g = make_scene(V, F)
     with pytest.raises(ValueError):
         g.set_option("block", 100)
+    for key in ("coop", "ray_coop"):
+        with pytest.raises(ValueError):
+            g.set_option(key, 2)
+        assert g.get_option(key) == 1
     with pytest.raises(RuntimeError, match="without silhouettes"):  # compute_silhouettes was not called
         g.find_closest_silhouette_points(np.zeros((1, 3), np.float32), np.ones(1, np.float32), np.zeros(1, np.int32),
                                          fcpw.GPUInteractionList())
