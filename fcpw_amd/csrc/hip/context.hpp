@@ -56,6 +56,8 @@ struct fcpw_hip_ctx {
     // options
     int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1, morton_bits = 24;
     int ray_coop = 1; // wave-cooperative leaves for ray queries
+    int ray_stack = 4, ray_hold_min = 8; // the cooperative ray kernels' LDS stack entries and leaf-pass threshold
+    int sil_hold_min = 8;                // the cooperative silhouette kernel's leaf-pass threshold
     int stats_on = 0, timing = 0;
     int n_cus = 256;
     // device scratch of fcpw_hip_build_bvh, kept (grow-only) for the next build

@@ -867,10 +867,10 @@ static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int oc
     if ((st = timing_begin(c, s))) return st;
     if (c->ray_coop) {
         const dim3 g = grid_for(c, n, 64);
-        const int k = c->stack >= 16 ? 16 : (c->stack >= 8 ? 8 : 4);
+        const int k = c->ray_stack;
 #define FCPW_RCOOP(D, K, ST)                                                                                  \
     hipLaunchKernelGGL((k_intersect_coop<D, K, ST>), g, dim3(64), 0, s, S, (const float2 *)rays, n, occlusion, \
-                       (float2 *)out, order, c->xcds, c->hold_min)
+                       (float2 *)out, order, c->xcds, c->ray_hold_min)
         if (c->stats_on) {
             if (c->dim == 3) FCPW_RCOOP(3, 16, true);
             else FCPW_RCOOP(2, 16, true);
@@ -1046,10 +1046,10 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
         const dim3 g = grid_for(c, n, 64);
         if (c->dim == 3)
             hipLaunchKernelGGL((k_closest_silhouettes_coop<3, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
-                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->hold_min);
+                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->sil_hold_min);
         else
             hipLaunchKernelGGL((k_closest_silhouettes_coop<2, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
-                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->hold_min);
+                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->sil_hold_min);
     } else if (c->dim == 3)
         hipLaunchKernelGGL((k_closest_silhouettes<3, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
                            (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, c->xcds);
@@ -1170,6 +1170,12 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "ray_coop") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "ray_coop must be 0 or 1");
         c->ray_coop = value;
+    } else if (k == "ray_stack") {
+        if (value != 4 && value != 8 && value != 16) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "ray_stack must be 4, 8 or 16");
+        c->ray_stack = value;
+    } else if (k == "ray_hold_min" || k == "sil_hold_min") {
+        if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be in [1, 64]", key);
+        (k == "ray_hold_min" ? c->ray_hold_min : c->sil_hold_min) = value;
     } else if (k == "xcds") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcds must be in [1, 64]");
         c->xcds = value;
@@ -1207,6 +1213,9 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "morton_bits") *value = c->morton_bits;
     else if (k == "hold_min") *value = c->hold_min;
     else if (k == "ray_coop") *value = c->ray_coop;
+    else if (k == "ray_stack") *value = c->ray_stack;
+    else if (k == "ray_hold_min") *value = c->ray_hold_min;
+    else if (k == "sil_hold_min") *value = c->sil_hold_min;
     else if (k == "stats") *value = c->stats_on;
     else if (k == "timing") *value = c->timing;
     else return FCPW_HIP_ERR_INVALID_ARGUMENT;

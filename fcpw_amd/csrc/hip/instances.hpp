@@ -1221,11 +1221,11 @@ static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t
         if (c->dim == 3)
             hipLaunchKernelGGL((k_instances_intersect_coop<3, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out, object_out, order,
-                               c->xcds, c->hold_min);
+                               c->xcds, c->ray_hold_min);
         else
             hipLaunchKernelGGL((k_instances_intersect_coop<2, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out, object_out, order,
-                               c->xcds, c->hold_min);
+                               c->xcds, c->ray_hold_min);
         HIPCHECK(c, hipGetLastError());
         return timing_end(c, s);
     }

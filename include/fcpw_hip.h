@@ -341,8 +341,10 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "coop"      1 = wave-cooperative leaf processing (default), 0 = one query per
                  lane with the leaf loop inline (closest points, on single-object and
                  instanced scenes, and closest silhouette points)
-     "ray_coop"  the same choice for ray queries (default 1); with 1 the ray
-                 kernel also reads "hold_min" and "stack" (4, 8 or 16 LDS entries)
+     "ray_coop"  the same choice for ray queries (default 1)
+     "ray_stack", "ray_hold_min"  the cooperative ray kernels' LDS stack entries (4, 8
+                 or 16, default 4) and leaf-pass threshold (1..64, default 8)
+     "sil_hold_min"  the cooperative silhouette kernel's leaf-pass threshold (default 8)
      "hold_min"  coop: lanes that must hold a leaf before the wave processes
                  leaves together (1..64, default 1)
      "stack"     traversal-stack entries per lane kept in LDS (default 8), the rest in cached

@@ -237,8 +237,8 @@ def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
     tmax[::7] = 0.3
     ref = ob.intersect_gpuorder(o, d, tmax, occlusion=occlusion)
     configs = [dict(ray_coop=0, block=b, stack=k, sort=srt) for b in (64, 256) for k in (8, 16) for srt in (0, 1)]
-    configs += [dict(ray_coop=1, stack=k, hold_min=h, sort=srt, xcds=x) for k in (4, 8, 16) for h in (1, 8, 64)
-                for srt in (0, 1) for x in (1, 8)]
+    configs += [dict(ray_coop=1, ray_stack=k, ray_hold_min=h, sort=srt, xcds=x) for k in (4, 8, 16)
+                for h in (1, 8, 64) for srt in (0, 1) for x in (1, 8)]
     for cfg in configs:
         for key, val in cfg.items():
             g.set_option(key, val)
@@ -249,7 +249,7 @@ def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
             assert_bit_equal(rec, ref)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(ray_coop=1, stack=8, hold_min=1, sort=1, xcds=8, block=128).items():
+    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, stack=8, sort=1, xcds=8, block=128).items():
         g.set_option(key, val)
 
 
@@ -378,6 +378,10 @@ def test_errors_are_reported(gpu):
         with pytest.raises(ValueError):
             g.set_option(key, 2)
         assert g.get_option(key) == 1
+    for key, bad in (("ray_stack", 5), ("ray_hold_min", 0), ("sil_hold_min", 65)):
+        with pytest.raises(ValueError):
+            g.set_option(key, bad)
+    assert (g.get_option("ray_stack"), g.get_option("ray_hold_min"), g.get_option("sil_hold_min")) == (4, 8, 8)
     with pytest.raises(RuntimeError, match="without silhouettes"):  # compute_silhouettes was not called
         g.find_closest_silhouette_points(np.zeros((1, 3), np.float32), np.ones(1, np.float32), np.zeros(1, np.int32),
                                          fcpw.GPUInteractionList())

@@ -201,11 +201,13 @@ def test_cooperative_and_per_lane_kernels_give_the_same_bits(gpu, name):
 
     for kw in ({}, dict(squared_min_radius=r2min, precision=2e-3)):
         base = run(dict(coop=0), **kw)
-        for cfg in (dict(coop=1, hold_min=1), dict(coop=1, hold_min=8), dict(coop=1, hold_min=64, sort=0)):
+        for cfg in (dict(coop=1, sil_hold_min=1), dict(coop=1, sil_hold_min=8),
+                    dict(coop=1, sil_hold_min=64, sort=0)):
             np.testing.assert_array_equal(run(cfg, **kw), base, err_msg=f"{cfg} {kw}")
         g.set_option("sort", 1)
     V2 = (V + np.random.default_rng(49).uniform(-0.01, 0.01, V.shape) * ext).astype(np.float32)
     s.update_object_vertices(V2, 0)
     g.refit(s)
-    np.testing.assert_array_equal(run(dict(coop=1, hold_min=1)), run(dict(coop=0)))
+    np.testing.assert_array_equal(run(dict(coop=1, sil_hold_min=1)), run(dict(coop=0)))
     g.set_option("coop", 1)
+    g.set_option("sil_hold_min", 8)
