@@ -54,10 +54,12 @@ struct fcpw_hip_ctx {
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
-    int block = 128, stack = 8, sort = 1, xcds = 8, coop = 1, hold_min = 1, morton_bits = 24;
+    int block = 128, stack = 8, sort = 1, xcds = 4, coop = 1, hold_min = 1, morton_bits = 24;
+    int xcd_chunk = 0; // log2 tiles per XCD chunk of the tile mapping (0: one range per XCD)
     int ray_coop = 1; // wave-cooperative leaves for ray queries
     int ray_stack = 4, ray_hold_min = 8; // the cooperative ray kernels' LDS stack entries and leaf-pass threshold
     int sil_hold_min = 8;                // the cooperative silhouette kernel's leaf-pass threshold
+    int sil_xcds = 8;                    // the silhouette kernels' XCD grouping
     int stats_on = 0, timing = 0;
     int n_cus = 256;
     // device scratch of fcpw_hip_build_bvh, kept (grow-only) for the next build
@@ -122,6 +124,10 @@ static void free_scene(fcpw_hip_ctx *c)
     c->counters = nullptr;
     c->uploaded = false;
 }
+
+// the tile-mapping argument of the traversal kernels (see xcd_tile)
+static inline int xcd_arg(const fcpw_hip_ctx *c) { return c->xcds | (c->xcd_chunk << 8); }
+static inline int sil_xcd_arg(const fcpw_hip_ctx *c) { return c->sil_xcds | (c->xcd_chunk << 8); }
 
 static DevScene dev_scene(fcpw_hip_ctx *c)
 {

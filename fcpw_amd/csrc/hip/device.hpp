@@ -493,12 +493,24 @@ __device__ __forceinline__ void write_interaction(float2 *o, const float p[3], c
 // returned a 64-bit value from two branches was compiled (ROCm 7.2 clang) with
 // the high word of the pass-through branch left holding a stale quotient, so
 // the blocks past the last full group of xcds computed out-of-range tiles.
-__device__ __forceinline__ int64_t xcd_tile(int xcds)
+// The argument packs the XCD count (bits 0-7) and a chunk shift c (bits 8-12).
+// With c = 0 each XCD owns one contiguous range of tiles; with c > 0 XCD j takes
+// chunks j, j + xcds, j + 2 xcds, ... of 2^c consecutive tiles, which keeps
+// some L2 locality while spreading every region of the curve over all XCDs.
+__device__ __forceinline__ int64_t xcd_tile(int xcd_arg)
 {
     const uint32_t b = blockIdx.x, G = gridDim.x;
-    const uint32_t x = xcds > 1 ? (uint32_t)xcds : 1u;
-    const uint32_t full = (G / x) * x;
-    const uint32_t t = b < full ? (b % x) * (full / x) + b / x : b;
+    const uint32_t xa = (uint32_t)xcd_arg & 0xFFu, cs = ((uint32_t)xcd_arg >> 8) & 31u;
+    const uint32_t x = xa > 1 ? xa : 1u;
+    if (cs == 0) {
+        const uint32_t full = (G / x) * x;
+        const uint32_t t = b < full ? (b % x) * (full / x) + b / x : b;
+        return (int64_t)(uint64_t)t;
+    }
+    const uint32_t C = 1u << cs, span = x * C;
+    const uint32_t full = (G / span) * span;
+    const uint32_t l = b / x;
+    const uint32_t t = b < full ? ((l / C) * x + b % x) * C + l % C : b;
     return (int64_t)(uint64_t)t;
 }
 

@@ -812,15 +812,15 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
         // counting pass: the coop kernel with per-wave counters (same traversal order)
         dim3 g = grid_for(c, n, 64);
         if (c->dim == 3)
-            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT, true>), g, dim3(64), 0, s, S, io, c->xcds,
+            hipLaunchKernelGGL((k_closest_points_coop<3, 16, OUT, true>), g, dim3(64), 0, s, S, io, xcd_arg(c),
                                c->hold_min);
         else
-            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT, true>), g, dim3(64), 0, s, S, io, c->xcds,
+            hipLaunchKernelGGL((k_closest_points_coop<2, 16, OUT, true>), g, dim3(64), 0, s, S, io, xcd_arg(c),
                                c->hold_min);
     } else if (c->coop) {
         dim3 g = grid_for(c, n, 64);
 #define FCPW_COOP(D, K) \
-    hipLaunchKernelGGL((k_closest_points_coop<D, K, OUT, false>), g, dim3(64), 0, s, S, io, c->xcds, c->hold_min)
+    hipLaunchKernelGGL((k_closest_points_coop<D, K, OUT, false>), g, dim3(64), 0, s, S, io, xcd_arg(c), c->hold_min)
         const int k = c->stack >= 16 ? 16 : c->stack; // 0, 4, 8 or 16 LDS entries
         if (c->dim == 3) {
             if (k == 0) FCPW_COOP(3, 0);
@@ -836,10 +836,10 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
 #undef FCPW_COOP
     } else if (c->dim == 3) {
         FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_closest_points<3, B, K, OUT>), grid_for(c, n, B), dim3(B), 0, s,
-                                                  S, io, c->xcds));
+                                                  S, io, xcd_arg(c)));
     } else {
         FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_closest_points<2, B, K, OUT>), grid_for(c, n, B), dim3(B), 0, s,
-                                                  S, io, c->xcds));
+                                                  S, io, xcd_arg(c)));
     }
     HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
@@ -870,7 +870,7 @@ static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int oc
         const int k = c->ray_stack;
 #define FCPW_RCOOP(D, K, ST)                                                                                  \
     hipLaunchKernelGGL((k_intersect_coop<D, K, ST>), g, dim3(64), 0, s, S, (const float2 *)rays, n, occlusion, \
-                       (float2 *)out, order, c->xcds, c->ray_hold_min)
+                       (float2 *)out, order, xcd_arg(c), c->ray_hold_min)
         if (c->stats_on) {
             if (c->dim == 3) FCPW_RCOOP(3, 16, true);
             else FCPW_RCOOP(2, 16, true);
@@ -886,16 +886,16 @@ static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int oc
 #undef FCPW_RCOOP
     } else if (c->stats_on && c->dim == 3)
         hipLaunchKernelGGL((k_intersect<3, 128, 16, true>), grid_for(c, n, 128), dim3(128), 0, s, S,
-                           (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds);
+                           (const float2 *)rays, n, occlusion, (float2 *)out, order, xcd_arg(c));
     else if (c->stats_on)
         hipLaunchKernelGGL((k_intersect<2, 128, 16, true>), grid_for(c, n, 128), dim3(128), 0, s, S,
-                           (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds);
+                           (const float2 *)rays, n, occlusion, (float2 *)out, order, xcd_arg(c));
     else if (c->dim == 3)
         FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<3, B, K, false>), grid_for(c, n, B), dim3(B), 0, s, S,
-                                            (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds));
+                                            (const float2 *)rays, n, occlusion, (float2 *)out, order, xcd_arg(c)));
     else
         FCPW_DISPATCH_BK(c, 0, hipLaunchKernelGGL((k_intersect<2, B, K, false>), grid_for(c, n, B), dim3(B), 0, s, S,
-                                            (const float2 *)rays, n, occlusion, (float2 *)out, order, c->xcds));
+                                            (const float2 *)rays, n, occlusion, (float2 *)out, order, xcd_arg(c)));
     HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
 }
@@ -1046,16 +1046,16 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
         const dim3 g = grid_for(c, n, 64);
         if (c->dim == 3)
             hipLaunchKernelGGL((k_closest_silhouettes_coop<3, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
-                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->sil_hold_min);
+                               flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c), c->sil_hold_min);
         else
             hipLaunchKernelGGL((k_closest_silhouettes_coop<2, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
-                               flip, n, r2min, prec, (float2 *)out, order, c->xcds, c->sil_hold_min);
+                               flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c), c->sil_hold_min);
     } else if (c->dim == 3)
         hipLaunchKernelGGL((k_closest_silhouettes<3, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
-                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, c->xcds);
+                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c));
     else
         hipLaunchKernelGGL((k_closest_silhouettes<2, 128, 8>), dim3(blocks), dim3(128), 0, s, S, T,
-                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, c->xcds);
+                           (const float4 *)spheres, flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c));
     HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
 }
@@ -1179,6 +1179,12 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "xcds") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcds must be in [1, 64]");
         c->xcds = value;
+    } else if (k == "sil_xcds") {
+        if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sil_xcds must be in [1, 64]");
+        c->sil_xcds = value;
+    } else if (k == "xcd_chunk") {
+        if (value < 0 || value > 16) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcd_chunk must be in [0, 16]");
+        c->xcd_chunk = value;
     } else if (k == "morton_bits") {
         if (value != 18 && value != 21 && value != 24)
             FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "morton_bits must be 18, 21 or 24");
@@ -1210,6 +1216,8 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "sort") *value = c->sort;
     else if (k == "coop") *value = c->coop;
     else if (k == "xcds") *value = c->xcds;
+    else if (k == "xcd_chunk") *value = c->xcd_chunk;
+    else if (k == "sil_xcds") *value = c->sil_xcds;
     else if (k == "morton_bits") *value = c->morton_bits;
     else if (k == "hold_min") *value = c->hold_min;
     else if (k == "ray_coop") *value = c->ray_coop;

@@ -1184,11 +1184,11 @@ static int launch_instances_closest(fcpw_hip_ctx *c, const void *spheres, int64_
         if (c->dim == 3)
             hipLaunchKernelGGL((k_instances_closest_coop<3, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
-                               order, c->xcds, c->hold_min);
+                               order, xcd_arg(c), c->hold_min);
         else
             hipLaunchKernelGGL((k_instances_closest_coop<2, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
-                               order, c->xcds, c->hold_min);
+                               order, xcd_arg(c), c->hold_min);
         HIPCHECK(c, hipGetLastError());
         return timing_end(c, s);
     }
@@ -1196,11 +1196,11 @@ static int launch_instances_closest(fcpw_hip_ctx *c, const void *spheres, int64_
     if (c->dim == 3)
         hipLaunchKernelGGL((k_instances_closest<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,
                            c->obj_table, c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out,
-                           object_out, order, c->xcds);
+                           object_out, order, xcd_arg(c));
     else
         hipLaunchKernelGGL((k_instances_closest<2, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,
                            c->obj_table, c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out,
-                           object_out, order, c->xcds);
+                           object_out, order, xcd_arg(c));
     HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
 }
@@ -1221,11 +1221,11 @@ static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t
         if (c->dim == 3)
             hipLaunchKernelGGL((k_instances_intersect_coop<3, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out, object_out, order,
-                               c->xcds, c->ray_hold_min);
+                               xcd_arg(c), c->ray_hold_min);
         else
             hipLaunchKernelGGL((k_instances_intersect_coop<2, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out, object_out, order,
-                               c->xcds, c->ray_hold_min);
+                               xcd_arg(c), c->ray_hold_min);
         HIPCHECK(c, hipGetLastError());
         return timing_end(c, s);
     }
@@ -1233,11 +1233,11 @@ static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t
     if (c->dim == 3)
         hipLaunchKernelGGL((k_instances_intersect<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,
                            c->inst, c->obj_table, c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out,
-                           object_out, order, c->xcds);
+                           object_out, order, xcd_arg(c));
     else
         hipLaunchKernelGGL((k_instances_intersect<2, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,
                            c->inst, c->obj_table, c->n_instances, (const float2 *)rays, n, occlusion, (float2 *)out,
-                           object_out, order, c->xcds);
+                           object_out, order, xcd_arg(c));
     HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
 }

@@ -353,7 +353,12 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "block"     threads per block when coop = 0 (64, 128, 256)
      "sort"      1 = order queries along a Morton curve before traversal (default)
      "morton_bits"  cells of that ordering grid, 2^18, 2^21 or 2^24 (default 24)
-     "xcds"      number of XCDs the block -> tile mapping groups by (1..64, default 8)
+     "xcds"      number of groups the block -> tile mapping splits the Morton-ordered
+                 tiles into (1..64, default 4: on the 8 XCDs each range is shared by two
+                 XCDs, which balances the ranges' unequal cost)
+     "sil_xcds"  the same for the silhouette kernels (default 8)
+     "xcd_chunk" 0 = one contiguous range per group (default); c > 0 = groups take
+                 chunks of 2^c tiles in turn (0..16)
    Measurement switches (off by default; results are unchanged):
      "stats"     1 = run the counting variant of the traversal kernels, which adds
                  the internal nodes expanded, primitives tested and queries to

@@ -72,7 +72,7 @@ def test_cooperative_and_per_lane_kernels_equal_oracle(gpu, name):
             assert_hits_equal(records(hits), obj, ref, ref_obj)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(coop=1, hold_min=1, sort=1, xcds=8).items():
+    for key, val in dict(coop=1, hold_min=1, sort=1, xcds=4).items():
         g.set_option(key, val)
 
 
@@ -98,7 +98,7 @@ def test_cooperative_and_per_lane_ray_kernels_equal_oracle(gpu, name, occlusion)
             assert_hits_equal(records(hits), obj, ref, ref_obj)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(ray_coop=1, ray_hold_min=8, sort=1, xcds=8).items():
+    for key, val in dict(ray_coop=1, ray_hold_min=8, sort=1, xcds=4).items():
         g.set_option(key, val)
 
 
