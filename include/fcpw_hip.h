@@ -355,7 +355,7 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "morton_bits"  cells of that ordering grid, 2^18, 2^21 or 2^24 (default 24)
      "xcds"      number of groups the block -> tile mapping splits the Morton-ordered
                  tiles into (1..64, default 4: on the 8 XCDs each range is shared by two
-                 XCDs, which balances the ranges' unequal cost)
+                 XCDs, measured faster than one range per XCD on the 1M-triangle torus)
      "sil_xcds"  the same for the silhouette kernels (default 8)
      "xcd_chunk" 0 = one contiguous range per group (default); c > 0 = groups take
                  chunks of 2^c tiles in turn (0..16)
