@@ -26,7 +26,7 @@ import torch
 __all__ = ["SCENE_CANDIDATES", "BATCH_CANDIDATES", "tune_scene", "tune_batch", "clear_cache"]
 
 SCENE_CANDIDATES = (
-    [dict(coop=1, stack=k, hold_min=h, xcds=x) for k in (4, 8, 16) for h in (1, 2, 4) for x in (4, 8)]
+    [dict(coop=1, stack=k, hold_min=h, xcds=x) for k in (8, 12, 16) for h in (1, 2, 4) for x in (4, 8)]
     + [dict(coop=0, stack=k, block=b, xcds=x) for k in (8, 16) for b in (64, 128) for x in (4, 8)]
 )
 BATCH_CANDIDATES = [dict(stack=k, hold_min=h, chunk=c, waves_per_cu=0)

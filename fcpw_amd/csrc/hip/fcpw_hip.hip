@@ -782,7 +782,7 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
 
 #define FCPW_DISPATCH_BK(c, V, ...)                                                                            \
     do {                                                                                                       \
-        const int _b = (c)->block, _k = (c)->stack < 8 ? 8 : (c)->stack;                                      \
+        const int _b = (c)->block, _k = (c)->stack < 8 || (c)->stack == 12 ? 8 : (c)->stack;                  \
         if (_b == 64 && _k == 8) { constexpr int B = 64, K = 8; __VA_ARGS__; }                               \
         else if (_b == 64 && _k == 16) { constexpr int B = 64, K = 16; __VA_ARGS__; }                        \
         else if (_b == 64 && _k == 24) { constexpr int B = 64, K = 24; __VA_ARGS__; }                        \
@@ -821,12 +821,14 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
         dim3 g = grid_for(c, n, 64);
 #define FCPW_COOP(D, K) \
     hipLaunchKernelGGL((k_closest_points_coop<D, K, OUT, false>), g, dim3(64), 0, s, S, io, xcd_arg(c), c->hold_min)
-        const int k = c->stack >= 16 ? 16 : c->stack; // 0, 4, 8 or 16 LDS entries
+        const int k = c->stack; // 0, 4, 8, 12, 16 or 24 LDS entries (2D: 24 and 12 run as 16)
         if (c->dim == 3) {
             if (k == 0) FCPW_COOP(3, 0);
             else if (k == 4) FCPW_COOP(3, 4);
             else if (k == 8) FCPW_COOP(3, 8);
-            else FCPW_COOP(3, 16);
+            else if (k == 12) FCPW_COOP(3, 12);
+            else if (k == 16) FCPW_COOP(3, 16);
+            else FCPW_COOP(3, 24);
         } else {
             if (k == 0) FCPW_COOP(2, 0);
             else if (k == 4) FCPW_COOP(2, 4);
@@ -1155,8 +1157,8 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
         if (value != 64 && value != 128 && value != 256) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
         c->block = value;
     } else if (k == "stack") {
-        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 24)
-            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 8, 16 or 24");
+        if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 8, 12, 16 or 24");
         c->stack = value;
     } else if (k == "sort") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sort must be 0 or 1");
