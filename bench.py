@@ -125,7 +125,7 @@ def cpu_baseline(V, F, q, r2, sample):
 # rocprofv3 summaries (no GPU needed): turn the SQLite output of a kernel-trace
 # run or of FETCH_SIZE / WRITE_SIZE counter runs into the text kept in profiles/
 # ---------------------------------------------------------------------------
-MAIN_KERNEL = "k_closest_points_coop<3, 8, 0, false>"
+MAIN_KERNEL = r"k_closest_points_coop<3, \d+, 0, false>"  # any LDS stack depth
 
 
 def _db(path):
@@ -168,7 +168,7 @@ def summarize_traffic(fetch_db, write_db, calib_bytes, commands, out):
     scaled by calib_bytes / (the counter's reading for a copy of calib_bytes)."""
     fcon, fpath = _db(fetch_db)
     wcon, wpath = _db(write_db)
-    pat = re.escape(MAIN_KERNEL)
+    pat = MAIN_KERNEL
     calib = r"copyBuffer|neg_kernel|direct_copy_kernel"  # read N bytes, write N bytes
     kf, nf, name = _counter(fcon, "FETCH_SIZE", pat)
     kw, nw, _ = _counter(wcon, "WRITE_SIZE", pat)
