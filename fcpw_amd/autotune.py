@@ -30,7 +30,7 @@ SCENE_CANDIDATES = (
     + [dict(coop=0, stack=k, block=b, xcds=x) for k in (8, 16) for b in (64, 128) for x in (4, 8)]
 )
 BATCH_CANDIDATES = [dict(stack=k, hold_min=h, chunk=c, waves_per_cu=0)
-                    for k in (4, 8, 16) for h in (1, 2) for c in (64, 256)]
+                    for k in (8, 12, 16) for h in (1, 2) for c in (64, 256)]
 
 _cache = {}
 

@@ -330,7 +330,7 @@ int fcpw_hip_batch_closest_points_compact_device(fcpw_hip_batch *batch, const fl
 int fcpw_hip_batch_distance_backward_device(fcpw_hip_batch *batch, const float *points, const int32_t *slot,
                                             const float *grad_d, float *grad_points, float *grad_vertices,
                                             void *stream);
-/* Batch knobs (none changes a result bit): "stack" (0, 4, 8, 16), "hold_min"
+/* Batch knobs (none changes a result bit): "stack" (0, 4, 8, 12, 16), "hold_min"
    (1..64), "chunk" (queries per work item: 64, 128, 256, 512), "sort" (0, 1),
    "waves_per_cu" (0 = as many as fit, else 1..32). */
 int fcpw_hip_batch_set_option(fcpw_hip_batch *batch, const char *key, int value);
