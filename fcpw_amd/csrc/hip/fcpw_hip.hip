@@ -1032,6 +1032,10 @@ int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *
     return FCPW_HIP_OK;
 }
 
+// LDS stack entries of the 3D cooperative silhouette kernel (4M queries on the
+// torus: 118.7 ms with 4, 120.1 with 0, 129.4 with 8, 132.0 with 12)
+constexpr int SIL_COOP_K3 = 4;
+
 static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32_t *flip, int64_t n, float r2min,
                               float prec, void *out, hipStream_t s)
 {
@@ -1047,7 +1051,7 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
         // record indices within a leaf travel in 10 bits of a leaf-pass item
         const dim3 g = grid_for(c, n, 64);
         if (c->dim == 3)
-            hipLaunchKernelGGL((k_closest_silhouettes_coop<3, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
+            hipLaunchKernelGGL((k_closest_silhouettes_coop<3, SIL_COOP_K3>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
                                flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c), c->sil_hold_min);
         else
             hipLaunchKernelGGL((k_closest_silhouettes_coop<2, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
