@@ -821,7 +821,7 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
         dim3 g = grid_for(c, n, 64);
 #define FCPW_COOP(D, K) \
     hipLaunchKernelGGL((k_closest_points_coop<D, K, OUT, false>), g, dim3(64), 0, s, S, io, xcd_arg(c), c->hold_min)
-        const int k = c->stack; // 0, 4, 8, 12, 16 or 24 LDS entries (2D: 24 and 12 run as 16)
+        const int k = c->stack; // 0, 4, 8, 12, 16 or 24 LDS entries (2D: 24 runs as 16)
         if (c->dim == 3) {
             if (k == 0) FCPW_COOP(3, 0);
             else if (k == 4) FCPW_COOP(3, 4);
@@ -833,6 +833,7 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
             if (k == 0) FCPW_COOP(2, 0);
             else if (k == 4) FCPW_COOP(2, 4);
             else if (k == 8) FCPW_COOP(2, 8);
+            else if (k == 12) FCPW_COOP(2, 12);
             else FCPW_COOP(2, 16);
         }
 #undef FCPW_COOP

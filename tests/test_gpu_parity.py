@@ -381,10 +381,13 @@ def test_errors_are_reported(gpu):
         with pytest.raises(ValueError):
             g.set_option(key, 2)
         assert g.get_option(key) == 1
-    for key, bad in (("ray_stack", 5), ("ray_hold_min", 0), ("sil_hold_min", 65)):
+    for key, bad in (("ray_stack", 5), ("ray_hold_min", 0), ("sil_hold_min", 65), ("stack", 10),
+                     ("xcd_chunk", 17), ("sil_xcds", 0)):
         with pytest.raises(ValueError):
             g.set_option(key, bad)
-    assert (g.get_option("ray_stack"), g.get_option("ray_hold_min"), g.get_option("sil_hold_min")) == (4, 8, 8)
+    defaults = dict(stack=12, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
+                    sil_xcds=8)
+    assert {k: g.get_option(k) for k in defaults} == defaults
     with pytest.raises(RuntimeError, match="without silhouettes"):  # compute_silhouettes was not called
         g.find_closest_silhouette_points(np.zeros((1, 3), np.float32), np.ones(1, np.float32), np.zeros(1, np.int32),
                                          fcpw.GPUInteractionList())
