@@ -291,6 +291,7 @@ static int batch_launch(fcpw_hip_batch *b, CpIO io, hipStream_t s)
     if (k == 0) return batch_launch_k<2, 0, OUT>(b, io, perm, s);
     if (k == 4) return batch_launch_k<2, 4, OUT>(b, io, perm, s);
     if (k == 8) return batch_launch_k<2, 8, OUT>(b, io, perm, s);
+    if (k == 12) return batch_launch_k<2, 12, OUT>(b, io, perm, s);
     return batch_launch_k<2, 16, OUT>(b, io, perm, s);
 }
 
