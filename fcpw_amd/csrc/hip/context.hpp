@@ -57,6 +57,7 @@ struct fcpw_hip_ctx {
     int block = 128, stack = 12, sort = 1, xcds = 4, coop = 1, hold_min = 1, morton_bits = 24;
     int xcd_chunk = 0; // log2 tiles per XCD chunk of the tile mapping (0: one range per XCD)
     int ray_coop = 1; // wave-cooperative leaves for ray queries
+    int ray_sort_dir = 1; // rays ordered by direction octant, then origin
     int ray_stack = 4, ray_hold_min = 8; // the cooperative ray kernels' LDS stack entries and leaf-pass threshold
     int sil_hold_min = 8;                // the cooperative silhouette kernel's leaf-pass threshold
     int sil_xcds = 8;                    // the silhouette kernels' XCD grouping

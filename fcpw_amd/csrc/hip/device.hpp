@@ -1257,13 +1257,21 @@ __device__ __forceinline__ uint32_t morton_cell(const float *q, float4 lo, float
     return spread2(quant(q[0], lo.x, inv.x, cells)) | (spread2(quant(q[1], lo.y, inv.y, cells)) << 1);
 }
 
+// dir_off >= 0: the records also hold a direction at that float offset, and the
+// signs of its components (the octant, or quadrant in 2D) form the code's top bits
 template <int DIM>
 __global__ void k_morton_hist(const float *pts, int stride, int64_t n, float4 lo, float4 inv, int axis_bits,
-                              uint32_t *codes, uint32_t *ranks, uint32_t *hist)
+                              uint32_t *codes, uint32_t *ranks, uint32_t *hist, int dir_off)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t code = morton_cell<DIM>(pts + i * stride, lo, inv, axis_bits);
+    if (dir_off >= 0) {
+        const float *d = pts + i * stride + dir_off;
+        uint32_t oct = (d[0] < 0.0f ? 1u : 0u) | (d[1] < 0.0f ? 2u : 0u);
+        if (DIM == 3) oct |= d[2] < 0.0f ? 4u : 0u;
+        code |= oct << (DIM * axis_bits);
+    }
     codes[i] = code;
     ranks[i] = atomicAdd(&hist[code], 1u); // the query's place within its cell
 }

@@ -745,13 +745,15 @@ static int timing_end(fcpw_hip_ctx *c, hipStream_t s)
 // Morton order of n query points (pts + i * stride floats); returns the
 // permutation, or nullptr when ordering is off or the batch is small
 static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n, hipStream_t s,
-                        const uint32_t **perm_out)
+                        const uint32_t **perm_out, int dir_off = -1)
 {
     *perm_out = nullptr;
     if (!c->sort || n < 4096) return FCPW_HIP_OK;
     if (n > (int64_t)UINT32_MAX) FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "query batches above 2^32 are not supported");
-    const int axis_bits = c->morton_bits / c->dim;
-    const int64_t cells = (int64_t)1 << (axis_bits * c->dim);
+    // rays: the direction's octant (quadrant) takes dim bits of the code, the origin the rest
+    const int dir_bits = dir_off >= 0 ? c->dim : 0;
+    const int axis_bits = (c->morton_bits - dir_bits) / c->dim;
+    const int64_t cells = (int64_t)1 << (axis_bits * c->dim + dir_bits);
     const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
     int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 8); // codes, then ranks
     if (!st) st = grow(c, &c->ws_perm, &c->ws_perm_bytes, (size_t)n * 4);
@@ -767,10 +769,10 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     float4 inv = make_float4(c->qinv[0], c->qinv[1], c->qinv[2], 0.0f);
     if (c->dim == 3)
         hipLaunchKernelGGL(k_morton_hist<3>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, axis_bits, codes,
-                           ranks, hist);
+                           ranks, hist, dir_off);
     else
         hipLaunchKernelGGL(k_morton_hist<2>, dim3(nb), dim3(256), 0, s, pts, stride, n, lo, inv, axis_bits, codes,
-                           ranks, hist);
+                           ranks, hist, dir_off);
     hipLaunchKernelGGL(k_scan_segment_sums, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, (int)segs);
     hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
@@ -865,7 +867,8 @@ static int launch_intersect(fcpw_hip_ctx *c, const void *rays, int64_t n, int oc
 {
     DevScene S = dev_scene(c);
     const uint32_t *order;
-    int st = morton_order(c, (const float *)rays, 10, n, s, &order); // ordered by origin
+    // ordered by origin, and by direction octant when ray_sort_dir is set
+    int st = morton_order(c, (const float *)rays, 10, n, s, &order, c->ray_sort_dir ? 3 : -1);
     if (st) return st;
     if ((st = timing_begin(c, s))) return st;
     if (c->ray_coop) {
@@ -1189,6 +1192,9 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "sil_xcds") {
         if (value < 1 || value > 64) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sil_xcds must be in [1, 64]");
         c->sil_xcds = value;
+    } else if (k == "ray_sort_dir") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "ray_sort_dir must be 0 or 1");
+        c->ray_sort_dir = value;
     } else if (k == "xcd_chunk") {
         if (value < 0 || value > 16) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcd_chunk must be in [0, 16]");
         c->xcd_chunk = value;
@@ -1224,6 +1230,7 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "coop") *value = c->coop;
     else if (k == "xcds") *value = c->xcds;
     else if (k == "xcd_chunk") *value = c->xcd_chunk;
+    else if (k == "ray_sort_dir") *value = c->ray_sort_dir;
     else if (k == "sil_xcds") *value = c->sil_xcds;
     else if (k == "morton_bits") *value = c->morton_bits;
     else if (k == "hold_min") *value = c->hold_min;

@@ -1214,7 +1214,8 @@ static int launch_instances_intersect(fcpw_hip_ctx *c, const void *rays, int64_t
     int st = instances_ready(c, "intersect");
     if (st) return st;
     const uint32_t *order;
-    st = morton_order(c, (const float *)rays, 10, n, s, &order); // ordered by origin
+    // ordered by origin, and by direction octant when ray_sort_dir is set
+    st = morton_order(c, (const float *)rays, 10, n, s, &order, c->ray_sort_dir ? 3 : -1);
     if (st) return st;
     if ((st = timing_begin(c, s))) return st;
     DevScene T = dev_scene(c);

@@ -345,6 +345,8 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "ray_stack", "ray_hold_min"  the cooperative ray kernels' LDS stack entries (4, 8
                  or 16, default 4) and leaf-pass threshold (1..64, default 8)
      "sil_hold_min"  the cooperative silhouette kernel's leaf-pass threshold (default 8)
+     "ray_sort_dir"  1 = rays ordered by direction octant (quadrant), then by origin
+                 along the Morton curve (default); 0 = by origin only
      "hold_min"  coop: lanes that must hold a leaf before the wave processes
                  leaves together (1..64, default 1)
      "stack"     traversal-stack entries per lane kept in LDS (default 12), the rest in

@@ -88,7 +88,8 @@ def test_cooperative_and_per_lane_ray_kernels_equal_oracle(gpu, name, occlusion)
     tmax[::4] = np.float32(2.5)
     ref, ref_obj = I.intersect(o, d, tmax, occlusion=occlusion)
     for cfg in (dict(ray_coop=1, ray_hold_min=1), dict(ray_coop=1, ray_hold_min=8), dict(ray_coop=1, ray_hold_min=64),
-                dict(ray_coop=1, ray_hold_min=1, sort=0, xcds=1), dict(ray_coop=0)):
+                dict(ray_coop=1, ray_hold_min=1, sort=0, xcds=1), dict(ray_coop=1, ray_sort_dir=0, sort=1),
+                dict(ray_coop=0)):
         for key, val in cfg.items():
             g.set_option(key, val)
         hits = fcpw.GPUInteractionList()
@@ -98,7 +99,7 @@ def test_cooperative_and_per_lane_ray_kernels_equal_oracle(gpu, name, occlusion)
             assert_hits_equal(records(hits), obj, ref, ref_obj)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(ray_coop=1, ray_hold_min=8, sort=1, xcds=4).items():
+    for key, val in dict(ray_coop=1, ray_hold_min=8, ray_sort_dir=1, sort=1, xcds=4).items():
         g.set_option(key, val)
 
 

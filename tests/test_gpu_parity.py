@@ -241,6 +241,8 @@ def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
     configs = [dict(ray_coop=0, block=b, stack=k, sort=srt) for b in (64, 256) for k in (8, 16) for srt in (0, 1)]
     configs += [dict(ray_coop=1, ray_stack=k, ray_hold_min=h, sort=srt, xcds=x) for k in (4, 8, 16)
                 for h in (1, 8, 64) for srt in (0, 1) for x in (1, 8)]
+    configs += [dict(ray_coop=rc, ray_sort_dir=sd, morton_bits=b) for rc in (0, 1) for sd in (0, 1)
+                for b in (18, 21, 24)]
     for cfg in configs:
         for key, val in cfg.items():
             g.set_option(key, val)
@@ -251,7 +253,8 @@ def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
             assert_bit_equal(rec, ref)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, stack=12, sort=1, xcds=4, block=128).items():
+    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, ray_sort_dir=1, stack=12, sort=1, xcds=4, block=128,
+                         morton_bits=24).items():
         g.set_option(key, val)
 
 
@@ -386,7 +389,7 @@ def test_errors_are_reported(gpu):
         with pytest.raises(ValueError):
             g.set_option(key, bad)
     defaults = dict(stack=12, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
-                    sil_xcds=8)
+                    sil_xcds=8, ray_sort_dir=1)
     assert {k: g.get_option(k) for k in defaults} == defaults
     with pytest.raises(RuntimeError, match="without silhouettes"):  # compute_silhouettes was not called
         g.find_closest_silhouette_points(np.zeros((1, 3), np.float32), np.ones(1, np.float32), np.zeros(1, np.int32),
