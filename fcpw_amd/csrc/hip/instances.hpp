@@ -300,7 +300,8 @@ __global__ __launch_bounds__(64) void k_instances_closest_coop(DevScene T, const
     }
     __syncthreads();
     while (true) {
-        if (active && !holding) {
+        // two traversal steps per wave iteration, as in closest_point_query_coop
+        for (int rep = 0; rep < 2; rep++) if (active && !holding) {
             if (start) {
                 // TransformedAggregate.findClosestPoint: map the query into the object's space
                 start = false;
@@ -714,7 +715,8 @@ __global__ __launch_bounds__(64) void k_instances_intersect_coop(DevScene T, con
     }
     __syncthreads();
     while (true) {
-        if (active && !holding) {
+        // two traversal steps per wave iteration, as in closest_point_query_coop
+        for (int rep = 0; rep < 2; rep++) if (active && !holding) {
             if (start) {
                 // TransformedAggregate.intersect: map the ray into the object's space
                 start = false;

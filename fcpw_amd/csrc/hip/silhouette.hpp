@@ -349,7 +349,8 @@ __global__ __launch_bounds__(64) void k_closest_silhouettes_coop(DevScene S, Sil
     FCPW_STACK_DECL(BLOCK, K);
     __syncthreads();
     while (true) {
-        if (active && !holding) {
+        // two traversal steps per wave iteration, as in closest_point_query_coop
+        for (int rep = 0; rep < 2; rep++) if (active && !holding) {
             if (!(live && cur_d2 <= r2)) {
                 live = false;
                 while (sp > 0) {
