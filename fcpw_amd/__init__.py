@@ -122,40 +122,52 @@ class _GPUSceneMixin:
         return self._find_closest_silhouette_points_arrays(*args, **kwargs)
 
     # -- device-resident queries (torch tensors) ---------------------------------
+    # Every tensor is checked on the host before its pointer reaches the C ABI,
+    # which can only check alignment: a wrong shape, dtype or device would
+    # otherwise become an out-of-bounds device access.
     def find_closest_points_device(self, spheres, out, record_normals=False, stream=None, object_index=None):
         """spheres: (n, 4) float32 CUDA tensor of GPUBoundingSphere records;
         out: (n, 10) float32 CUDA tensor receiving GPUInteraction records;
         object_index: optional (n,) int32 CUDA tensor receiving object indices."""
-        n = spheres.shape[0]
+        dev = self._device()
+        n = _check_tensor(spheres, None, 4, _F32, "spheres", dev)
+        _check_tensor(out, n, 10, _F32, "out", dev)
         self._closest_points_device(spheres.data_ptr(), n, bool(record_normals), out.data_ptr(),
-                                    _stream_handle(spheres, stream), _object_ptr(object_index, n))
+                                    _stream_handle(spheres, stream), _object_ptr(object_index, n, dev))
 
     def intersect_device(self, rays, out, check_for_occlusion=False, stream=None, object_index=None):
         """rays: (n, 10) float32 CUDA tensor of GPURay records; out: (n, 10) float32;
         object_index: optional (n,) int32 CUDA tensor receiving object indices."""
-        n = rays.shape[0]
+        dev = self._device()
+        n = _check_tensor(rays, None, 10, _F32, "rays", dev)
+        _check_tensor(out, n, 10, _F32, "out", dev)
         self._intersect_device(rays.data_ptr(), n, bool(check_for_occlusion), out.data_ptr(),
-                               _stream_handle(rays, stream), _object_ptr(object_index, n))
+                               _stream_handle(rays, stream), _object_ptr(object_index, n, dev))
 
     def find_closest_silhouette_points_device(self, spheres, flip_normal_orientation, out, squared_min_radius=0.0,
                                               precision=1e-3, stream=None):
         """spheres: (n, 4) GPUBoundingSphere records, flip_normal_orientation: (n,) int32/uint32,
         out: (n, 10) GPUInteraction records (CUDA tensors)."""
-        n = spheres.shape[0]
+        dev = self._device()
+        n = _check_tensor(spheres, None, 4, _F32, "spheres", dev)
         fl = flip_normal_orientation
-        if fl.shape[0] != n or not fl.is_contiguous() or fl.element_size() != 4:
-            raise ValueError("flip_normal_orientation must be a contiguous (n,) 32-bit tensor")
+        _check_tensor(fl, n, 0, _I32, "flip_normal_orientation", dev)
+        _check_tensor(out, n, 10, _F32, "out", dev)
         self._closest_silhouette_points_device(spheres.data_ptr(), fl.data_ptr(), n, float(squared_min_radius),
                                                float(precision), out.data_ptr(), _stream_handle(spheres, stream))
 
     def intersect_spheres_device(self, spheres, rand_nums, out, stream=None, object_index=None):
         """spheres: (n, 4) GPUBoundingSphere records, rand_nums: (n, 3) float32 in [0, 1),
         out: (n, 10) GPUInteraction records, object_index: optional (n,) int32 (CUDA tensors)."""
-        n = spheres.shape[0]
-        if rand_nums.shape[0] != n or rand_nums.shape[-1] != 3 or not rand_nums.is_contiguous():
-            raise ValueError("rand_nums must be a contiguous (n, 3) tensor")
+        dev = self._device()
+        n = _check_tensor(spheres, None, 4, _F32, "spheres", dev)
+        _check_tensor(rand_nums, n, 3, _F32, "rand_nums", dev)
+        _check_tensor(out, n, 10, _F32, "out", dev)
         self._intersect_spheres_device(spheres.data_ptr(), rand_nums.data_ptr(), n, out.data_ptr(),
-                                       _stream_handle(spheres, stream), _object_ptr(object_index, n))
+                                       _stream_handle(spheres, stream), _object_ptr(object_index, n, dev))
+
+    def _device(self):
+        return int(self._info("device"))
 
     def set_option(self, key, value):
         self._set_option(key, int(value))
@@ -167,11 +179,38 @@ class _GPUSceneMixin:
         return self._info(key)
 
 
-def _object_ptr(t, n):
+_F32 = ("float32",)
+_I32 = ("int32", "uint32")
+
+
+def _check_tensor(t, rows, cols, dtypes, name, device=None):
+    """Checks that t is a contiguous CUDA tensor of one of `dtypes` (names) on
+    `device`, with `rows` rows (None: any) and `cols` columns (0: a 1-D tensor).
+    Returns its row count.  Raises TypeError / ValueError before any launch."""
+    if not hasattr(t, "data_ptr") or not hasattr(t, "is_cuda"):
+        raise TypeError(f"{name} must be a torch tensor")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if str(t.dtype).replace("torch.", "") not in dtypes:
+        raise TypeError(f"{name} must have dtype {' or '.join(dtypes)}, not {t.dtype}")
+    if device is not None and t.device.index != device:
+        raise ValueError(f"{name} is on cuda:{t.device.index}, the scene on cuda:{device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if cols == 0:
+        if t.dim() != 1:
+            raise ValueError(f"{name} must be a 1-D tensor")
+    elif t.dim() != 2 or t.shape[1] != cols:
+        raise ValueError(f"{name} must have shape (n, {cols}), not {tuple(t.shape)}")
+    if rows is not None and t.shape[0] != rows:
+        raise ValueError(f"{name} must have {rows} rows, not {t.shape[0]}")
+    return int(t.shape[0])
+
+
+def _object_ptr(t, n, device=None):
     if t is None:
         return 0
-    if t.shape[0] != n or not t.is_contiguous() or t.element_size() != 4:
-        raise ValueError("object_index must be a contiguous (n,) 32-bit tensor")
+    _check_tensor(t, n, 0, _I32, "object_index", device)
     return t.data_ptr()
 
 

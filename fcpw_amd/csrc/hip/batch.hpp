@@ -211,14 +211,22 @@ static int batch_grow(fcpw_hip_batch *b, void **buf, size_t *cap, size_t bytes)
     return FCPW_HIP_OK;
 }
 
-// table up to date with the member scenes (a re-upload moves their buffers)
+// table up to date with the member scenes (a re-upload moves their buffers).  A
+// member may have been re-transferred since fcpw_hip_batch_create, so the
+// checks made there are repeated: an instanced upload (its records index
+// instances, not primitives) or a change of dimension or device is rejected
+// here, on the host, before anything is launched.
 static int batch_ready(fcpw_hip_batch *b, const char *fn)
 {
     if (!b) return FCPW_HIP_ERR_INVALID_ARGUMENT;
     if (!b->have_counts) FAIL(b, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: call fcpw_hip_batch_set_queries first", fn);
     for (size_t s = 0; s < b->scenes.size(); s++) {
-        if (!b->scenes[s]->uploaded) FAIL(b, FCPW_HIP_ERR_NOT_UPLOADED, "%s: member scene %zu has no BVH", fn, s);
-        if (b->scenes[s]->generation != b->generations[s]) b->dirty = true;
+        const fcpw_hip_ctx *m = b->scenes[s];
+        if (!m->uploaded || m->instanced)
+            FAIL(b, FCPW_HIP_ERR_NOT_UPLOADED, "%s: member scene %zu has no uploaded single-object BVH", fn, s);
+        if (m->dim != b->dim || m->device != b->device)
+            FAIL(b, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s: member scene %zu changed dimension or device", fn, s);
+        if (m->generation != b->generations[s]) b->dirty = true;
     }
     if (b->dirty) return batch_table(b);
     return FCPW_HIP_OK;

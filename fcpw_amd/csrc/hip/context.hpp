@@ -16,6 +16,7 @@ struct fcpw_hip_ctx {
     int dim = 0;
     bool uploaded = false;
     uint64_t generation = 0; // bumped by every upload (batches re-read the buffers)
+    uint64_t positions_version = 0; // bumped by every upload and vertex-position update
     std::string err;
     // device buffers
     float4 *records = nullptr, *root = nullptr, *prims = nullptr;

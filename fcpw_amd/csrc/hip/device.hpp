@@ -20,10 +20,10 @@
 // shrink r^2 by each child's farthest-corner distance, continue with the nearer
 // child and push the farther one.  Each stack entry carries the child's link and
 // its squared (or parametric) distance, so a pop is pruned without refetching
-// the node.  The top K entries of the stack live in LDS (K x 8 bytes per lane,
-// lane-interleaved so a wave's accesses hit distinct banks); entries beyond K
-// spill to a private array.  Depth <= 63 (leaf forced at depth 62,
-// bvh.inl:170), so 64 entries always suffice.
+// the node.  The newest K entries of the stack live in LDS (K x 8 bytes per
+// lane, lane-interleaved so a wave's accesses hit distinct banks) as a sliding
+// window; older entries spill to a private array.  Depth <= 63 (leaf forced at
+// depth 62, bvh.inl:170), so 64 entries always suffice.
 #include <hip/hip_runtime.h>
 
 #include <float.h>
@@ -69,17 +69,26 @@ struct DevInstance {
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
 
 // ---------------------------------------------------------------------------
-// traversal stack: entries [0, K) in LDS, lane-interleaved (entry s of lane t
-// at lds[s * BLOCK + t], so a wave's accesses fall in distinct banks), entries
-// [K, 64) in a private spill array.  The stack pointer stays in a register and
-// the LDS array is declared in the traversal function itself, so the accesses
-// compile to ds_read/ds_write rather than generic (flat) loads.
+// traversal stack: the NEWEST (up to) K entries live in LDS, lane-interleaved
+// (the entry of stack index i of lane t at lds[(i mod K) * BLOCK + t], so a
+// wave's accesses fall in distinct banks); older entries sit in a private
+// spill array at their stack index.  Entries [stack_lo, sp) are in LDS and
+// [0, stack_lo) in the spill array, with sp - stack_lo <= K.  A push onto a
+// full window first moves the window's oldest entry -- the one in the LDS slot
+// the push is about to write -- to the spill array; a pop below the window
+// reads the spill array and leaves the window empty.  Pushes and pops at the
+// top of a deep stack therefore stay in LDS, and each entry is spilled at most
+// once per push.  The stack pointers stay in registers and the LDS array is
+// declared in the traversal function itself, so the accesses compile to
+// ds_read/ds_write rather than generic (flat) loads.  K = 0 keeps every entry
+// in the spill array.
 // ---------------------------------------------------------------------------
 #define FCPW_STACK_DECL(BLOCK, K)                                                                              \
     __shared__ uint64_t lds_stack[(K) > 0 ? (K) * (BLOCK) : 1];                                              \
     uint32_t stack_spill[2 * (STACK_TOTAL - (K))];                                                           \
     const int stack_lane = threadIdx.x;                                                                       \
-    int sp = 0
+    int sp = 0;                                                                                               \
+    int stack_lo = 0
 
 // An entry packs (link, distance bits) into 64 bits.  The cold spill path moves
 // the entry as two 32-bit halves: an access of a different type from the LDS
@@ -91,11 +100,18 @@ __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f)
         ON_FULL;                                                                                              \
     } else {                                                                                                  \
         const uint32_t l_ = (uint32_t)(LINK), d_ = __float_as_uint(D);                                        \
-        if (sp < (K)) {                                                                                       \
-            lds_stack[sp * (BLOCK) + stack_lane] = (uint64_t)l_ | ((uint64_t)d_ << 32);                       \
+        if ((K) == 0) {                                                                                       \
+            stack_spill[2 * sp] = l_;                                                                         \
+            stack_spill[2 * sp + 1] = d_;                                                                     \
         } else {                                                                                              \
-            stack_spill[2 * (sp - (K))] = l_;                                                                 \
-            stack_spill[2 * (sp - (K)) + 1] = d_;                                                             \
+            const int slot_ = ((K) > 0 ? sp % ((K) > 0 ? (K) : 1) : 0) * (BLOCK) + stack_lane;               \
+            if (sp - stack_lo == (K)) {                                                                       \
+                const uint64_t old_ = lds_stack[slot_];                                                       \
+                stack_spill[2 * stack_lo] = (uint32_t)old_;                                                   \
+                stack_spill[2 * stack_lo + 1] = (uint32_t)(old_ >> 32);                                       \
+                stack_lo++;                                                                                   \
+            }                                                                                                 \
+            lds_stack[slot_] = (uint64_t)l_ | ((uint64_t)d_ << 32);                                           \
         }                                                                                                     \
         sp++;                                                                                                 \
     }
@@ -103,13 +119,14 @@ __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f)
 #define FCPW_POP(BLOCK, K, LINK, D)                                                                            \
     do {                                                                                                      \
         sp--;                                                                                                 \
-        if (sp < (K)) {                                                                                       \
-            const uint64_t e_ = lds_stack[sp * (BLOCK) + stack_lane];                                         \
+        if ((K) > 0 && sp >= stack_lo) {                                                                     \
+            const uint64_t e_ = lds_stack[((K) > 0 ? sp % ((K) > 0 ? (K) : 1) : 0) * (BLOCK) + stack_lane];  \
             (LINK) = (uint32_t)e_;                                                                            \
             (D) = __uint_as_float((uint32_t)(e_ >> 32));                                                      \
         } else {                                                                                              \
-            (LINK) = stack_spill[2 * (sp - (K))];                                                             \
-            (D) = __uint_as_float(stack_spill[2 * (sp - (K)) + 1]);                                           \
+            (LINK) = stack_spill[2 * sp];                                                                     \
+            (D) = __uint_as_float(stack_spill[2 * sp + 1]);                                                   \
+            stack_lo = sp;                                                                                    \
         }                                                                                                     \
     } while (0)
 
@@ -1517,9 +1534,13 @@ __device__ __forceinline__ void distance_backward_one(const float4 *prims, const
             float sc = gd / dist;
             for (int k = 0; k < DIM; k++) g[k] = sc * diff[k];
             if (grad_vertices) {
+                // hardware FP32 atomic add (global_atomic_add_f32, no return value) rather
+                // than the compare-and-swap loop atomicAdd compiles to without
+                // -munsafe-fp-atomics; grad_vertices is device (coarse-grained) memory
                 for (int v = 0; v < DIM; v++) {
                     int32_t vid = prim_vids[(size_t)s * DIM + v];
-                    for (int k = 0; k < DIM; k++) atomicAdd(&grad_vertices[(size_t)vid * DIM + k], -w[v] * g[k]);
+                    for (int k = 0; k < DIM; k++)
+                        (void)unsafeAtomicAdd(&grad_vertices[(size_t)vid * DIM + k], -w[v] * g[k]);
                 }
             }
         }

@@ -254,6 +254,7 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     c->node_leaf_off = std::move(node_leaf_off);
     c->uploaded = true;
     c->generation++;
+    c->positions_version++;
     return FCPW_HIP_OK;
 }
 
@@ -547,6 +548,7 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
     c->node_leaf_off = std::move(T.node_leaf_off);
     c->uploaded = true;
     c->generation++;
+    c->positions_version++;
     return FCPW_HIP_OK;
 }
 
@@ -661,6 +663,7 @@ int fcpw_hip_update_positions(fcpw_hip_ctx *c, const float *positions, int n_ver
              n_vertices);
     HIPCHECK(c, hipSetDevice(c->device));
     HIPCHECK(c, hipMemcpy(c->V, positions, (size_t)n_vertices * c->dim * sizeof(float), hipMemcpyHostToDevice));
+    c->positions_version++;
     st = launch_rebuild(c, 0);
     if (st) return st;
     HIPCHECK(c, hipDeviceSynchronize());
@@ -677,6 +680,7 @@ int fcpw_hip_update_positions_device(fcpw_hip_ctx *c, const float *positions, in
     hipStream_t s = (hipStream_t)stream;
     HIPCHECK(c, hipMemcpyAsync(c->V, positions, (size_t)n_vertices * c->dim * sizeof(float),
                                hipMemcpyDeviceToDevice, s));
+    c->positions_version++;
     return launch_rebuild(c, s);
 }
 
@@ -1256,6 +1260,9 @@ int fcpw_hip_get_info(const fcpw_hip_ctx *c, const char *key, int64_t *value)
     else if (k == "max_depth") *value = c->max_depth;
     else if (k == "n_cus") *value = c->n_cus;
     else if (k == "uploaded") *value = c->uploaded ? 1 : 0;
+    else if (k == "generation") *value = (int64_t)c->generation;
+    else if (k == "positions_version") *value = (int64_t)c->positions_version;
+    else if (k == "device") *value = c->device;
     else if (k == "n_instances") *value = c->instanced ? c->n_instances : 0;
     else if (k == "n_objects") *value = c->instanced ? (int64_t)c->inst_objects.size() : (c->uploaded ? 1 : 0);
     else if (k.rfind("stat_", 0) == 0) {

@@ -17,8 +17,13 @@ Queries with d == 0 contribute zero gradient (the distance is not
 differentiable there).
 
 Passing V makes the forward pass upload V to the scene and refit the BVH on the
-device first, so optimising V in a loop keeps the BVH consistent.  The scene
-must not be changed between a forward pass and its backward pass.
+device first, so optimising V in a loop keeps the BVH consistent.  The backward
+pass reads the scene's current primitives, so it must see the geometry of its
+own forward pass: the forward records the scene's positions version, and a
+backward that finds another one (a second forward, an update of the vertices
+or a new transfer in between) uploads the V of its forward pass again and
+refits (the scene then holds that V), or raises RuntimeError when its forward
+pass was given no V, instead of returning gradients of the wrong geometry.
 """
 import torch
 
@@ -58,20 +63,29 @@ class ClosestPointDistance(torch.autograd.Function):
         scene._closest_points_compact_device(x.data_ptr(), 0, n, d.data_ptr(), p.data_ptr(), prim.data_ptr(),
                                              slot.data_ptr(), s)
         ctx.scene = scene
+        ctx.positions_version = scene.info("positions_version")
         ctx.n_vertices = None if V is None else V.shape[0]
-        ctx.save_for_backward(x, slot)
+        ctx.save_for_backward(x, slot, V if V is not None else torch.empty(0, device=x.device))
         ctx.mark_non_differentiable(p, prim)
         return d, p, prim
 
     @staticmethod
     def backward(ctx, grad_d, grad_p, grad_prim):
-        x, slot = ctx.saved_tensors
+        x, slot, V = ctx.saved_tensors
         scene = ctx.scene
         dim = scene.DIM
         need_x = ctx.needs_input_grad[0]
         need_v = ctx.needs_input_grad[1] and ctx.n_vertices is not None
         if grad_d is None or not (need_x or need_v):
             return None, None, None
+        if scene.info("positions_version") != ctx.positions_version:
+            if ctx.n_vertices is None:
+                raise RuntimeError("closest_point_distance: the scene's vertex positions changed between the forward "
+                                   "pass and its backward pass, and the forward pass was given no V to restore")
+            s = _stream()
+            scene._update_positions_device(V.data_ptr(), V.shape[0], s)
+            scene._refit_device(s)
+            ctx.positions_version = scene.info("positions_version")
         grad_d = grad_d.contiguous().to(torch.float32)
         gx = torch.empty_like(x) if need_x else None
         gV = torch.zeros((ctx.n_vertices, dim), dtype=torch.float32, device=x.device) if need_v else None

@@ -235,7 +235,9 @@ int fcpw_hip_closest_points_compact_device(fcpw_hip_ctx *ctx, const float *point
  *   grad_points[i]  = grad_d[i] * (x - p) / d                 (n x dim, overwritten)
  *   grad_vertices[v] += -grad_d[i] * b_k * (x - p) / d        (n_vertices x dim, FP32 atomics)
  * in the caller's vertex order.  Queries with d == 0 or slot < 0 contribute 0.
- * Either output may be NULL.
+ * Either output may be NULL.  grad_vertices must be device memory from
+ * hipMalloc (coarse-grained): the accumulation uses the hardware FP32 atomic
+ * add, whose summation order is not fixed.
  */
 int fcpw_hip_distance_backward_device(fcpw_hip_ctx *ctx, const float *points, const int32_t *slot,
                                       const float *grad_d, int64_t n, float *grad_points, float *grad_vertices,
@@ -372,7 +374,9 @@ int fcpw_hip_set_option(fcpw_hip_ctx *ctx, const char *key, int value);
 int fcpw_hip_get_option(const fcpw_hip_ctx *ctx, const char *key, int *value);
 
 /* Uploaded BVH facts: n_records (internal nodes), n_leaves, n_prims, n_vertices,
-   dim, max_depth, n_cus, n_objects, n_instances (0 unless instanced; on a scene of
+   dim, max_depth, n_cus, device (the HIP device the context was created on),
+   generation (bumped by every upload, which moves the device buffers),
+   positions_version (bumped by every upload and vertex-position update), n_objects, n_instances (0 unless instanced; on a scene of
    several objects the tree facts are the instance tree's: n_prims = instances,
    n_vertices = 0).  The instanced kernels run with fixed shapes (128 threads, 8
    LDS stack entries per object traversal, 4 for the instance tree); of the

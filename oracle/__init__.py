@@ -21,7 +21,11 @@ Pinning (each function's check lives in tests/test_oracle_*.py):
   * CPU and GPU traversal    -- equal to brute force (index except at FP64 ties);
   * rays                     -- analytic axis-aligned hits and the FP64 brute force;
   * refit                    -- bit equality with a fresh build's boxes on the
-    same topology.
+    same topology;
+  * vectorized aggregate (``VectorizedBvh``, fcpw_oracle_mbvh.cpp: the 4-wide
+    MBVH with SIMD leaf groups of mbvh.inl / wide_query_operations.h) -- the
+    collapsed tree's invariants, and its distances equal to the scalar path's
+    and the brute force within 1e-6 (tests/aggregate_tests.cpp:562,790-807).
 Bit equality with the reference binary itself is "parity unpinned": the Eigen
 commit the reference pins is unknown and nothing in its tests fixes FP32 bits.
 """
@@ -93,6 +97,16 @@ def _declare(L):
     L.oracle_build.restype = c_void
     L.oracle_build.argtypes = [c_int, _f32p, c_int, _i32p, c_int, c_int, c_int, c_int]
     L.oracle_free.argtypes = [c_void]
+    L.oracle_build_packed.restype = c_void
+    L.oracle_build_packed.argtypes = [c_int, _f32p, c_int, _i32p, c_int, c_int, c_int, c_int, c_int]
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    for w in (8, 16):
+        getattr(L, f"oracle_mbvh_build_w{w}").restype = c_void
+        getattr(L, f"oracle_mbvh_build_w{w}").argtypes = [c_void]
+        getattr(L, f"oracle_mbvh_free_w{w}").argtypes = [c_void]
+        getattr(L, f"oracle_mbvh_info_w{w}").argtypes = [c_void, i64p, i64p, i64p]
+        getattr(L, f"oracle_mbvh_export_w{w}").argtypes = [c_void, _f32p, _f32p, _i32p, _i32p]
+        getattr(L, f"oracle_mbvh_closest_points_w{w}").argtypes = [c_void, _f32p, _f32p, c_int, c_int, _hitp, i64p]
     for name in ("oracle_num_nodes", "oracle_num_leaves", "oracle_max_depth"):
         getattr(L, name).restype = c_int
         getattr(L, name).argtypes = [c_void]
@@ -300,6 +314,79 @@ class Bvh:
         out = np.zeros(len(q), HIT_DTYPE)
         lib().oracle_brute_closest_f32(self._h, q, r2, len(q), nthreads, out)
         return out
+
+
+def host_simd_width():
+    """FCPW_SIMD_WIDTH the reference's CMake would pick on this host
+    (CMakeLists.txt:88-100): 16 with AVX-512, else 8 (AVX2)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            flags = next((ln for ln in fh if ln.startswith("flags")), "")
+    except OSError:
+        flags = ""
+    f = set(flags.split())
+    return 16 if {"avx512f", "avx512vl", "avx512dq"} <= f else 8
+
+
+class VectorizedBvh:
+    """Reference Scene<DIM> with one object built with vectorize = true: the
+    SAH tree with leaves packed to `width` primitives (fcpw.inl:812-827),
+    collapsed into 4-wide nodes with SIMD leaf groups (mbvh.inl:56-147,171-245),
+    queried by the wide traversal (mbvh.inl:745-803,2034-2197)."""
+
+    def __init__(self, positions, indices, width=None, heuristic=SURFACE_AREA, n_buckets=8):
+        self.width = int(width or host_simd_width())
+        if self.width not in (8, 16):
+            raise ValueError("width must be 8 or 16")
+        self.positions = _f32(positions)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int32)
+        self.dim = self.positions.shape[1]
+        pack = 1 if heuristic != LONGEST_AXIS_CENTER else 0  # LongestAxisCenter builds unpacked (fcpw.inl:822-824)
+        self._bin = lib().oracle_build_packed(self.dim, self.positions, len(self.positions), self.indices,
+                                              len(self.indices), heuristic, self.width, n_buckets, pack)
+        if not self._bin:
+            raise ValueError("oracle_build_packed rejected the input")
+        self._m = getattr(lib(), f"oracle_mbvh_build_w{self.width}")(self._bin)
+
+    def __del__(self):
+        if getattr(self, "_m", None):
+            getattr(lib(), f"oracle_mbvh_free_w{self.width}")(self._m)
+            self._m = None
+        if getattr(self, "_bin", None):
+            lib().oracle_free(self._bin)
+            self._bin = None
+
+    def info(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        getattr(lib(), f"oracle_mbvh_info_w{self.width}")(self._m, ctypes.byref(a), ctypes.byref(b),
+                                                           ctypes.byref(c))
+        return {"n_nodes": a.value, "n_leaf_groups": b.value, "max_depth": c.value}
+
+    def export(self):
+        """(bmin, bmax: (n_nodes, 3, 4); child: (n_nodes, 4); leaf_prim: (n_leaf_groups, width))"""
+        inf = self.info()
+        n, g = inf["n_nodes"], inf["n_leaf_groups"]
+        bmin, bmax = np.zeros((n, 3, 4), np.float32), np.zeros((n, 3, 4), np.float32)
+        child, leaf_prim = np.zeros((n, 4), np.int32), np.zeros((max(g, 1), self.width), np.int32)
+        getattr(lib(), f"oracle_mbvh_export_w{self.width}")(self._m, bmin, bmax, child, leaf_prim)
+        return bmin, bmax, child, leaf_prim[:g]
+
+    def binary_nodes(self):
+        """the packed binary tree the 4-wide one is collapsed from"""
+        n = lib().oracle_num_nodes(self._bin)
+        pmin, pmax = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        nrefs, offset = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        lib().oracle_export_nodes(self._bin, pmin, pmax, nrefs, offset)
+        return pmin, pmax, nrefs, offset
+
+    def closest_points(self, q, r2=None, nthreads=0, visited=False):
+        q = _f32(q, self.dim)
+        r2 = np.full(len(q), np.inf, np.float32) if r2 is None else _f32(r2).ravel()
+        out = np.zeros(len(q), HIT_DTYPE)
+        v = ctypes.c_int64()
+        getattr(lib(), f"oracle_mbvh_closest_points_w{self.width}")(self._m, q, r2, len(q), nthreads, out,
+                                                                     ctypes.byref(v))
+        return (out, v.value) if visited else out
 
 
 def brute_closest_f64(positions, indices, q, nthreads=0):

@@ -493,6 +493,7 @@ static int ray_segment(const float *pa, const float *pb, const float *o, const f
 typedef struct {
     oracle_bvh *h;
     int heuristic, leaf_size, n_buckets;
+    int pack_leaves, depth_guess; /* packLeaves / depthGuess (bvh.inl:440-455) */
     box_t *ref_boxes;
     float *ref_cent;  /* dim floats per reference */
     box_t *bkt_box, *rbkt_box;
@@ -512,11 +513,15 @@ static void swap_refs(build_ctx *b, int i, int j)
     }
 }
 
-/* computeSplitCost (bvh.inl:3-37) with packLeaves = false (fcpw.inl:812-819) */
-static float split_cost(build_ctx *b, const box_t *bl, const box_t *br, int nl, int nr)
+/* computeSplitCost (bvh.inl:3-37); packLeaves is set for the vectorized
+   aggregate's SAH build (fcpw.inl:812-827) */
+static float split_cost(build_ctx *b, const box_t *bl, const box_t *br, int nl, int nr, int depth)
 {
     int dim = b->h->dim;
     float cost = MAXF;
+    if (b->pack_leaves && depth > 0 && ((float)b->depth_guess / depth) < 1.5f && nl % b->leaf_size != 0 &&
+        nr % b->leaf_size != 0)
+        return cost;
     if (b->heuristic == ORACLE_SURFACE_AREA) {
         cost = (float)nl * box_surface_area(bl, dim) + (float)nr * box_surface_area(br, dim);
     } else if (b->heuristic == ORACLE_OVERLAP_SURFACE_AREA) {
@@ -535,8 +540,8 @@ static float split_cost(build_ctx *b, const box_t *bl, const box_t *br, int nl, 
 }
 
 /* computeObjectSplit (bvh.inl:39-111) */
-static void object_split(build_ctx *b, const box_t *bb, const box_t *bc, int start, int end, int *split_dim,
-                         float *split_coord)
+static void object_split(build_ctx *b, const box_t *bb, const box_t *bc, int start, int end, int depth,
+                         int *split_dim, float *split_coord)
 {
     int dim = b->h->dim, nb = b->n_buckets;
     float split_cost_best = MAXF;
@@ -571,7 +576,7 @@ static void object_split(build_ctx *b, const box_t *bb, const box_t *bc, int sta
                 box_expand_box(&left, &b->bkt_box[k - 1], dim);
                 nl += b->bkt_cnt[k - 1];
                 if (nl > 0 && b->rbkt_cnt[k] > 0) {
-                    float cost = split_cost(b, &left, &b->rbkt_box[k], nl, b->rbkt_cnt[k]);
+                    float cost = split_cost(b, &left, &b->rbkt_box[k], nl, b->rbkt_cnt[k], depth);
                     if (cost < split_cost_best) {
                         split_cost_best = cost;
                         *split_dim = d;
@@ -587,7 +592,7 @@ static void object_split(build_ctx *b, const box_t *bb, const box_t *bc, int sta
     }
 }
 
-/* performObjectSplit (bvh.inl:113-140), packLeaves = false */
+/* performObjectSplit (bvh.inl:113-140) */
 static int object_partition(build_ctx *b, int start, int end, int split_dim, float split_coord)
 {
     int dim = b->h->dim, mid = start;
@@ -597,7 +602,13 @@ static int object_partition(build_ctx *b, int start, int end, int split_dim, flo
             mid++;
         }
     }
-    if (mid == start || mid == end) mid = start + (end - start) / 2;
+    if (mid == start || mid == end) {
+        mid = start + (end - start) / 2;
+        if (b->pack_leaves) { /* one branch a multiple of the leaf size (bvh.inl:132-136) */
+            while ((mid - start) % b->leaf_size != 0 && mid < end) mid++;
+            if (mid == end) mid = start + (end - start) / 2;
+        }
+    }
     return mid;
 }
 
@@ -643,7 +654,7 @@ static void build_recursive(build_ctx *b, int parent, int is_right, int start, i
     if (leaf) return;
     int split_dim;
     float split_coord;
-    object_split(b, &bb, &bc, start, end, &split_dim, &split_coord);
+    object_split(b, &bb, &bc, start, end, depth, &split_dim, &split_coord);
     int mid = object_partition(b, start, end, split_dim, split_coord);
     build_recursive(b, cur, 0, start, mid, depth + 1);
     build_recursive(b, cur, 1, mid, end, depth + 1);
@@ -675,6 +686,12 @@ static void sort_soup_positions(oracle_bvh *h, const float *user_pos, const int3
 oracle_bvh *oracle_build(int dim, const float *positions, int n_vertices, const int32_t *indices, int n_prims,
                          int heuristic, int leaf_size, int n_buckets)
 {
+    return oracle_build_packed(dim, positions, n_vertices, indices, n_prims, heuristic, leaf_size, n_buckets, 0);
+}
+
+oracle_bvh *oracle_build_packed(int dim, const float *positions, int n_vertices, const int32_t *indices,
+                                int n_prims, int heuristic, int leaf_size, int n_buckets, int pack_leaves)
+{
     if ((dim != 2 && dim != 3) || n_prims <= 0 || n_vertices <= 0) return NULL;
     oracle_bvh *h = (oracle_bvh *)calloc(1, sizeof(oracle_bvh));
     h->dim = dim;
@@ -692,6 +709,8 @@ oracle_bvh *oracle_build(int dim, const float *positions, int n_vertices, const 
     b.heuristic = heuristic;
     b.leaf_size = leaf_size;
     b.n_buckets = n_buckets;
+    b.pack_leaves = pack_leaves;
+    b.depth_guess = (int)log2((double)n_prims); /* depthGuess = std::log2(primitives.size()) */
     b.ref_boxes = (box_t *)malloc((size_t)n_prims * sizeof(box_t));
     b.ref_cent = (float *)malloc((size_t)n_prims * dim * sizeof(float));
     b.bkt_box = (box_t *)malloc((size_t)n_buckets * sizeof(box_t));
@@ -722,6 +741,24 @@ void oracle_free(oracle_bvh *h)
 }
 
 int oracle_num_nodes(const oracle_bvh *h) { return h->n_nodes; }
+int oracle_dim(const oracle_bvh *h) { return h->dim; }
+int oracle_num_prims(const oracle_bvh *h) { return h->n_prims; }
+float oracle_box_surface_area(const float *pmin, const float *pmax, int dim)
+{
+    box_t b;
+    for (int d = 0; d < 3; d++) {
+        b.pmin[d] = d < dim ? pmin[d] : 0.0f;
+        b.pmax[d] = d < dim ? pmax[d] : 0.0f;
+    }
+    return box_surface_area(&b, dim);
+}
+void oracle_export_prim_vertices(const oracle_bvh *h, float *out)
+{
+    for (int s = 0; s < h->n_prims; s++)
+        for (int c = 0; c < h->k; c++)
+            memcpy(out + ((size_t)s * h->k + c) * h->dim,
+                   h->positions + (size_t)h->prim_vidx[(size_t)s * h->k + c] * h->dim, h->dim * sizeof(float));
+}
 int oracle_num_leaves(const oracle_bvh *h) { return h->n_leaves; }
 int oracle_max_depth(const oracle_bvh *h) { return h->max_depth; }
 

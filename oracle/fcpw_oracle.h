@@ -60,9 +60,20 @@ typedef struct oracle_bvh oracle_bvh;
 oracle_bvh *oracle_build(int dim, const float *positions, int n_vertices,
                          const int32_t *indices, int n_prims,
                          int heuristic, int leaf_size, int n_buckets);
+/* the same with packLeaves (bvh.inl:3-37,113-140): the build of the vectorized
+   aggregate passes pack_leaves = 1 for the SAH heuristics and leaf_size =
+   FCPW_SIMD_WIDTH (fcpw.inl:812-827) */
+oracle_bvh *oracle_build_packed(int dim, const float *positions, int n_vertices, const int32_t *indices,
+                                int n_prims, int heuristic, int leaf_size, int n_buckets, int pack_leaves);
 void oracle_free(oracle_bvh *h);
 
 int oracle_num_nodes(const oracle_bvh *h);
+int oracle_dim(const oracle_bvh *h);
+int oracle_num_prims(const oracle_bvh *h);
+/* BoundingBox::surfaceArea (bounding_volumes.h:164-167) of a box given as 3 floats */
+float oracle_box_surface_area(const float *pmin, const float *pmax, int dim);
+/* per leaf slot, the positions of its dim vertices (n_prims x dim x dim floats) */
+void oracle_export_prim_vertices(const oracle_bvh *h, float *out);
 int oracle_num_leaves(const oracle_bvh *h);
 int oracle_max_depth(const oracle_bvh *h);
 
@@ -177,6 +188,18 @@ void oracle_brute_ray_f64(int dim, const float *positions, int n_vertices, const
 /* FP64 distance from each query to one given primitive (tie classification) */
 void oracle_prim_dist_f64(int dim, const float *positions, const int32_t *indices,
                           const float *queries, const int32_t *prims, int n, double *d);
+
+/* ---- the vectorized CPU aggregate (fcpw_oracle_mbvh.cpp), compiled for 8
+   lanes (AVX2 + FMA, suffix _w8) and 16 lanes (AVX-512, suffix _w16) ---- */
+#define ORACLE_MBVH_DECL(W)                                                                                 \
+    void *oracle_mbvh_build_w##W(const oracle_bvh *h);                                                     \
+    void oracle_mbvh_free_w##W(void *m);                                                                    \
+    void oracle_mbvh_info_w##W(const void *m, int64_t *n_nodes, int64_t *n_leaf_groups, int64_t *max_depth); \
+    void oracle_mbvh_export_w##W(const void *m, float *bmin, float *bmax, int32_t *child, int32_t *leaf_prim); \
+    void oracle_mbvh_closest_points_w##W(const void *m, const float *queries, const float *r2, int n,       \
+                                         int nthreads, oracle_hit *out, int64_t *visited);
+ORACLE_MBVH_DECL(8)
+ORACLE_MBVH_DECL(16)
 
 #ifdef __cplusplus
 }

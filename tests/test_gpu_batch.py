@@ -215,6 +215,78 @@ def test_batch_picks_up_a_reuploaded_member(gpu):
     np.testing.assert_array_equal(got["d"][3000:], exp["d"])
 
 
+def test_batch_rejects_a_member_retransferred_as_a_scene_of_several_objects(gpu):
+    """a member transferred again from a scene of two objects holds an instance
+    tree, not a triangle BVH: the batch refuses it on the host, before any launch"""
+    from tests.test_host_instances import make_scene
+    V1, F1 = meshes.small_icosphere(2)
+    V2, F2 = meshes.small_torus(32, 16)
+    s1, g1 = transfer(V1, F1)
+    s2, g2 = transfer(V2, F2)
+    batch = BatchedScenes([g1, g2])
+    batch.set_queries([100, 100])
+    sph = torch.zeros((200, 4), device="cuda")
+    sph[:, 3] = float("inf")
+    out = torch.empty((200, 10), device="cuda")
+    batch.find_closest_points_device(sph, out)
+    graph = batch.capture(lambda: batch.find_closest_points_device(sph, out))
+    graph.replay()
+    g2.transfer_to_gpu(make_scene([(V1, F1), (V2, F2)], [[], []]))
+    assert g2.info("n_objects") == 2
+    with pytest.raises(RuntimeError, match="single-object"):
+        batch.find_closest_points_device(sph, out)
+    with pytest.raises(RuntimeError, match="capture the graph again"):
+        graph.replay()  # the captured table points at the freed single-object buffers
+    x = torch.zeros((200, 3), device="cuda")
+    with pytest.raises(RuntimeError, match="single-object"):
+        batch.closest_points_compact_device(x, torch.empty(200, device="cuda"))
+    torch.cuda.synchronize()
+
+
+def test_batch_graph_refuses_replay_after_set_queries(gpu):
+    V1, F1 = meshes.small_icosphere(2)
+    _, g1 = transfer(V1, F1)
+    batch = BatchedScenes([g1])
+    batch.set_queries([64])
+    sph = torch.zeros((64, 4), device="cuda")
+    out = torch.empty((64, 10), device="cuda")
+    graph = batch.capture(lambda: batch.find_closest_points_device(sph, out))
+    graph.replay()
+    batch.set_queries([5000])
+    with pytest.raises(RuntimeError, match="capture the graph again"):
+        graph.replay()
+    torch.cuda.synchronize()
+
+
+def test_batch_tensor_arguments_are_checked_before_launch(gpu):
+    V, F = meshes.small_icosphere(1)
+    _, g = transfer(V, F)
+    batch = BatchedScenes([g])
+    batch.set_queries([32])
+    x = torch.zeros((32, 3), device="cuda")
+    f = torch.zeros(32, device="cuda")
+    i = torch.zeros(32, dtype=torch.int32, device="cuda")
+    nv = g.info("n_vertices")
+    bad = [
+        lambda: batch.closest_points_compact_device(x, d=torch.zeros(31, device="cuda")),
+        lambda: batch.closest_points_compact_device(x, p=torch.zeros((32, 2), device="cuda")),
+        lambda: batch.closest_points_compact_device(x, prim=f),
+        lambda: batch.closest_points_compact_device(x, slot=torch.zeros(32, dtype=torch.int64, device="cuda")),
+        lambda: batch.closest_points_compact_device(x, r2=torch.zeros((32, 1), device="cuda")),
+        lambda: batch.closest_points_compact_device(x[:, :2].contiguous()),
+        lambda: batch.distance_backward_device(x, f, f),
+        lambda: batch.distance_backward_device(x, i, f, grad_points=torch.zeros((32, 3), device="cuda")[::1, :2]),
+        lambda: batch.distance_backward_device(x, i, f, grad_vertices=torch.zeros((nv - 1, 3), device="cuda")),
+        lambda: batch.distance_backward_device(x.cpu(), i, f),
+    ]
+    for k, call in enumerate(bad):
+        with pytest.raises((TypeError, ValueError)):
+            call()
+    batch.distance_backward_device(x, i, f, grad_points=torch.zeros((32, 3), device="cuda"),
+                                   grad_vertices=torch.zeros((nv, 3), device="cuda"))
+    torch.cuda.synchronize()
+
+
 def test_batch_errors(gpu):
     V, F = meshes.small_icosphere(1)
     _, g3 = transfer(V, F)

@@ -86,3 +86,84 @@ def test_scene_far_from_the_origin(gpu):
     hits = fcpw.GPUInteractionList()
     g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits)
     assert_same(records(hits), oracle.Bvh(V, F).closest_points_gpuorder(q))
+
+
+def test_device_entry_points_check_their_tensors_before_launch(gpu):
+    """shape, dtype, contiguity and device of every tensor argument are checked
+    on the host (the C ABI can only check alignment)"""
+    import torch
+    V, F = meshes.small_icosphere(1)
+    s = fcpw.Scene3D()
+    s.set_object_count(1)
+    s.set_object_vertices(V, 0)
+    s.set_object_triangles(F, 0)
+    s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    g = fcpw.GPUScene3D("")
+    g.transfer_to_gpu(s)
+    n = 64
+    sph = torch.zeros((n, 4), device="cuda")
+    out = torch.zeros((n, 10), device="cuda")
+    rays = torch.zeros((n, 10), device="cuda")
+    rnd = torch.zeros((n, 3), device="cuda")
+    bad = [
+        lambda: g.find_closest_points_device(sph, torch.zeros((n - 1, 10), device="cuda")),
+        lambda: g.find_closest_points_device(sph, torch.zeros((n, 9), device="cuda")),
+        lambda: g.find_closest_points_device(sph, torch.zeros((n, 10), dtype=torch.float64, device="cuda")),
+        lambda: g.find_closest_points_device(sph, torch.zeros((10, n), device="cuda").t()),
+        lambda: g.find_closest_points_device(sph.cpu(), out),
+        lambda: g.find_closest_points_device(sph[:, :3], out),
+        lambda: g.find_closest_points_device(sph, out, object_index=torch.zeros(n, dtype=torch.int64, device="cuda")),
+        lambda: g.find_closest_points_device(sph, out, object_index=torch.zeros(n - 1, dtype=torch.int32, device="cuda")),
+        lambda: g.intersect_device(rays[:, :9], out),
+        lambda: g.intersect_device(rays, out[: n // 2]),
+        lambda: g.intersect_spheres_device(sph, rnd[:, :2], out),
+        lambda: g.intersect_spheres_device(sph, rnd, out.cpu()),
+        lambda: g.find_closest_points_device(np.zeros((n, 4), np.float32), out),
+    ]
+    for call in bad:
+        with pytest.raises((TypeError, ValueError)):
+            call()
+    g.find_closest_points_device(sph, out)
+    g.intersect_spheres_device(sph, rnd, out)
+    torch.cuda.synchronize()
+
+
+def test_backward_sees_the_geometry_of_its_own_forward(gpu):
+    """a second forward with other V before the first backward: the first
+    backward restores its V (and refits), so both gradients equal those of a
+    forward/backward pair run alone; without V to restore it raises"""
+    import torch
+    from fcpw_amd.torch_ops import closest_point_distance
+    V, F = meshes.small_icosphere(2)
+    s = fcpw.Scene3D()
+    s.set_object_count(1)
+    s.set_object_vertices(V, 0)
+    s.set_object_triangles(F, 0)
+    s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+    g = fcpw.GPUScene3D("")
+    g.transfer_to_gpu(s)
+    x = torch.from_numpy(meshes.queries_around(V, 256, seed=3)).cuda()
+    V1 = torch.from_numpy(V).cuda()
+    V2 = (V1 * 1.01).contiguous()
+
+    def alone(Vv):
+        a = Vv.clone().requires_grad_(True)
+        d, _, _ = closest_point_distance(g, x, a)
+        d.sum().backward()
+        return a.grad.clone()
+    ref1, ref2 = alone(V1), alone(V2)
+    a1, a2 = V1.clone().requires_grad_(True), V2.clone().requires_grad_(True)
+    d1, _, _ = closest_point_distance(g, x, a1)
+    d2, _, _ = closest_point_distance(g, x, a2)
+    d1.sum().backward()
+    d2.sum().backward()
+    # FP32 atomics sum in no fixed order: equal within rounding
+    torch.testing.assert_close(a1.grad, ref1, rtol=1e-5, atol=1e-6)
+    torch.testing.assert_close(a2.grad, ref2, rtol=1e-5, atol=1e-6)
+    assert (ref1 - ref2).abs().max() > 1e-3
+    xg = x.clone().requires_grad_(True)
+    d3, _, _ = closest_point_distance(g, xg)  # no V: the scene's current geometry
+    g._update_positions_device(V1.data_ptr(), V1.shape[0], torch.cuda.current_stream().cuda_stream)
+    with pytest.raises(RuntimeError, match="no V to restore"):
+        d3.sum().backward()
+    torch.cuda.synchronize()
