@@ -67,7 +67,10 @@ def parse():
     p.add_argument("--grid", type=int, nargs=2, default=[1024, 512], help="torus U V (2UV triangles)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=1 << 19, help="queries timed on the CPU path")
-    p.add_argument("--no-extra", action="store_true", help="forward only (skip fwd+bwd and rays)")
+    p.add_argument("--no-extra", action="store_true", help="forward only (skip fwd+bwd, rays and the other configs)")
+    p.add_argument("--legs", default="all",
+                   help="comma-separated extra legs of the headline run: fwd_bwd, e2e, rays, spheres, silhouettes, "
+                        "build, instances, configs (star2d / diff / batch sub-objects); default all")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="after the timed work, copy a buffer of this size 3x (counter calibration)")
     for k in ("block", "stack", "sort", "coop", "hold_min", "xcds"):
@@ -75,6 +78,10 @@ def parse():
     p.add_argument("--summarize-kernels", metavar="DB", help="write a kernel-trace summary and exit")
     p.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                    help="write the HBM traffic summary from two counter runs and exit")
+    p.add_argument("--summarize-counters", nargs="+", metavar="DB",
+                   help="write a per-kernel summary of rocprofv3 --pmc runs (one counter group each) and exit")
+    p.add_argument("--kernel", action="append", default=[], help="kernel name pattern (regex) for --summarize-counters")
+    p.add_argument("--workload-note", default="", help="workload line of a summary")
     p.add_argument("--calib-bytes", type=int, default=0, help="bytes of the calibration copy (summaries)")
     p.add_argument("--command", action="append", default=[], help="command lines recorded in a summary")
     p.add_argument("--out", default=None, help="summary file")
@@ -101,24 +108,89 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(V, F, q, r2, sample):
-    """Reference CPU path on this host (oracle = restatement of bvh.inl)."""
+def host_record():
+    """The host the CPU baseline runs on: model, sockets, cores, SMT, usable threads."""
+    rec = {"model": None, "sockets": None, "physical_cores": None, "smt": None,
+           "online_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+           "simd": None}
+    try:
+        phys, cores, flags = set(), set(), ""
+        with open("/proc/cpuinfo") as fh:
+            cur = {}
+            for line in fh:
+                if ":" not in line:
+                    if cur:
+                        phys.add(cur.get("physical id"))
+                        cores.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                cur[k] = v
+                if k == "model name" and rec["model"] is None:
+                    rec["model"] = v
+                if k == "flags" and not flags:
+                    flags = v
+        rec["sockets"] = len(phys)
+        rec["physical_cores"] = len(cores)
+        rec["smt"] = (rec["online_cpus"] or 0) > len(cores) > 0
+        f = set(flags.split())
+        rec["simd"] = "avx512" if {"avx512f", "avx512vl", "avx512dq"} <= f else ("avx2" if "avx2" in f else "none")
+    except OSError:
+        pass
+    return rec
+
+
+def _median_rate(fn, n, runs):
+    """n / median time of `runs` calls of fn, after one warm-up call"""
+    fn()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return n / float(np.median(times)), [round(n / t, 1) for t in times]
+
+
+def cpu_baseline(V, F, q, r2, sample, runs=5):
+    """The reference CPU path on this host's cores, BVH build excluded: (1) the
+    restated non-vectorized Scene<3>::findClosestPoints (binary SAH BVH, static
+    thread partition as fcpw.inl:1547-1559) and (2) the restated vectorized
+    aggregate (4-wide MBVH with SIMD leaf groups, mbvh.inl / wide_query_operations.h),
+    each on all usable threads and on one thread.  The sample is every k-th query
+    of the whole batch, so it covers the uniform and the near-surface half in
+    their proportions; the one-thread sample is 1/16 of it."""
     import oracle
+    threads = min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30)
+    n = min(sample, len(q))
+    stride = max(1, len(q) // n)
+    qs, rs = np.ascontiguousarray(q[::stride][:n]), np.ascontiguousarray(r2[::stride][:n])
+    n1 = max(1, n // 16)
+    q1, r1 = np.ascontiguousarray(qs[::16][:n1]), np.ascontiguousarray(rs[::16][:n1])
     t0 = time.time()
     ob = oracle.Bvh(V, F)
-    log(f"[cpu] oracle BVH built in {time.time() - t0:.1f}s")
-    cores = os.cpu_count() or 1
-    n = min(sample, len(q))
-    qs, rs = q[:n], r2[:n]
-    ob.closest_points_cpu(qs[: min(n, 4096)], rs[: min(n, 4096)], nthreads=cores)  # warm up
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        ob.closest_points_cpu(qs, rs, nthreads=cores)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": n / t, "unit": "queries/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} of the same queries, median of 3 runs, {cores} threads, BVH build excluded"}
+    log(f"[cpu] scalar BVH built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    vb = oracle.VectorizedBvh(V, F)
+    log(f"[cpu] vectorized BVH ({vb.width} lanes, {vb.info()}) built in {time.time() - t0:.1f}s")
+    res = {}
+    for kind, fn in (("scalar", lambda a, b, t: ob.closest_points_cpu(a, b, nthreads=t)),
+                     ("vectorized", lambda a, b, t: vb.closest_points(a, b, nthreads=t))):
+        v_all, runs_all = _median_rate(lambda: fn(qs, rs, threads), n, runs)
+        v_one, runs_one = _median_rate(lambda: fn(q1, r1, 1), n1, runs)
+        res[kind] = {"value": v_all, "threads": threads, "value_1_thread": v_one, "runs": runs_all,
+                     "runs_1_thread": runs_one}
+        log(f"[cpu] {kind}: {v_all:.3e} queries/s on {threads} threads, {v_one:.3e} on 1 thread")
+    # the two paths answer the same (the reference's own vectorized check, |dd| <= 1e-6)
+    a, b = ob.closest_points_cpu(q1, r1, nthreads=threads), vb.closest_points(q1, r1, nthreads=threads)
+    dd = float(np.abs(a["d"].astype(np.float64) - b["d"]).max())
+    best = max(("scalar", "vectorized"), key=lambda k: res[k]["value"])
+    return {"value": res[best]["value"], "unit": "queries/s", "cores": threads, "kind": "port",
+            "path": best, "scalar": res["scalar"],
+            "vectorized": {**res["vectorized"], "simd_width": vb.width, "mbvh": vb.info()},
+            "max_abs_d_scalar_vs_vectorized": dd, "host": host_record(),
+            "sample": f"every {stride}th of the same queries ({n}; uniform and near-surface halves alike), "
+                      f"{n1} of those for the 1-thread figure, median of {runs} runs after a warm-up run, "
+                      f"BVH build excluded; value = the faster path ({best}) on {threads} threads"}
 
 
 # ---------------------------------------------------------------------------
@@ -192,6 +264,36 @@ def summarize_traffic(fetch_db, write_db, calib_bytes, commands, out):
     _emit(lines, out)
 
 
+def summarize_counters(dbs, kernels, commands, workload, out):
+    """Per-dispatch counter values (mean over the dispatches of each kernel with
+    the largest grid) from several rocprofv3 --pmc runs, one counter group per
+    run, for each kernel pattern; plus the L2 hit rate from TCC_HIT / TCC_MISS."""
+    lines = ["# rocprofv3 counters per dispatch", f"# workload: {workload}"]
+    lines += [f"# command: {c}" for c in commands]
+    for pat in kernels:
+        vals, name = {}, None
+        for db in dbs:
+            con, _ = _db(db)
+            names = [r[0] for r in con.execute("select distinct counter_name from counters_collection")]
+            for cn in names:
+                try:
+                    v, n, kname = _counter(con, cn, pat, pick_largest_grid=True)
+                except SystemExit:
+                    continue
+                vals[cn] = (v, n)
+                name = kname
+        if not vals:
+            continue
+        lines += ["", f"kernel: {name}"]
+        for cn in sorted(vals):
+            v, n = vals[cn]
+            lines.append(f"  {cn:28s} {v:16.6g}   ({n} dispatches)")
+        if "TCC_HIT_sum" in vals and "TCC_MISS_sum" in vals:
+            h, m = vals["TCC_HIT_sum"][0], vals["TCC_MISS_sum"][0]
+            lines.append(f"  {'L2_hit_rate':28s} {h / max(h + m, 1):16.4f}   (TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum))")
+    _emit(lines, out)
+
+
 def _emit(lines, out):
     text = "\n".join(lines) + "\n"
     if out:
@@ -215,6 +317,9 @@ def main():
     args = parse()
     if args.summarize_kernels:
         summarize_kernels(args.summarize_kernels, " && ".join(args.command), args.out)
+        return
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters, args.kernel, args.command, args.workload_note, args.out)
         return
     if args.summarize_traffic:
         summarize_traffic(*args.summarize_traffic, args.calib_bytes, args.command, args.out)
@@ -314,146 +419,192 @@ def run_torus(args, torch):
                                   f"half within 0.05 of the surface), r2=inf, forward",
                       "triangles": len(F), "queries": nq, **opts}}
 
+    legs = set(args.legs.split(",")) if args.legs != "all" else None
+
+    def want(leg):
+        return not args.no_extra and (legs is None or leg in legs)
+
     if not args.no_extra:
-        # ---- forward + backward (compact forward, adjoint kernel for dL/dx and dL/dV)
-        x = sph[:, :3].contiguous()
-        d = torch.empty(nq, dtype=torch.float32, device=dev)
-        p = torch.empty((nq, 3), dtype=torch.float32, device=dev)
-        prim = torch.empty(nq, dtype=torch.int32, device=dev)
-        slot = torch.empty(nq, dtype=torch.int32, device=dev)
-        gd = torch.ones(nq, dtype=torch.float32, device=dev)
-        gx = torch.empty_like(x)
-        gV = torch.zeros((len(V), 3), dtype=torch.float32, device=dev)
-        h = stream.cuda_stream
+        if want("fwd_bwd"):
+            # ---- forward + backward (compact forward, adjoint kernel for dL/dx and dL/dV)
+            x = sph[:, :3].contiguous()
+            d = torch.empty(nq, dtype=torch.float32, device=dev)
+            p = torch.empty((nq, 3), dtype=torch.float32, device=dev)
+            prim = torch.empty(nq, dtype=torch.int32, device=dev)
+            slot = torch.empty(nq, dtype=torch.int32, device=dev)
+            gd = torch.ones(nq, dtype=torch.float32, device=dev)
+            gx = torch.empty_like(x)
+            gV = torch.zeros((len(V), 3), dtype=torch.float32, device=dev)
+            h = stream.cuda_stream
 
-        def fwd_bwd():
-            gpu._closest_points_compact_device(x.data_ptr(), 0, nq, d.data_ptr(), p.data_ptr(), prim.data_ptr(),
-                                               slot.data_ptr(), h)
-            gV.zero_()
-            gpu._distance_backward_device(x.data_ptr(), slot.data_ptr(), gd.data_ptr(), nq, gx.data_ptr(),
-                                          gV.data_ptr(), h)
-        t_fb = timed(torch, stream, fwd_bwd, args.steps, args.warmup)
-        res["fwd_bwd"] = {"value": nq / t_fb, "unit": "queries/s", "ms_per_step": t_fb * 1e3,
-                          "ratio_to_fwd": t_fb * 1e3 / ms_step}
-        log(f"[fwd+bwd] {t_fb * 1e3:.3f} ms/step, {nq / t_fb:.4e} queries/s ({t_fb * 1e3 / ms_step:.2f}x fwd)")
-        # same closest points as the record-layout forward
-        rec = d_out.view(-1)[: 4096 * 10].view(4096, 10)
-        assert torch.equal(rec[:, 8], d[:4096]), "compact and record forward passes differ"
-        del x, d, p, prim, slot, gd, gx, gV
+            def fwd_bwd():
+                gpu._closest_points_compact_device(x.data_ptr(), 0, nq, d.data_ptr(), p.data_ptr(), prim.data_ptr(),
+                                                   slot.data_ptr(), h)
+                gV.zero_()
+                gpu._distance_backward_device(x.data_ptr(), slot.data_ptr(), gd.data_ptr(), nq, gx.data_ptr(),
+                                              gV.data_ptr(), h)
+            t_fb = timed(torch, stream, fwd_bwd, args.steps, args.warmup)
+            res["fwd_bwd"] = {"value": nq / t_fb, "unit": "queries/s", "ms_per_step": t_fb * 1e3,
+                              "ratio_to_fwd": t_fb * 1e3 / ms_step}
+            log(f"[fwd+bwd] {t_fb * 1e3:.3f} ms/step, {nq / t_fb:.4e} queries/s ({t_fb * 1e3 / ms_step:.2f}x fwd)")
+            # same closest points as the record-layout forward
+            rec = d_out.view(-1)[: 4096 * 10].view(4096, 10)
+            assert torch.equal(rec[:, 8], d[:4096]), "compact and record forward passes differ"
+            del x, d, p, prim, slot, gd, gx, gV
 
-        # ---- rays
-        rr = synth.Rng(synth.seed(2, 3))
-        o = synth.f32(synth.uniform_in_box(nq, lo, hi, 1.0, rr))
-        dr = synth.f32(rr.sphere(nq, 3))
-        rays = torch.empty((nq, 10), dtype=torch.float32, device=dev)
-        rays[:, 0:3] = torch.from_numpy(o).to(dev)
-        rays[:, 3:6] = torch.from_numpy(dr).to(dev)
-        rays[:, 6:9] = 1.0 / rays[:, 3:6]
-        rays[:, 9] = float("inf")
-        ray_out = torch.empty((nq, 10), dtype=torch.float32, device=dev)
-        del sph
+        if want("e2e"):
+            # ---- end to end through the drop-in API: host query arrays in, host records
+            # out (GPUScene::findClosestPoints, fcpw_gpu.inl:219-274: pack, upload,
+            # kernel, readback), against the device-resident forward above
+            hits = fcpw.GPUInteractionList()
+            gpu.find_closest_points(q[:4096], r2[:4096], hits)  # warm-up
+            e2e = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                gpu.find_closest_points(q, r2, hits)
+                e2e.append(time.perf_counter() - t0)
+            t_e2e = float(np.median(e2e))
+            assert np.array_equal(hits.get_distances()[:4096], d_out[:4096, 8].cpu().numpy()), "API and device differ"
+            res["end_to_end"] = {"value": nq / t_e2e, "unit": "queries/s", "ms_per_call": t_e2e * 1e3,
+                                 "ratio_to_device_resident": t_e2e * 1e3 / ms_step,
+                                 "what": "GPUScene3D.find_closest_points(points, r2, hits) on the same 16M host "
+                                         "queries: GPUBoundingSphere packing, host-to-device copy, ordering and "
+                                         "kernel, copy of the GPUInteraction records back into the list; median of 3"}
+            log(f"[e2e] API call {t_e2e * 1e3:.1f} ms ({nq / t_e2e:.4e} queries/s, {t_e2e * 1e3 / ms_step:.2f}x the "
+                f"device-resident step)")
+            del hits
 
-        def ray_step():
-            gpu.intersect_device(rays, ray_out, stream=stream)
-        t_r = timed(torch, stream, ray_step, args.steps, args.warmup)
-        res["rays"] = {"value": nq / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
-                       "workload": f"{nq} rays, origins uniform in the AABB, directions uniform on S^2, tMax=inf",
-                       "ray_coop": gpu.get_option("ray_coop")}
-        log(f"[rays] {t_r * 1e3:.3f} ms/step, {nq / t_r:.4e} rays/s (ray_coop={gpu.get_option('ray_coop')})")
-        # the other ray kernel on the same rays: same bits, its own time
-        first = ray_out.clone()
-        other = 1 - gpu.get_option("ray_coop")
-        gpu.set_option("ray_coop", other)
-        t_o = timed(torch, stream, ray_step, args.steps, args.warmup)
-        assert torch.equal(first.view(torch.int32), ray_out.view(torch.int32)), "ray kernels differ"
-        gpu.set_option("ray_coop", 1 - other)
-        res["rays_other_kernel"] = {"value": nq / t_o, "unit": "rays/s", "ms_per_step": t_o * 1e3,
-                                    "ray_coop": other}
-        log(f"[rays] ray_coop={other}: {t_o * 1e3:.3f} ms/step, {nq / t_o:.4e} rays/s")
-        del rays, first
+        if want("rays"):
+            # ---- rays
+            rr = synth.Rng(synth.seed(2, 3))
+            o = synth.f32(synth.uniform_in_box(nq, lo, hi, 1.0, rr))
+            dr = synth.f32(rr.sphere(nq, 3))
+            rays = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+            rays[:, 0:3] = torch.from_numpy(o).to(dev)
+            rays[:, 3:6] = torch.from_numpy(dr).to(dev)
+            rays[:, 6:9] = 1.0 / rays[:, 3:6]
+            rays[:, 9] = float("inf")
+            ray_out = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+            del sph
 
-        # ---- stochastic sphere intersection at the query points, radius 0.1
-        balls = _spheres(torch, q, dev)
-        balls[:, 3] = 0.01
-        rnd = torch.rand((nq, 3), dtype=torch.float32, device=dev)
+            def ray_step():
+                gpu.intersect_device(rays, ray_out, stream=stream)
+            t_r = timed(torch, stream, ray_step, args.steps, args.warmup)
+            res["rays"] = {"value": nq / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
+                           "workload": f"{nq} rays, origins uniform in the AABB, directions uniform on S^2, tMax=inf",
+                           "ray_coop": gpu.get_option("ray_coop")}
+            log(f"[rays] {t_r * 1e3:.3f} ms/step, {nq / t_r:.4e} rays/s (ray_coop={gpu.get_option('ray_coop')})")
+            # the other ray kernel on the same rays: same bits, its own time
+            first = ray_out.clone()
+            other = 1 - gpu.get_option("ray_coop")
+            gpu.set_option("ray_coop", other)
+            t_o = timed(torch, stream, ray_step, args.steps, args.warmup)
+            assert torch.equal(first.view(torch.int32), ray_out.view(torch.int32)), "ray kernels differ"
+            gpu.set_option("ray_coop", 1 - other)
+            res["rays_other_kernel"] = {"value": nq / t_o, "unit": "rays/s", "ms_per_step": t_o * 1e3,
+                                        "ray_coop": other}
+            log(f"[rays] ray_coop={other}: {t_o * 1e3:.3f} ms/step, {nq / t_o:.4e} rays/s")
+            del rays, first, ray_out
 
-        def sphere_step():
-            gpu.intersect_spheres_device(balls, rnd, ray_out, stream=stream)
-        t_s = timed(torch, stream, sphere_step, args.steps, args.warmup)
-        found = float((ray_out[:, 9].view(torch.int32) != -1).float().mean())
-        res["sphere_intersection"] = {"value": nq / t_s, "unit": "queries/s", "ms_per_step": t_s * 1e3,
-                                      "workload": f"{nq} spheres of radius 0.1 at the closest-point queries, "
-                                                  f"uniform random numbers; {found:.3f} find a primitive"}
-        log(f"[spheres] {t_s * 1e3:.3f} ms/step, {nq / t_s:.4e} queries/s, found {found:.3f}")
-        del balls, rnd, ray_out
+        if want("spheres"):
+            # ---- stochastic sphere intersection at the query points, radius 0.1
+            balls = _spheres(torch, q, dev)
+            balls[:, 3] = 0.01
+            rnd = torch.rand((nq, 3), dtype=torch.float32, device=dev)
+            ray_out = torch.empty((nq, 10), dtype=torch.float32, device=dev)
 
-        # ---- closest silhouette points on the same torus built with silhouettes,
-        # every fourth closest-point query (uniform and near-surface alike)
-        t0 = time.time()
-        ss = fcpw.Scene3D()
-        ss.set_object_count(1)
-        ss.set_object_vertices(V, 0)
-        ss.set_object_triangles(F, 0)
-        ss.compute_silhouettes()
-        ss.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
-        gs = fcpw.GPUScene3D(fcpw.slang_directory_path)
-        gs.transfer_to_gpu(ss)
-        log(f"[silhouettes] silhouettes + cones built and transferred in {time.time() - t0:.1f}s")
-        qs = q[::4]
-        ns = len(qs)
-        balls = _spheres(torch, qs, dev)
-        flip = torch.zeros(ns, dtype=torch.int32, device=dev)
-        sil_out = torch.empty((ns, 10), dtype=torch.float32, device=dev)
+            def sphere_step():
+                gpu.intersect_spheres_device(balls, rnd, ray_out, stream=stream)
+            t_s = timed(torch, stream, sphere_step, args.steps, args.warmup)
+            found = float((ray_out[:, 9].view(torch.int32) != -1).float().mean())
+            res["sphere_intersection"] = {"value": nq / t_s, "unit": "queries/s", "ms_per_step": t_s * 1e3,
+                                          "workload": f"{nq} spheres of radius 0.1 at the closest-point queries, "
+                                                      f"uniform random numbers; {found:.3f} find a primitive"}
+            log(f"[spheres] {t_s * 1e3:.3f} ms/step, {nq / t_s:.4e} queries/s, found {found:.3f}")
+            del balls, rnd, ray_out
 
-        def sil_step():
-            gs.find_closest_silhouette_points_device(balls, flip, sil_out, stream=stream)
-        t_sil = timed(torch, stream, sil_step, args.steps, args.warmup)
-        found = float((sil_out[:, 9].view(torch.int32) != -1).float().mean())
-        res["closest_silhouette"] = {"value": ns / t_sil, "unit": "queries/s", "ms_per_step": t_sil * 1e3,
-                                     "workload": f"{ns} queries (every 4th closest-point query), r2=inf, "
-                                                 f"flip=0; {found:.3f} find a silhouette"}
-        log(f"[silhouettes] {t_sil * 1e3:.3f} ms/step, {ns / t_sil:.4e} queries/s, found {found:.3f}")
-        # the one-query-per-lane kernel on the same queries: same records, its own time
-        first = sil_out.clone()
-        gs.set_option("coop", 0)
-        t_sil1 = timed(torch, stream, sil_step, args.steps, args.warmup)
-        assert torch.equal(first.view(torch.int32), sil_out.view(torch.int32)), "silhouette kernels differ"
-        gs.set_option("coop", 1)
-        res["closest_silhouette"]["per_lane_ms_per_step"] = t_sil1 * 1e3
-        log(f"[silhouettes] one query per lane: {t_sil1 * 1e3:.3f} ms/step")
-        del balls, flip, sil_out, gs, ss, first
+        if want("silhouettes"):
+            # ---- closest silhouette points on the same torus built with silhouettes,
+            # every fourth closest-point query (uniform and near-surface alike)
+            t0 = time.time()
+            ss = fcpw.Scene3D()
+            ss.set_object_count(1)
+            ss.set_object_vertices(V, 0)
+            ss.set_object_triangles(F, 0)
+            ss.compute_silhouettes()
+            ss.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+            gs = fcpw.GPUScene3D(fcpw.slang_directory_path)
+            gs.transfer_to_gpu(ss)
+            log(f"[silhouettes] silhouettes + cones built and transferred in {time.time() - t0:.1f}s")
+            qs = q[::4]
+            ns = len(qs)
+            balls = _spheres(torch, qs, dev)
+            flip = torch.zeros(ns, dtype=torch.int32, device=dev)
+            sil_out = torch.empty((ns, 10), dtype=torch.float32, device=dev)
 
-        # ---- BVH construction: the reference's SAH tree built on the device
-        # (geometry upload included) against the host build + transfer
-        sb = fcpw.Scene3D()
-        sb.set_object_count(1)
-        sb.set_object_vertices(V, 0)
-        sb.set_object_triangles(F, 0)
-        gb = fcpw.GPUScene3D(fcpw.slang_directory_path)
-        gb.build_on_gpu(sb)
-        torch.cuda.synchronize()
-        t_dev = []
-        for _ in range(5):
-            t0 = time.perf_counter()
+            def sil_step():
+                gs.find_closest_silhouette_points_device(balls, flip, sil_out, stream=stream)
+            t_sil = timed(torch, stream, sil_step, args.steps, args.warmup)
+            found = float((sil_out[:, 9].view(torch.int32) != -1).float().mean())
+            res["closest_silhouette"] = {"value": ns / t_sil, "unit": "queries/s", "ms_per_step": t_sil * 1e3,
+                                         "workload": f"{ns} queries (every 4th closest-point query), r2=inf, "
+                                                     f"flip=0; {found:.3f} find a silhouette"}
+            log(f"[silhouettes] {t_sil * 1e3:.3f} ms/step, {ns / t_sil:.4e} queries/s, found {found:.3f}")
+            # the one-query-per-lane kernel on the same queries: same records, its own time
+            first = sil_out.clone()
+            gs.set_option("coop", 0)
+            t_sil1 = timed(torch, stream, sil_step, args.steps, args.warmup)
+            assert torch.equal(first.view(torch.int32), sil_out.view(torch.int32)), "silhouette kernels differ"
+            gs.set_option("coop", 1)
+            res["closest_silhouette"]["per_lane_ms_per_step"] = t_sil1 * 1e3
+            log(f"[silhouettes] one query per lane: {t_sil1 * 1e3:.3f} ms/step")
+            del balls, flip, sil_out, gs, ss, first
+
+        if want("build"):
+            # ---- BVH construction: the reference's SAH tree built on the device
+            # (geometry upload included) against the host build + transfer
+            sb = fcpw.Scene3D()
+            sb.set_object_count(1)
+            sb.set_object_vertices(V, 0)
+            sb.set_object_triangles(F, 0)
+            gb = fcpw.GPUScene3D(fcpw.slang_directory_path)
             gb.build_on_gpu(sb)
             torch.cuda.synchronize()
-            t_dev.append(time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        sb.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
-        t_hb = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        gb.transfer_to_gpu(sb)
-        torch.cuda.synchronize()
-        t_tr = time.perf_counter() - t0
-        t_db = float(np.median(t_dev))
-        res["bvh_build"] = {"device_ms": t_db * 1e3, "host_build_ms": t_hb * 1e3, "host_transfer_ms": t_tr * 1e3,
-                            "speedup": (t_hb + t_tr) / t_db,
-                            "workload": f"{len(F)} triangles, SAH, the same tree either way; device time includes "
-                                        f"the geometry upload (median of 5)"}
-        log(f"[build] device {t_db * 1e3:.2f} ms vs host build {t_hb * 1e3:.1f} ms + transfer {t_tr * 1e3:.1f} ms")
-        del gb, sb
+            t_dev = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                gb.build_on_gpu(sb)
+                torch.cuda.synchronize()
+                t_dev.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            sb.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
+            t_hb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            gb.transfer_to_gpu(sb)
+            torch.cuda.synchronize()
+            t_tr = time.perf_counter() - t0
+            t_db = float(np.median(t_dev))
+            res["bvh_build"] = {"device_ms": t_db * 1e3, "host_build_ms": t_hb * 1e3, "host_transfer_ms": t_tr * 1e3,
+                                "speedup": (t_hb + t_tr) / t_db,
+                                "workload": f"{len(F)} triangles, SAH, the same tree either way; device time includes "
+                                            f"the geometry upload (median of 5)"}
+            log(f"[build] device {t_db * 1e3:.2f} ms vs host build {t_hb * 1e3:.1f} ms + transfer {t_tr * 1e3:.1f} ms")
+            del gb, sb
 
-        res["instances"] = run_instances(args, torch, fcpw, synth, dev, stream)
+        if want("instances"):
+            res["instances"] = run_instances(args, torch, fcpw, synth, dev, stream)
+
+    if want("configs"):
+        # the other configurations, each as a sub-object of the headline line
+        keep = ("value", "unit", "ms_per_step", "config", "rays", "refit", "forward_only", "ratio_step_to_forward",
+                "loss_every_10_steps", "eager", "separate_launches", "speedup_vs_separate", "fwd_bwd",
+                "closest_after_refit_ms", "metric")
+        for name, fn in (("star2d", run_star2d), ("diff", run_diff), ("batch", run_batch)):
+            t0 = time.time()
+            sub = fn(args, torch)
+            res[name] = {k: v for k, v in sub.items() if k in keep}
+            log(f"[{name}] done in {time.time() - t0:.1f}s")
+            torch.cuda.empty_cache()
 
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20

@@ -351,8 +351,9 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
                  along the Morton curve (default); 0 = by origin only
      "hold_min"  coop: lanes that must hold a leaf before the wave processes
                  leaves together (1..64, default 1)
-     "stack"     traversal-stack entries per lane kept in LDS (default 12), the rest in
-                 cached scratch: 0, 4, 8, 12, 16 or 24 (the 2D coop kernel uses 16 for 24,
+     "stack"     traversal-stack entries per lane kept in LDS (default 8): the newest
+                 entries, as a sliding window; older ones are spilled once to cached
+                 scratch: 0, 4, 8, 12, 16 or 24 (the 2D coop kernel uses 16 for 24,
                  the one-query-per-lane kernel 8 for 0, 4 and 12)
      "block"     threads per block when coop = 0 (64, 128, 256)
      "sort"      1 = order queries along a Morton curve before traversal (default)

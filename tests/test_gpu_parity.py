@@ -388,7 +388,7 @@ def test_errors_are_reported(gpu):
                      ("xcd_chunk", 17), ("sil_xcds", 0)):
         with pytest.raises(ValueError):
             g.set_option(key, bad)
-    defaults = dict(stack=12, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
+    defaults = dict(stack=8, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
                     sil_xcds=8, ray_sort_dir=1)
     assert {k: g.get_option(k) for k in defaults} == defaults
     with pytest.raises(RuntimeError, match="without silhouettes"):  # compute_silhouettes was not called
