@@ -4,6 +4,15 @@
   _fcpw_amd<ext-suffix>.so    pybind11 host module (Scene, GPUScene), linked to it
 
 Everything is compiled in-tree so the built objects travel with the source.
+
+``python fcpw_amd/_build.py --sanitize`` builds the host code with
+AddressSanitizer and UndefinedBehaviorSanitizer instead, for the CPU test
+suite: the pybind11 module (the host BVH, silhouette and instance builders and
+the GPUScene front end) and the oracle, staged with copies of the Python
+sources under build/sanitize/ so the in-tree build is left alone.  The HIP
+library is linked unsanitized (its device code cannot be, and the CPU tests
+only reach its argument checks).  It prints the command that runs the CPU
+tests against the staged copy.
 """
 import os
 import subprocess
@@ -55,6 +64,47 @@ def build_all():
     build_ext()
 
 
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized():
+    """ASan + UBSan builds of the host module and the oracle, staged under build/sanitize/."""
+    import pybind11
+    import shutil
+    stage = os.path.join(ROOT, "build", "sanitize")
+    if os.path.isdir(stage):
+        shutil.rmtree(stage)
+    ignore = shutil.ignore_patterns("__pycache__", "*.so", "*.o", "_san", "obj")
+    shutil.copytree(HERE, os.path.join(stage, "fcpw_amd"), ignore=ignore)
+    shutil.copytree(os.path.join(ROOT, "oracle"), os.path.join(stage, "oracle"), ignore=ignore)
+    shutil.copytree(os.path.join(ROOT, "tests"), os.path.join(stage, "tests"), ignore=ignore)
+    shutil.copytree(os.path.join(ROOT, "include"), os.path.join(stage, "include"))
+    for f in ("pytest.ini", "__graft_entry__.py"):
+        shutil.copy(os.path.join(ROOT, f), stage)
+    if not os.path.exists(HIP_LIB):
+        build_hip()
+    os.makedirs(os.path.join(stage, "fcpw_amd", "lib"), exist_ok=True)
+    shutil.copy(HIP_LIB, os.path.join(stage, "fcpw_amd", "lib"))
+    py_inc = sysconfig.get_paths()["include"]
+    _run(["g++", *SAN_FLAGS, "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden", "-ffp-contract=off",
+          f"-I{py_inc}", f"-I{pybind11.get_include()}", os.path.join(HERE, "csrc", "py", "bindings.cpp"),
+          f"-L{os.path.dirname(HIP_LIB)}", "-lfcpw_hip", "-Wl,-rpath,$ORIGIN/lib",
+          "-o", os.path.join(stage, "fcpw_amd", os.path.basename(EXT))])
+    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "SANITIZE=1"])
+    shutil.copy(os.path.join(ROOT, "oracle", "_san", "liboracle.so"), os.path.join(stage, "oracle", "liboracle.so"))
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    ubsan = subprocess.run(["gcc", "-print-file-name=libubsan.so"], capture_output=True, text=True).stdout.strip()
+    cmd = (f"cd {stage} && LD_PRELOAD={asan}:{ubsan} ASAN_OPTIONS=detect_leaks=0:detect_odr_violation=0 "
+           f"UBSAN_OPTIONS=print_stacktrace=1:halt_on_error=1 python -m pytest tests -m 'not gpu' -x -q "
+           f"-p no:cacheprovider")
+    print(cmd)
+    return cmd
+
+
 if __name__ == "__main__":
-    build_all()
+    if "--sanitize" in sys.argv:
+        build_sanitized()
+    else:
+        build_all()
     sys.exit(0)
