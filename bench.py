@@ -445,8 +445,20 @@ def run_torus(args, torch):
                                               gV.data_ptr(), h)
             t_fb = timed(torch, stream, fwd_bwd, args.steps, args.warmup)
             res["fwd_bwd"] = {"value": nq / t_fb, "unit": "queries/s", "ms_per_step": t_fb * 1e3,
-                              "ratio_to_fwd": t_fb * 1e3 / ms_step}
+                              "ratio_to_fwd": t_fb * 1e3 / ms_step,
+                              "bwd_order": gpu.get_option("bwd_order"), "bwd_reduce": gpu.get_option("bwd_reduce")}
             log(f"[fwd+bwd] {t_fb * 1e3:.3f} ms/step, {nq / t_fb:.4e} queries/s ({t_fb * 1e3 / ms_step:.2f}x fwd)")
+            # the adjoint's other query orders on the same step (same dL/dx, dL/dV up to summation order)
+            alt = {}
+            for order, reduce in ((0, 0), (1, 0)):
+                gpu.set_option("bwd_order", order)
+                gpu.set_option("bwd_reduce", reduce)
+                alt[f"order{order}_reduce{reduce}_ms_per_step"] = timed(torch, stream, fwd_bwd, args.steps,
+                                                                         args.warmup) * 1e3
+            gpu.set_option("bwd_order", 1)
+            gpu.set_option("bwd_reduce", 1)
+            res["fwd_bwd"]["other_adjoint_orders"] = alt
+            log(f"[fwd+bwd] other adjoint orders: {alt}")
             # same closest points as the record-layout forward
             rec = d_out.view(-1)[: 4096 * 10].view(4096, 10)
             assert torch.equal(rec[:, 8], d[:4096]), "compact and record forward passes differ"

@@ -51,6 +51,10 @@ struct fcpw_hip_ctx {
     // query ordering workspace (grown on demand, kept for the context's lifetime)
     void *ws_codes = nullptr, *ws_perm = nullptr, *ws_hist = nullptr, *ws_sums = nullptr;
     size_t ws_codes_bytes = 0, ws_perm_bytes = 0, ws_hist_bytes = 0, ws_sums_bytes = 0;
+    // number of queries of the batch whose Morton permutation ws_perm holds (-1: none);
+    // the adjoint walks the queries in that order when its batch has the same size
+    int64_t ws_perm_n = -1;
+    int bwd_order = 1, bwd_reduce = 1; // adjoint: the forward's query order, wave pre-reduction
     float qlo[3] = {0, 0, 0}, qinv[3] = {1, 1, 1}; // Morton grid: 1.5x the scene box
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;

@@ -1499,6 +1499,66 @@ __global__ void k_refit(float4 *records, float4 *root, const float4 *prims, cons
 
 // Adjoint of d(x, mesh): recompute p and its barycentric weights from the saved
 // leaf slot, then d(d)/dx = (x - p)/d and d(d)/dV_k = -b_k (x - p)/d.
+// The terms of one query: the slot (-1 when it contributes nothing), dL/dx and,
+// per vertex k of the primitive, its id and -b_k dL/dx.
+template <int DIM>
+struct AdjointTerms {
+    int32_t slot;
+    float g[DIM];
+    float c[DIM][DIM];
+    int32_t vid[DIM];
+};
+
+template <int DIM>
+__device__ __forceinline__ void adjoint_terms(const float4 *prims, const int32_t *prim_vids, int n_prims,
+                                              const float *points, const int32_t *slots, const float *grad_d,
+                                              int64_t i, AdjointTerms<DIM> &T)
+{
+    int32_t s = slots[i];
+    if (s >= n_prims) s = -1; // not a slot of this scene
+    for (int k = 0; k < DIM; k++) T.g[k] = 0.0f;
+    T.slot = -1;
+    float gd = grad_d[i];
+    if (s < 0) return;
+    float dist, w[3];
+    float diff[3] = {0.0f, 0.0f, 0.0f};
+    if (DIM == 3) {
+        const float4 *t = prims + (size_t)s * 3;
+        float4 a = t[0], b = t[1], c = t[2];
+        v3 x = {points[i * 3], points[i * 3 + 1], points[i * 3 + 2]}, p;
+        float t0, t1;
+        dist = closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, t0, t1);
+        w[0] = t0; w[1] = t1; w[2] = 1.0f - t0 - t1;
+        diff[0] = x.x - p.x; diff[1] = x.y - p.y; diff[2] = x.z - p.z;
+    } else {
+        float4 q = prims[s];
+        v2 x = {points[i * 2], points[i * 2 + 1]}, p;
+        float t;
+        dist = closest_point_segment({q.x, q.y}, {q.z, q.w}, x, p, t);
+        w[0] = 1.0f - t; w[1] = t;
+        diff[0] = x.x - p.x; diff[1] = x.y - p.y;
+    }
+    if (!(dist > 0.0f)) return;
+    float sc = gd / dist;
+    for (int k = 0; k < DIM; k++) T.g[k] = sc * diff[k];
+    for (int v = 0; v < DIM; v++) {
+        T.vid[v] = prim_vids[(size_t)s * DIM + v];
+        for (int k = 0; k < DIM; k++) T.c[v][k] = -w[v] * T.g[k];
+    }
+    T.slot = s;
+}
+
+// hardware FP32 atomic add (global_atomic_add_f32, no return value) rather
+// than the compare-and-swap loop atomicAdd compiles to without
+// -munsafe-fp-atomics; grad_vertices is device (coarse-grained) memory
+template <int DIM>
+__device__ __forceinline__ void adjoint_scatter(const AdjointTerms<DIM> &T, float *grad_vertices)
+{
+    for (int v = 0; v < DIM; v++)
+        for (int k = 0; k < DIM; k++)
+            (void)unsafeAtomicAdd(&grad_vertices[(size_t)T.vid[v] * DIM + k], T.c[v][k]);
+}
+
 // one query of the adjoint (shared by the single-scene and batched kernels)
 template <int DIM>
 __device__ __forceinline__ void distance_backward_one(const float4 *prims, const int32_t *prim_vids, int n_prims,
@@ -1506,47 +1566,61 @@ __device__ __forceinline__ void distance_backward_one(const float4 *prims, const
                                                       const float *grad_d, int64_t i, float *grad_points,
                                                       float *grad_vertices)
 {
-    int32_t s = slots[i];
-    if (s >= n_prims) s = -1; // not a slot of this scene
-    float g[DIM];
-    for (int k = 0; k < DIM; k++) g[k] = 0.0f;
-    float gd = grad_d[i];
-    if (s >= 0) {
-        float dist, w[3];
-        float diff[3] = {0.0f, 0.0f, 0.0f};
-        if (DIM == 3) {
-            const float4 *t = prims + (size_t)s * 3;
-            float4 a = t[0], b = t[1], c = t[2];
-            v3 x = {points[i * 3], points[i * 3 + 1], points[i * 3 + 2]}, p;
-            float t0, t1;
-            dist = closest_point_triangle({a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}, x, p, t0, t1);
-            w[0] = t0; w[1] = t1; w[2] = 1.0f - t0 - t1;
-            diff[0] = x.x - p.x; diff[1] = x.y - p.y; diff[2] = x.z - p.z;
-        } else {
-            float4 q = prims[s];
-            v2 x = {points[i * 2], points[i * 2 + 1]}, p;
-            float t;
-            dist = closest_point_segment({q.x, q.y}, {q.z, q.w}, x, p, t);
-            w[0] = 1.0f - t; w[1] = t;
-            diff[0] = x.x - p.x; diff[1] = x.y - p.y;
-        }
-        if (dist > 0.0f) {
-            float sc = gd / dist;
-            for (int k = 0; k < DIM; k++) g[k] = sc * diff[k];
-            if (grad_vertices) {
-                // hardware FP32 atomic add (global_atomic_add_f32, no return value) rather
-                // than the compare-and-swap loop atomicAdd compiles to without
-                // -munsafe-fp-atomics; grad_vertices is device (coarse-grained) memory
-                for (int v = 0; v < DIM; v++) {
-                    int32_t vid = prim_vids[(size_t)s * DIM + v];
-                    for (int k = 0; k < DIM; k++)
-                        (void)unsafeAtomicAdd(&grad_vertices[(size_t)vid * DIM + k], -w[v] * g[k]);
-                }
-            }
-        }
-    }
+    AdjointTerms<DIM> T;
+    adjoint_terms<DIM>(prims, prim_vids, n_prims, points, slots, grad_d, i, T);
+    if (grad_vertices && T.slot >= 0) adjoint_scatter<DIM>(T, grad_vertices);
     if (grad_points)
-        for (int k = 0; k < DIM; k++) grad_points[i * DIM + k] = g[k];
+        for (int k = 0; k < DIM; k++) grad_points[i * DIM + k] = T.g[k];
+}
+
+// The adjoint over the queries in a given order -- the Morton order of the
+// preceding forward pass, so that a wave's queries read neighbouring primitives
+// and scatter into neighbouring vertices (entry k handles query order[k]).
+// With REDUCE, a run of consecutive lanes on the same primitive first sums its
+// vertex terms (a segmented scan across the wave) and only the run's last lane
+// issues the atomics.
+template <int DIM, bool REDUCE>
+__global__ __launch_bounds__(256) void k_distance_backward_ordered(const float4 *prims, const int32_t *prim_vids,
+                                                                   const float *points, const int32_t *slots,
+                                                                   const float *grad_d, int64_t n, int n_prims,
+                                                                   const uint32_t *order, float *grad_points,
+                                                                   float *grad_vertices)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = k < n ? (int64_t)order[k] : n;
+    const bool valid = i < n;
+    AdjointTerms<DIM> T;
+    T.slot = -1;
+    if (valid) {
+        adjoint_terms<DIM>(prims, prim_vids, n_prims, points, slots, grad_d, i, T);
+        if (grad_points)
+            for (int c = 0; c < DIM; c++) grad_points[i * DIM + c] = T.g[c];
+    }
+    if (!grad_vertices) return;
+    if constexpr (!REDUCE) {
+        if (T.slot >= 0) adjoint_scatter<DIM>(T, grad_vertices);
+    } else {
+        // every lane of the wave takes part in the shuffles; a lane with nothing
+        // to add carries slot -1 and zero terms
+        const int lane = (int)(threadIdx.x & 63u);
+        if (T.slot < 0)
+            for (int v = 0; v < DIM; v++)
+                for (int c = 0; c < DIM; c++) T.c[v][c] = 0.0f;
+        const int prev = __shfl_up(T.slot, 1, 64);
+        const bool head = lane == 0 || prev != T.slot;
+        const uint64_t heads = __ballot(head);
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int start = 63 - __clzll(heads & upto); // the first lane of this lane's run
+        for (int off = 1; off < 64; off <<= 1) {
+            for (int v = 0; v < DIM; v++)
+                for (int c = 0; c < DIM; c++) {
+                    const float t = __shfl_up(T.c[v][c], off, 64);
+                    if (lane - off >= start) T.c[v][c] += t;
+                }
+        }
+        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+        if (tail && T.slot >= 0) adjoint_scatter<DIM>(T, grad_vertices);
+    }
 }
 
 template <int DIM>

@@ -760,6 +760,7 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     const int64_t cells = (int64_t)1 << (axis_bits * c->dim + dir_bits);
     const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
     int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 8); // codes, then ranks
+    c->ws_perm_n = -1;
     if (!st) st = grow(c, &c->ws_perm, &c->ws_perm_bytes, (size_t)n * 4);
     if (!st) st = grow(c, &c->ws_hist, &c->ws_hist_bytes, (size_t)cells * 4);
     if (!st) st = grow(c, &c->ws_sums, &c->ws_sums_bytes, (size_t)segs * 4);
@@ -782,6 +783,7 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
     hipLaunchKernelGGL(k_morton_place, dim3(nb), dim3(256), 0, s, codes, ranks, n, hist, perm);
     HIPCHECK(c, hipGetLastError());
+    c->ws_perm_n = n;
     *perm_out = perm;
     return FCPW_HIP_OK;
 }
@@ -1151,12 +1153,26 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *c, const float *points, cons
     hipStream_t s = (hipStream_t)stream;
     int threads = 256;
     unsigned blocks = (unsigned)((n + threads - 1) / threads);
-    if (c->dim == 3)
-        hipLaunchKernelGGL(k_distance_backward<3>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, points,
-                           slot, grad_d, n, c->n_prims, grad_points, grad_vertices);
-    else
-        hipLaunchKernelGGL(k_distance_backward<2>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, points,
-                           slot, grad_d, n, c->n_prims, grad_points, grad_vertices);
+    // The Morton permutation the forward pass of a batch of this size left in the
+    // workspace (any permutation of [0, n) gives the same sums up to FP32
+    // rounding order; this one keeps a wave's primitives and vertices together).
+    // Stream order makes it the one of the latest query call on this context.
+    const uint32_t *order = (c->bwd_order && c->ws_perm_n == n) ? (const uint32_t *)c->ws_perm : nullptr;
+#define FCPW_BWD(D)                                                                                           \
+    do {                                                                                                      \
+        if (!order)                                                                                           \
+            hipLaunchKernelGGL(k_distance_backward<D>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, \
+                               points, slot, grad_d, n, c->n_prims, grad_points, grad_vertices);                  \
+        else if (c->bwd_reduce)                                                                               \
+            hipLaunchKernelGGL((k_distance_backward_ordered<D, true>), dim3(blocks), dim3(threads), 0, s, c->prims, \
+                               c->prim_vids, points, slot, grad_d, n, c->n_prims, order, grad_points, grad_vertices); \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_distance_backward_ordered<D, false>), dim3(blocks), dim3(threads), 0, s, c->prims, \
+                               c->prim_vids, points, slot, grad_d, n, c->n_prims, order, grad_points, grad_vertices); \
+    } while (0)
+    if (c->dim == 3) FCPW_BWD(3);
+    else FCPW_BWD(2);
+#undef FCPW_BWD
     HIPCHECK(c, hipGetLastError());
     return FCPW_HIP_OK;
 }
@@ -1168,6 +1184,9 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     if (k == "block") {
         if (value != 64 && value != 128 && value != 256) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
         c->block = value;
+    } else if (k == "bwd_order" || k == "bwd_reduce") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be 0 or 1", key);
+        (k == "bwd_order" ? c->bwd_order : c->bwd_reduce) = value;
     } else if (k == "stack") {
         if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
             FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 8, 12, 16 or 24");
@@ -1230,6 +1249,8 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     std::string k(key);
     if (k == "block") *value = c->block;
     else if (k == "stack") *value = c->stack;
+    else if (k == "bwd_order") *value = c->bwd_order;
+    else if (k == "bwd_reduce") *value = c->bwd_reduce;
     else if (k == "sort") *value = c->sort;
     else if (k == "coop") *value = c->coop;
     else if (k == "xcds") *value = c->xcds;

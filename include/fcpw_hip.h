@@ -364,6 +364,12 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "sil_xcds"  the same for the silhouette kernels (default 8)
      "xcd_chunk" 0 = one contiguous range per group (default); c > 0 = groups take
                  chunks of 2^c tiles in turn (0..16)
+     "bwd_order" 1 = the adjoint walks the queries in the Morton order the latest
+                 query call on this context computed, when that batch had the same
+                 size (default); 0 = in the caller's order
+     "bwd_reduce"  1 = in that order, consecutive queries on the same primitive sum
+                 their vertex terms across the wave before the atomics (default)
+                 (both change only the FP32 summation order of grad_vertices)
    Measurement switches (off by default; results are unchanged):
      "stats"     1 = run the counting variant of the traversal kernels, which adds
                  the internal nodes expanded, primitives tested and queries to

@@ -234,3 +234,40 @@ def test_forward_is_deterministic_and_backward_accumulates(gpu):
     Vt.grad = None
     d2.sum().backward()
     torch.testing.assert_close(Vt.grad, g1, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("name", ["icosphere", "star2d"])
+def test_adjoint_orders_agree(gpu, name):
+    """the adjoint walked in the caller's order, in the forward's Morton order and
+    in that order with the wave pre-reduction: dL/dx bit for bit, dL/dV equal up
+    to FP32 summation order; a backward after a forward of another size falls
+    back to the caller's order"""
+    V, F = meshes.small_icosphere(4) if name == "icosphere" else meshes.small_star(4096)
+    s, g = make(V, F)
+    q = near_queries(V, F, 60000, 0.1, 12)
+    x = torch.from_numpy(q).cuda()
+    dim = V.shape[1]
+    w = torch.rand(len(q), device="cuda")
+    grads = []
+    for order, reduce in ((0, 0), (1, 0), (1, 1)):
+        g.set_option("bwd_order", order)
+        g.set_option("bwd_reduce", reduce)
+        xg = x.clone().requires_grad_(True)
+        Vt = torch.from_numpy(V).cuda().requires_grad_(True)
+        d, _, _ = closest_point_distance(g, xg, Vt)
+        (w * d).sum().backward()
+        grads.append((xg.grad.clone(), Vt.grad.clone()))
+    g.set_option("bwd_order", 1)
+    g.set_option("bwd_reduce", 1)
+    for gx, gV in grads[1:]:
+        assert torch.equal(gx, grads[0][0])
+        torch.testing.assert_close(gV, grads[0][1], rtol=1e-5, atol=1e-6)
+    # another query call in between, of another size: the permutation is not used
+    xg = x.clone().requires_grad_(True)
+    Vt = torch.from_numpy(V).cuda().requires_grad_(True)
+    d, _, _ = closest_point_distance(g, xg, Vt)
+    closest_point_distance(g, x[:5000])
+    (w * d).sum().backward()
+    assert torch.equal(xg.grad, grads[0][0])
+    torch.testing.assert_close(Vt.grad, grads[0][1], rtol=1e-5, atol=1e-6)
+    assert gV.shape == (len(V), dim)
