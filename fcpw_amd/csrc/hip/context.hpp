@@ -54,6 +54,11 @@ struct fcpw_hip_ctx {
     // number of queries of the batch whose Morton permutation ws_perm holds (-1: none);
     // the adjoint walks the queries in that order when its batch has the same size
     int64_t ws_perm_n = -1;
+    // device address of the allocation-free entry point's scratch array on this
+    // context's device (shared by every context on the device), or null with the
+    // reason in scratch_err
+    void *scratch = nullptr;
+    std::string scratch_err;
     int bwd_order = 1, bwd_reduce = 1; // adjoint: the forward's query order, wave pre-reduction
     float qlo[3] = {0, 0, 0}, qinv[3] = {1, 1, 1}; // Morton grid: 1.5x the scene box
     // facts

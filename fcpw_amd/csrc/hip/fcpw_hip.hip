@@ -3,10 +3,15 @@
 // adjoint, options), from batch.hpp the batched multi-scene launch and from
 // instances.hpp scenes of several objects and transformed instances.
 // Device code lives in device.hpp, the per-scene context in context.hpp.
+#include <mutex>
+
 #include "device.hpp"
 #include "silhouette.hpp"
 #include "context.hpp"
 #include "gpu_build.hpp"
+
+// the allocation-free entry point's scratch lookup (defined with the array below)
+static void *noalloc_scratch_ptr(fcpw_hip_ctx *c);
 
 // scenes of several objects (instances.hpp)
 static int instances_refit(fcpw_hip_ctx *c, hipStream_t s);
@@ -288,6 +293,15 @@ int fcpw_hip_create(int device, fcpw_hip_ctx **out)
     if (e == hipSuccess) e = hipMemset(c->flags, 0, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void **)&c->stats, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->stats, 0, 8 * sizeof(unsigned long long));
+    // the allocation-free entry point's scratch array, looked up here once per
+    // device; a failure only disables that entry point, whose calls report it
+    if (e == hipSuccess) {
+        c->scratch = noalloc_scratch_ptr(c);
+        if (!c->scratch) {
+            c->scratch_err = c->err;
+            c->err.clear();
+        }
+    }
     if (e != hipSuccess) {
         g_create_error = hipGetErrorString(e);
         if (c->flags) (void)hipFree(c->flags);
@@ -748,26 +762,50 @@ static int timing_end(fcpw_hip_ctx *c, hipStream_t s)
 
 // Morton order of n query points (pts + i * stride floats); returns the
 // permutation, or nullptr when ordering is off or the batch is small
+// Cells of the ordering grid: morton_bits in total, of which the direction's
+// octant (quadrant) takes dim bits for rays (dir_off >= 0)
+static int64_t morton_cells(int dim, int morton_bits, int dir_off, int *axis_bits)
+{
+    const int dir_bits = dir_off >= 0 ? dim : 0;
+    *axis_bits = (morton_bits - dir_bits) / dim;
+    return (int64_t)1 << (*axis_bits * dim + dir_bits);
+}
+
+// Enqueues the counting sort of n queries along the Morton curve into the given
+// buffers: codes and ranks (2n words), perm (n), hist (cells), sums (segments).
+static int morton_order_into(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n, hipStream_t s, int morton_bits,
+                             int dir_off, uint32_t *codes, uint32_t *perm, uint32_t *hist, uint32_t *sums);
+
 static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n, hipStream_t s,
                         const uint32_t **perm_out, int dir_off = -1)
 {
     *perm_out = nullptr;
     if (!c->sort || n < 4096) return FCPW_HIP_OK;
     if (n > (int64_t)UINT32_MAX) FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "query batches above 2^32 are not supported");
-    // rays: the direction's octant (quadrant) takes dim bits of the code, the origin the rest
-    const int dir_bits = dir_off >= 0 ? c->dim : 0;
-    const int axis_bits = (c->morton_bits - dir_bits) / c->dim;
-    const int64_t cells = (int64_t)1 << (axis_bits * c->dim + dir_bits);
+    int axis_bits;
+    const int64_t cells = morton_cells(c->dim, c->morton_bits, dir_off, &axis_bits);
     const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
-    int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 8); // codes, then ranks
     c->ws_perm_n = -1;
+    int st = grow(c, &c->ws_codes, &c->ws_codes_bytes, (size_t)n * 8); // codes, then ranks
     if (!st) st = grow(c, &c->ws_perm, &c->ws_perm_bytes, (size_t)n * 4);
     if (!st) st = grow(c, &c->ws_hist, &c->ws_hist_bytes, (size_t)cells * 4);
     if (!st) st = grow(c, &c->ws_sums, &c->ws_sums_bytes, (size_t)segs * 4);
     if (st) return st;
-    uint32_t *codes = (uint32_t *)c->ws_codes, *perm = (uint32_t *)c->ws_perm, *hist = (uint32_t *)c->ws_hist;
+    st = morton_order_into(c, pts, stride, n, s, c->morton_bits, dir_off, (uint32_t *)c->ws_codes,
+                           (uint32_t *)c->ws_perm, (uint32_t *)c->ws_hist, (uint32_t *)c->ws_sums);
+    if (st) return st;
+    c->ws_perm_n = n;
+    *perm_out = (const uint32_t *)c->ws_perm;
+    return FCPW_HIP_OK;
+}
+
+static int morton_order_into(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n, hipStream_t s, int morton_bits,
+                             int dir_off, uint32_t *codes, uint32_t *perm, uint32_t *hist, uint32_t *sums)
+{
+    int axis_bits;
+    const int64_t cells = morton_cells(c->dim, morton_bits, dir_off, &axis_bits);
+    const int64_t segs = (cells + SCAN_SEG - 1) / SCAN_SEG;
     uint32_t *ranks = codes + n;
-    uint32_t *sums = (uint32_t *)c->ws_sums;
     unsigned nb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, hist, 0u, cells);
     float4 lo = make_float4(c->qlo[0], c->qlo[1], c->qlo[2], 0.0f);
@@ -783,8 +821,6 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
     hipLaunchKernelGGL(k_scan_segment_apply, dim3((unsigned)segs), dim3(SCAN_THREADS), 0, s, hist, cells, sums);
     hipLaunchKernelGGL(k_morton_place, dim3(nb), dim3(256), 0, s, codes, ranks, n, hist, perm);
     HIPCHECK(c, hipGetLastError());
-    c->ws_perm_n = n;
-    *perm_out = perm;
     return FCPW_HIP_OK;
 }
 
@@ -809,14 +845,25 @@ static int morton_order(fcpw_hip_ctx *c, const float *pts, int stride, int64_t n
 // closest-point launch for either output policy; io.in holds spheres (4
 // floats per query) for interaction records, points (dim floats) otherwise
 template <int OUT>
+static int launch_closest_kernel(fcpw_hip_ctx *c, const CpIO &io, hipStream_t s, bool stats);
+
+template <int OUT>
 static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
 {
-    DevScene S = dev_scene(c);
     int st = morton_order(c, io.in, OUT == CP_OUT_INTERACTION ? 4 : c->dim, io.n, s, &io.order);
     if (st) return st;
-    const int64_t n = io.n;
     if ((st = timing_begin(c, s))) return st;
-    if (c->stats_on) {
+    if ((st = launch_closest_kernel<OUT>(c, io, s, c->stats_on))) return st;
+    return timing_end(c, s);
+}
+
+// the traversal kernel alone (the queries ordered by io.order when set)
+template <int OUT>
+static int launch_closest_kernel(fcpw_hip_ctx *c, const CpIO &io, hipStream_t s, bool stats)
+{
+    DevScene S = dev_scene(c);
+    const int64_t n = io.n;
+    if (stats) {
         // counting pass: the coop kernel with per-wave counters (same traversal order)
         dim3 g = grid_for(c, n, 64);
         if (c->dim == 3)
@@ -853,7 +900,45 @@ static int launch_closest(fcpw_hip_ctx *c, CpIO io, hipStream_t s)
                                                   S, io, xcd_arg(c)));
     }
     HIPCHECK(c, hipGetLastError());
-    return timing_end(c, s);
+    return FCPW_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free closest-point entry point: the query-ordering
+// workspace in a file-scope device array instead of the context's hipMalloc'ed,
+// grow-on-demand buffers.  Every GPU the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for that device (noalloc_scratch_ptr), never
+// through the array's name.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_QUERIES = (int64_t)1 << 21; // 2,097,152 queries per call
+constexpr int NOALLOC_MORTON_BITS = 18;                  // a 2^18-cell ordering grid
+constexpr int64_t NOALLOC_CELLS = (int64_t)1 << NOALLOC_MORTON_BITS;
+constexpr int64_t NOALLOC_SEGS = (NOALLOC_CELLS + SCAN_SEG - 1) / SCAN_SEG;
+// 32-bit words: codes and ranks (2 per query), perm (1 per query), histogram, segment sums
+constexpr int64_t NOALLOC_WORDS = 3 * NOALLOC_MAX_QUERIES + NOALLOC_CELLS + NOALLOC_SEGS;
+__device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+static std::mutex g_scratch_mutex;
+static std::vector<void *> g_scratch_ptr; // per device; null until looked up
+
+// The device address of `scratch` on c's device, looked up once per device with
+// that device current; null (and c's error set) when the lookup fails.
+static void *noalloc_scratch_ptr(fcpw_hip_ctx *c)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= c->device) g_scratch_ptr.resize(c->device + 1, nullptr);
+    if (!g_scratch_ptr[c->device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipSetDevice(c->device);
+        if (e == hipSuccess) e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            c->err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_ptr[c->device] = ptr;
+    }
+    return g_scratch_ptr[c->device];
 }
 
 static int launch_closest_records(fcpw_hip_ctx *c, const void *spheres, int64_t n, int record_normals, void *out,
@@ -934,6 +1019,43 @@ int fcpw_hip_closest_points_device(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphe
         FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device: spheres must be 16-byte and out 8-byte aligned");
     if (n == 0) return FCPW_HIP_OK;
     return launch_closest_records(c, spheres, n, record_normals, out, (hipStream_t)stream);
+}
+
+int fcpw_hip_closest_points_device_noalloc(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                           int record_normals, fcpw_hip_interaction *out, void *stream)
+{
+    int st = check_single(c, "closest_points_device_noalloc");
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!spheres || !out)))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "closest_points_device_noalloc: bad arguments");
+    if (n > NOALLOC_MAX_QUERIES)
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "closest_points_device_noalloc: %lld queries exceed the scratch maximum of %lld per call",
+             (long long)n, (long long)NOALLOC_MAX_QUERIES);
+    if (((uintptr_t)spheres & 15) || ((uintptr_t)out & 7))
+        FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT,
+             "closest_points_device_noalloc: spheres must be 16-byte and out 8-byte aligned");
+    if (!c->scratch)
+        FAIL(c, FCPW_HIP_ERR_HIP, "closest_points_device_noalloc: the scratch array's address is unknown (%s)",
+             c->scratch_err.c_str());
+    if (n == 0) return FCPW_HIP_OK;
+    hipStream_t s = (hipStream_t)stream;
+    CpIO io = {};
+    io.in = (const float *)spheres;
+    io.n = n;
+    io.out = (float2 *)out;
+    io.record_normals = record_normals;
+    if (c->sort && n >= 4096) {
+        // codes | ranks | perm | histogram | segment sums; the histogram is cleared
+        // by the fill kernel morton_order_into starts with
+        uint32_t *codes = (uint32_t *)c->scratch;
+        uint32_t *perm = codes + 2 * NOALLOC_MAX_QUERIES, *hist = perm + NOALLOC_MAX_QUERIES;
+        uint32_t *sums = hist + NOALLOC_CELLS;
+        st = morton_order_into(c, io.in, 4, n, s, NOALLOC_MORTON_BITS, -1, codes, perm, hist, sums);
+        if (st) return st;
+        io.order = perm;
+    }
+    return launch_closest_kernel<CP_OUT_INTERACTION>(c, io, s, false);
 }
 
 int fcpw_hip_closest_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *spheres, int64_t n, int record_normals,

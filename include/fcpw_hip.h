@@ -154,6 +154,23 @@ int fcpw_hip_closest_points_device(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sp
                                    int record_normals, fcpw_hip_interaction *out, void *stream);
 
 /*
+ * fcpw_hip_closest_points_device without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory during the call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as fcpw_hip_closest_points_device
+ * on a single-object scene; the query-ordering workspace lives in a
+ * file-scope device array of the library (about 25 MiB per GPU, held until
+ * the process ends) instead of the context's grow-on-demand buffers, with a
+ * 2^18-cell ordering grid.  At most 2,097,152 queries per call (more returns
+ * FCPW_HIP_ERR_INVALID_ARGUMENT before anything is launched).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from different contexts) race.  The "timing" and
+ * "stats" switches do not apply.
+ */
+int fcpw_hip_closest_points_device_noalloc(fcpw_hip_ctx *ctx, const fcpw_hip_bounding_sphere *spheres, int64_t n,
+                                           int record_normals, fcpw_hip_interaction *out, void *stream);
+
+/*
  * Replaces GPUScene::intersect(rays) (fcpw_gpu.inl:137-156) and the
  * rayIntersection entry point (bvh.cs.slang:36-57).  With check_for_occlusion
  * the first hit ends the query and only its index is written (bvh.slang:183-187).
