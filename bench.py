@@ -75,6 +75,8 @@ def parse():
                    help="after the timed work, copy a buffer of this size 3x (counter calibration)")
     for k in ("block", "stack", "sort", "coop", "hold_min", "xcds"):
         p.add_argument("--" + k.replace("_", "-"), type=int, default=None)
+    p.add_argument("--opt", action="append", default=[], metavar="KEY=VALUE",
+                   help="a GPUScene option set on every scene the run builds (repeatable)")
     p.add_argument("--summarize-kernels", metavar="DB", help="write a kernel-trace summary and exit")
     p.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                    help="write the HBM traffic summary from two counter runs and exit")
@@ -86,6 +88,17 @@ def parse():
     p.add_argument("--command", action="append", default=[], help="command lines recorded in a summary")
     p.add_argument("--out", default=None, help="summary file")
     return p.parse_args()
+
+
+EXTRA_OPTS = []
+
+
+def apply_opts(g):
+    """the --opt settings, on a freshly transferred GPUScene"""
+    for kv in EXTRA_OPTS:
+        k, v = kv.split("=")
+        g.set_option(k, int(v))
+    return g
 
 
 def algorithmic_bytes(n_internal, n_prims, sphere_in=16, record_out=40, node_bytes=32, prim_bytes=40):
@@ -315,6 +328,7 @@ def timed(torch, stream, fn, steps, warmup):
 
 def main():
     args = parse()
+    EXTRA_OPTS.extend(args.opt)
     if args.summarize_kernels:
         summarize_kernels(args.summarize_kernels, " && ".join(args.command), args.out)
         return
@@ -363,6 +377,7 @@ def run_torus(args, torch):
     scene.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
     gpu = fcpw.GPUScene3D(fcpw.slang_directory_path)
     gpu.transfer_to_gpu(scene)
+    apply_opts(gpu)
     for k in ("block", "stack", "sort", "coop", "hold_min", "xcds"):
         val = getattr(args, k)
         if val is not None:
@@ -547,6 +562,7 @@ def run_torus(args, torch):
             ss.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
             gs = fcpw.GPUScene3D(fcpw.slang_directory_path)
             gs.transfer_to_gpu(ss)
+            apply_opts(gs)
             log(f"[silhouettes] silhouettes + cones built and transferred in {time.time() - t0:.1f}s")
             qs = q[::4]
             ns = len(qs)
@@ -665,6 +681,7 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     si.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
     gi = fcpw.GPUScene3D(fcpw.slang_directory_path)
     gi.transfer_to_gpu(si)
+    apply_opts(gi)
     t_inst_build = time.time() - t0
     # the same instances baked into one mesh (FP64 transform, rounded once)
     W = np.concatenate([V.astype(np.float64) @ M[:3, :3].T.astype(np.float64) + M[:3, 3] for M in mats])
@@ -739,6 +756,7 @@ def _scene(fcpw, V, F):
     s.build(fcpw.AggregateType.Bvh_SurfaceArea, False)
     g = fcpw.GPUScene3D("") if dim == 3 else fcpw.GPUScene2D("")
     g.transfer_to_gpu(s)
+    apply_opts(g)
     return s, g
 
 

@@ -681,25 +681,21 @@ __device__ __forceinline__ void coop_begin(const DevScene &S, CoopShared &sh, Co
                 (L).n_internal++;                                                                             \
                 float a0, a1, b0, b1;                                                                         \
                 box_sqdist(loA, hiA, (L).x, a0, a1);                                                          \
-                bool oA = a0 <= (L).r2;                                                                       \
+                const bool oA = a0 <= (L).r2;                                                                 \
                 (L).r2 = fminf((L).r2, a1);                                                                   \
                 box_sqdist(loB, hiB, (L).x, b0, b1);                                                          \
-                bool oB = b0 <= (L).r2;                                                                       \
+                const bool oB = b0 <= (L).r2;                                                                 \
                 (L).r2 = fminf((L).r2, b1);                                                                   \
+                /* continue with B when only B overlaps, or both do and B is strictly nearer; */            \
+                /* with neither, live turns false and the next step pops */                                 \
+                const bool bFirst = oB && (!oA || b0 < a0);                                                   \
                 (L).live = oA || oB;                                                                          \
+                (L).cur = bFirst ? lB : lA;                                                                   \
+                (L).cur_d2 = bFirst ? b0 : a0;                                                                \
                 if (oA && oB) {                                                                               \
-                    uint32_t nearL = lA, farL = lB;                                                           \
-                    float nearD = a0, farD = b0;                                                              \
-                    if (b0 < a0) { nearL = lB; farL = lA; nearD = b0; farD = a0; }                           \
+                    const uint32_t farL = bFirst ? lA : lB;                                                   \
+                    const float farD = bFirst ? a0 : b0;                                                      \
                     FCPW_PUSH(BLOCK, K, farL, farD, { OVERFLOW = true; (L).active = false; (L).live = false; }); \
-                    (L).cur = nearL;                                                                          \
-                    (L).cur_d2 = nearD;                                                                       \
-                } else if (oA) {                                                                              \
-                    (L).cur = lA;                                                                             \
-                    (L).cur_d2 = a0;                                                                          \
-                } else if (oB) {                                                                              \
-                    (L).cur = lB;                                                                             \
-                    (L).cur_d2 = b0;                                                                          \
                 }                                                                                             \
             }                                                                                                 \
         }                                                                                                     \

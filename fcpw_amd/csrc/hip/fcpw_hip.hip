@@ -1182,10 +1182,15 @@ static int launch_silhouettes(fcpw_hip_ctx *c, const void *spheres, const uint32
     if (c->coop && c->sil_leaf_max <= 1023) {
         // record indices within a leaf travel in 10 bits of a leaf-pass item
         const dim3 g = grid_for(c, n, 64);
-        if (c->dim == 3)
-            hipLaunchKernelGGL((k_closest_silhouettes_coop<3, SIL_COOP_K3>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
-                               flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c), c->sil_hold_min);
-        else
+#define FCPW_SIL3(K)                                                                                          \
+    hipLaunchKernelGGL((k_closest_silhouettes_coop<3, K>), g, dim3(64), 0, s, S, T, (const float4 *)spheres, flip, n, \
+                       r2min, prec, (float2 *)out, order, sil_xcd_arg(c), c->sil_hold_min)
+        if (c->dim == 3) {
+            if (c->sil_stack == 8) FCPW_SIL3(8);
+            else if (c->sil_stack == 12) FCPW_SIL3(12);
+            else FCPW_SIL3(SIL_COOP_K3);
+        } else
+#undef FCPW_SIL3
             hipLaunchKernelGGL((k_closest_silhouettes_coop<2, 8>), g, dim3(64), 0, s, S, T, (const float4 *)spheres,
                                flip, n, r2min, prec, (float2 *)out, order, sil_xcd_arg(c), c->sil_hold_min);
     } else if (c->dim == 3)
@@ -1306,6 +1311,10 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     if (k == "block") {
         if (value != 64 && value != 128 && value != 256) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
         c->block = value;
+    } else if (k == "sil_stack" || k == "inst_stack") {
+        if (k == "sil_stack" ? (value != 4 && value != 8 && value != 12) : (value != 8 && value != 12))
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be %s", key, k == "sil_stack" ? "4, 8 or 12" : "8 or 12");
+        (k == "sil_stack" ? c->sil_stack : c->inst_stack) = value;
     } else if (k == "bwd_order" || k == "bwd_reduce") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be 0 or 1", key);
         (k == "bwd_order" ? c->bwd_order : c->bwd_reduce) = value;
@@ -1372,6 +1381,8 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     if (k == "block") *value = c->block;
     else if (k == "stack") *value = c->stack;
     else if (k == "bwd_order") *value = c->bwd_order;
+    else if (k == "sil_stack") *value = c->sil_stack;
+    else if (k == "inst_stack") *value = c->inst_stack;
     else if (k == "bwd_reduce") *value = c->bwd_reduce;
     else if (k == "sort") *value = c->sort;
     else if (k == "coop") *value = c->coop;

@@ -1186,7 +1186,11 @@ static int launch_instances_closest(fcpw_hip_ctx *c, const void *spheres, int64_
     if (c->coop && c->max_depth + c->inst_obj_depth < STACK_TOTAL) {
         // one stack serves both levels, so the two trees' depths must fit it together
         const dim3 g = grid_for(c, n, 64);
-        if (c->dim == 3)
+        if (c->dim == 3 && c->inst_stack == 8)
+            hipLaunchKernelGGL((k_instances_closest_coop<3, 8>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
+                               c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
+                               order, xcd_arg(c), c->hold_min);
+        else if (c->dim == 3)
             hipLaunchKernelGGL((k_instances_closest_coop<3, INST_COOP_K3>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
                                c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
                                order, xcd_arg(c), c->hold_min);

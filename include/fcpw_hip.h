@@ -364,6 +364,9 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "ray_stack", "ray_hold_min"  the cooperative ray kernels' LDS stack entries (4, 8
                  or 16, default 4) and leaf-pass threshold (1..64, default 8)
      "sil_hold_min"  the cooperative silhouette kernel's leaf-pass threshold (default 8)
+     "sil_stack", "inst_stack"  LDS stack entries of the 3D cooperative silhouette kernel
+                 (4, 8 or 12, default 4) and of the 3D cooperative instanced closest-point
+                 kernel (8 or 12, default 12)
      "ray_sort_dir"  1 = rays ordered by direction octant (quadrant), then by origin
                  along the Morton curve (default); 0 = by origin only
      "hold_min"  coop: lanes that must hold a leaf before the wave processes

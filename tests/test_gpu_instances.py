@@ -62,7 +62,8 @@ def test_cooperative_and_per_lane_kernels_equal_oracle(gpu, name):
     r2[1::3] = np.float32(0.25)
     ref, ref_obj = I.closest_points(q, r2, record_normal=True)
     for cfg in (dict(coop=1, hold_min=1), dict(coop=1, hold_min=8), dict(coop=1, hold_min=64),
-                dict(coop=1, hold_min=1, sort=0, xcds=1), dict(coop=0)):
+                dict(coop=1, hold_min=1, sort=0, xcds=1), dict(coop=1, inst_stack=8, sort=1, xcds=4),
+                dict(coop=0, inst_stack=12)):
         for key, val in cfg.items():
             g.set_option(key, val)
         hits = fcpw.GPUInteractionList()

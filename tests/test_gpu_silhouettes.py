@@ -202,7 +202,8 @@ def test_cooperative_and_per_lane_kernels_give_the_same_bits(gpu, name):
     for kw in ({}, dict(squared_min_radius=r2min, precision=2e-3)):
         base = run(dict(coop=0), **kw)
         for cfg in (dict(coop=1, sil_hold_min=1), dict(coop=1, sil_hold_min=8),
-                    dict(coop=1, sil_hold_min=64, sort=0)):
+                    dict(coop=1, sil_hold_min=64, sort=0), dict(coop=1, sil_stack=8, sort=1),
+                    dict(coop=1, sil_stack=12, sil_hold_min=1), dict(sil_stack=4)):
             np.testing.assert_array_equal(run(cfg, **kw), base, err_msg=f"{cfg} {kw}")
         g.set_option("sort", 1)
     V2 = (V + np.random.default_rng(49).uniform(-0.01, 0.01, V.shape) * ext).astype(np.float32)
