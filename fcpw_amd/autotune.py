@@ -5,10 +5,13 @@ the same order), so tuning is a pure timing search:
 
 ``tune_scene(gpu_scene, spheres)``
     times the single-scene closest-point launch over candidate settings of
+    ``layout`` (the order of the tree's records in memory: DFS or treelets),
     ``coop`` (wave-cooperative leaves or not), ``stack`` (traversal-stack
     entries kept in LDS), ``hold_min`` (held leaves before a cooperative leaf
     pass), ``xcds`` (how many ranges of the ordered queries the XCDs split
     into) and, without coop, ``block``; applies and returns the fastest.
+    Candidates are grouped by layout, so the records are reordered once per
+    layout.
 
 ``tune_batch(batch, spheres, out)``
     the same for a ``BatchedScenes`` persistent launch, over ``stack``,
@@ -26,8 +29,9 @@ import torch
 __all__ = ["SCENE_CANDIDATES", "BATCH_CANDIDATES", "tune_scene", "tune_batch", "clear_cache"]
 
 SCENE_CANDIDATES = (
-    [dict(coop=1, stack=k, hold_min=h, xcds=x) for k in (8, 12, 16) for h in (1, 2, 4) for x in (4, 8)]
-    + [dict(coop=0, stack=k, block=b, xcds=x) for k in (8, 16) for b in (64, 128) for x in (4, 8)]
+    [dict(layout=L, coop=1, stack=k, hold_min=h, xcds=x) for L in (0, 1) for k in (8, 12, 16) for h in (1, 2, 4)
+     for x in (4, 8)]
+    + [dict(layout=0, coop=0, stack=k, block=b, xcds=x) for k in (8, 16) for b in (64, 128) for x in (4, 8)]
 )
 BATCH_CANDIDATES = [dict(stack=k, hold_min=h, chunk=c, waves_per_cu=0)
                     for k in (8, 12, 16) for h in (1, 2) for c in (64, 256)]

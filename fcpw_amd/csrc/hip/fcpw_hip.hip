@@ -12,6 +12,8 @@
 
 // the allocation-free entry point's scratch lookup (defined with the array below)
 static void *noalloc_scratch_ptr(fcpw_hip_ctx *c);
+// the record layout of an uploaded tree (defined before upload_tree)
+static int relayout_records(fcpw_hip_ctx *c, int layout);
 
 // scenes of several objects (instances.hpp)
 static int instances_refit(fcpw_hip_ctx *c, hipStream_t s);
@@ -260,7 +262,7 @@ static int gpu_build(fcpw_hip_ctx *c, const float *positions, int n_vertices, co
     c->uploaded = true;
     c->generation++;
     c->positions_version++;
-    return FCPW_HIP_OK;
+    return c->layout ? relayout_records(c, c->layout) : FCPW_HIP_OK;
 }
 
 extern "C" {
@@ -468,6 +470,108 @@ static int tree_records(fcpw_hip_ctx *c, const char *fn, int dim, int n_nodes, c
     return FCPW_HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Record layouts.  Records are numbered by their internal node; a layout is an
+// order of those numbers in memory:
+//   0  DFS: internal nodes in the preorder of the flat tree (first child first),
+//      as fcpw_hip_upload_bvh and fcpw_hip_build_bvh produce them;
+//   1  treelets: a node, its internal children and grandchildren stored
+//      contiguously (three levels, up to 7 records = 448 bytes in 3D) in
+//      breadth-first order, the treelets in DFS order of their roots, so that
+//      both children of a record lie next to it rather than only the first.
+// Relayout renumbers the records in place on the device: every internal link,
+// the root link, the refit schedule (parents, leaf slots), the silhouette cones
+// and the host table of node slots.  Only record indices change, so the
+// traversal visits the same nodes in the same order and every result and count
+// is unchanged.
+// ---------------------------------------------------------------------------
+static int relayout_records(fcpw_hip_ctx *c, int layout)
+{
+    const int n_rec = c->n_records;
+    if (n_rec <= 1) return FCPW_HIP_OK; // nothing to reorder
+    const int REC = c->dim == 3 ? 4 : 3;
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipDeviceSynchronize());
+    std::vector<float4> rec((size_t)n_rec * REC), root(2);
+    std::vector<int32_t> parent(n_rec), leaf_refs(c->n_leaves);
+    HIPCHECK(c, hipMemcpy(rec.data(), c->records, rec.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(root.data(), c->root, 2 * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(parent.data(), c->rec_parent, parent.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(leaf_refs.data(), c->leaf_refs, leaf_refs.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    auto link_at = [&](int r, int side) -> uint32_t & {
+        float &f = c->dim == 3 ? (side == 0 ? rec[(size_t)r * 4].w : rec[(size_t)r * 4 + 2].w)
+                               : (side == 0 ? rec[(size_t)r * 3 + 2].x : rec[(size_t)r * 3 + 2].y);
+        return *reinterpret_cast<uint32_t *>(&f);
+    };
+    uint32_t &root_link = *reinterpret_cast<uint32_t *>(c->dim == 3 ? &root[0].w : &root[1].x);
+    if (root_link & LEAF_BIT) return FCPW_HIP_OK;
+    // the target order, from the topology
+    std::vector<int32_t> order;
+    order.reserve(n_rec);
+    std::vector<int32_t> todo{(int32_t)root_link};
+    while (!todo.empty()) {
+        const int t = todo.back();
+        todo.pop_back();
+        if (layout == 0) {
+            order.push_back(t);
+            for (int side = 1; side >= 0; side--) { // first child on top of the stack
+                const uint32_t l = link_at(t, side);
+                if (!(l & LEAF_BIT)) todo.push_back((int32_t)l);
+            }
+        } else {
+            std::vector<int32_t> level{t}, frontier;
+            for (int d = 0; d < 3 && !level.empty(); d++) {
+                std::vector<int32_t> next;
+                for (int r : level) {
+                    order.push_back(r);
+                    for (int side = 0; side < 2; side++) {
+                        const uint32_t l = link_at(r, side);
+                        if (!(l & LEAF_BIT)) next.push_back((int32_t)l);
+                    }
+                }
+                if (d == 2) frontier = next;
+                level.swap(next);
+            }
+            for (auto it = frontier.rbegin(); it != frontier.rend(); ++it) todo.push_back(*it);
+        }
+    }
+    if ((int)order.size() != n_rec) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "relayout: the records do not form one tree");
+    std::vector<int32_t> new_of(n_rec, -1);
+    for (int k = 0; k < n_rec; k++) new_of[order[k]] = k;
+    auto slot_map = [&](int32_t slot) { return slot < 0 || slot >= 2 * n_rec ? slot : new_of[slot >> 1] * 2 + (slot & 1); };
+    std::vector<float4> rec2(rec.size());
+    std::vector<int32_t> parent2(n_rec);
+    for (int r = 0; r < n_rec; r++) {
+        for (int side = 0; side < 2; side++) {
+            uint32_t &l = link_at(r, side);
+            if (!(l & LEAF_BIT)) l = (uint32_t)new_of[l];
+        }
+        for (int j = 0; j < REC; j++) rec2[(size_t)new_of[r] * REC + j] = rec[(size_t)r * REC + j];
+        parent2[new_of[r]] = slot_map(parent[r]);
+    }
+    root_link = (uint32_t)new_of[root_link];
+    for (int32_t &l : leaf_refs) l = slot_map(l);
+    for (int32_t &sl : c->node_slot) sl = slot_map(sl);
+    HIPCHECK(c, hipMemcpy(c->records, rec2.data(), rec2.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->root, root.data(), 2 * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->rec_parent, parent2.data(), parent2.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->leaf_refs, leaf_refs.data(), leaf_refs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (c->has_silhouettes) {
+        // two float4 per child slot, the root's slot (2 * n_rec) last and unmoved
+        std::vector<float4> cones((size_t)(2 * n_rec + 1) * 2), cones2(cones.size());
+        HIPCHECK(c, hipMemcpy(cones.data(), c->cones, cones.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int32_t sl = 0; sl <= 2 * n_rec; sl++) {
+            const int32_t to = slot_map(sl);
+            cones2[(size_t)to * 2] = cones[(size_t)sl * 2];
+            cones2[(size_t)to * 2 + 1] = cones[(size_t)sl * 2 + 1];
+        }
+        HIPCHECK(c, hipMemcpy(c->cones, cones2.data(), cones2.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    HIPCHECK(c, hipDeviceSynchronize());
+    c->generation++; // batches re-read their tables
+    return FCPW_HIP_OK;
+}
+
 // allocates and fills the tree buffers of c (records, root, leaf-slot index,
 // refit schedule); the scene's previous buffers must have been freed
 static int upload_tree(fcpw_hip_ctx *c, const TreeRecords &T, const int32_t *slot_index, int n_slots)
@@ -563,7 +667,7 @@ int fcpw_hip_upload_bvh(fcpw_hip_ctx *c, int dim, int n_nodes, const float *pmin
     c->uploaded = true;
     c->generation++;
     c->positions_version++;
-    return FCPW_HIP_OK;
+    return c->layout ? relayout_records(c, c->layout) : FCPW_HIP_OK;
 }
 
 int fcpw_hip_build_bvh(fcpw_hip_ctx *c, int dim, const float *positions, int n_vertices, const int32_t *indices,
@@ -1311,6 +1415,15 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     if (k == "block") {
         if (value != 64 && value != 128 && value != 256) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
         c->block = value;
+    } else if (k == "layout") {
+        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "layout must be 0 or 1");
+        if (c->uploaded && c->instanced)
+            FAIL(c, FCPW_HIP_ERR_UNSUPPORTED, "layout: not supported on a scene of several objects or instances");
+        if (c->uploaded && value != c->layout) {
+            int st = relayout_records(c, value);
+            if (st) return st;
+        }
+        c->layout = value;
     } else if (k == "sil_stack" || k == "inst_stack") {
         if (k == "sil_stack" ? (value != 4 && value != 8 && value != 12) : (value != 8 && value != 12))
             FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be %s", key, k == "sil_stack" ? "4, 8 or 12" : "8 or 12");
@@ -1382,6 +1495,7 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "stack") *value = c->stack;
     else if (k == "bwd_order") *value = c->bwd_order;
     else if (k == "sil_stack") *value = c->sil_stack;
+    else if (k == "layout") *value = c->layout;
     else if (k == "inst_stack") *value = c->inst_stack;
     else if (k == "bwd_reduce") *value = c->bwd_reduce;
     else if (k == "sort") *value = c->sort;

@@ -364,6 +364,11 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "ray_stack", "ray_hold_min"  the cooperative ray kernels' LDS stack entries (4, 8
                  or 16, default 4) and leaf-pass threshold (1..64, default 8)
      "sil_hold_min"  the cooperative silhouette kernel's leaf-pass threshold (default 8)
+     "layout"    the order of the tree's records in HBM: 0 = DFS order of the internal
+                 nodes (default), 1 = treelets (a node, its internal children and
+                 grandchildren contiguous, treelets in DFS order); setting it on an
+                 uploaded single-object scene reorders the records in place, and later
+                 uploads and device builds of the context use it
      "sil_stack", "inst_stack"  LDS stack entries of the 3D cooperative silhouette kernel
                  (4, 8 or 12, default 4) and of the 3D cooperative instanced closest-point
                  kernel (8 or 12, default 12)

@@ -316,7 +316,8 @@ def test_autotuner_keeps_results_and_caches(gpu, world3):
     ref = out.clone()
     autotune.clear_cache()
     best, times = autotune.tune_scene(g, sp, reps=1)
-    assert times and set(best) <= {"coop", "stack", "hold_min", "block", "xcds"}
+    assert times and set(best) <= {"layout", "coop", "stack", "hold_min", "block", "xcds"}
+    assert {dict(k)["layout"] for k in times} == {0, 1}
     g.find_closest_points_device(sp, out)
     torch.cuda.synchronize()
     assert torch.equal(out.view(torch.int32), ref.view(torch.int32))  # misses carry UINT_MAX (NaN as float)

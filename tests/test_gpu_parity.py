@@ -223,7 +223,7 @@ def test_options_do_not_change_results(gpu, cache):
             assert_bit_equal(rec, ob.closest_points_gpuorder(q, r2))
         else:
             np.testing.assert_array_equal(rec, base, err_msg=str(cfg))
-    for key, val in dict(coop=1, stack=12, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
+    for key, val in dict(coop=1, stack=8, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
         g.set_option(key, val)
 
 
@@ -253,7 +253,7 @@ def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
             assert_bit_equal(rec, ref)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, ray_sort_dir=1, stack=12, sort=1, xcds=4, block=128,
+    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, ray_sort_dir=1, stack=8, sort=1, xcds=4, block=128,
                          morton_bits=24).items():
         g.set_option(key, val)
 
@@ -300,7 +300,7 @@ def test_every_query_of_a_ragged_batch_is_written(gpu, cache, n):
         rec = out.cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel()
         assert_bit_equal(rec, ref)
         np.testing.assert_array_equal(d.cpu().numpy(), ref["d"], err_msg=str(cfg))
-    for key, val in dict(coop=1, stack=12, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
+    for key, val in dict(coop=1, stack=8, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
         g.set_option(key, val)
     o, dr = meshes.rays_from(V, n, seed=12)
     ref = ob.intersect_gpuorder(o, dr)
