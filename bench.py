@@ -465,13 +465,13 @@ def run_torus(args, torch):
             log(f"[fwd+bwd] {t_fb * 1e3:.3f} ms/step, {nq / t_fb:.4e} queries/s ({t_fb * 1e3 / ms_step:.2f}x fwd)")
             # the adjoint's other query orders on the same step (same dL/dx, dL/dV up to summation order)
             alt = {}
-            for order, reduce in ((0, 0), (1, 0)):
+            for order, reduce in ((0, 0), (1, 0), (1, 1)):
                 gpu.set_option("bwd_order", order)
                 gpu.set_option("bwd_reduce", reduce)
                 alt[f"order{order}_reduce{reduce}_ms_per_step"] = timed(torch, stream, fwd_bwd, args.steps,
                                                                          args.warmup) * 1e3
             gpu.set_option("bwd_order", 1)
-            gpu.set_option("bwd_reduce", 1)
+            gpu.set_option("bwd_reduce", res["fwd_bwd"]["bwd_reduce"])
             res["fwd_bwd"]["other_adjoint_orders"] = alt
             log(f"[fwd+bwd] other adjoint orders: {alt}")
             # same closest points as the record-layout forward

@@ -59,7 +59,7 @@ struct fcpw_hip_ctx {
     // reason in scratch_err
     void *scratch = nullptr;
     std::string scratch_err;
-    int bwd_order = 1, bwd_reduce = 1; // adjoint: the forward's query order, wave pre-reduction
+    int bwd_order = 1, bwd_reduce = 2; // adjoint: the forward's query order, per-wave vertex sums in LDS
     float qlo[3] = {0, 0, 0}, qinv[3] = {1, 1, 1}; // Morton grid: 1.5x the scene box
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;

@@ -1572,10 +1572,13 @@ __device__ __forceinline__ void distance_backward_one(const float4 *prims, const
 // The adjoint over the queries in a given order -- the Morton order of the
 // preceding forward pass, so that a wave's queries read neighbouring primitives
 // and scatter into neighbouring vertices (entry k handles query order[k]).
-// With REDUCE, a run of consecutive lanes on the same primitive first sums its
+// REDUCE 1: a run of consecutive lanes on the same primitive first sums its
 // vertex terms (a segmented scan across the wave) and only the run's last lane
-// issues the atomics.
-template <int DIM, bool REDUCE>
+// issues the atomics.  REDUCE 2: every vertex term of the wave goes into a
+// per-wave hash table in LDS keyed by vertex id (LDS atomics), and each vertex
+// the wave touched gets one global atomic per coordinate.
+constexpr int BWD_HASH = 256; // entries per wave: more than the 64 x 3 keys a wave inserts
+template <int DIM, int REDUCE>
 __global__ __launch_bounds__(256) void k_distance_backward_ordered(const float4 *prims, const int32_t *prim_vids,
                                                                    const float *points, const int32_t *slots,
                                                                    const float *grad_d, int64_t n, int n_prims,
@@ -1593,8 +1596,38 @@ __global__ __launch_bounds__(256) void k_distance_backward_ordered(const float4 
             for (int c = 0; c < DIM; c++) grad_points[i * DIM + c] = T.g[c];
     }
     if (!grad_vertices) return;
-    if constexpr (!REDUCE) {
+    if constexpr (REDUCE == 0) {
         if (T.slot >= 0) adjoint_scatter<DIM>(T, grad_vertices);
+    } else if constexpr (REDUCE == 2) {
+        // one table per wave of the 256-thread block; every thread reaches both barriers
+        __shared__ uint32_t hkey[4][BWD_HASH];
+        __shared__ float hval[4][BWD_HASH][DIM];
+        const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+        for (int e = lane; e < BWD_HASH; e += 64) {
+            hkey[w][e] = 0xFFFFFFFFu;
+            for (int c = 0; c < DIM; c++) hval[w][e][c] = 0.0f;
+        }
+        __syncthreads();
+        if (T.slot >= 0) {
+            for (int v = 0; v < DIM; v++) {
+                const uint32_t key = (uint32_t)T.vid[v];
+                uint32_t h = (key * 2654435761u) >> 24; // 8 bits: BWD_HASH = 256
+                // at most 64 * DIM < BWD_HASH keys, so an empty entry is always found
+                while (true) {
+                    const uint32_t old = atomicCAS(&hkey[w][h], 0xFFFFFFFFu, key);
+                    if (old == 0xFFFFFFFFu || old == key) break;
+                    h = (h + 1u) & (BWD_HASH - 1);
+                }
+                for (int c = 0; c < DIM; c++) atomicAdd(&hval[w][h][c], T.c[v][c]);
+            }
+        }
+        __syncthreads();
+        for (int e = lane; e < BWD_HASH; e += 64) {
+            const uint32_t key = hkey[w][e];
+            if (key != 0xFFFFFFFFu)
+                for (int c = 0; c < DIM; c++)
+                    (void)unsafeAtomicAdd(&grad_vertices[(size_t)key * DIM + c], hval[w][e][c]);
+        }
     } else {
         // every lane of the wave takes part in the shuffles; a lane with nothing
         // to add carries slot -1 and zero terms

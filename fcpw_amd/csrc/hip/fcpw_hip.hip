@@ -1394,11 +1394,14 @@ int fcpw_hip_distance_backward_device(fcpw_hip_ctx *c, const float *points, cons
         if (!order)                                                                                           \
             hipLaunchKernelGGL(k_distance_backward<D>, dim3(blocks), dim3(threads), 0, s, c->prims, c->prim_vids, \
                                points, slot, grad_d, n, c->n_prims, grad_points, grad_vertices);                  \
-        else if (c->bwd_reduce)                                                                               \
-            hipLaunchKernelGGL((k_distance_backward_ordered<D, true>), dim3(blocks), dim3(threads), 0, s, c->prims, \
+        else if (c->bwd_reduce == 2)                                                                          \
+            hipLaunchKernelGGL((k_distance_backward_ordered<D, 2>), dim3(blocks), dim3(threads), 0, s, c->prims,  \
+                               c->prim_vids, points, slot, grad_d, n, c->n_prims, order, grad_points, grad_vertices); \
+        else if (c->bwd_reduce == 1)                                                                          \
+            hipLaunchKernelGGL((k_distance_backward_ordered<D, 1>), dim3(blocks), dim3(threads), 0, s, c->prims,  \
                                c->prim_vids, points, slot, grad_d, n, c->n_prims, order, grad_points, grad_vertices); \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_distance_backward_ordered<D, false>), dim3(blocks), dim3(threads), 0, s, c->prims, \
+            hipLaunchKernelGGL((k_distance_backward_ordered<D, 0>), dim3(blocks), dim3(threads), 0, s, c->prims,  \
                                c->prim_vids, points, slot, grad_d, n, c->n_prims, order, grad_points, grad_vertices); \
     } while (0)
     if (c->dim == 3) FCPW_BWD(3);
@@ -1429,7 +1432,8 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
             FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be %s", key, k == "sil_stack" ? "4, 8 or 12" : "8 or 12");
         (k == "sil_stack" ? c->sil_stack : c->inst_stack) = value;
     } else if (k == "bwd_order" || k == "bwd_reduce") {
-        if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be 0 or 1", key);
+        if (value != 0 && value != 1 && !(k == "bwd_reduce" && value == 2))
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be 0 or 1%s", key, k == "bwd_reduce" ? " or 2" : "");
         (k == "bwd_order" ? c->bwd_order : c->bwd_reduce) = value;
     } else if (k == "stack") {
         if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)

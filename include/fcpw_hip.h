@@ -392,9 +392,11 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "bwd_order" 1 = the adjoint walks the queries in the Morton order the latest
                  query call on this context computed, when that batch had the same
                  size (default); 0 = in the caller's order
-     "bwd_reduce"  1 = in that order, consecutive queries on the same primitive sum
-                 their vertex terms across the wave before the atomics (default)
-                 (both change only the FP32 summation order of grad_vertices)
+     "bwd_reduce"  how a wave of that order combines its vertex terms before the
+                 global FP32 atomics: 2 = sums per vertex in an LDS hash table, one
+                 atomic per vertex and coordinate (default); 1 = consecutive queries
+                 on the same primitive summed by a segmented wave scan; 0 = none
+                 (both options change only the FP32 summation order of grad_vertices)
    Measurement switches (off by default; results are unchanged):
      "stats"     1 = run the counting variant of the traversal kernels, which adds
                  the internal nodes expanded, primitives tested and queries to
