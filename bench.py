@@ -23,6 +23,17 @@ The printed JSON line carries
   instances    a scene of 256 transformed instances of a 20,480-triangle
                icosphere, 4,194,304 closest-point queries through the two-level
                kernel, against the same triangles baked into one mesh
+  end_to_end   GPUScene3D.find_closest_points(points, r2, hits) on the same
+               16M queries from host memory: packing, upload, ordering, kernel
+               and the copy of the records back into the list (median of 3)
+  star2d       the 2D workload: a 262,144-segment star polygon, 8,388,608
+               queries, plus rays and ten refit steps after vertex perturbations
+  diff         the differentiable loop: a 200,000-triangle geodesic sphere,
+               4,194,304 near-surface queries, 100 steps of vertex upload +
+               device refit + forward + adjoint + Adam
+  batch        256 meshes (1k-500k triangles, 1k-1M queries each, log-uniform)
+               in one persistent launch replayed from a HIP graph, against 256
+               separate launches; and the batch's fwd+bwd
   roofline     algorithmic bytes per query x queries per launch / the traversal
                kernel's mean launch duration (HIP events on the launch stream),
                against MI355X HBM bandwidth.  Bytes per query
@@ -30,9 +41,17 @@ The printed JSON line carries
                (primitives tested) counted by the kernel's own counting variant
                over the whole batch.  traffic = measured HBM bytes per launch from
                the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (null if absent)
-  cpu_baseline the reference CPU path (restatement of Scene<3>::findClosestPoints,
-               binary SAH BVH, static thread partition) timed on this host's cores
-               over a bounded sample of the same queries
+  cpu_baseline the reference CPU path timed on this host's cores over a strided
+               sample of the same queries (both halves), median of 5 runs:
+               "scalar", the restatement of Scene<3>::findClosestPoints with the
+               binary SAH BVH, and "vectorized", the MBVH aggregate (4-wide nodes,
+               SIMD leaf groups); each with the thread count, a 1-thread figure on
+               a smaller sample, and the host record (CPU model, sockets, SMT)
+
+Options: --legs selects the extra legs, --opt KEY=VALUE sets a GPUScene option on
+every scene, --workload runs one of the other configs alone, and the
+--summarize-* modes turn rocprofv3 databases into the text summaries kept under
+profiles/.
 """
 import argparse
 import json
@@ -221,6 +240,8 @@ def _db(path):
         if not found:
             raise SystemExit(f"no rocprofv3 database under {path}")
         path = found[0]
+    elif not os.path.isfile(path):
+        raise SystemExit(f"no rocprofv3 database at {path}")
     return sqlite3.connect(path), path
 
 
@@ -417,6 +438,10 @@ def run_torus(args, torch):
     fwd()
     I = gpu.info("stat_internal_nodes") / gpu.info("stat_queries")
     P = gpu.info("stat_prim_tests") / gpu.info("stat_queries")
+    waves = max(gpu.info("stat_waves"), 1)
+    log(f"[fwd] per wave: {gpu.info('stat_wave_iterations') / waves:.1f} loop iterations, "
+        f"{gpu.info('stat_leaf_passes') / waves:.1f} leaf passes, {gpu.info('stat_leaf_groups') / waves:.1f} "
+        f"64-item groups (one query per lane, no refill)")
     gpu.set_option("stats", 0)
     B = algorithmic_bytes(I, P)
     achieved = B * nq / kernel_s / 1e9

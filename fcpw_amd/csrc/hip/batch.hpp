@@ -111,19 +111,31 @@ __global__ __launch_bounds__(64) void k_batch_closest_coop(const BatchScene *T, 
     }
 }
 
+// The adjoint of the batch: one wave per 64 queries of one scene, taken in the
+// scene's Morton order when perm holds it (the permutation of the latest
+// forward; queries of a wave then share primitives and vertices), else in the
+// caller's order.  The wave's vertex terms are summed per vertex in LDS before
+// one global atomic per touched vertex and coordinate.
 template <int DIM>
 __global__ __launch_bounds__(64) void k_batch_distance_backward(const BatchScene *T, int n_scenes,
                                                                 const float *points, const int32_t *slots,
-                                                                const float *grad_d, float *grad_points,
-                                                                float *grad_vertices)
+                                                                const float *grad_d, const uint32_t *perm,
+                                                                float *grad_points, float *grad_vertices)
 {
+    __shared__ uint32_t hkey[BWD_HASH];
+    __shared__ float hval[BWD_HASH][DIM];
     const int t = batch_find(T, n_scenes, (int64_t)blockIdx.x, [](const BatchScene &b) { return b.c64_begin; });
     const BatchScene &B = T[t];
-    int64_t local = ((int64_t)blockIdx.x - B.c64_begin) * 64 + threadIdx.x;
-    if (local >= B.n_q) return;
-    distance_backward_one<DIM>(as_global(B.S.prims), as_global(B.prim_vids), B.n_prims, points, slots, grad_d,
-                               B.q_begin + local,
-                               grad_points, grad_vertices ? grad_vertices + (size_t)B.vert_offset * DIM : nullptr);
+    const int64_t local = ((int64_t)blockIdx.x - B.c64_begin) * 64 + threadIdx.x;
+    AdjointTerms<DIM> A;
+    A.slot = -1;
+    if (local < B.n_q) {
+        const int64_t i = perm ? (int64_t)perm[B.p_begin + local] : B.q_begin + local;
+        adjoint_terms<DIM>(as_global(B.S.prims), as_global(B.prim_vids), B.n_prims, points, slots, grad_d, i, A);
+        if (grad_points)
+            for (int c = 0; c < DIM; c++) grad_points[i * DIM + c] = A.g[c];
+    }
+    if (grad_vertices) adjoint_hash_scatter<DIM>(A, hkey, hval, grad_vertices + (size_t)B.vert_offset * DIM);
 }
 
 } // namespace fcpw_hip
@@ -138,6 +150,7 @@ struct fcpw_hip_batch {
     std::vector<uint64_t> generations;  // of the scenes when the table was built
     std::vector<int64_t> counts;        // queries per scene, caller order
     bool have_counts = false, dirty = true;
+    bool perm_valid = false; // perm holds a Morton order of the current table's query ranges
     std::vector<BatchScene> host_table; // table order
     BatchScene *table = nullptr;
     int64_t n_total = 0, n_items = 0, n_c64 = 0, n_vertices_total = 0;
@@ -153,6 +166,7 @@ struct fcpw_hip_batch {
 static int batch_table(fcpw_hip_batch *b)
 {
     const int n = (int)b->scenes.size();
+    b->perm_valid = false; // the scenes' ranges in the permutation may move
     std::vector<int> order(n);
     std::vector<double> cost(n);
     for (int s = 0; s < n; s++) {
@@ -256,6 +270,7 @@ static int batch_order(fcpw_hip_batch *b, const float *pts, int stride, hipStrea
                        b->n_total, hist, perm);
     HIPCHECK(b, hipGetLastError());
     *perm_out = perm;
+    b->perm_valid = true;
     return FCPW_HIP_OK;
 }
 
@@ -420,12 +435,14 @@ int fcpw_hip_batch_distance_backward_device(fcpw_hip_batch *b, const float *poin
     if (!points || !slot || !grad_d) FAIL(b, FCPW_HIP_ERR_INVALID_ARGUMENT, "batch_distance_backward_device: null input");
     hipStream_t s = (hipStream_t)stream;
     const int n = (int)b->scenes.size();
+    // any permutation of a scene's range serves; the latest forward's Morton order is the fast one
+    const uint32_t *perm = b->perm_valid ? (const uint32_t *)b->perm : nullptr;
     if (b->dim == 3)
         hipLaunchKernelGGL(k_batch_distance_backward<3>, dim3((unsigned)b->n_c64), dim3(64), 0, s, b->table, n,
-                           points, slot, grad_d, grad_points, grad_vertices);
+                           points, slot, grad_d, perm, grad_points, grad_vertices);
     else
         hipLaunchKernelGGL(k_batch_distance_backward<2>, dim3((unsigned)b->n_c64), dim3(64), 0, s, b->table, n,
-                           points, slot, grad_d, grad_points, grad_vertices);
+                           points, slot, grad_d, perm, grad_points, grad_vertices);
     HIPCHECK(b, hipGetLastError());
     return FCPW_HIP_OK;
 }

@@ -68,6 +68,15 @@ struct DevInstance {
 
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
 
+// min(a, b) of two squared distances: for floats in [+0, +inf] the bit
+// patterns order as unsigned integers, so an integer min gives fminf's value
+// (and a NaN b, whose pattern lies above +inf, leaves a, as fminf does)
+// without the NaN canonicalisation fminf compiles to
+__device__ __forceinline__ float fmin_sq(float a, float b)
+{
+    return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+}
+
 // ---------------------------------------------------------------------------
 // traversal stack: the NEWEST (up to) K entries live in LDS, lane-interleaved
 // (the entry of stack index i of lane t at lds[(i mod K) * BLOCK + t], so a
@@ -104,7 +113,7 @@ __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f)
             stack_spill[2 * sp] = l_;                                                                         \
             stack_spill[2 * sp + 1] = d_;                                                                     \
         } else {                                                                                              \
-            const int slot_ = ((K) > 0 ? sp % ((K) > 0 ? (K) : 1) : 0) * (BLOCK) + stack_lane;               \
+            const int slot_ = (int)((uint32_t)sp % (uint32_t)((K) > 0 ? (K) : 1)) * (BLOCK) + stack_lane;    \
             if (sp - stack_lo == (K)) {                                                                       \
                 const uint64_t old_ = lds_stack[slot_];                                                       \
                 stack_spill[2 * stack_lo] = (uint32_t)old_;                                                   \
@@ -120,7 +129,7 @@ __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f)
     do {                                                                                                      \
         sp--;                                                                                                 \
         if ((K) > 0 && sp >= stack_lo) {                                                                     \
-            const uint64_t e_ = lds_stack[((K) > 0 ? sp % ((K) > 0 ? (K) : 1) : 0) * (BLOCK) + stack_lane];  \
+            const uint64_t e_ = lds_stack[(int)((uint32_t)sp % (uint32_t)((K) > 0 ? (K) : 1)) * (BLOCK) + stack_lane]; \
             (LINK) = (uint32_t)e_;                                                                            \
             (D) = __uint_as_float((uint32_t)(e_ >> 32));                                                      \
         } else {                                                                                              \
@@ -642,7 +651,8 @@ __device__ __forceinline__ void coop_begin(const DevScene &S, CoopShared &sh, Co
     box_sqdist(lo, hi, x, d2min, d2max);
     L.x = x;
     L.active = valid && (d2min <= r2);
-    L.r2 = L.active ? fminf(r2, d2max) : r2;
+    // an active lane has r2 >= d2min >= +0; -0 becomes +0, which every comparison treats alike
+    L.r2 = L.active ? fmin_sq(r2 == 0.0f ? 0.0f : r2, d2max) : r2;
     L.cur_d2 = L.active ? d2min : 0.0f;
     L.live = L.active;
     L.holding = false;
@@ -682,10 +692,10 @@ __device__ __forceinline__ void coop_begin(const DevScene &S, CoopShared &sh, Co
                 float a0, a1, b0, b1;                                                                         \
                 box_sqdist(loA, hiA, (L).x, a0, a1);                                                          \
                 const bool oA = a0 <= (L).r2;                                                                 \
-                (L).r2 = fminf((L).r2, a1);                                                                   \
+                (L).r2 = fmin_sq((L).r2, a1);                                                                 \
                 box_sqdist(loB, hiB, (L).x, b0, b1);                                                          \
                 const bool oB = b0 <= (L).r2;                                                                 \
-                (L).r2 = fminf((L).r2, b1);                                                                   \
+                (L).r2 = fmin_sq((L).r2, b1);                                                                 \
                 /* continue with B when only B overlaps, or both do and B is strictly nearer; */            \
                 /* with neither, live turns false and the next step pops */                                 \
                 const bool bFirst = oB && (!oA || b0 < a0);                                                   \
@@ -751,7 +761,7 @@ __device__ __forceinline__ uint32_t coop_leaf_pass(const DevScene &S, CoopShared
                     L.n_prims++;
                     float d = sh.d[idx[jj]];
                     if (d * d <= L.r2) {
-                        L.r2 = fminf(L.r2, d * d);
+                        L.r2 = d * d; // min(r2, d * d) when d * d <= r2
                         L.best = (int32_t)(L.loff + j0 + jj);
                     }
                 }
@@ -1578,6 +1588,42 @@ __device__ __forceinline__ void distance_backward_one(const float4 *prims, const
 // per-wave hash table in LDS keyed by vertex id (LDS atomics), and each vertex
 // the wave touched gets one global atomic per coordinate.
 constexpr int BWD_HASH = 256; // entries per wave: more than the 64 x 3 keys a wave inserts
+
+// Sums the vertex terms of a wave's lanes per vertex id in the wave's LDS hash
+// table (hkey / hval, BWD_HASH entries), then adds each touched vertex to
+// grad_vertices with one global atomic per coordinate.  Every thread of the
+// block calls it (it holds two block barriers); a lane with T.slot < 0 adds
+// nothing.
+template <int DIM>
+__device__ __forceinline__ void adjoint_hash_scatter(const AdjointTerms<DIM> &T, uint32_t *hkey, float (*hval)[DIM],
+                                                     float *grad_vertices)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+    for (int e = lane; e < BWD_HASH; e += 64) {
+        hkey[e] = 0xFFFFFFFFu;
+        for (int c = 0; c < DIM; c++) hval[e][c] = 0.0f;
+    }
+    __syncthreads();
+    if (T.slot >= 0) {
+        for (int v = 0; v < DIM; v++) {
+            const uint32_t key = (uint32_t)T.vid[v];
+            uint32_t h = (key * 2654435761u) >> 24; // 8 bits: BWD_HASH = 256
+            // at most 64 * DIM < BWD_HASH keys, so an empty entry is always found
+            while (true) {
+                const uint32_t old = atomicCAS(&hkey[h], 0xFFFFFFFFu, key);
+                if (old == 0xFFFFFFFFu || old == key) break;
+                h = (h + 1u) & (BWD_HASH - 1);
+            }
+            for (int c = 0; c < DIM; c++) atomicAdd(&hval[h][c], T.c[v][c]);
+        }
+    }
+    __syncthreads();
+    for (int e = lane; e < BWD_HASH; e += 64) {
+        const uint32_t key = hkey[e];
+        if (key != 0xFFFFFFFFu)
+            for (int c = 0; c < DIM; c++) (void)unsafeAtomicAdd(&grad_vertices[(size_t)key * DIM + c], hval[e][c]);
+    }
+}
 template <int DIM, int REDUCE>
 __global__ __launch_bounds__(256) void k_distance_backward_ordered(const float4 *prims, const int32_t *prim_vids,
                                                                    const float *points, const int32_t *slots,
@@ -1599,35 +1645,11 @@ __global__ __launch_bounds__(256) void k_distance_backward_ordered(const float4 
     if constexpr (REDUCE == 0) {
         if (T.slot >= 0) adjoint_scatter<DIM>(T, grad_vertices);
     } else if constexpr (REDUCE == 2) {
-        // one table per wave of the 256-thread block; every thread reaches both barriers
+        // one table per wave of the 256-thread block
         __shared__ uint32_t hkey[4][BWD_HASH];
         __shared__ float hval[4][BWD_HASH][DIM];
-        const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
-        for (int e = lane; e < BWD_HASH; e += 64) {
-            hkey[w][e] = 0xFFFFFFFFu;
-            for (int c = 0; c < DIM; c++) hval[w][e][c] = 0.0f;
-        }
-        __syncthreads();
-        if (T.slot >= 0) {
-            for (int v = 0; v < DIM; v++) {
-                const uint32_t key = (uint32_t)T.vid[v];
-                uint32_t h = (key * 2654435761u) >> 24; // 8 bits: BWD_HASH = 256
-                // at most 64 * DIM < BWD_HASH keys, so an empty entry is always found
-                while (true) {
-                    const uint32_t old = atomicCAS(&hkey[w][h], 0xFFFFFFFFu, key);
-                    if (old == 0xFFFFFFFFu || old == key) break;
-                    h = (h + 1u) & (BWD_HASH - 1);
-                }
-                for (int c = 0; c < DIM; c++) atomicAdd(&hval[w][h][c], T.c[v][c]);
-            }
-        }
-        __syncthreads();
-        for (int e = lane; e < BWD_HASH; e += 64) {
-            const uint32_t key = hkey[w][e];
-            if (key != 0xFFFFFFFFu)
-                for (int c = 0; c < DIM; c++)
-                    (void)unsafeAtomicAdd(&grad_vertices[(size_t)key * DIM + c], hval[w][e][c]);
-        }
+        const int w = (int)(threadIdx.x >> 6);
+        adjoint_hash_scatter<DIM>(T, hkey[w], hval[w], grad_vertices);
     } else {
         // every lane of the wave takes part in the shuffles; a lane with nothing
         // to add carries slot -1 and zero terms

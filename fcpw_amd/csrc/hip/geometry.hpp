@@ -31,13 +31,27 @@ __host__ __device__ __forceinline__ v3 cross(v3 a, v3 b)
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// A value whose square is min(u, v)^2, for non-NaN u, v: on the device the
+// median of (u, v, -FLT_MAX), one v_med3_f32 that (unlike fminf) needs no NaN
+// canonicalisation of operands computed in another basic block.  It differs
+// from min(u, v) only when that is -inf, where it gives -FLT_MAX, whose square
+// rounds to +inf as well.
+__host__ __device__ __forceinline__ float min_med3(float u, float v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(u, v, -FLT_MAX);
+#else
+    return fminf(u, v);
+#endif
+}
+
 // BoundingBox.overlap (bounding-volumes.slang:43-53): squared distance to the
 // box (0 inside) and to its farthest corner
 __host__ __device__ __forceinline__ void box_sqdist(v3 lo, v3 hi, v3 c, float &d2min, float &d2max)
 {
     v3 u = sub(lo, c), v = sub(c, hi);
     v3 a = {fmaxf(fmaxf(u.x, v.x), 0.0f), fmaxf(fmaxf(u.y, v.y), 0.0f), fmaxf(fmaxf(u.z, v.z), 0.0f)};
-    v3 b = {fminf(u.x, v.x), fminf(u.y, v.y), fminf(u.z, v.z)};
+    v3 b = {min_med3(u.x, v.x), min_med3(u.y, v.y), min_med3(u.z, v.z)};
     d2min = dot(a, a);
     d2max = dot(b, b);
 }
@@ -45,7 +59,7 @@ __host__ __device__ __forceinline__ void box_sqdist(v2 lo, v2 hi, v2 c, float &d
 {
     v2 u = sub(lo, c), v = sub(c, hi);
     v2 a = {fmaxf(fmaxf(u.x, v.x), 0.0f), fmaxf(fmaxf(u.y, v.y), 0.0f)};
-    v2 b = {fminf(u.x, v.x), fminf(u.y, v.y)};
+    v2 b = {min_med3(u.x, v.x), min_med3(u.y, v.y)};
     d2min = dot(a, a);
     d2max = dot(b, b);
 }

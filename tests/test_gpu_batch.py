@@ -149,6 +149,49 @@ def test_batch_compact_forward_and_adjoint_equal_single(gpu, world3):
         a = b
 
 
+def test_batch_adjoint_in_morton_and_caller_order(gpu, world3):
+    """the adjoint walks each scene's queries in the latest forward's Morton
+    order, or in the caller's order when set_queries has invalidated that order
+    since: the same dL/dx bit for bit, dL/dV up to the FP32 summation order"""
+    items, q_all, sph = world3
+    n = len(q_all)
+    x = torch.from_numpy(q_all).cuda()
+    counts = [len(it[4]) for it in items]
+    batch = BatchedScenes([it[3] for it in items])
+    batch.set_queries(counts)
+    d = torch.empty(n, device="cuda")
+    slot = torch.empty(n, dtype=torch.int32, device="cuda")
+    prim = torch.empty(n, dtype=torch.int32, device="cuda")
+    batch.closest_points_compact_device(x, d, prim=prim, slot=slot)
+    wn = np.random.default_rng(6).uniform(0.5, 1.5, n).astype(np.float32)
+    w = torch.from_numpy(wn).cuda()
+    nv = int(batch.vertex_offsets[-1])
+    # per vertex, the sum of |terms| landing on it (each term is at most |w| in size):
+    # FP32 sums of one vertex in two orders differ by a few ulp of it
+    pr = prim.cpu().numpy()
+    abs_sum = np.zeros(nv)
+    a = 0
+    for k, it in enumerate(items):
+        b = a + len(it[4])
+        ok = pr[a:b] >= 0
+        ids = it[1][pr[a:b][ok]].ravel() + int(batch.vertex_offsets[k])
+        abs_sum += np.bincount(ids, np.repeat(wn[a:b][ok], 3), nv)
+        a = b
+    abs_sum = torch.from_numpy(abs_sum.astype(np.float32)).cuda()[:, None]
+    runs = []
+    for reset in (False, True):
+        if reset:
+            batch.set_queries(counts)  # the ranges are rebuilt: the order of the forward no longer applies
+        gx = torch.empty_like(x)
+        gV = torch.zeros((nv, 3), device="cuda")
+        batch.distance_backward_device(x, slot, w, gx, gV)
+        torch.cuda.synchronize()
+        runs.append((gx, gV))
+    assert torch.equal(runs[0][0], runs[1][0])
+    assert ((runs[0][1] - runs[1][1]).abs() <= 1e-5 * abs_sum + 1e-12).all()
+    assert (runs[0][1].abs() > 1e-5 * abs_sum).float().mean() > 0.5
+
+
 def test_batch_graph_replays_equal_eager(gpu, world3):
     items, q_all, sph = world3
     batch = BatchedScenes([it[3] for it in items])
