@@ -29,12 +29,12 @@ import torch
 __all__ = ["SCENE_CANDIDATES", "BATCH_CANDIDATES", "tune_scene", "tune_batch", "clear_cache"]
 
 SCENE_CANDIDATES = (
-    [dict(layout=L, coop=1, stack=k, hold_min=h, xcds=x) for L in (0, 1) for k in (8, 12, 16) for h in (1, 2, 4)
+    [dict(layout=L, coop=1, stack=k, hold_min=h, xcds=x) for L in (0, 1) for k in (4, 6, 8, 12) for h in (1, 2, 4)
      for x in (4, 8)]
     + [dict(layout=0, coop=0, stack=k, block=b, xcds=x) for k in (8, 16) for b in (64, 128) for x in (4, 8)]
 )
 BATCH_CANDIDATES = [dict(stack=k, hold_min=h, chunk=c, waves_per_cu=0)
-                    for k in (8, 12, 16) for h in (1, 2) for c in (64, 256)]
+                    for k in (6, 8, 12) for h in (1, 2) for c in (64, 256)]
 
 _cache = {}
 

@@ -158,9 +158,9 @@ struct fcpw_hip_batch {
     void *codes = nullptr, *perm = nullptr, *hist = nullptr, *sums = nullptr;
     size_t codes_bytes = 0, perm_bytes = 0, hist_bytes = 0, sums_bytes = 0;
     int n_cus = 256;
-    int occupancy[2][5][2] = {}; // resident waves per CU by [dim 2/3][stack 0/4/8/16/12][output], 0 = not queried
+    int occupancy[2][6][2] = {}; // resident waves per CU by [dim 2/3][stack 0/4/8/16/12/6][output], 0 = not queried
     // options
-    int stack = 8, hold_min = 1, chunk = 64, sort = 1, waves_per_cu = 0;
+    int stack = 6, hold_min = 1, chunk = 64, sort = 1, waves_per_cu = 0;
 };
 
 static int batch_table(fcpw_hip_batch *b)
@@ -280,7 +280,7 @@ static int batch_launch_k(fcpw_hip_batch *b, const CpIO &io, const uint32_t *per
     auto kern = k_batch_closest_coop<DIM, K, OUT>;
     int per_cu = b->waves_per_cu;
     if (per_cu <= 0) {
-        int &occ = b->occupancy[DIM - 2][K == 0 ? 0 : K == 4 ? 1 : K == 8 ? 2 : K == 16 ? 3 : 4][OUT];
+        int &occ = b->occupancy[DIM - 2][K == 0 ? 0 : K == 4 ? 1 : K == 8 ? 2 : K == 16 ? 3 : K == 12 ? 4 : 5][OUT];
         if (occ == 0) {
             int blocks = 0;
             HIPCHECK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, 64, 0));
@@ -307,12 +307,14 @@ static int batch_launch(fcpw_hip_batch *b, CpIO io, hipStream_t s)
     if (b->dim == 3) {
         if (k == 0) return batch_launch_k<3, 0, OUT>(b, io, perm, s);
         if (k == 4) return batch_launch_k<3, 4, OUT>(b, io, perm, s);
+        if (k == 6) return batch_launch_k<3, 6, OUT>(b, io, perm, s);
         if (k == 8) return batch_launch_k<3, 8, OUT>(b, io, perm, s);
         if (k == 12) return batch_launch_k<3, 12, OUT>(b, io, perm, s);
         return batch_launch_k<3, 16, OUT>(b, io, perm, s);
     }
     if (k == 0) return batch_launch_k<2, 0, OUT>(b, io, perm, s);
     if (k == 4) return batch_launch_k<2, 4, OUT>(b, io, perm, s);
+    if (k == 6) return batch_launch_k<2, 6, OUT>(b, io, perm, s);
     if (k == 8) return batch_launch_k<2, 8, OUT>(b, io, perm, s);
     if (k == 12) return batch_launch_k<2, 12, OUT>(b, io, perm, s);
     return batch_launch_k<2, 16, OUT>(b, io, perm, s);
@@ -452,8 +454,8 @@ int fcpw_hip_batch_set_option(fcpw_hip_batch *b, const char *key, int value)
     if (!b || !key) return FCPW_HIP_ERR_INVALID_ARGUMENT;
     std::string k(key);
     if (k == "stack") {
-        if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16)
-            FAIL(b, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 8, 12 or 16");
+        if (value != 0 && value != 4 && value != 6 && value != 8 && value != 12 && value != 16)
+            FAIL(b, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 6, 8, 12 or 16");
         b->stack = value;
     } else if (k == "hold_min") {
         if (value < 1 || value > 64) FAIL(b, FCPW_HIP_ERR_INVALID_ARGUMENT, "hold_min must be in [1, 64]");

@@ -64,7 +64,7 @@ struct fcpw_hip_ctx {
     // facts
     int n_records = 0, n_leaves = 0, n_prims = 0, n_vertices = 0, max_depth = 0;
     // options
-    int block = 128, stack = 8, sort = 1, xcds = 4, coop = 1, hold_min = 1, morton_bits = 24;
+    int block = 128, stack = 6, sort = 1, xcds = 4, coop = 1, hold_min = 1, morton_bits = 24;
     int xcd_chunk = 0; // log2 tiles per XCD chunk of the tile mapping (0: one range per XCD)
     int ray_coop = 1; // wave-cooperative leaves for ray queries
     int ray_sort_dir = 1; // rays ordered by direction octant, then origin

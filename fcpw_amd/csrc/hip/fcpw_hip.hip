@@ -980,10 +980,11 @@ static int launch_closest_kernel(fcpw_hip_ctx *c, const CpIO &io, hipStream_t s,
         dim3 g = grid_for(c, n, 64);
 #define FCPW_COOP(D, K) \
     hipLaunchKernelGGL((k_closest_points_coop<D, K, OUT, false>), g, dim3(64), 0, s, S, io, xcd_arg(c), c->hold_min)
-        const int k = c->stack; // 0, 4, 8, 12, 16 or 24 LDS entries (2D: 24 runs as 16)
+        const int k = c->stack; // 0, 4, 6, 8, 12, 16 or 24 LDS entries (2D: 24 runs as 16)
         if (c->dim == 3) {
             if (k == 0) FCPW_COOP(3, 0);
             else if (k == 4) FCPW_COOP(3, 4);
+            else if (k == 6) FCPW_COOP(3, 6);
             else if (k == 8) FCPW_COOP(3, 8);
             else if (k == 12) FCPW_COOP(3, 12);
             else if (k == 16) FCPW_COOP(3, 16);
@@ -991,6 +992,7 @@ static int launch_closest_kernel(fcpw_hip_ctx *c, const CpIO &io, hipStream_t s,
         } else {
             if (k == 0) FCPW_COOP(2, 0);
             else if (k == 4) FCPW_COOP(2, 4);
+            else if (k == 6) FCPW_COOP(2, 6);
             else if (k == 8) FCPW_COOP(2, 8);
             else if (k == 12) FCPW_COOP(2, 12);
             else FCPW_COOP(2, 16);
@@ -1436,8 +1438,8 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
             FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "%s must be 0 or 1%s", key, k == "bwd_reduce" ? " or 2" : "");
         (k == "bwd_order" ? c->bwd_order : c->bwd_reduce) = value;
     } else if (k == "stack") {
-        if (value != 0 && value != 4 && value != 8 && value != 12 && value != 16 && value != 24)
-            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 8, 12, 16 or 24");
+        if (value != 0 && value != 4 && value != 6 && value != 8 && value != 12 && value != 16 && value != 24)
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "stack must be 0, 4, 6, 8, 12, 16 or 24");
         c->stack = value;
     } else if (k == "sort") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "sort must be 0 or 1");

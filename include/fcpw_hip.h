@@ -349,7 +349,7 @@ int fcpw_hip_batch_closest_points_compact_device(fcpw_hip_batch *batch, const fl
 int fcpw_hip_batch_distance_backward_device(fcpw_hip_batch *batch, const float *points, const int32_t *slot,
                                             const float *grad_d, float *grad_points, float *grad_vertices,
                                             void *stream);
-/* Batch knobs (none changes a result bit): "stack" (0, 4, 8, 12, 16), "hold_min"
+/* Batch knobs (none changes a result bit): "stack" (0, 4, 6, 8, 12, 16; default 6), "hold_min"
    (1..64), "chunk" (queries per work item: 64, 128, 256, 512), "sort" (0, 1),
    "waves_per_cu" (0 = as many as fit, else 1..32). */
 int fcpw_hip_batch_set_option(fcpw_hip_batch *batch, const char *key, int value);
@@ -376,10 +376,10 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
                  along the Morton curve (default); 0 = by origin only
      "hold_min"  coop: lanes that must hold a leaf before the wave processes
                  leaves together (1..64, default 1)
-     "stack"     traversal-stack entries per lane kept in LDS (default 8): the newest
+     "stack"     traversal-stack entries per lane kept in LDS (default 6): the newest
                  entries, as a sliding window; older ones are spilled once to cached
-                 scratch: 0, 4, 8, 12, 16 or 24 (the 2D coop kernel uses 16 for 24,
-                 the one-query-per-lane kernel 8 for 0, 4 and 12)
+                 scratch: 0, 4, 6, 8, 12, 16 or 24 (the 2D coop kernel uses 16 for 24,
+                 the one-query-per-lane kernel 8 for 0, 4, 6 and 12)
      "block"     threads per block when coop = 0 (64, 128, 256)
      "sort"      1 = order queries along a Morton curve before traversal (default)
      "morton_bits"  cells of that ordering grid, 2^18, 2^21 or 2^24 (default 24)

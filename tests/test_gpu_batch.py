@@ -74,9 +74,10 @@ def single_reference(items, sph):
     return out.cpu().numpy()
 
 
-BATCH_OPTIONS = [dict(), dict(stack=0), dict(stack=4), dict(stack=12), dict(stack=16), dict(hold_min=2), dict(hold_min=64),
+BATCH_OPTIONS = [dict(), dict(stack=0), dict(stack=4), dict(stack=8), dict(stack=12), dict(stack=16), dict(hold_min=2),
+                 dict(hold_min=64),
                  dict(chunk=256), dict(chunk=512), dict(sort=0), dict(waves_per_cu=1), dict(waves_per_cu=3, chunk=128)]
-DEFAULTS = dict(stack=8, hold_min=1, chunk=64, sort=1, waves_per_cu=0)
+DEFAULTS = dict(stack=6, hold_min=1, chunk=64, sort=1, waves_per_cu=0)
 
 
 @pytest.mark.parametrize("world_name", ["world3", "world2"])

@@ -206,7 +206,7 @@ def test_options_do_not_change_results(gpu, cache):
     q = meshes.queries_around(V, 20000, seed=5)
     r2 = np.full(len(q), np.inf, np.float32)
     configs = [dict(coop=0, block=b, stack=k, sort=srt) for b in (64, 128, 256) for k in (8, 16, 24) for srt in (0, 1)]
-    configs += [dict(coop=1, stack=k, hold_min=h, sort=srt, xcds=x) for k in (0, 4, 8, 12, 16, 24)
+    configs += [dict(coop=1, stack=k, hold_min=h, sort=srt, xcds=x) for k in (0, 4, 6, 8, 12, 16, 24)
                 for h in (1, 2, 8, 64) for srt in (0, 1) for x in (1, 8)]
     configs += [dict(coop=1, stack=8, hold_min=1, sort=1, xcds=8, morton_bits=b) for b in (18, 21, 24)]
     # chunked tile mappings, both kernels
@@ -223,7 +223,7 @@ def test_options_do_not_change_results(gpu, cache):
             assert_bit_equal(rec, ob.closest_points_gpuorder(q, r2))
         else:
             np.testing.assert_array_equal(rec, base, err_msg=str(cfg))
-    for key, val in dict(coop=1, stack=8, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
+    for key, val in dict(coop=1, stack=6, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
         g.set_option(key, val)
 
 
@@ -253,7 +253,7 @@ def test_ray_options_do_not_change_results(gpu, cache, name, occlusion):
             assert_bit_equal(rec, ref)
         except AssertionError as e:
             raise AssertionError(f"{cfg}: {e}") from None
-    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, ray_sort_dir=1, stack=8, sort=1, xcds=4, block=128,
+    for key, val in dict(ray_coop=1, ray_stack=4, ray_hold_min=8, ray_sort_dir=1, stack=6, sort=1, xcds=4, block=128,
                          morton_bits=24).items():
         g.set_option(key, val)
 
@@ -300,7 +300,7 @@ def test_every_query_of_a_ragged_batch_is_written(gpu, cache, n):
         rec = out.cpu().numpy().view(fcpw.INTERACTION_DTYPE).ravel()
         assert_bit_equal(rec, ref)
         np.testing.assert_array_equal(d.cpu().numpy(), ref["d"], err_msg=str(cfg))
-    for key, val in dict(coop=1, stack=8, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
+    for key, val in dict(coop=1, stack=6, hold_min=1, sort=1, xcds=4, xcd_chunk=0, block=128, morton_bits=24).items():
         g.set_option(key, val)
     o, dr = meshes.rays_from(V, n, seed=12)
     ref = ob.intersect_gpuorder(o, dr)
@@ -389,7 +389,7 @@ def test_errors_are_reported(gpu):
                      ("bwd_reduce", -1)):
         with pytest.raises(ValueError):
             g.set_option(key, bad)
-    defaults = dict(stack=8, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
+    defaults = dict(stack=6, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
                     sil_stack=4, inst_stack=12, bwd_order=1, bwd_reduce=2,
                     sil_xcds=8, ray_sort_dir=1)
     assert {k: g.get_option(k) for k in defaults} == defaults
