@@ -84,7 +84,7 @@ __global__ __launch_bounds__(64) void k_batch_morton_hist(const BatchScene *T, i
 // left, so waves that drew short queries take more items and the batch ends
 // when the total work does, not when the slowest scene's launch would.
 template <int DIM, int K, int OUT>
-__global__ __launch_bounds__(64) void k_batch_closest_coop(const BatchScene *T, int n_scenes, CpIO io,
+__global__ __launch_bounds__(64) FCPW_EIGHT_WAVES void k_batch_closest_coop(const BatchScene *T, int n_scenes, CpIO io,
                                                            const uint32_t *perm, uint32_t *queue, int64_t n_items,
                                                            int chunk_q, int hold_min)
 {

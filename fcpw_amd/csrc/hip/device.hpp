@@ -68,6 +68,13 @@ struct DevInstance {
 
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
 
+// Asks the compiler to fit a kernel in 64 VGPRs, so that a SIMD can hold eight
+// of its waves.  Applied only where the kernel fits without spilling registers
+// to scratch (the private segment size of the build stays that of the
+// traversal stack) and measured faster: the cooperative ray kernel went from
+// 71 VGPRs and the cooperative silhouette kernel from 65.
+#define FCPW_EIGHT_WAVES __attribute__((amdgpu_waves_per_eu(8, 8)))
+
 // min(a, b) of two squared distances: for floats in [+0, +inf] the bit
 // patterns order as unsigned integers, so an integer min gives fminf's value
 // (and a NaN b, whose pattern lies above +inf, leaves a, as fminf does)
@@ -1032,7 +1039,7 @@ __device__ __forceinline__ uint32_t ray_coop_leaf_pass(const DevScene &S, CoopSh
 }
 
 template <int DIM, int K, bool STATS>
-__global__ __launch_bounds__(64) void k_intersect_coop(DevScene S, const float2 *rays, int64_t n, int occlusion,
+__global__ __launch_bounds__(64) FCPW_EIGHT_WAVES void k_intersect_coop(DevScene S, const float2 *rays, int64_t n, int occlusion,
                                                        float2 *out, const uint32_t *order, int xcds, int hold_min)
 {
     using G = Geo<DIM>;

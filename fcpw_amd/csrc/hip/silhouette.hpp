@@ -313,7 +313,7 @@ struct SilShared {
 };
 
 template <int DIM, int K>
-__global__ __launch_bounds__(64) void k_closest_silhouettes_coop(DevScene S, SilScene T, const float4 *spheres,
+__global__ __launch_bounds__(64) FCPW_EIGHT_WAVES void k_closest_silhouettes_coop(DevScene S, SilScene T, const float4 *spheres,
                                                                  const uint32_t *flip, int64_t n, float r2min,
                                                                  float prec, float2 *out, const uint32_t *order,
                                                                  int xcds, int hold_min)
