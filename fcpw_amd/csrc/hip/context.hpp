@@ -45,9 +45,13 @@ struct fcpw_hip_ctx {
     int inst_obj_depth = 0; // deepest object tree
     std::vector<int32_t> node_slot;     // flat node -> child slot (record * 2 + side), root = 2 * n_records
     std::vector<int32_t> node_leaf_off; // flat node -> first leaf slot, -1 for internal nodes
-    // staging for the host-pointer entry points
+    // staging for the host-pointer entry points: device buffers, and pinned host
+    // bounce buffers with one event per chunk of a device-to-host copy
     void *stage_in = nullptr, *stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    void *pin_in = nullptr, *pin_out = nullptr;
+    size_t pin_in_bytes = 0, pin_out_bytes = 0;
+    std::vector<hipEvent_t> pin_ev;
     // query ordering workspace (grown on demand, kept for the context's lifetime)
     void *ws_codes = nullptr, *ws_perm = nullptr, *ws_hist = nullptr, *ws_sums = nullptr;
     size_t ws_codes_bytes = 0, ws_perm_bytes = 0, ws_hist_bytes = 0, ws_sums_bytes = 0;
@@ -155,6 +159,120 @@ static int grow(fcpw_hip_ctx *c, void **buf, size_t *cap, size_t bytes)
     *cap = 0;
     HIPCHECK(c, hipMalloc(buf, bytes));
     *cap = bytes;
+    return FCPW_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host <-> device copies of query and result arrays for the host-pointer entry
+// points.  Caller memory is pageable, so a copy goes through pinned bounce
+// buffers of the context (hipHostMalloc, grown on demand): the array is cut
+// into chunks, host threads copy the chunks between the caller's memory and
+// the bounce buffer, and each chunk's DMA overlaps the host copies of the
+// others.  Small arrays take a plain hipMemcpy.
+// ---------------------------------------------------------------------------
+constexpr size_t STAGE_CHUNK = (size_t)4 << 20;
+
+// host threads for the bounce copies: the CPUs this process may run on, at most 16
+static int stage_threads()
+{
+    static const int n = [] {
+        int k = 0;
+        cpu_set_t set;
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) k = CPU_COUNT(&set);
+        if (k < 1) k = (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(k, 16));
+    }();
+    return n;
+}
+
+static int pin_grow(fcpw_hip_ctx *c, void **buf, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return FCPW_HIP_OK;
+    if (*buf) (void)hipHostFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    HIPCHECK(c, hipHostMalloc(buf, bytes, hipHostMallocDefault));
+    *cap = bytes;
+    return FCPW_HIP_OK;
+}
+
+// runs body(k) for chunks k = 0 .. n_chunks - 1 on the stage threads (chunk k on
+// thread k mod threads, in increasing k); returns the first nonzero status
+template <typename F>
+static int stage_parallel(int64_t n_chunks, F &&body)
+{
+    const int nt = (int)std::min<int64_t>(stage_threads(), n_chunks);
+    std::atomic<int> status{0};
+    auto work = [&](int t) {
+        for (int64_t k = t; k < n_chunks; k += nt) {
+            const int st = body(k);
+            if (st) {
+                int expected = 0;
+                status.compare_exchange_strong(expected, st);
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &t : th) t.join();
+    return status.load();
+}
+
+// host src -> device dst, stream-ordered on s before whatever is launched after it
+static int stage_copy_in(fcpw_hip_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (bytes <= STAGE_CHUNK) {
+        HIPCHECK(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return FCPW_HIP_OK;
+    }
+    // the bounce buffer may still feed the DMAs of an earlier copy queued on s
+    HIPCHECK(c, hipStreamSynchronize(s));
+    int st = pin_grow(c, &c->pin_in, &c->pin_in_bytes, bytes);
+    if (st) return st;
+    const int64_t n_chunks = (int64_t)((bytes + STAGE_CHUNK - 1) / STAGE_CHUNK);
+    char *pin = (char *)c->pin_in;
+    st = stage_parallel(n_chunks, [&](int64_t k) {
+        const size_t off = (size_t)k * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+        memcpy(pin + off, (const char *)src + off, len);
+        return hipMemcpyAsync((char *)dst + off, pin + off, len, hipMemcpyHostToDevice, s) == hipSuccess
+                   ? FCPW_HIP_OK
+                   : FCPW_HIP_ERR_HIP;
+    });
+    if (st) FAIL(c, st, "host-to-device copy of %zu bytes failed", bytes);
+    return FCPW_HIP_OK;
+}
+
+// device src -> host dst after the work already queued on s; returns when dst holds it
+static int stage_copy_out(fcpw_hip_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (bytes <= STAGE_CHUNK) {
+        HIPCHECK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HIPCHECK(c, hipStreamSynchronize(s));
+        return FCPW_HIP_OK;
+    }
+    int st = pin_grow(c, &c->pin_out, &c->pin_out_bytes, bytes);
+    if (st) return st;
+    const int64_t n_chunks = (int64_t)((bytes + STAGE_CHUNK - 1) / STAGE_CHUNK);
+    while ((int64_t)c->pin_ev.size() < n_chunks) {
+        hipEvent_t e = nullptr;
+        HIPCHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->pin_ev.push_back(e);
+    }
+    char *pin = (char *)c->pin_out;
+    for (int64_t k = 0; k < n_chunks; k++) {
+        const size_t off = (size_t)k * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+        HIPCHECK(c, hipMemcpyAsync(pin + off, (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+        HIPCHECK(c, hipEventRecord(c->pin_ev[k], s));
+    }
+    st = stage_parallel(n_chunks, [&](int64_t k) {
+        const size_t off = (size_t)k * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - off);
+        if (hipEventSynchronize(c->pin_ev[k]) != hipSuccess) return (int)FCPW_HIP_ERR_HIP;
+        memcpy((char *)dst + off, pin + off, len);
+        return (int)FCPW_HIP_OK;
+    });
+    if (st) FAIL(c, st, "device-to-host copy of %zu bytes failed", bytes);
     return FCPW_HIP_OK;
 }
 

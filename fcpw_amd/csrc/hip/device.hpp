@@ -69,10 +69,11 @@ struct DevInstance {
 __device__ __forceinline__ uint32_t link_of(float f) { return __float_as_uint(f); }
 
 // Asks the compiler to fit a kernel in 64 VGPRs, so that a SIMD can hold eight
-// of its waves.  Applied only where the kernel fits without spilling registers
-// to scratch (the private segment size of the build stays that of the
-// traversal stack) and measured faster: the cooperative ray kernel went from
-// 71 VGPRs and the cooperative silhouette kernel from 65.
+// of its waves.  Applied where that measured faster: the cooperative ray kernel
+// (from 71 VGPRs) and the cooperative silhouette kernel (from 65) fit without
+// spilling registers to scratch, the batch kernel (from 79) with 32 bytes of
+// spills per lane.  The instanced ray kernel (101 VGPRs) spilled 144 bytes per
+// lane and ran 2.2x slower, so it goes without.
 #define FCPW_EIGHT_WAVES __attribute__((amdgpu_waves_per_eu(8, 8)))
 
 // min(a, b) of two squared distances: for floats in [+0, +inf] the bit

@@ -3,7 +3,12 @@
 // adjoint, options), from batch.hpp the batched multi-scene launch and from
 // instances.hpp scenes of several objects and transformed instances.
 // Device code lives in device.hpp, the per-scene context in context.hpp.
+#include <sched.h>
+
+#include <algorithm>
+#include <atomic>
 #include <mutex>
+#include <thread>
 
 #include "device.hpp"
 #include "silhouette.hpp"
@@ -325,6 +330,9 @@ int fcpw_hip_destroy(fcpw_hip_ctx *c)
     void *ws[] = {c->stage_in, c->stage_out, c->ws_codes, c->ws_perm, c->ws_hist, c->ws_sums};
     for (void *b : ws)
         if (b) (void)hipFree(b);
+    if (c->pin_in) (void)hipHostFree(c->pin_in);
+    if (c->pin_out) (void)hipHostFree(c->pin_out);
+    for (hipEvent_t e : c->pin_ev) (void)hipEventDestroy(e);
     for (void *b : c->build_buf)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->ev_begin) (void)hipEventDestroy(e);
@@ -1177,10 +1185,11 @@ int fcpw_hip_closest_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *sph
     if (st) return st;
     st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), hipMemcpyHostToDevice));
+    st = stage_copy_in(c, c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), 0);
+    if (st) return st;
     st = launch_closest_records(c, c->stage_in, n, record_normals, c->stage_out, 0);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), 0))) return st;
     return check_overflow(c);
 }
 
@@ -1209,10 +1218,11 @@ int fcpw_hip_intersect(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_t n, int
     if (st) return st;
     st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(c->stage_in, rays, (size_t)n * sizeof(fcpw_hip_ray), hipMemcpyHostToDevice));
+    st = stage_copy_in(c, c->stage_in, rays, (size_t)n * sizeof(fcpw_hip_ray), 0);
+    if (st) return st;
     st = launch_intersect(c, c->stage_in, n, occlusion, c->stage_out, 0);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), 0))) return st;
     return check_overflow(c);
 }
 
@@ -1262,11 +1272,11 @@ int fcpw_hip_intersect_spheres(fcpw_hip_ctx *c, const fcpw_hip_bounding_sphere *
     if (st) return st;
     st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, sb, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy((char *)c->stage_in + sb, rand_nums, rb, hipMemcpyHostToDevice));
+    if ((st = stage_copy_in(c, c->stage_in, spheres, sb, 0))) return st;
+    if ((st = stage_copy_in(c, (char *)c->stage_in + sb, rand_nums, rb, 0))) return st;
     st = launch_spheres(c, c->stage_in, (const float *)((char *)c->stage_in + sb), n, c->stage_out, 0);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), 0))) return st;
     return FCPW_HIP_OK;
 }
 
@@ -1348,12 +1358,12 @@ int fcpw_hip_closest_silhouette_points(fcpw_hip_ctx *c, const fcpw_hip_bounding_
     if (st) return st;
     st = grow(c, &c->stage_out, &c->stage_out_bytes, (size_t)n * sizeof(fcpw_hip_interaction));
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, sb, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy((char *)c->stage_in + sb, flip, fb, hipMemcpyHostToDevice));
+    if ((st = stage_copy_in(c, c->stage_in, spheres, sb, 0))) return st;
+    if ((st = stage_copy_in(c, (char *)c->stage_in + sb, flip, fb, 0))) return st;
     st = launch_silhouettes(c, c->stage_in, (const uint32_t *)((char *)c->stage_in + sb), n, squared_min_radius,
                             precision, c->stage_out, 0);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, (size_t)n * sizeof(fcpw_hip_interaction), 0))) return st;
     return FCPW_HIP_OK;
 }
 

@@ -1382,13 +1382,13 @@ int fcpw_hip_closest_points_objects(fcpw_hip_ctx *c, const fcpw_hip_bounding_sph
     st = grow(c, &c->stage_out, &c->stage_out_bytes, rec + (size_t)n * sizeof(int32_t));
     if (st) return st;
     int32_t *obj_dev = object_index ? (int32_t *)((char *)c->stage_out + rec) : nullptr;
-    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), hipMemcpyHostToDevice));
+    if ((st = stage_copy_in(c, c->stage_in, spheres, (size_t)n * sizeof(fcpw_hip_bounding_sphere), 0))) return st;
     st = fcpw_hip_closest_points_objects_device(c, (const fcpw_hip_bounding_sphere *)c->stage_in, n, record_normals,
                                                 (fcpw_hip_interaction *)c->stage_out, obj_dev, nullptr);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, rec, hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, rec, 0))) return st;
     if (object_index)
-        HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if ((st = stage_copy_out(c, object_index, obj_dev, (size_t)n * sizeof(int32_t), 0))) return st;
     return check_overflow(c);
 }
 
@@ -1423,13 +1423,13 @@ int fcpw_hip_intersect_objects(fcpw_hip_ctx *c, const fcpw_hip_ray *rays, int64_
     st = grow(c, &c->stage_out, &c->stage_out_bytes, rec + (size_t)n * sizeof(int32_t));
     if (st) return st;
     int32_t *obj_dev = object_index ? (int32_t *)((char *)c->stage_out + rec) : nullptr;
-    HIPCHECK(c, hipMemcpy(c->stage_in, rays, (size_t)n * sizeof(fcpw_hip_ray), hipMemcpyHostToDevice));
+    if ((st = stage_copy_in(c, c->stage_in, rays, (size_t)n * sizeof(fcpw_hip_ray), 0))) return st;
     st = fcpw_hip_intersect_objects_device(c, (const fcpw_hip_ray *)c->stage_in, n, occlusion,
                                            (fcpw_hip_interaction *)c->stage_out, obj_dev, nullptr);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, rec, hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, rec, 0))) return st;
     if (object_index)
-        HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if ((st = stage_copy_out(c, object_index, obj_dev, (size_t)n * sizeof(int32_t), 0))) return st;
     return check_overflow(c);
 }
 
@@ -1471,15 +1471,15 @@ int fcpw_hip_intersect_spheres_objects(fcpw_hip_ctx *c, const fcpw_hip_bounding_
     st = grow(c, &c->stage_out, &c->stage_out_bytes, rec + (size_t)n * sizeof(int32_t));
     if (st) return st;
     int32_t *obj_dev = object_index ? (int32_t *)((char *)c->stage_out + rec) : nullptr;
-    HIPCHECK(c, hipMemcpy(c->stage_in, spheres, sb, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy((char *)c->stage_in + sb, rand_nums, rb, hipMemcpyHostToDevice));
+    if ((st = stage_copy_in(c, c->stage_in, spheres, sb, 0))) return st;
+    if ((st = stage_copy_in(c, (char *)c->stage_in + sb, rand_nums, rb, 0))) return st;
     st = fcpw_hip_intersect_spheres_objects_device(c, (const fcpw_hip_bounding_sphere *)c->stage_in,
                                                    (const fcpw_hip_float3 *)((char *)c->stage_in + sb), n,
                                                    (fcpw_hip_interaction *)c->stage_out, obj_dev, nullptr);
     if (st) return st;
-    HIPCHECK(c, hipMemcpy(out, c->stage_out, rec, hipMemcpyDeviceToHost));
+    if ((st = stage_copy_out(c, out, c->stage_out, rec, 0))) return st;
     if (object_index)
-        HIPCHECK(c, hipMemcpy(object_index, obj_dev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if ((st = stage_copy_out(c, object_index, obj_dev, (size_t)n * sizeof(int32_t), 0))) return st;
     return FCPW_HIP_OK;
 }
 

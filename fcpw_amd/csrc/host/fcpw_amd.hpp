@@ -515,7 +515,8 @@ public:
                    std::vector<GPUInteraction> &interactions, bool checkForOcclusion = false,
                    std::vector<int32_t> *objectIndices = nullptr)
     {
-        std::vector<GPURay> rays((size_t)n);
+        std::vector<GPURay> &rays = rayBuf_;
+        rays.resize((size_t)n);
         parallel_rows(n, [&](int64_t s, int64_t e) {
             for (int64_t i = s; i < e; i++) {
                 const float *o = rayOrigins + i * DIM, *d = rayDirections + i * DIM;
@@ -531,7 +532,7 @@ public:
                    bool checkForOcclusion = false, std::vector<int32_t> *objectIndices = nullptr)
     {
         require();
-        interactions.assign(rays.size(), GPUInteraction());
+        interactions.resize(rays.size()); // the device writes every record, hit or miss
         if (objectIndices) objectIndices->assign(rays.size(), -1);
         hip_check(ctx_,
                   fcpw_hip_intersect_objects(ctx_, (const fcpw_hip_ray *)rays.data(), (int64_t)rays.size(),
@@ -545,7 +546,8 @@ public:
                           int64_t n, std::vector<GPUInteraction> &interactions,
                           std::vector<int32_t> *objectIndices = nullptr)
     {
-        std::vector<GPUBoundingSphere> spheres((size_t)n);
+        std::vector<GPUBoundingSphere> &spheres = sphereBuf_;
+        spheres.resize((size_t)n);
         std::vector<float3> rnd((size_t)n);
         parallel_rows(n, [&](int64_t s, int64_t e) {
             for (int64_t i = s; i < e; i++) {
@@ -564,7 +566,7 @@ public:
         if (randNums.size() != spheres.size())
             throw std::invalid_argument("intersect(spheres): one rand_nums entry per sphere");
         require();
-        interactions.assign(spheres.size(), GPUInteraction());
+        interactions.resize(spheres.size()); // the device writes every record, hit or miss
         if (objectIndices) objectIndices->assign(spheres.size(), -1);
         hip_check(ctx_,
                   fcpw_hip_intersect_spheres_objects(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
@@ -580,7 +582,8 @@ public:
                            std::vector<GPUInteraction> &interactions, bool recordNormals = false,
                            std::vector<int32_t> *objectIndices = nullptr)
     {
-        std::vector<GPUBoundingSphere> spheres((size_t)n);
+        std::vector<GPUBoundingSphere> &spheres = sphereBuf_;
+        spheres.resize((size_t)n);
         parallel_rows(n, [&](int64_t s, int64_t e) {
             for (int64_t i = s; i < e; i++) {
                 const float *q = queryPoints + i * DIM;
@@ -595,7 +598,7 @@ public:
                            bool recordNormals = false, std::vector<int32_t> *objectIndices = nullptr)
     {
         require();
-        interactions.assign(spheres.size(), GPUInteraction());
+        interactions.resize(spheres.size()); // the device writes every record, hit or miss
         if (objectIndices) objectIndices->assign(spheres.size(), -1);
         hip_check(ctx_,
                   fcpw_hip_closest_points_objects(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
@@ -611,7 +614,8 @@ public:
                                      std::vector<GPUInteraction> &interactions, float squaredMinRadius = 0.0f,
                                      float precision = 1e-3f)
     {
-        std::vector<GPUBoundingSphere> spheres((size_t)n);
+        std::vector<GPUBoundingSphere> &spheres = sphereBuf_;
+        spheres.resize((size_t)n);
         std::vector<uint32_t> flip((size_t)n);
         parallel_rows(n, [&](int64_t s, int64_t e) {
             for (int64_t i = s; i < e; i++) {
@@ -631,7 +635,7 @@ public:
         if (flipNormalOrientation.size() != spheres.size())
             throw std::invalid_argument("findClosestSilhouettePoints: one flipNormalOrientation entry per sphere");
         require();
-        interactions.assign(spheres.size(), GPUInteraction());
+        interactions.resize(spheres.size()); // the device writes every record, hit or miss
         hip_check(ctx_,
                   fcpw_hip_closest_silhouette_points(ctx_, (const fcpw_hip_bounding_sphere *)spheres.data(),
                                                      flipNormalOrientation.data(), (int64_t)spheres.size(),
@@ -650,6 +654,11 @@ public:
     bool instanced() const { return instanced_; }
 
 private:
+    // query records packed from caller arrays, kept between calls so that a
+    // repeated call of the same size reuses memory it has already touched
+    std::vector<GPUBoundingSphere> sphereBuf_;
+    std::vector<GPURay> rayBuf_;
+
     // fcpw_hip_build_bvh into ctx; with exportTree the tree is returned as the
     // host build would have made it
     FlatBvh<DIM> buildTree(fcpw_hip_ctx *ctx, const std::vector<float> &positions, const std::vector<int32_t> &indices,
