@@ -75,7 +75,7 @@ struct fcpw_hip_ctx {
     int ray_stack = 4, ray_hold_min = 8; // the cooperative ray kernels' LDS stack entries and leaf-pass threshold
     int sil_stack = 4, inst_stack = 12;  // LDS stack entries of the 3D cooperative silhouette and instance kernels
     int layout = 0;                      // record order in memory: 0 DFS, 1 treelets of three levels
-    int sil_hold_min = 8;                // the cooperative silhouette kernel's leaf-pass threshold
+    int sil_hold_min = 3;                // the cooperative silhouette kernel's leaf-pass threshold
     int sil_xcds = 8;                    // the silhouette kernels' XCD grouping
     int stats_on = 0, timing = 0;
     int n_cus = 256;

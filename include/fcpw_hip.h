@@ -363,7 +363,7 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "ray_coop"  the same choice for ray queries (default 1)
      "ray_stack", "ray_hold_min"  the cooperative ray kernels' LDS stack entries (4, 8
                  or 16, default 4) and leaf-pass threshold (1..64, default 8)
-     "sil_hold_min"  the cooperative silhouette kernel's leaf-pass threshold (default 8)
+     "sil_hold_min"  the cooperative silhouette kernel's leaf-pass threshold (default 3)
      "layout"    the order of the tree's records in HBM: 0 = DFS order of the internal
                  nodes (default), 1 = treelets (a node, its internal children and
                  grandchildren contiguous, treelets in DFS order); setting it on an

@@ -389,7 +389,7 @@ def test_errors_are_reported(gpu):
                      ("bwd_reduce", -1)):
         with pytest.raises(ValueError):
             g.set_option(key, bad)
-    defaults = dict(stack=6, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=8,
+    defaults = dict(stack=6, hold_min=1, xcds=4, xcd_chunk=0, ray_stack=4, ray_hold_min=8, sil_hold_min=3,
                     sil_stack=4, inst_stack=12, bwd_order=1, bwd_reduce=2,
                     sil_xcds=8, ray_sort_dir=1)
     assert {k: g.get_option(k) for k in defaults} == defaults

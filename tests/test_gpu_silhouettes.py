@@ -201,7 +201,7 @@ def test_cooperative_and_per_lane_kernels_give_the_same_bits(gpu, name):
 
     for kw in ({}, dict(squared_min_radius=r2min, precision=2e-3)):
         base = run(dict(coop=0), **kw)
-        for cfg in (dict(coop=1, sil_hold_min=1), dict(coop=1, sil_hold_min=8),
+        for cfg in (dict(coop=1, sil_hold_min=1), dict(coop=1, sil_hold_min=3), dict(coop=1, sil_hold_min=8),
                     dict(coop=1, sil_hold_min=64, sort=0), dict(coop=1, sil_stack=8, sort=1),
                     dict(coop=1, sil_stack=12, sil_hold_min=1), dict(sil_stack=4)):
             np.testing.assert_array_equal(run(cfg, **kw), base, err_msg=f"{cfg} {kw}")
@@ -211,4 +211,4 @@ def test_cooperative_and_per_lane_kernels_give_the_same_bits(gpu, name):
     g.refit(s)
     np.testing.assert_array_equal(run(dict(coop=1, sil_hold_min=1)), run(dict(coop=0)))
     g.set_option("coop", 1)
-    g.set_option("sil_hold_min", 8)
+    g.set_option("sil_hold_min", 3)
