@@ -30,8 +30,11 @@ EXT = os.path.join(HERE, "_fcpw_amd" + sysconfig.get_config_var("EXT_SUFFIX"))
 # -ffp-contract=off: every product and sum rounded separately, so the kernels
 # reproduce the reference arithmetic step for step (and match the oracle bit
 # for bit).  -packed-fp32-ops: no packed FP32 instructions are generated.
+# -Wno-pass-failed: the eight-waves occupancy hint cannot be met by the kernel
+# instances whose LDS stack window is too large for eight waves per SIMD; the
+# compiler then keeps its own register budget, which is the intended outcome.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-pass-failed",
              f"--offload-arch={ARCH}"]
 
 

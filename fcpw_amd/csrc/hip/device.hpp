@@ -1028,7 +1028,7 @@ __device__ __forceinline__ uint32_t ray_coop_leaf_pass(const DevScene &S, CoopSh
                             L.active = false;
                             break;
                         }
-                        L.tmax = fminf(L.tmax, t);
+                        L.tmax = t; // min(tmax, t) when t <= tmax
                     }
                 }
             }

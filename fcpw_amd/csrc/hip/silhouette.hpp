@@ -430,7 +430,7 @@ __global__ __launch_bounds__(64) FCPW_EIGHT_WAVES void k_closest_silhouettes_coo
                         if (index == best || r2min >= r2) continue;
                         const float d = sh.d[idx[jj]];
                         if (d * d <= r2) {
-                            r2 = fminf(r2, d * d);
+                            r2 = d * d; // min(r2, d * d) when d * d <= r2
                             best = index;
                             best_rec = (int32_t)(loff + j0 + jj);
                         }
