@@ -28,12 +28,15 @@ def torus():
     return V, F, s, g
 
 
-def test_closest_points_through_a_reused_list(gpu, torus):
+@pytest.mark.parametrize("n", [262_144, 262_145, N])
+def test_closest_points_through_a_reused_list(gpu, torus, n):
+    """262,144 spheres fill exactly one 4 MiB chunk (a plain copy in), 262,145
+    add a 16-byte second chunk; the records (40 bytes each) span several chunks"""
     V, F, s, g = torus
     ob = oracle.Bvh(V, F)
-    qa = meshes.queries_around(V, N, seed=91)
-    qb = meshes.queries_around(V, N, seed=92)
-    r2 = np.full(N, np.inf, np.float32)
+    qa = meshes.queries_around(V, n, seed=91)
+    qb = meshes.queries_around(V, n, seed=92)
+    r2 = np.full(n, np.inf, np.float32)
     r2[::3] = 1e-3
     hits = fcpw.GPUInteractionList()
     g.find_closest_points(qa, r2, hits)
@@ -43,13 +46,13 @@ def test_closest_points_through_a_reused_list(gpu, torus):
     assert_bit_equal(as_records(hits), ref)
     assert (ref["index"] < 0).any() and (ref["index"] >= 0).any()
     # the device entry point on the same queries
-    sph = np.zeros((N, 4), np.float32)
+    sph = np.zeros((n, 4), np.float32)
     sph[:, :3] = qb
     sph[:, 3] = r2
-    out = torch.empty((N, 10), dtype=torch.float32, device="cuda")
+    out = torch.empty((n, 10), dtype=torch.float32, device="cuda")
     g.find_closest_points_device(torch.from_numpy(sph).cuda(), out)
     torch.cuda.synchronize()
-    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), as_records(hits).view(np.uint32).reshape(N, 10))
+    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), as_records(hits).view(np.uint32).reshape(n, 10))
 
 
 def test_rays_and_occlusion_through_a_reused_list(gpu, torus):
