@@ -214,7 +214,13 @@ static int stage_parallel(int64_t n_chunks, F &&body)
         }
     };
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    int started = 1;
+    try {
+        for (; started < nt; started++) th.emplace_back(work, started);
+    } catch (...) {
+        // no more threads: this thread also takes the shares that found none
+    }
+    for (int t = started; t < nt; t++) work(t);
     work(0);
     for (auto &t : th) t.join();
     return status.load();
