@@ -737,6 +737,12 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     t_i1 = timed(torch, stream, inst_step, args.steps, args.warmup)
     gi.set_option("coop", 1)
     assert torch.equal(first.view(torch.int32), out_i.view(torch.int32)), "instance kernels differ"
+    # one launch with the caller's radii (no lowering from each group's first query): same records
+    seed_group = gi.get_option("inst_seed")
+    gi.set_option("inst_seed", 0)
+    t_i0 = timed(torch, stream, inst_step, args.steps, args.warmup)
+    gi.set_option("inst_seed", seed_group)
+    assert torch.equal(first.view(torch.int32), out_i.view(torch.int32)), "lowered radii changed a record"
     del first
     t_f = timed(torch, stream, flat_step, args.steps, args.warmup)
     # the baked mesh through the one-query-per-lane kernel, the instanced kernels' structure
@@ -759,7 +765,8 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     del first, rays
     log(f"[instances] {n_queries} rays: {t_r * 1e3:.3f} ms ({n_queries / t_r:.4e} rays/s; one ray per lane "
         f"{t_r1 * 1e3:.3f} ms), {ray_hits:.3f} hit")
-    log(f"[instances] one query per lane: {t_i1 * 1e3:.3f} ms")
+    log(f"[instances] one query per lane: {t_i1 * 1e3:.3f} ms; one launch with the caller's radii "
+        f"(inst_seed 0): {t_i0 * 1e3:.3f} ms")
     log(f"[instances] {n_inst} instances x {len(F)} tris: two-level {t_i * 1e3:.3f} ms "
         f"({n_queries / t_i:.4e} q/s) vs baked mesh {t_f * 1e3:.3f} ms ({n_queries / t_f:.4e} q/s; "
         f"one query per lane {t_f1 * 1e3:.3f} ms); "
@@ -767,6 +774,7 @@ def run_instances(args, torch, fcpw, synth, dev, stream, n_inst=256, n_queries=1
     return {"value": n_queries / t_i, "unit": "queries/s", "ms_per_step": t_i * 1e3,
             "baked_mesh_value": n_queries / t_f, "baked_mesh_ms_per_step": t_f * 1e3,
             "baked_mesh_per_lane_ms_per_step": t_f1 * 1e3, "per_lane_ms_per_step": t_i1 * 1e3,
+            "inst_seed": seed_group, "one_launch_ms_per_step": t_i0 * 1e3,
             "rays": {"value": n_queries / t_r, "unit": "rays/s", "ms_per_step": t_r * 1e3,
                      "per_lane_ms_per_step": t_r1 * 1e3, "hit_fraction": ray_hits,
                      "workload": f"{n_queries} rays, origins uniform in [-9, 9]^3, directions uniform, tMax=inf"},

@@ -77,6 +77,7 @@ struct fcpw_hip_ctx {
     int layout = 0;                      // record order in memory: 0 DFS, 1 treelets of three levels
     int sil_hold_min = 3;                // the cooperative silhouette kernel's leaf-pass threshold
     int sil_xcds = 8;                    // the silhouette kernels' XCD grouping
+    int inst_seed = 1024;                // instanced closest points: ordered queries per representative wave (0: off)
     int stats_on = 0, timing = 0;
     int n_cus = 256;
     // device scratch of fcpw_hip_build_bvh, kept (grow-only) for the next build

@@ -1478,6 +1478,10 @@ int fcpw_hip_set_option(fcpw_hip_ctx *c, const char *key, int value)
     } else if (k == "ray_sort_dir") {
         if (value != 0 && value != 1) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "ray_sort_dir must be 0 or 1");
         c->ray_sort_dir = value;
+    } else if (k == "inst_seed") {
+        if (value != 0 && (value < 64 || value > (1 << 20)))
+            FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "inst_seed must be 0 or in [64, 2^20]");
+        c->inst_seed = value;
     } else if (k == "xcd_chunk") {
         if (value < 0 || value > 16) FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "xcd_chunk must be in [0, 16]");
         c->xcd_chunk = value;
@@ -1520,6 +1524,7 @@ int fcpw_hip_get_option(const fcpw_hip_ctx *c, const char *key, int *value)
     else if (k == "xcd_chunk") *value = c->xcd_chunk;
     else if (k == "ray_sort_dir") *value = c->ray_sort_dir;
     else if (k == "sil_xcds") *value = c->sil_xcds;
+    else if (k == "inst_seed") *value = c->inst_seed;
     else if (k == "morton_bits") *value = c->morton_bits;
     else if (k == "hold_min") *value = c->hold_min;
     else if (k == "ray_coop") *value = c->ray_coop;

@@ -118,23 +118,96 @@ __device__ __forceinline__ bool instance_closest(const DevInstance *inst, const 
     return true;
 }
 
+// Closest points on a scene of instances run as two launches over the queries'
+// Morton order when "inst_seed" = B > 0.  The order is cut into blocks of B
+// queries; the first launch answers, per block, the 64 consecutive queries in
+// its middle (the whole block when it is shorter: its representatives), with
+// their own radii and one coherent wave per block.  The second launch answers
+// the other queries, each with its squared radius lowered to
+//   U = ((d_rep + |x - x_rep|) (1 + 1e-3) + 1e-4 (1 + max_k |x_k|))^2,
+// x_rep being the representative nearest in the order and d_rep its reported
+// distance (no lowering when that query found nothing).  The records are those
+// of the reference's walk:
+//  - a lower starting radius never adds an acceptance.  Along the reference's
+//    visit order the lowered walk's radius stays at or below the reference's
+//    (it skips only nodes whose box distance exceeds its own radius, and those
+//    nodes' bounds and primitives could not lower it further), and every
+//    acceptance test is monotone in the radius: the object walk's radius is the
+//    world radius mapped by xf_r2, whose rounded steps are each monotone;
+//  - the reference reports its last acceptance W, so the lowered walk reports W
+//    when it still accepts W, i.e. when U maps to at least W's squared distance
+//    in its object's space.  By the triangle inequality a point of the scene lies
+//    within d_rep + |x - x_rep| of x (up to the rounding of the mapped points,
+//    a few units in the last place of the coordinates), so W's distance, which
+//    is the smallest to that rounding, is below the unmargined bound; the 1e-3
+//    relative and 1e-4 (1 + |x|) absolute margins exceed that rounding and the
+//    rounding of xf_r2 by orders of magnitude.
+// A lowered radius prunes most of the object walks that the reference's
+// box-ordered instance walk starts with loose radii.
+//
+// Block b's representatives: positions [r0, r0 + nr) of the order.
+__device__ __forceinline__ void seed_reps(int64_t b, int block, int64_t n, int64_t &r0, int64_t &nr)
+{
+    const int64_t len = std::min<int64_t>(block, n - b * block);
+    nr = std::min<int64_t>(64, len);
+    r0 = b * block + (len - nr) / 2;
+}
+
+// The ordered position lane k answers in launch `pass` (false: nothing to answer).
+__device__ __forceinline__ bool seed_position(int64_t k, int64_t n_launch, int64_t n, int block, int pass,
+                                              int64_t &pos)
+{
+    pos = k;
+    if (k >= n_launch) return false;
+    if (!block) return true;
+    int64_t r0, nr;
+    if (pass == 0) {
+        seed_reps(k >> 6, block, n, r0, nr);
+        pos = r0 + (k & 63);
+        return (k & 63) < nr;
+    }
+    seed_reps(k / block, block, n, r0, nr);
+    return k < r0 || k >= r0 + nr;
+}
+
+template <int DIM>
+__device__ __forceinline__ float seeded_r2(const float4 *spheres, const float2 *out, const uint32_t *order,
+                                           int64_t pos, int64_t n, int block, float r2, typename Geo<DIM>::vec x)
+{
+    int64_t r0, nr;
+    seed_reps(pos / block, block, n, r0, nr);
+    const uint32_t rep = order[std::min(std::max(pos, r0), r0 + nr - 1)];
+    const float dr = out[(size_t)rep * 5 + 4].x;
+    if (!(dr < FLT_MAX)) return r2;
+    const float4 sr = spheres[rep];
+    float q[3];
+    store_p(x, q);
+    const float e0 = q[0] - sr.x, e1 = q[1] - sr.y, e2 = DIM == 3 ? q[2] - sr.z : 0.0f;
+    const float m = fmaxf(fmaxf(fabsf(q[0]), fabsf(q[1])), DIM == 3 ? fabsf(q[2]) : 0.0f);
+    const float b = (dr + sqrtf(e0 * e0 + e1 * e1 + e2 * e2)) * 1.001f + 1e-4f * (1.0f + m);
+    return fminf(r2, b * b);
+}
+
 // __launch_bounds__(BLOCK, 6): six waves per SIMD (80 VGPRs); measured 36 ms
 // against 42 ms unbounded (113 VGPRs, four waves) on 256 instances
 template <int DIM, int BLOCK, int K, int KT>
 __global__ __launch_bounds__(BLOCK, 6) void k_instances_closest(DevScene T, const DevInstance *inst, const DevScene *objs,
                                                              int n_inst, const float4 *spheres, int64_t n,
                                                              int record_normals, float2 *out, int32_t *object_out,
-                                                             const uint32_t *order, int xcds)
+                                                             const uint32_t *order, int xcds, int64_t n_all,
+                                                             int block, int pass)
 {
     using G = Geo<DIM>;
     using vec = typename G::vec;
     const int64_t k = xcd_tile(xcds) * BLOCK + threadIdx.x;
-    if (k >= n) return;
-    const int64_t i = order ? (int64_t)order[k] : k;
+    int64_t pos;
+    if (!seed_position(k, n, n_all, block, pass, pos)) return;
+    const int64_t i = order ? (int64_t)order[pos] : pos;
     const float4 s4 = spheres[i];
     const float q[3] = {s4.x, s4.y, s4.z};
     const vec x = G::load_point(q);
     float r2 = s4.w;
+    if (block && pass == 1) r2 = seeded_r2<DIM>(spheres, out, order, pos, n_all, block, r2, x);
     int best_inst = -1, best_obj = -1;
     int32_t best_slot = -1;
     float best_d = FLT_MAX, bp[3] = {0.0f, 0.0f, 0.0f}, buv[2] = {0.0f, 0.0f};
@@ -244,7 +317,8 @@ __global__ __launch_bounds__(64) void k_instances_closest_coop(DevScene T, const
                                                                const DevScene *objs, int n_inst,
                                                                const float4 *spheres, int64_t n, int record_normals,
                                                                float2 *out, int32_t *object_out,
-                                                               const uint32_t *order, int xcds, int hold_min)
+                                                               const uint32_t *order, int xcds, int hold_min,
+                                                               int64_t n_all, int block, int pass)
 {
     using G = Geo<DIM>;
     using vec = typename G::vec;
@@ -252,8 +326,9 @@ __global__ __launch_bounds__(64) void k_instances_closest_coop(DevScene T, const
     __shared__ CoopShared sh;
     const int lane = threadIdx.x & 63;
     const int64_t k = xcd_tile(xcds) * 64 + threadIdx.x;
-    const bool valid = k < n;
-    const int64_t i = valid ? (order ? (int64_t)order[k] : k) : 0;
+    int64_t pos;
+    const bool valid = seed_position(k, n, n_all, block, pass, pos);
+    const int64_t i = valid ? (order ? (int64_t)order[pos] : pos) : 0;
     float q[3] = {0.0f, 0.0f, 0.0f};
     float r2 = -1.0f;
     if (valid) {
@@ -262,6 +337,7 @@ __global__ __launch_bounds__(64) void k_instances_closest_coop(DevScene T, const
         r2 = s4.w;
     }
     const vec x = G::load_point(q);
+    if (valid && block && pass == 1) r2 = seeded_r2<DIM>(spheres, out, order, pos, n_all, block, r2, x);
     int best_inst = -1;
     int32_t best_slot = -1;
     // traversal state of either level
@@ -1183,34 +1259,36 @@ static int launch_instances_closest(fcpw_hip_ctx *c, const void *spheres, int64_
     if (st) return st;
     if ((st = timing_begin(c, s))) return st;
     DevScene T = dev_scene(c);
-    if (c->coop && c->max_depth + c->inst_obj_depth < STACK_TOTAL) {
-        // one stack serves both levels, so the two trees' depths must fit it together
-        const dim3 g = grid_for(c, n, 64);
-        if (c->dim == 3 && c->inst_stack == 8)
-            hipLaunchKernelGGL((k_instances_closest_coop<3, 8>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
-                               c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
-                               order, xcd_arg(c), c->hold_min);
-        else if (c->dim == 3)
-            hipLaunchKernelGGL((k_instances_closest_coop<3, INST_COOP_K3>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
-                               c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
-                               order, xcd_arg(c), c->hold_min);
-        else
-            hipLaunchKernelGGL((k_instances_closest_coop<2, INST_K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,
-                               c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out, object_out,
-                               order, xcd_arg(c), c->hold_min);
+    // with ordered queries and inst_seed = B: each block's representatives, then
+    // the rest with radii lowered from them (seeded_r2); otherwise one launch
+    const int block = order && c->inst_seed > 0 ? c->inst_seed : 0;
+    const bool coop = c->coop && c->max_depth + c->inst_obj_depth < STACK_TOTAL;
+    for (int pass = block ? 0 : 1; pass < 2; pass++) {
+        const int64_t m = pass == 0 ? (n + block - 1) / block * 64 : n;
+        if (coop) {
+            // one stack serves both levels, so the two trees' depths must fit it together
+            const dim3 g = grid_for(c, m, 64);
+#define FCPW_ICOOP(D, K)                                                                                       \
+    hipLaunchKernelGGL((k_instances_closest_coop<D, K>), g, dim3(64), 0, s, T, c->inst, c->obj_table,          \
+                       c->n_instances, (const float4 *)spheres, m, record_normals, (float2 *)out, object_out, \
+                       order, xcd_arg(c), c->hold_min, n, block, pass)
+            if (c->dim == 3 && c->inst_stack == 8) FCPW_ICOOP(3, 8);
+            else if (c->dim == 3) FCPW_ICOOP(3, INST_COOP_K3);
+            else FCPW_ICOOP(2, INST_K);
+#undef FCPW_ICOOP
+        } else {
+            const dim3 g = grid_for(c, m, INST_BLOCK);
+            if (c->dim == 3)
+                hipLaunchKernelGGL((k_instances_closest<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,
+                                   c->inst, c->obj_table, c->n_instances, (const float4 *)spheres, m, record_normals,
+                                   (float2 *)out, object_out, order, xcd_arg(c), n, block, pass);
+            else
+                hipLaunchKernelGGL((k_instances_closest<2, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T,
+                                   c->inst, c->obj_table, c->n_instances, (const float4 *)spheres, m, record_normals,
+                                   (float2 *)out, object_out, order, xcd_arg(c), n, block, pass);
+        }
         HIPCHECK(c, hipGetLastError());
-        return timing_end(c, s);
     }
-    dim3 g = grid_for(c, n, INST_BLOCK);
-    if (c->dim == 3)
-        hipLaunchKernelGGL((k_instances_closest<3, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,
-                           c->obj_table, c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out,
-                           object_out, order, xcd_arg(c));
-    else
-        hipLaunchKernelGGL((k_instances_closest<2, INST_BLOCK, INST_K, INST_KT>), g, dim3(INST_BLOCK), 0, s, T, c->inst,
-                           c->obj_table, c->n_instances, (const float4 *)spheres, n, record_normals, (float2 *)out,
-                           object_out, order, xcd_arg(c));
-    HIPCHECK(c, hipGetLastError());
     return timing_end(c, s);
 }
 

@@ -389,6 +389,12 @@ int fcpw_hip_batch_get_option(const fcpw_hip_batch *batch, const char *key, int 
      "sil_xcds"  the same for the silhouette kernels (default 8)
      "xcd_chunk" 0 = one contiguous range per group (default); c > 0 = groups take
                  chunks of 2^c tiles in turn (0..16)
+     "inst_seed" closest points on a scene of instances with ordered queries run
+                 as two launches: per block of B queries of the Morton order, the
+                 64 in its middle first, then the others with their radius lowered
+                 to an upper bound from the nearest of those (its distance plus the
+                 distance between the two queries, with margins), which never
+                 changes a record; B = 1024 by default, 64..2^20, 0 = one launch
      "bwd_order" 1 = the adjoint walks the queries in the Morton order the latest
                  query call on this context computed, when that batch had the same
                  size (default); 0 = in the caller's order

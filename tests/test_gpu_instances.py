@@ -104,6 +104,40 @@ def test_cooperative_and_per_lane_ray_kernels_equal_oracle(gpu, name, occlusion)
         g.set_option(key, val)
 
 
+@pytest.mark.parametrize("name", scenes.NAMES + ["crowd3d"])
+def test_lowered_radii_give_the_oracle_records(gpu, name):
+    """inst_seed = B: per block of B ordered queries the 64 in its middle
+    first, then the others with their radius lowered to a bound from the
+    nearest of those; for every B (every query a representative, blocks that
+    are not a multiple of 64, a last partial block), both kernels, radii
+    unbounded, bounded and too small to find anything (representatives that
+    miss), the records are the oracle's, which runs the reference's walk with
+    the caller's radii"""
+    objs, tr, s, g, bv, I = setup(name)
+    q = scenes.queries(objs, tr, 20000, seed=59)
+    rng = np.random.default_rng(60)
+    r2 = np.full(len(q), np.inf, np.float32)
+    r2[1::3] = rng.uniform(0.0, 0.25, len(r2[1::3])).astype(np.float32)
+    r2[2::7] = np.float32(1e-8)
+    ref, ref_obj = I.closest_points(q, r2, record_normal=True)
+    assert (ref_obj < 0).any() and (ref_obj >= 0).any()
+    try:
+        for coop in (1, 0):
+            for seed in (0, 64, 100, 1024, 1 << 20):
+                g.set_option("coop", coop)
+                g.set_option("inst_seed", seed)
+                hits = fcpw.GPUInteractionList()
+                obj = np.full(len(q), -7, np.int32)
+                g.find_closest_points(q, r2, hits, record_normals=True, object_indices=obj)
+                try:
+                    assert_hits_equal(records(hits), obj, ref, ref_obj)
+                except AssertionError as e:
+                    raise AssertionError(f"coop={coop} inst_seed={seed}: {e}") from None
+    finally:
+        g.set_option("coop", 1)
+        g.set_option("inst_seed", 1024)
+
+
 @pytest.mark.parametrize("name", ["mixed3d", "many3d", "mixed2d"])
 def test_bounded_radii_equal_oracle(gpu, name):
     objs, tr, s, g, bv, I = setup(name)
