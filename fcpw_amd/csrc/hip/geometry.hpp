@@ -100,60 +100,70 @@ __host__ __device__ __forceinline__ bool box_ray(v2 lo, v2 hi, v2 o, v2 d, v2 di
 }
 
 // findClosestPointTriangle (geometry.slang:251-332): Ericson's region test;
-// t = barycentric weights of (pa, pb)
+// t = barycentric weights of (pa, pb).  The regions are tested in the
+// reference's order with its expressions; the edge and face regions share one
+// division (num / den with the region's operands: v = d1 / (d1 - d3),
+// w = d2 / (d2 - d6), w = (d4 - d3) / ((d4 - d3) + (d5 - d6)),
+// denom = 1 / (va + vb + vc)) and every region one distance, so a wave whose
+// lanes fall in several regions runs each of these once.
 __host__ __device__ __forceinline__ float closest_point_triangle(v3 pa, v3 pb, v3 pc, v3 x, v3 &p, float &t0, float &t1)
 {
-    v3 ab = sub(pb, pa), ac = sub(pc, pa), ax = sub(x, pa);
-    float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    const v3 ab = sub(pb, pa), ac = sub(pc, pa), ax = sub(x, pa);
+    const float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    // region: 0 vertex (p set), 1 edge ab, 2 edge ac, 3 edge bc, 4 face
+    int region = 0;
+    float num = 1.0f, den = 1.0f, vb = 0.0f, vc = 0.0f;
     if (d1 <= 0.0f && d2 <= 0.0f) {
         t0 = 1.0f; t1 = 0.0f; p = pa;
-        v3 r = sub(x, p);
-        return sqrtf(dot(r, r));
+    } else {
+        const v3 bx = sub(x, pb);
+        const float d3 = dot(ab, bx), d4 = dot(ac, bx);
+        if (d3 >= 0.0f && d4 <= d3) {
+            t0 = 0.0f; t1 = 1.0f; p = pb;
+        } else {
+            const v3 cx = sub(x, pc);
+            const float d5 = dot(ab, cx), d6 = dot(ac, cx);
+            if (d6 >= 0.0f && d5 <= d6) {
+                t0 = 0.0f; t1 = 0.0f; p = pc;
+            } else {
+                vc = d1 * d4 - d3 * d2;
+                if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+                    region = 1; num = d1; den = d1 - d3;
+                } else {
+                    vb = d5 * d2 - d1 * d6;
+                    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+                        region = 2; num = d2; den = d2 - d6;
+                    } else {
+                        const float va = d3 * d6 - d5 * d4;
+                        if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+                            region = 3; num = d4 - d3; den = (d4 - d3) + (d5 - d6);
+                        } else {
+                            region = 4; den = va + vb + vc;
+                        }
+                    }
+                }
+            }
+        }
     }
-    v3 bx = sub(x, pb);
-    float d3 = dot(ab, bx), d4 = dot(ac, bx);
-    if (d3 >= 0.0f && d4 <= d3) {
-        t0 = 0.0f; t1 = 1.0f; p = pb;
-        v3 r = sub(x, p);
-        return sqrtf(dot(r, r));
+    if (region) {
+        const float q = num / den;
+        if (region == 1) {
+            t0 = 1.0f - q; t1 = q;
+            p = {pa.x + ab.x * q, pa.y + ab.y * q, pa.z + ab.z * q};
+        } else if (region == 2) {
+            t0 = 1.0f - q; t1 = 0.0f;
+            p = {pa.x + ac.x * q, pa.y + ac.y * q, pa.z + ac.z * q};
+        } else if (region == 3) {
+            t0 = 0.0f; t1 = 1.0f - q;
+            const v3 bc = sub(pc, pb);
+            p = {pb.x + bc.x * q, pb.y + bc.y * q, pb.z + bc.z * q};
+        } else {
+            const float v = vb * q, w = vc * q;
+            t0 = 1.0f - v - w; t1 = v;
+            p = {(pa.x + ab.x * v) + ac.x * w, (pa.y + ab.y * v) + ac.y * w, (pa.z + ab.z * v) + ac.z * w};
+        }
     }
-    v3 cx = sub(x, pc);
-    float d5 = dot(ab, cx), d6 = dot(ac, cx);
-    if (d6 >= 0.0f && d5 <= d6) {
-        t0 = 0.0f; t1 = 0.0f; p = pc;
-        v3 r = sub(x, p);
-        return sqrtf(dot(r, r));
-    }
-    float vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-        float v = d1 / (d1 - d3);
-        t0 = 1.0f - v; t1 = v;
-        p = {pa.x + ab.x * v, pa.y + ab.y * v, pa.z + ab.z * v};
-        v3 r = sub(x, p);
-        return sqrtf(dot(r, r));
-    }
-    float vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-        float w = d2 / (d2 - d6);
-        t0 = 1.0f - w; t1 = 0.0f;
-        p = {pa.x + ac.x * w, pa.y + ac.y * w, pa.z + ac.z * w};
-        v3 r = sub(x, p);
-        return sqrtf(dot(r, r));
-    }
-    float va = d3 * d6 - d5 * d4;
-    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-        t0 = 0.0f; t1 = 1.0f - w;
-        v3 bc = sub(pc, pb);
-        p = {pb.x + bc.x * w, pb.y + bc.y * w, pb.z + bc.z * w};
-        v3 r = sub(x, p);
-        return sqrtf(dot(r, r));
-    }
-    float denom = 1.0f / (va + vb + vc);
-    float v = vb * denom, w = vc * denom;
-    t0 = 1.0f - v - w; t1 = v;
-    p = {(pa.x + ab.x * v) + ac.x * w, (pa.y + ab.y * v) + ac.y * w, (pa.z + ab.z * v) + ac.z * w};
-    v3 r = sub(x, p);
+    const v3 r = sub(x, p);
     return sqrtf(dot(r, r));
 }
 
