@@ -725,7 +725,7 @@ int fcpw_hip_upload_silhouettes(fcpw_hip_ctx *c, int n_nodes, const fcpw_hip_bou
         const fcpw_hip_bounding_cone &k = cones[i];
         const int slot = c->node_slot[i];
         cone_buf[(size_t)slot * 2] = make_float4(k.axis.x, k.axis.y, k.axis.z, k.half_angle);
-        cone_buf[(size_t)slot * 2 + 1] = make_float4(k.radius, 0.0f, 0.0f, 0.0f);
+        cone_buf[(size_t)slot * 2 + 1] = cone_aux(k.axis.x, k.axis.y, k.axis.z, k.half_angle, k.radius);
         const int32_t cnt = n_silhouettes[i], off = silhouette_offset[i];
         if (cnt < 0 || off < 0 || (int64_t)off + cnt > n_refs)
             FAIL(c, FCPW_HIP_ERR_INVALID_ARGUMENT, "upload_silhouettes: node %d silhouette range out of bounds", i);

@@ -4,8 +4,10 @@
 //
 // Device layout (next to the closest-point records of device.hpp):
 //   cones     per record two child entries, then one entry for the root; an
-//             entry is two float4 {axis.xyz, halfAngle} {radius, 0, 0, 0}
-//             (GPUBoundingCone, bvh_interop_structures.h:41-52)
+//             entry is two float4 {axis.xyz, halfAngle} {radius, sin halfAngle,
+//             cos halfAngle, max(1, |axis|)} (GPUBoundingCone,
+//             bvh_interop_structures.h:41-52, plus three values of its own
+//             for cone_overlap_bounded)
 //   sil_leaf  per leaf, indexed by the leaf's first primitive slot: uint2
 //             {first silhouette record, count}
 //   sils      silhouette records in reference order (a silhouette shared by two
@@ -69,6 +71,95 @@ __device__ __forceinline__ bool cone_overlap(float4 cone, float radius, v3 o, v3
     d = project_to_plane(v, e);
     const float hs = half + atan2f(d, s);
     return hs >= PI_2_F ? true : in_range(PI_2_F, d_axis - hs, d_axis + hs);
+}
+
+// The second float4 of a cone entry (written by the upload and by the device
+// cone refit): the reference's radius, then sin and cos of the half angle and
+// max(1, |axis|) (NaN for a NaN axis, which sends every test to the reference form).
+__host__ __device__ inline float4 cone_aux(float ax, float ay, float az, float half, float radius)
+{
+    const float n = sqrtf(ax * ax + ay * ay + az * az);
+    return make_float4(radius, sinf(half), cosf(half), n <= 1.0f ? 1.0f : n);
+}
+
+// BoundingCone.overlap decided without inverse trigonometry or division where
+// the decision is clear by a margin.  With v = c - o, l = |v| and
+// cc = dot(axis, v / l), the reference accepts (past its two early accepts)
+// iff hs = half + beta >= pi/2 or |acos(cc) - pi/2| <= hs, where
+//   beta = asin(radius / l)                         when l > radius,
+//   beta = atan2(d, s), s = l - dot(e, |v / l|),     when l <= radius
+//          d = projectToPlane(v / l, e)             (accept at once if s <= 0);
+// its first test, |acos(cc) - pi/2| <= half, is implied, also in floats, as
+// hs >= half and rounding is monotone.  In sines, with sh = sin(half),
+// ch = cos(half):
+//   l > radius, w = sqrt(l^2 - radius^2):
+//     hs >= pi/2            <=>  radius - l ch >= 0
+//     asin|cc| <= hs < pi/2 <=>  (sh w + ch radius) - |dot(axis, v)| >= 0
+//   l <= radius, rho = sqrt(d^2 + s^2):
+//     s <= 0                <=>  l^2 - dot(e, |v|) <= 0
+//     hs >= pi/2            <=>  d sh - s ch >= 0
+//     asin|cc| <= hs < pi/2 <=>  l (sh s + ch d) - |dot(axis, v)| rho >= 0
+// (sin(half + beta) times l, or times l rho).  d, s and 1/l are computed here
+// with the hardware reciprocal; a side decides only when it clears zero by
+// CONE_MARGIN times its scale (l, l^2 or l (rho + l), times max(1, |axis|)).
+// The float error of these sides, and the reference's own (rounding of cc and
+// of the angle sums, the last-place errors of acos, asin and atan2, mapped to
+// sines, where no error is larger than in angles and near |cc| = 1 the error of
+// acos shrinks to 2 |dcc|), stay below 1e-6 of that scale; inside the margin,
+// or when a side is NaN, the reference's form decides.  So the decision is the
+// reference's everywhere.  c, v and l are computed as the reference computes
+// them, so both take the same branch on l > radius.
+constexpr float CONE_MARGIN = 2e-5f;
+
+// the reference's form out of line: rarely reached, and kept out of the
+// bounded test's register allocation
+__device__ __noinline__ bool cone_overlap_ref(float4 cone, float radius, v3 o, v3 lo, v3 hi, float dist_to_box)
+{
+    return cone_overlap(cone, radius, o, lo, hi, dist_to_box);
+}
+
+__device__ __forceinline__ bool cone_overlap_bounded(float4 cone, float4 aux, v3 o, v3 lo, v3 hi, float dist_to_box)
+{
+    if (cone.w >= PI_2_F || dist_to_box < FLT_EPSILON) return true;
+    const v3 c = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+    const v3 v = sub(c, o);
+    const float l = sqrtf(dot(v, v));
+    const float radius = aux.x, sh = aux.y, ch = aux.z;
+    const float a = fabsf(dot(v3{cone.x, cone.y, cone.z}, v));
+    if (l > radius) {
+        const float m = CONE_MARGIN * l * aux.w;
+        const float t = radius - l * ch;
+        const float w = __builtin_amdgcn_sqrtf((l - radius) * (l + radius));
+        const float g = (sh * w + ch * radius) - a;
+        if (t >= m || g >= m) return true;
+        if (t <= -m && g <= -m) return false;
+    } else {
+        const v3 e = sub(hi, c);
+        const float sl = l * l - dot(e, v3{fabsf(v.x), fabsf(v.y), fabsf(v.z)});
+        const float ms = CONE_MARGIN * l * l;
+        if (sl <= -ms) return true;
+        if (sl >= ms) {
+            // projectToPlane(v / l, e) with the reciprocal
+            const float il = __builtin_amdgcn_rcpf(l);
+            const v3 n = {v.x * il, v.y * il, v.z * il};
+            const float sg = n.z >= 0.0f ? 1.0f : -1.0f;
+            const float pa = -__builtin_amdgcn_rcpf(sg + n.z);
+            const float pb = n.x * n.y * pa;
+            const v3 b1 = {1.0f + sg * n.x * n.x * pa, sg * pb, -sg * n.x};
+            const v3 b2 = {pb, sg + n.y * n.y * pa, -n.y};
+            const float r1 = dot(e, v3{fabsf(b1.x), fabsf(b1.y), fabsf(b1.z)});
+            const float r2 = dot(e, v3{fabsf(b2.x), fabsf(b2.y), fabsf(b2.z)});
+            const float d = __builtin_amdgcn_sqrtf(r1 * r1 + r2 * r2);
+            const float s = sl * il;
+            const float rho = __builtin_amdgcn_sqrtf(d * d + s * s);
+            const float sc = CONE_MARGIN * (rho + l) * aux.w;
+            const float t = d * sh - s * ch;
+            const float g = l * (sh * s + ch * d) - a * rho;
+            if (t >= sc || g >= sc * l) return true;
+            if (t <= -sc && g <= -sc * l) return false;
+        }
+    }
+    return cone_overlap_ref(cone, radius, o, lo, hi, dist_to_box);
 }
 
 struct SilRecord {
@@ -181,8 +272,9 @@ __device__ __forceinline__ float silhouette_distance(const SilRecord &s, v3 x, b
 
 // The children of record cur that the silhouette traversal visits (bvh.slang:580-627):
 // a child with a valid cone whose box overlaps the sphere and whose cone passes
-// BoundingCone.overlap; a0, b0 are the children's squared box distances.
-template <int DIM>
+// BoundingCone.overlap (the reference's form, or cone_overlap_bounded, which
+// decides the same); a0, b0 are the children's squared box distances.
+template <int DIM, bool BOUNDED>
 __device__ __forceinline__ void silhouette_children(const DevScene &S, const SilScene &T, uint32_t cur, v3 x,
                                                     typename Geo<DIM>::vec xb, float r2, bool &oA, bool &oB,
                                                     float &a0, float &b0, uint32_t &lA, uint32_t &lB)
@@ -200,13 +292,15 @@ __device__ __forceinline__ void silhouette_children(const DevScene &S, const Sil
     if (oA) {
         box_sqdist(loA, hiA, xb, a0, unused);
         oA = a0 <= r2;
-        if (oA) oA = cone_overlap(cA, rA.x, x, to3(loA), to3(hiA), a0);
+        if (oA) oA = BOUNDED ? cone_overlap_bounded(cA, rA, x, to3(loA), to3(hiA), a0)
+                             : cone_overlap(cA, rA.x, x, to3(loA), to3(hiA), a0);
     }
     oB = cB.w >= 0.0f;
     if (oB) {
         box_sqdist(loB, hiB, xb, b0, unused);
         oB = b0 <= r2;
-        if (oB) oB = cone_overlap(cB, rB.x, x, to3(loB), to3(hiB), b0);
+        if (oB) oB = BOUNDED ? cone_overlap_bounded(cB, rB, x, to3(loB), to3(hiB), b0)
+                             : cone_overlap(cB, rB.x, x, to3(loB), to3(hiB), b0);
     }
 }
 
@@ -264,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest_silhouettes(DevScene S, SilSc
                     uint32_t lA, lB;
                     float a0, b0;
                     bool oA, oB;
-                    silhouette_children<DIM>(S, T, cur, x, xb, r2, oA, oB, a0, b0, lA, lB);
+                    silhouette_children<DIM, false>(S, T, cur, x, xb, r2, oA, oB, a0, b0, lA, lB);
                     if (oA && oB) {
                         uint32_t nearL = lA, farL = lB;
                         float nearD = a0, farD = b0;
@@ -304,7 +398,9 @@ __global__ __launch_bounds__(BLOCK) void k_closest_silhouettes(DevScene S, SilSc
 // classification depends on r2, so the replay is exact; the winner's point is
 // recomputed at the end.  Items are (lane << 10 | record within the leaf), so
 // leaves hold at most 1023 records (the launch falls back to the per-lane
-// kernel otherwise).
+// kernel otherwise).  The cone tests take cone_overlap_bounded; the per-lane
+// kernel keeps the reference's form, so equal results of the two kernels check
+// one form against the other.
 // ---------------------------------------------------------------------------
 struct SilShared {
     uint16_t item[256];
@@ -370,7 +466,7 @@ __global__ __launch_bounds__(64) FCPW_EIGHT_WAVES void k_closest_silhouettes_coo
                 uint32_t lA, lB;
                 float a0, b0;
                 bool oA, oB;
-                silhouette_children<DIM>(S, T, cur, x, xb, r2, oA, oB, a0, b0, lA, lB);
+                silhouette_children<DIM, true>(S, T, cur, x, xb, r2, oA, oB, a0, b0, lA, lB);
                 live = oA || oB;
                 if (oA && oB) {
                     uint32_t nearL = lA, farL = lB;
@@ -529,7 +625,7 @@ __device__ __forceinline__ Cone load_cone(const float4 *cones, int64_t k)
 __device__ __forceinline__ void store_cone(float4 *cones, int64_t k, const Cone &c)
 {
     cones[k * 2] = make_float4(c.axis.x, c.axis.y, c.axis.z, c.half);
-    cones[k * 2 + 1] = make_float4(c.radius, 0.0f, 0.0f, 0.0f);
+    cones[k * 2 + 1] = cone_aux(c.axis.x, c.axis.y, c.axis.z, c.half, c.radius);
 }
 
 // box of child slot ref (rec * 2 + slot) or of the root (ref < 0), z = 0 in 2D
