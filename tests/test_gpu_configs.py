@@ -150,7 +150,8 @@ def test_adjoint_at_config_size_against_fp64(gpu, geodesic):
     # the stencil, within (4h + |p+ - p-|)/d of its value at x.  Where the
     # closest point jumps (|p+ - p-| > 20h) the stencil straddles a medial
     # ridge -- a tie between distant parts of the noisy surface, where d is not
-    # differentiable -- and the point is excluded (SURVEY's tie rule), counted.
+    # differentiable -- and the point is excluded (a tie is not a gradient
+    # error), counted.
     far = pts[dbest > 0.01]
     qq = np.concatenate([q64[far] + s_ * H * e for e in np.eye(3) for s_ in (1.0, -1.0)])
     d_pm, p_pm, i_pm = oracle.brute_closest_f64(V, F, qq.astype(np.float32))
