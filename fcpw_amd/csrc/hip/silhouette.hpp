@@ -108,7 +108,10 @@ __host__ __device__ inline float4 cone_aux(float ax, float ay, float az, float h
 // acos shrinks to 2 |dcc|), stay below 1e-6 of that scale; inside the margin,
 // or when a side is NaN, the reference's form decides.  So the decision is the
 // reference's everywhere.  c, v and l are computed as the reference computes
-// them, so both take the same branch on l > radius.
+// them, so both take the same branch on l > radius, and the analysis starts
+// from that v: its own rounding is shared.  Past the early accept on
+// dist_to_box the query lies outside the box, so l >= sqrt(FLT_EPSILON) and
+// no margin underflows.
 constexpr float CONE_MARGIN = 2e-5f;
 
 // the reference's form out of line: rarely reached, and kept out of the
