@@ -212,3 +212,58 @@ def test_cooperative_and_per_lane_kernels_give_the_same_bits(gpu, name):
     np.testing.assert_array_equal(run(dict(coop=1, sil_hold_min=1)), run(dict(coop=0)))
     g.set_option("coop", 1)
     g.set_option("sil_hold_min", 3)
+
+
+@pytest.mark.parametrize("name", ["torus", "walker2d", "bunny"])
+def test_cone_test_in_sines_near_its_boundaries(gpu, name):
+    """The cooperative kernel decides the cone test in sines with margins and
+    the per-lane kernel in the reference's trigonometric form: queries placed on
+    the bounding spheres of the cones (|c - x| = radius up to a relative 1e-7,
+    1e-5 and 1e-3 either side, where the two formulas switch between asin and
+    atan2) and just outside the boxes (where the early accept on the box
+    distance stops) give the same bits in both kernels, before and after a
+    device refit (which leaves non-unit cone axes)."""
+    if name not in CASES:
+        pytest.skip(f"no case {name}")
+    V, F = CASES[name]
+    dim = V.shape[1]
+    s = product_scene(V, F)
+    g = gpu_scene(s, dim)
+    b = oracle.Bvh(V, F)
+    b.build_silhouettes()
+    cones = b.silhouettes()[0]
+    pmin, pmax, nrefs, _ = b.nodes()
+    rng = np.random.default_rng(50)
+    keep = np.nonzero((cones[:, 3] >= 0) & (cones[:, 3] < np.pi / 2) & (cones[:, 4] > 0))[0]
+    pick = rng.choice(keep, min(len(keep), 3000), replace=False)
+    c = 0.5 * (pmin[pick, :dim] + pmax[pick, :dim]).astype(np.float64)
+    rad = cones[pick, 4].astype(np.float64)
+    u = rng.normal(size=(len(pick), dim))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    qs = [c + (rad * (1 + f))[:, None] * u for f in (0.0, 1e-7, -1e-7, 1e-5, -1e-5, 1e-3, -1e-3)]
+    half = 0.5 * (pmax[pick, :dim] - pmin[pick, :dim]).astype(np.float64)
+    qs.append(c + (half + 1e-3 * (1 + np.abs(c))) * np.sign(u))  # just outside each box, by a corner
+    q = np.vstack(qs).astype(np.float32)
+    sph = torch.zeros((len(q), 4), dtype=torch.float32, device="cuda")
+    sph[:, :dim] = torch.from_numpy(q).cuda()
+    sph[:, 3] = float("inf")
+    fl = torch.from_numpy(rng.integers(0, 2, len(q)).astype(np.int32)).cuda()
+
+    def run(coop):
+        g.set_option("coop", coop)
+        out = torch.full((len(q), 10), -7.0, dtype=torch.float32, device="cuda")
+        g.find_closest_silhouette_points_device(sph, fl, out)
+        torch.cuda.synchronize()
+        return out.cpu().numpy().view(np.uint32)
+
+    try:
+        first = run(1)
+        np.testing.assert_array_equal(first, run(0))
+        assert (first[:, 9] != 0xFFFFFFFF).mean() > 0.5  # most queries find a silhouette point
+        ext = float(np.abs(V).max())
+        V2 = (V + np.random.default_rng(51).uniform(-0.01, 0.01, V.shape) * ext).astype(np.float32)
+        s.update_object_vertices(V2, 0)
+        g.refit(s)
+        np.testing.assert_array_equal(run(1), run(0))
+    finally:
+        g.set_option("coop", 1)
