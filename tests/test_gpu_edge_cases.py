@@ -167,3 +167,37 @@ def test_backward_sees_the_geometry_of_its_own_forward(gpu):
     with pytest.raises(RuntimeError, match="no V to restore"):
         d3.sum().backward()
     torch.cuda.synchronize()
+
+
+def test_triangle_regions_and_their_boundaries(gpu):
+    """The triangle routine's seven Voronoi regions and the planes between them:
+    the closed-form answers of the unit right triangle (vertex, edge and face
+    regions) within FP32 rounding, and a grid of queries with coordinates in
+    {-1, -0.5, 0, 0.25, 0.5, 1, 1.5} (many exactly on a region boundary, where
+    the reference's <= and >= decide), plus queries at the vertices and edge
+    midpoints of a soup nudged by +-1 ulp, bit-exact against the oracle (d, p,
+    uv, n, index)"""
+    from tests.test_oracle_primitives import TRI_CASES, TRI_F, TRI_V
+    scene, g = make_scene(TRI_V, TRI_F)
+    q = np.array([c[0] for c in TRI_CASES], np.float32)
+    hits = fcpw.GPUInteractionList()
+    g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits, record_normals=True)
+    rec = records(hits)
+    for k, (_, p, uv, d) in enumerate(TRI_CASES):
+        np.testing.assert_allclose(rec["p"][k], p, atol=1e-7)
+        np.testing.assert_allclose(rec["uv"][k], uv, atol=1e-7)
+        assert abs(rec["d"][k] - d) <= 1e-6 * d
+    vals = np.array([-1.0, -0.5, 0.0, 0.25, 0.5, 1.0, 1.5], np.float32)
+    grid = np.stack(np.meshgrid(vals, vals, np.array([-1.0, 0.0, 1.0], np.float32), indexing="ij"), -1)
+    q = grid.reshape(-1, 3).astype(np.float32)
+    g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits, record_normals=True)
+    assert_same(records(hits), oracle.Bvh(TRI_V, TRI_F).closest_points_gpuorder(q, record_normal=True))
+    V, F = meshes.triangle_soup(400)
+    scene, g = make_scene(V, F)
+    rng = np.random.default_rng(52)
+    tri = V[F]
+    pts = np.concatenate([tri.reshape(-1, 3), 0.5 * (tri + np.roll(tri, 1, axis=1)).reshape(-1, 3)])
+    q = np.concatenate([pts, np.nextafter(pts, pts + 1), np.nextafter(pts, pts - 1),
+                        pts + rng.normal(scale=1e-3, size=pts.shape)]).astype(np.float32)
+    g.find_closest_points(q, np.full(len(q), np.inf, np.float32), hits, record_normals=True)
+    assert_same(records(hits), oracle.Bvh(V, F).closest_points_gpuorder(q, record_normal=True))
