@@ -1,6 +1,6 @@
 """The host-pointer entry points (GPUScene.find_closest_points, intersect,
 sphere intersection, closest silhouette points) copy queries in and records out
-through pinned bounce buffers in chunks of 4 MiB, and reuse the caller's record
+through pinned bounce buffers in chunks of 8 MiB, and reuse the caller's record
 list: with more queries than one chunk holds (and a ragged last chunk), two
 calls in a row on the same list give the bits of a fresh list and of the
 device entry point, and the second call leaves nothing of the first."""
@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 fcpw = pytest.importorskip("fcpw_amd")
 
-N = 300_001  # 12 MB of records: three chunks, the last one partial
+N = 300_001  # 12 MB of records: two chunks, the last one partial
 
 
 @pytest.fixture(scope="module")
@@ -28,9 +28,9 @@ def torus():
     return V, F, s, g
 
 
-@pytest.mark.parametrize("n", [262_144, 262_145, N])
+@pytest.mark.parametrize("n", [524_288, 524_289, N])
 def test_closest_points_through_a_reused_list(gpu, torus, n):
-    """262,144 spheres fill exactly one 4 MiB chunk (a plain copy in), 262,145
+    """524,288 spheres fill exactly one 8 MiB chunk (a plain copy in), 524,289
     add a 16-byte second chunk; the records (40 bytes each) span several chunks"""
     V, F, s, g = torus
     ob = oracle.Bvh(V, F)
